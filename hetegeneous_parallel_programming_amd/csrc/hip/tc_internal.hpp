@@ -1,0 +1,94 @@
+// Internal state of the device library shared by tc_runtime.cpp,
+// tc_batch.cpp, tc_server.cpp and tc_tune.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tc.h"
+#include "tc_kernels.hpp"
+
+namespace tcr {
+
+// Records the calling thread's error text and returns `st`.
+tc_status fail(tc_status st, const std::string& msg);
+
+#define TC_HIP_TRY(expr)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::tcr::fail(TC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Device allocation with an out-of-memory status instead of an abort.
+tc_status dev_alloc(void** p, size_t bytes, const char* what);
+void dev_free(void* p);
+
+// Plan scratch (binning output and counters) sized for P pivots / k outputs.
+struct PlanScratch {
+  uint8_t* bin_of = nullptr;
+  uint64_t* items = nullptr;
+  int64_t cap = 0;
+  uint32_t* counters = nullptr;
+  uint64_t* outs = nullptr;
+  int64_t outs_cap = 0;
+  tc_status ensure(int64_t pivots, int64_t nouts);
+  void release();
+  tck::PlanBufs bufs() const { return tck::PlanBufs{bin_of, items, counters, outs}; }
+};
+
+// Makes ctx's device current for the scope of one API call.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+tck::Tune to_kernel_tune(const tc_tune& t);
+tc_status check_tune(const tc_tune& t);
+tc_tune default_tune();
+
+}  // namespace tcr
+
+struct tc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int cus = 0;
+  int count_grid = 0;   // full-residency grid of the persistent kernel
+  int plain_grid = 0;   // full-residency grid of the plain kernel
+  tc_tune tune{};
+  std::recursive_mutex mu;
+  std::string name;
+  tcr::PlanScratch scratch;
+  uint64_t* pinned = nullptr;  // 64 B count + 64 B bin counters
+  int grid() const { return tune.grid > 0 ? tune.grid : count_grid; }
+};
+
+struct tc_graph {
+  tc_ctx* ctx = nullptr;
+  int64_t* rp = nullptr;
+  int32_t* ci = nullptr;
+  int32_t n = 0;
+  int64_t nnz = 0;
+  int32_t row_begin = 0, row_end = 0;
+  int32_t max_out = 0;
+  tck::GraphDesc* d_desc = nullptr;
+  int64_t bytes = 0;
+};

@@ -1,0 +1,430 @@
+// Triangle-counting kernels for gfx950 (CDNA4, wave64).
+//
+// The counted quantity is the reference's bs_warp_vertex loop nest
+// (bs_warp_vertex.cuh:13-22): sum over pivots u, over v in N+(u), of
+// |N+(u) ∩ N+(v)| on the degree-ordered DAG, i.e. every triangle once.
+//
+// Work unit.  A group of G lanes (G = 8, 16, 32 or 64) owns one pivot u.
+// It stages N+(u) in LDS, then walks the edges (u, v) in batches of G: lane
+// j of the batch reads rowptr[v], rowptr[v+1] for its edge, a group scan of
+// the d+(v) gives each edge's slice of a flat index space, and the group's
+// lanes then stride that space with coalesced int32 reads of the N+(v)
+// lists.  Each read neighbour w is binary-searched in the LDS copy of N+(u);
+// the shorter list of the pair (the streamed N+(v) when d+(v) <= d+(u)) is
+// the one whose elements are looked up, as in the reference's warp_intersect
+// (successful_push_pull/graph_gpu.h:228-252), and every element of N+(v) is
+// read exactly once per edge.  Four lookups per lane are kept in flight so
+// that the HBM/L2 reads and the LDS search chains overlap.
+//
+// Scheduling.  A planning pass bins pivots by d+(u) and by the wedge work
+// T(u) = sum of d+(v) over v in N+(u); light pivots go to 8/16/32-lane
+// groups, medium ones to a wave, heavy ones to a whole workgroup whose four
+// waves share one LDS copy of N+(u).  A persistent grid then pulls tiles
+// (bin, run of pivots) from a device queue, heaviest bins first.  Counts are
+// u64 sums, so the result does not depend on tile order.
+#include <climits>
+
+#include "tc_kernels.hpp"
+
+namespace tck {
+
+// Orders LDS traffic between lanes of one wave (a wave's LDS operations are
+// performed in issue order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exclusive scan of x over the G lanes of a group; total = group sum.
+template <int G>
+__device__ __forceinline__ int group_excl_scan(int x, int gl, int& total) {
+  int incl = x;
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int y = __shfl_up(incl, d, G);
+    if (gl >= d) incl += y;
+  }
+  total = __shfl(incl, G - 1, G);
+  return incl - x;
+}
+
+// Largest k in [0, G) with sP[k] <= t (sP non-decreasing, sP[0] = 0 <= t).
+template <int G>
+__device__ __forceinline__ int find_edge(const int32_t* sP, int t) {
+  int k = 0;
+#pragma unroll
+  for (int step = G / 2; step >= 1; step >>= 1)
+    if (sP[k + step] <= t) k += step;
+  return k;
+}
+
+constexpr int kInFlight = 4;  // lookups per lane per iteration
+
+// Counts, for the edges j = jfirst, jfirst + jstride, ... of pivot u (list
+// A[0, da), in LDS or HBM), the elements of N+(v) found in A.  sP/sOff are
+// this group's G-entry LDS scratch.  Returns this lane's share of the hits.
+template <int G>
+__device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
+                                                const int32_t* __restrict__ ci, const int32_t* A,
+                                                int da, int jfirst, int jstride, int32_t* sP,
+                                                int64_t* sOff, int gl) {
+  uint32_t hits = 0;
+  for (int j0 = jfirst; j0 < da; j0 += jstride) {
+    const int j = j0 + gl;
+    int c = 0;
+    int64_t off = 0;
+    if (j < da) {
+      const int v = A[j];
+      const int64_t b = rp[v];
+      c = (int)(rp[v + 1] - b);
+      off = b;
+    }
+    int total;
+    const int excl = group_excl_scan<G>(c, gl, total);
+    sP[gl] = excl;
+    sOff[gl] = off;
+    wave_sync();
+    for (int t0 = 0; t0 < total; t0 += G * kInFlight) {
+      int w[kInFlight];
+#pragma unroll
+      for (int q = 0; q < kInFlight; ++q) {
+        const int t = t0 + q * G + gl;
+        w[q] = INT_MIN;  // ids are >= 0: never found
+        if (t < total) {
+          const int k = find_edge<G>(sP, t);
+          w[q] = ci[sOff[k] + (t - sP[k])];
+        }
+      }
+      int lo[kInFlight];
+#pragma unroll
+      for (int q = 0; q < kInFlight; ++q) lo[q] = 0;
+      for (int len = da; len > 1;) {
+        const int half = len >> 1;
+#pragma unroll
+        for (int q = 0; q < kInFlight; ++q) lo[q] = (A[lo[q] + half] <= w[q]) ? lo[q] + half : lo[q];
+        len -= half;
+      }
+#pragma unroll
+      for (int q = 0; q < kInFlight; ++q) hits += (A[lo[q]] == w[q]) ? 1u : 0u;
+    }
+    wave_sync();
+  }
+  return hits;
+}
+
+__device__ __forceinline__ uint64_t shfl_down_u64(uint64_t x, int d) {
+  const uint32_t lo = __shfl_down((uint32_t)x, d, 64);
+  const uint32_t hi = __shfl_down((uint32_t)(x >> 32), d, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Sum over the workgroup; result valid in thread 0.  All threads must call.
+__device__ __forceinline__ uint64_t block_sum(uint64_t x, uint64_t* red) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += shfl_down_u64(x, d);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  uint64_t s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kWaves; ++i) s += red[i];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// zero fill (used instead of memset nodes inside captured graphs)
+// ---------------------------------------------------------------------------
+__global__ void zero_kernel(uint32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// planning: bin every pivot, then scatter (graph, row) items grouped by bin
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_graph(const GraphDesc* descs, int ngraphs, int64_t p) {
+  int lo = 0, hi = ngraphs - 1;  // largest g with plan_base <= p
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].plan_base <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int choose_bin(const int64_t* rp, const int32_t* ci, int64_t rb,
+                                          int64_t da, const Tune& t) {
+  if (da < 2) return kNoBin;
+  if (da > kCapBlock) return BIN_GLOBAL;
+  if (da > kCapWave) return BIN_BLOCK;
+  int64_t work = 0;
+  for (int64_t e = rb; e < rb + da; ++e) {
+    const int v = ci[e];
+    work += rp[v + 1] - rp[v];
+    if (work > t.t_wave) return BIN_BLOCK;
+  }
+  if (work == 0) return kNoBin;
+  if (da > kCapG32 || work > t.t_g32) return BIN_WAVE;
+  if (da > kCapG16 || work > t.t_g16) return BIN_G32;
+  if (da > kCapG8 || work > t.t_g8) return BIN_G16;
+  return BIN_G8;
+}
+
+__global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
+  __shared__ uint32_t s_cnt[kNumBins];
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (threadIdx.x < kNumBins) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  if (p < a.P) {
+    const int g = find_graph(a.descs, a.ngraphs, p);
+    const GraphDesc d = a.descs[g];
+    const int64_t u = d.row_begin + (p - d.plan_base);
+    const int64_t* rp = a.rp + d.rp_off;
+    const int32_t* ci = a.ci + d.ci_off;
+    const int64_t rb = rp[u];
+    const int bin = choose_bin(rp, ci, rb, rp[u + 1] - rb, a.tune);
+    a.bufs.bin_of[p] = (uint8_t)bin;
+    if (bin != kNoBin) atomicAdd(&s_cnt[bin], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumBins && s_cnt[threadIdx.x])
+    atomicAdd(&a.bufs.counters[kCntBinCount + threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
+  __shared__ uint32_t s_cnt[kNumBins];
+  __shared__ uint32_t s_base[kNumBins];
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (threadIdx.x < kNumBins) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int bin = kNoBin;
+  uint32_t rank = 0;
+  if (p < a.P) {
+    bin = a.bufs.bin_of[p];
+    if (bin != kNoBin) rank = atomicAdd(&s_cnt[bin], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumBins && s_cnt[threadIdx.x])
+    s_base[threadIdx.x] = atomicAdd(&a.bufs.counters[kCntBinCursor + threadIdx.x], s_cnt[threadIdx.x]);
+  __syncthreads();
+  if (bin != kNoBin) {
+    int64_t seg = 0;
+    for (int b = 0; b < bin; ++b) seg += a.bufs.counters[kCntBinCount + b];
+    const int g = find_graph(a.descs, a.ngraphs, p);
+    const GraphDesc d = a.descs[g];
+    const int64_t u = d.row_begin + (p - d.plan_base);
+    a.bufs.items[seg + s_base[bin] + rank] = ((uint64_t)(uint32_t)g << 32) | (uint32_t)u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// persistent count
+// ---------------------------------------------------------------------------
+struct __align__(16) CountLds {
+  int32_t A[kCapBlock];   // pivot-list slots (layout depends on the tile's bin)
+  int32_t P[kBlock];      // per-lane edge prefix
+  int64_t Off[kBlock];    // per-lane N+(v) offset
+  uint64_t red[kWaves];
+  uint32_t tile;
+};
+
+// Per-lane running count; in batch mode it is flushed to outs[g] whenever
+// the lane moves to a pivot of another graph.
+struct Acc {
+  uint64_t sum = 0;
+  int g = -1;
+  __device__ __forceinline__ void to_graph(int ng, bool batch, uint64_t* outs) {
+    if (batch && ng != g) {
+      if (sum) atomicAdd(&outs[g], sum);
+      sum = 0;
+      g = ng;
+    }
+  }
+};
+
+template <int G, int CAP>
+__device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* items, int64_t first,
+                                           int64_t last, CountLds& L, bool batch, Acc& acc) {
+  constexpr int NGRP = kBlock / G;
+  const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+  int32_t* slot = L.A + grp * CAP;
+  int32_t* sP = L.P + grp * G;
+  int64_t* sOff = L.Off + grp * G;
+  for (int64_t i = first + grp; i < last; i += NGRP) {
+    const uint64_t item = items[i];
+    const int g = (int)(item >> 32);
+    const int u = (int)(uint32_t)item;
+    const GraphDesc d = a.descs[g];
+    const int64_t* rp = a.rp + d.rp_off;
+    const int32_t* ci = a.ci + d.ci_off;
+    const int64_t rb = rp[u];
+    const int da = (int)(rp[u + 1] - rb);
+    for (int x = gl; x < da; x += G) slot[x] = ci[rb + x];
+    wave_sync();
+    const uint32_t h = pivot_edges<G>(rp, ci, slot, da, 0, G, sP, sOff, gl);
+    acc.to_graph(g, batch, a.bufs.outs);
+    acc.sum += h;
+    wave_sync();
+  }
+}
+
+template <bool kLdsPivot>
+__device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, CountLds& L, bool batch,
+                                            Acc& acc) {
+  const int g = (int)(item >> 32);
+  const int u = (int)(uint32_t)item;
+  const GraphDesc d = a.descs[g];
+  const int64_t* rp = a.rp + d.rp_off;
+  const int32_t* ci = a.ci + d.ci_off;
+  const int64_t rb = rp[u];
+  const int da = (int)(rp[u + 1] - rb);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t h;
+  if (kLdsPivot) {
+    for (int x = threadIdx.x; x < da; x += kBlock) L.A[x] = ci[rb + x];
+    __syncthreads();
+    h = pivot_edges<64>(rp, ci, L.A, da, wid * 64, kBlock, L.P + wid * 64, L.Off + wid * 64, lane);
+  } else {
+    h = pivot_edges<64>(rp, ci, ci + rb, da, wid * 64, kBlock, L.P + wid * 64, L.Off + wid * 64, lane);
+  }
+  acc.to_graph(g, batch, a.bufs.outs);
+  acc.sum += h;
+}
+
+__global__ __launch_bounds__(kBlock) void count_kernel(LaunchArgs a) {
+  __shared__ CountLds L;
+  const bool batch = a.ngraphs > 1;
+  // Tile table: bins in order, each split into runs of tile_piv pivots.
+  int64_t cnt[kNumBins], seg[kNumBins], tiles_end[kNumBins];
+  int tile_piv[kNumBins];
+  const int r = a.tune.rounds > 0 ? a.tune.rounds : 1;
+  tile_piv[BIN_GLOBAL] = 1;
+  tile_piv[BIN_BLOCK] = 1;
+  tile_piv[BIN_WAVE] = kWaves;
+  tile_piv[BIN_G32] = (kBlock / 32) * r;
+  tile_piv[BIN_G16] = (kBlock / 16) * r;
+  tile_piv[BIN_G8] = (kBlock / 8) * r;
+  int64_t s = 0, tacc = 0;
+#pragma unroll
+  for (int b = 0; b < kNumBins; ++b) {
+    cnt[b] = a.bufs.counters[kCntBinCount + b];
+    seg[b] = s;
+    s += cnt[b];
+    tacc += (cnt[b] + tile_piv[b] - 1) / tile_piv[b];
+    tiles_end[b] = tacc;
+  }
+  const int64_t ntiles = tacc;
+  Acc acc;
+  acc.g = 0;
+  for (;;) {
+    if (threadIdx.x == 0) L.tile = atomicAdd(&a.bufs.counters[kCntTileHead], 1u);
+    __syncthreads();
+    const int64_t t = L.tile;
+    if (t >= ntiles) break;
+    int b = 0;
+    while (t >= tiles_end[b]) ++b;
+    const int64_t tfirst = (b == 0) ? 0 : tiles_end[b - 1];
+    const int64_t first = (t - tfirst) * tile_piv[b];
+    const int64_t last = min(first + (int64_t)tile_piv[b], cnt[b]);
+    const uint64_t* items = a.bufs.items + seg[b];
+    switch (b) {
+      case BIN_GLOBAL: block_pivot<false>(a, items[first], L, batch, acc); break;
+      case BIN_BLOCK: block_pivot<true>(a, items[first], L, batch, acc); break;
+      case BIN_WAVE: group_tile<64, kCapWave>(a, items, first, last, L, batch, acc); break;
+      case BIN_G32: group_tile<32, kCapG32>(a, items, first, last, L, batch, acc); break;
+      case BIN_G16: group_tile<16, kCapG16>(a, items, first, last, L, batch, acc); break;
+      default: group_tile<8, kCapG8>(a, items, first, last, L, batch, acc); break;
+    }
+    __syncthreads();
+  }
+  if (batch) {
+    if (acc.sum) atomicAdd(&a.bufs.outs[acc.g], acc.sum);
+  } else {
+    const uint64_t tot = block_sum(acc.sum, L.red);
+    if (threadIdx.x == 0 && tot) atomicAdd(&a.bufs.outs[0], tot);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plain kernel: one wave64 per pivot, grid-stride (bs_warp_vertex.cuh:2-35)
+// ---------------------------------------------------------------------------
+struct __align__(16) PlainLds {
+  int32_t A[kWaves][kCapWave];
+  int32_t P[kBlock];
+  int64_t Off[kBlock];
+  uint64_t red[kWaves];
+};
+
+__global__ __launch_bounds__(kBlock) void plain_kernel(const int64_t* __restrict__ rp,
+                                                       const int32_t* __restrict__ ci,
+                                                       int32_t row_begin, int32_t row_end,
+                                                       uint64_t* out) {
+  __shared__ PlainLds L;
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * kWaves;
+  uint64_t acc = 0;
+  for (int64_t u = row_begin + (int64_t)blockIdx.x * kWaves + wid; u < row_end; u += nw) {
+    const int64_t rb = rp[u];
+    const int da = (int)(rp[u + 1] - rb);
+    if (da < 2) continue;
+    if (da <= kCapWave) {
+      for (int x = lane; x < da; x += 64) L.A[wid][x] = ci[rb + x];
+      wave_sync();
+      acc += pivot_edges<64>(rp, ci, L.A[wid], da, 0, 64, L.P + wid * 64, L.Off + wid * 64, lane);
+      wave_sync();
+    } else {
+      acc += pivot_edges<64>(rp, ci, ci + rb, da, 0, 64, L.P + wid * 64, L.Off + wid * 64, lane);
+    }
+  }
+  const uint64_t tot = block_sum(acc, L.red);
+  if (threadIdx.x == 0 && tot) atomicAdd(out, tot);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_zero(void* p, int64_t words32, hipStream_t s) {
+  if (words32 <= 0) return hipSuccess;
+  int64_t blocks = (words32 + kBlock - 1) / kBlock;
+  if (blocks > 4096) blocks = 4096;
+  zero_kernel<<<(unsigned)blocks, kBlock, 0, s>>>((uint32_t*)p, words32);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
+  if (a.P <= 0) return hipSuccess;
+  const int64_t blocks = (a.P + kBlock - 1) / kBlock;
+  plan_bin_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  plan_scatter_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const LaunchArgs& a, int grid, hipStream_t s) {
+  count_kernel<<<grid, kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
+                        uint64_t* out, int grid, hipStream_t s) {
+  plain_kernel<<<grid, kBlock, 0, s>>>(rp, ci, row_begin, row_end, out);
+  return hipGetLastError();
+}
+
+int count_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel, kBlock, 0) != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}
+
+int plain_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, plain_kernel, kBlock, 0) != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}
+
+}  // namespace tck

@@ -1,0 +1,337 @@
+// Device runtime of the triangle-counting library: contexts, graphs in HBM
+// and the single-graph count entry points of tc.h.
+//
+// Reference counterparts: TCSolver (successful_push_pull/multigpu_nvshmem_pushpull.cu:29-85:
+// device check, full-residency grid, device graph, zeroed count, launch,
+// copy back) and GraphGPU::init (successful_push_pull/graph_gpu.h:84-137).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "tc_internal.hpp"
+
+namespace tcr {
+
+static thread_local std::string g_err;
+
+tc_status fail(tc_status st, const std::string& msg) {
+  g_err = msg;
+  return st;
+}
+
+tc_status dev_alloc(void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? TC_ENOMEM : TC_EDEVICE,
+                std::string("hipMalloc(") + what + ", " + std::to_string(bytes) + " B): " +
+                    hipGetErrorString(e));
+  }
+  return TC_OK;
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+tc_status PlanScratch::ensure(int64_t pivots, int64_t nouts) {
+  if (!counters) {
+    tc_status st = dev_alloc((void**)&counters, sizeof(uint32_t) * 64, "plan counters");
+    if (st != TC_OK) return st;
+  }
+  if (pivots > cap) {
+    dev_free(bin_of);
+    dev_free(items);
+    bin_of = nullptr;
+    items = nullptr;
+    cap = 0;
+    int64_t c = std::max<int64_t>(pivots, 1024);
+    tc_status st = dev_alloc((void**)&bin_of, (size_t)c, "pivot bins");
+    if (st != TC_OK) return st;
+    st = dev_alloc((void**)&items, sizeof(uint64_t) * (size_t)c, "pivot items");
+    if (st != TC_OK) return st;
+    cap = c;
+  }
+  if (nouts > outs_cap) {
+    dev_free(outs);
+    outs = nullptr;
+    outs_cap = 0;
+    int64_t c = std::max<int64_t>(nouts, 16);
+    tc_status st = dev_alloc((void**)&outs, sizeof(uint64_t) * (size_t)c, "counts");
+    if (st != TC_OK) return st;
+    outs_cap = c;
+  }
+  return TC_OK;
+}
+
+void PlanScratch::release() {
+  dev_free(bin_of);
+  dev_free(items);
+  dev_free(counters);
+  dev_free(outs);
+  *this = PlanScratch();
+}
+
+tc_tune default_tune() {
+  tc_tune t;
+  t.t_g8 = 256;
+  t.t_g16 = 1024;
+  t.t_g32 = 4096;
+  t.t_wave = 16384;
+  t.rounds = 4;
+  t.grid = 0;
+  return t;
+}
+
+tc_status check_tune(const tc_tune& t) {
+  if (t.t_g8 < 0 || t.t_g16 < t.t_g8 || t.t_g32 < t.t_g16 || t.t_wave < t.t_g32)
+    return fail(TC_EINVAL, "tune thresholds must satisfy 0 <= t_g8 <= t_g16 <= t_g32 <= t_wave");
+  if (t.rounds < 1 || t.rounds > 64) return fail(TC_EINVAL, "tune rounds must be in [1, 64]");
+  if (t.grid < 0 || t.grid > (1 << 20)) return fail(TC_EINVAL, "tune grid must be in [0, 2^20]");
+  return TC_OK;
+}
+
+tck::Tune to_kernel_tune(const tc_tune& t) {
+  return tck::Tune{t.t_g8, t.t_g16, t.t_g32, t.t_wave, t.rounds};
+}
+
+// Host-side checks shared by every upload: shape (and more when asked), and
+// the per-batch int32 element total (64 lists of at most max_out ids).
+static tc_status check_dag(const tc_csr_view* dag, uint32_t check, int32_t* max_out) {
+  if (!dag) return fail(TC_EINVAL, "null csr view");
+  tc_status st = tch_validate(dag, check | TC_CHECK_SHAPE);
+  if (st != TC_OK) return fail(st, tch_last_error());
+  int64_t mo = 0;
+  for (int64_t v = 0; v < dag->n_rows; ++v) mo = std::max<int64_t>(mo, dag->rowptr[v + 1] - dag->rowptr[v]);
+  if (mo * 64 >= ((int64_t)1 << 31))
+    return fail(TC_ERANGE, "DAG out-degree too large (max_out * 64 must fit in int32)");
+  *max_out = (int32_t)mo;
+  return TC_OK;
+}
+
+static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_begin, int32_t row_end,
+                            int32_t max_out, tc_graph** out) {
+  std::unique_ptr<tc_graph> g(new tc_graph);
+  g->ctx = ctx;
+  g->n = dag->n_rows;
+  g->nnz = dag->nnz;
+  g->row_begin = row_begin;
+  g->row_end = row_end;
+  g->max_out = max_out;
+  auto cleanup = [&]() {
+    dev_free(g->rp);
+    dev_free(g->ci);
+    dev_free(g->d_desc);
+  };
+  tc_status st = dev_alloc((void**)&g->rp, sizeof(int64_t) * ((size_t)g->n + 1), "rowptr");
+  if (st == TC_OK) st = dev_alloc((void**)&g->ci, sizeof(int32_t) * (size_t)std::max<int64_t>(g->nnz, 1), "colidx");
+  if (st == TC_OK) st = dev_alloc((void**)&g->d_desc, sizeof(tck::GraphDesc), "graph desc");
+  if (st != TC_OK) {
+    cleanup();
+    return st;
+  }
+  tck::GraphDesc d{0, 0, 0, row_begin, row_end};
+  hipError_t e = hipMemcpy(g->rp, dag->rowptr, sizeof(int64_t) * ((size_t)g->n + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess && g->nnz > 0)
+    e = hipMemcpy(g->ci, dag->colidx, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_desc, &d, sizeof(d), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(TC_EDEVICE, std::string("graph upload: ") + hipGetErrorString(e));
+  }
+  g->bytes = sizeof(int64_t) * ((int64_t)g->n + 1) + sizeof(int32_t) * g->nnz + sizeof(tck::GraphDesc);
+  *out = g.release();
+  return TC_OK;
+}
+
+static tc_status finish_stats(tc_ctx* ctx, tc_stats* st, double t0, bool planned, int64_t pivots) {
+  if (!st) return TC_OK;
+  std::memset(st, 0, sizeof(*st));
+  st->total_ms = now_ms() - t0;
+  float ms = 0;
+  if (planned) {
+    TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    st->plan_ms = ms;
+    const uint32_t* c = (const uint32_t*)(ctx->pinned + 8);
+    for (int b = 0; b < TC_NUM_BINS; ++b) st->bin_pivots[b] = c[tck::kCntBinCount + b];
+  }
+  TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+  st->count_ms = ms;
+  st->pivots = pivots;
+  return TC_OK;
+}
+
+}  // namespace tcr
+
+using namespace tcr;
+
+extern "C" {
+
+const char* tc_last_error(const tc_ctx*) { return tcr::g_err.c_str(); }
+
+tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
+  if (!out) return fail(TC_EINVAL, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(TC_EDEVICE, std::string("no HIP device available: ") + hipGetErrorString(e));
+  if (device < 0 || device >= ndev)
+    return fail(TC_EDEVICE, "device " + std::to_string(device) + " not present (" + std::to_string(ndev) + " visible)");
+  std::unique_ptr<tc_ctx> c(new tc_ctx);
+  c->device = device;
+  DeviceGuard guard(device);
+  hipDeviceProp_t prop;
+  TC_HIP_TRY(hipGetDeviceProperties(&prop, device));
+  c->cus = prop.multiProcessorCount;
+  c->name = std::string(prop.name) + " (" + prop.gcnArchName + ", " + std::to_string(c->cus) + " CUs)";
+  TC_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
+  TC_HIP_TRY(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));
+  c->count_grid = tck::count_blocks_per_cu() * c->cus;
+  c->plain_grid = tck::plain_blocks_per_cu() * c->cus;
+  c->tune = default_tune();
+  *out = c.release();
+  return TC_OK;
+}
+
+void tc_ctx_destroy(tc_ctx* ctx) {
+  if (!ctx) return;
+  {
+    DeviceGuard guard(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->scratch.release();
+    for (auto& ev : ctx->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+tc_status tc_get_tune(tc_ctx* ctx, tc_tune* out) {
+  if (!ctx || !out) return fail(TC_EINVAL, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  *out = ctx->tune;
+  return TC_OK;
+}
+
+tc_status tc_set_tune(tc_ctx* ctx, const tc_tune* t) {
+  if (!ctx || !t) return fail(TC_EINVAL, "null argument");
+  tc_status st = check_tune(*t);
+  if (st != TC_OK) return st;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  ctx->tune = *t;
+  return TC_OK;
+}
+
+tc_status tc_device_name(tc_ctx* ctx, char* buf, int32_t len) {
+  if (!ctx || !buf || len <= 0) return fail(TC_EINVAL, "null argument");
+  std::snprintf(buf, (size_t)len, "%s", ctx->name.c_str());
+  return TC_OK;
+}
+
+tc_status tc_graph_upload(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, tc_graph** out) {
+  if (!ctx || !out) return fail(TC_EINVAL, "null argument");
+  int32_t max_out = 0;
+  tc_status st = check_dag(dag, check, &max_out);
+  if (st != TC_OK) return st;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  return upload_csr(ctx, dag, 0, dag->n_rows, max_out, out);
+}
+
+tc_status tc_partition_upload(tc_ctx* ctx, const tch_partition_view* part, uint32_t check,
+                              tc_graph** out) {
+  if (!ctx || !part || !out) return fail(TC_EINVAL, "null argument");
+  tch_csr* full = nullptr;
+  tc_status st = tch_partition_assemble(part, &full);
+  if (st != TC_OK) return fail(st, tch_last_error());
+  tc_csr_view v;
+  tch_csr_view(full, &v);
+  int32_t max_out = 0;
+  st = check_dag(&v, check, &max_out);
+  if (st == TC_OK) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    st = upload_csr(ctx, &v, part->row_begin, part->row_end, max_out, out);
+  }
+  tch_csr_free(full);
+  return st;
+}
+
+tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) {
+  if (!g) return TC_OK;
+  if (ctx && g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  tc_ctx* c = g->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DeviceGuard guard(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(g->rp);
+  dev_free(g->ci);
+  dev_free(g->d_desc);
+  delete g;
+  return TC_OK;
+}
+
+int64_t tc_graph_bytes(const tc_graph* g) { return g ? g->bytes : 0; }
+
+tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  const double t0 = now_ms();
+  const int64_t P = (int64_t)g->row_end - g->row_begin;
+  tc_status st = ctx->scratch.ensure(P, 1);
+  if (st != TC_OK) return st;
+  tck::LaunchArgs a;
+  a.descs = g->d_desc;
+  a.ngraphs = 1;
+  a.P = P;
+  a.rp = g->rp;
+  a.ci = g->ci;
+  a.tune = to_kernel_tune(ctx->tune);
+  a.bufs = ctx->scratch.bufs();
+  hipStream_t s = ctx->stream;
+  TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
+  TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
+  TC_HIP_TRY(tck::launch_plan(a, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  TC_HIP_TRY(tck::launch_count(a, ctx->grid(), s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,
+                            hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipStreamSynchronize(s));
+  *total = ctx->pinned[0];
+  return finish_stats(ctx, stats, t0, true, P);
+}
+
+tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  const double t0 = now_ms();
+  const int64_t P = (int64_t)g->row_end - g->row_begin;
+  tc_status st = ctx->scratch.ensure(0, 1);
+  if (st != TC_OK) return st;
+  hipStream_t s = ctx->stream;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->plain_grid, (P + tck::kWaves - 1) / tck::kWaves));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  TC_HIP_TRY(tck::launch_zero(ctx->scratch.outs, 2, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, ctx->scratch.outs, grid, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->scratch.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipStreamSynchronize(s));
+  *total = ctx->pinned[0];
+  return finish_stats(ctx, stats, t0, false, P);
+}
+
+}  // extern "C"

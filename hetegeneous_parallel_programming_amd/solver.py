@@ -1,0 +1,265 @@
+"""Device-side API: the Python face of ``libtc_hip.so`` (C ABI in
+``include/tc.h``).
+
+:func:`TCSolver` mirrors the reference's boundary
+``void TCSolver(Graph &g, uint64_t &total, int n_gpus, int chunk_size)``
+(successful_push_pull/main.cc:5, multigpu_nvshmem_pushpull.cu:29-85) and
+returns the u64 total.  Every count runs the hand-written HIP kernels; if
+the library is missing the import of this module's entry points raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _native as nat
+from ._native import CsrView, PartitionView, TcError, host
+from .graph import Graph, HaloPartition
+
+NUM_BINS = 6
+BIN_NAMES = ("global", "workgroup", "wave", "g32", "g16", "g8")
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("total_ms", nat.c_dbl), ("plan_ms", nat.c_dbl), ("count_ms", nat.c_dbl),
+                ("pivots", nat.c_i64), ("bin_pivots", nat.c_i64 * NUM_BINS)]
+
+    def as_dict(self) -> dict:
+        return {"total_ms": self.total_ms, "plan_ms": self.plan_ms, "count_ms": self.count_ms,
+                "pivots": self.pivots,
+                "bin_pivots": dict(zip(BIN_NAMES, list(self.bin_pivots)))}
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("t_g8", nat.c_i32), ("t_g16", nat.c_i32), ("t_g32", nat.c_i32),
+                ("t_wave", nat.c_i32), ("rounds", nat.c_i32), ("grid", nat.c_i32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "Tune":
+        t = cls()
+        for f, _ in cls._fields_:
+            setattr(t, f, int(d[f]))
+        return t
+
+
+_dev = None
+
+
+def device_lib():
+    """Load libtc_hip.so; raises RuntimeError when it has not been built."""
+    global _dev
+    if _dev is not None:
+        return _dev
+    host()  # dependency, loaded RTLD_GLOBAL first
+    if not os.path.exists(nat.HIP_LIB):
+        raise RuntimeError(f"{nat.HIP_LIB} is missing: the HIP extension must be built "
+                           "(python -m hetegeneous_parallel_programming_amd.build)")
+    L = ctypes.CDLL(nat.HIP_LIB)
+    P, PP = ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)
+    S = nat._sig
+    S(L, "tc_ctx_create", nat.c_i32, [nat.c_i32, PP])
+    S(L, "tc_ctx_destroy", None, [P])
+    S(L, "tc_last_error", ctypes.c_char_p, [P])
+    S(L, "tc_get_tune", nat.c_i32, [P, ctypes.POINTER(Tune)])
+    S(L, "tc_set_tune", nat.c_i32, [P, ctypes.POINTER(Tune)])
+    S(L, "tc_device_name", nat.c_i32, [P, ctypes.c_char_p, nat.c_i32])
+    S(L, "tc_graph_upload", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_u32, PP])
+    S(L, "tc_partition_upload", nat.c_i32, [P, ctypes.POINTER(PartitionView), nat.c_u32, PP])
+    S(L, "tc_graph_free", nat.c_i32, [P, P])
+    S(L, "tc_graph_bytes", nat.c_i64, [P])
+    S(L, "tc_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
+    S(L, "tc_count_plain", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
+    S(L, "tc_batch_create", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_i32, nat.c_u32, PP])
+    S(L, "tc_batch_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
+    S(L, "tc_batch_count_each", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(nat.c_dbl),
+                                            ctypes.POINTER(Stats)])
+    S(L, "tc_batch_free", nat.c_i32, [P, P])
+    S(L, "tc_batch_bytes", nat.c_i64, [P])
+    _dev = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != nat.TC_OK:
+        raise TcError(status, device_lib().tc_last_error(None).decode())
+
+
+class DeviceGraph:
+    """A DAG (or a partition with its halo cache) resident in HBM."""
+
+    def __init__(self, ctx: "Context", handle: int, rows: int, edges: int):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p(handle)
+        self.rows = rows
+        self.edges = edges
+
+    @property
+    def bytes(self) -> int:
+        return int(device_lib().tc_graph_bytes(self._h))
+
+    def free(self) -> None:
+        if self._h is not None and self._h.value and self.ctx._h is not None:
+            _check(device_lib().tc_graph_free(self.ctx._h, self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many DAGs resident at once; counted by one captured persistent launch."""
+
+    def __init__(self, ctx: "Context", handle: int, n: int):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p(handle)
+        self.n = n
+
+    def count(self):
+        out = np.zeros(self.n, dtype=np.uint64)
+        st = Stats()
+        _check(device_lib().tc_batch_count(self.ctx._h, self._h,
+                                           out.ctypes.data_as(nat.u64p), ctypes.byref(st)))
+        return out, st.as_dict()
+
+    def count_each(self):
+        """Baseline: one plain launch + copy back + sync per graph."""
+        out = np.zeros(self.n, dtype=np.uint64)
+        ms = np.zeros(self.n, dtype=np.float64)
+        st = Stats()
+        _check(device_lib().tc_batch_count_each(self.ctx._h, self._h,
+                                                out.ctypes.data_as(nat.u64p),
+                                                ms.ctypes.data_as(ctypes.POINTER(nat.c_dbl)),
+                                                ctypes.byref(st)))
+        return out, ms, st.as_dict()
+
+    @property
+    def bytes(self) -> int:
+        return int(device_lib().tc_batch_bytes(self._h))
+
+    def free(self) -> None:
+        if self._h is not None and self._h.value and self.ctx._h is not None:
+            _check(device_lib().tc_batch_free(self.ctx._h, self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device, one HIP stream, the tuning parameters and plan scratch."""
+
+    def __init__(self, device: int = 0):
+        L = device_lib()
+        h = ctypes.c_void_p()
+        _check(L.tc_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check(device_lib().tc_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    @property
+    def tune(self) -> dict:
+        t = Tune()
+        _check(device_lib().tc_get_tune(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    @tune.setter
+    def tune(self, d: dict) -> None:
+        cur = self.tune
+        cur.update(d)
+        t = Tune.from_dict(cur)
+        _check(device_lib().tc_set_tune(self._h, ctypes.byref(t)))
+
+    def upload(self, g: Graph, check: int = nat.TC_CHECK_SHAPE) -> DeviceGraph:
+        h = ctypes.c_void_p()
+        _check(device_lib().tc_graph_upload(self._h, ctypes.byref(g.view()), check,
+                                            ctypes.byref(h)))
+        return DeviceGraph(self, h.value, g.V(), g.E())
+
+    def upload_arrays(self, rowptr, colidx, check: int = nat.TC_CHECK_SHAPE) -> DeviceGraph:
+        rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+        ci = np.ascontiguousarray(colidx, dtype=np.int32)
+        v = CsrView(rp.ctypes.data_as(nat.i64p), ci.ctypes.data_as(nat.i32p), rp.size - 1, ci.size)
+        h = ctypes.c_void_p()
+        _check(device_lib().tc_graph_upload(self._h, ctypes.byref(v), check, ctypes.byref(h)))
+        return DeviceGraph(self, h.value, rp.size - 1, ci.size)
+
+    def upload_partition(self, p: HaloPartition, check: int = nat.TC_CHECK_SHAPE) -> DeviceGraph:
+        h = ctypes.c_void_p()
+        _check(device_lib().tc_partition_upload(self._h, ctypes.byref(p.view()), check,
+                                                ctypes.byref(h)))
+        return DeviceGraph(self, h.value, p.row_end - p.row_begin, int(p.local_colidx.size))
+
+    def count(self, dg: DeviceGraph):
+        total = nat.c_u64(0)
+        st = Stats()
+        _check(device_lib().tc_count(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
+        return int(total.value), st.as_dict()
+
+    def count_plain(self, dg: DeviceGraph):
+        total = nat.c_u64(0)
+        st = Stats()
+        _check(device_lib().tc_count_plain(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
+        return int(total.value), st.as_dict()
+
+    def batch(self, graphs: list, check: int = nat.TC_CHECK_SHAPE) -> Batch:
+        views = (CsrView * max(len(graphs), 1))()
+        for i, g in enumerate(graphs):
+            views[i] = g.view()
+        h = ctypes.c_void_p()
+        _check(device_lib().tc_batch_create(self._h, views, len(graphs), check, ctypes.byref(h)))
+        return Batch(self, h.value, len(graphs))
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            device_lib().tc_ctx_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def TCSolver(g: Graph, n_gpus: int = 1, chunk_size: int = 1024, ctx: Context | None = None) -> int:
+    """Count the triangles of DAG `g` on the GPU; returns the u64 total.
+
+    Mirrors ``TCSolver(Graph&, uint64_t& total, int n_gpus, int chunk_size)``
+    (successful_push_pull/main.cc:5).  Only one GPU is driven here: n_gpus
+    other than 1 raises.  chunk_size is accepted for signature parity; the
+    reference's GPU solvers ignore it too (main.cc:25,28).
+    """
+    if n_gpus != 1:
+        raise TcError(nat.TC_EINVAL, "this build drives exactly one GPU (n_gpus must be 1)")
+    own = ctx is None
+    c = Context(0) if own else ctx
+    try:
+        dg = c.upload(g)
+        total, _ = c.count(dg)
+        dg.free()
+        return total
+    finally:
+        if own:
+            c.close()

@@ -1,0 +1,67 @@
+"""GPU parity of the batched persistent kernel and the partition (halo) count."""
+import numpy as np
+import pytest
+
+import oracle
+from hetegeneous_parallel_programming_amd import Graph, PartitionedGraph
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_matches_oracle(ctx):
+    graphs = Graph.batch(1, 0, 96)
+    want = np.array([oracle.tc_count(g.out_rowptr(), g.out_colidx()) for g in graphs], np.uint64)
+    b = ctx.batch(graphs)
+    for rep in range(3):  # replays of the captured graph
+        got, st = b.count()
+        assert np.array_equal(got, want), (rep, np.nonzero(got != want))
+    each, ms, _ = b.count_each()
+    assert np.array_equal(each, want)
+    assert (ms > 0).all()
+    b.free()
+
+
+def test_batch_with_empty_and_tiny_graphs(ctx):
+    gs = [Graph.from_pairs(1, np.zeros(0, np.int32), np.zeros(0, np.int32)).oriented(),
+          Graph.from_pairs(3, np.array([0, 1, 2], np.int32), np.array([1, 2, 0], np.int32)).oriented(),
+          Graph.rmat(4096, 16, 5).oriented(),
+          Graph.from_pairs(10, np.zeros(0, np.int32), np.zeros(0, np.int32)).oriented(),
+          Graph.er(3000, 30, 8).oriented()]
+    want = np.array([oracle.tc_count(g.out_rowptr(), g.out_colidx()) for g in gs], np.uint64)
+    b = ctx.batch(gs)
+    got, _ = b.count()
+    assert np.array_equal(got, want)
+    assert np.array_equal(b.count_each()[0], want)
+    b.free()
+
+
+def test_batch_interleaved_with_single_counts(ctx):
+    graphs = Graph.batch(7, 100, 32)
+    want = np.array([oracle.tc_count(g.out_rowptr(), g.out_colidx()) for g in graphs], np.uint64)
+    b = ctx.batch(graphs)
+    single = ctx.upload(graphs[3])
+    for _ in range(3):
+        assert np.array_equal(b.count()[0], want)
+        assert ctx.count(single)[0] == want[3]
+    single.free()
+    b.free()
+
+
+@pytest.mark.parametrize("k", [2, 4, 5])
+def test_partition_halo_counts(ctx, k):
+    g = Graph.rmat(1 << 16, 16, 3).oriented()
+    total = oracle.tc_count(g.out_rowptr(), g.out_colidx())
+    pg = PartitionedGraph(g, k)
+    got_sum = 0
+    for p in range(k):
+        part = pg.halo_partition(p)
+        want = oracle.tc_count_halo(part.row_begin, part.row_end, part.local_rowptr,
+                                    part.local_colidx, part.halo_ids, part.halo_rowptr,
+                                    part.halo_colidx)
+        dg = ctx.upload_partition(part)
+        got, st = ctx.count(dg)
+        assert got == want, (p, got, want)
+        assert ctx.count_plain(dg)[0] == want
+        dg.free()
+        got_sum += got
+    assert got_sum == total

@@ -1,0 +1,136 @@
+"""GPU parity: the HIP count kernels against the CPU oracle (bit-exact u64).
+
+Every case builds its DAG with the product's host library, counts it on the
+device through the C ABI (tc_count = degree-binned persistent kernel,
+tc_count_plain = one wave per pivot) and compares with the oracle's OpenMP
+count on the same arrays, or with a closed form.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from hetegeneous_parallel_programming_amd import Graph
+from hetegeneous_parallel_programming_amd._native import TcError
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(ctx, g):
+    dg = ctx.upload(g)
+    a, st = ctx.count(dg)
+    b, _ = ctx.count_plain(dg)
+    dg.free()
+    return a, b, st
+
+
+def _clique_pairs(n):
+    src, dst = zip(*itertools.combinations(range(n), 2))
+    return np.array(src, np.int32), np.array(dst, np.int32)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_er_16384_d16(ctx, seed):
+    g = Graph.er(16384, 16, seed).oriented()
+    want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
+    a, b, st = _both(ctx, g)
+    assert a == want and b == want, (a, b, want, st)
+
+
+@pytest.mark.parametrize("n,ef,seed", [(1 << 12, 8, 1), (1 << 14, 16, 2), (1 << 16, 16, 3),
+                                       (50000, 24, 4), (1 << 17, 8, 5)])
+def test_rmat(ctx, n, ef, seed):
+    g = Graph.rmat(n, ef, seed).oriented()
+    want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
+    a, b, st = _both(ctx, g)
+    assert a == want and b == want, (a, b, want, st)
+
+
+@pytest.mark.parametrize("n", [3, 4, 64, 300, 1100])
+def test_clique(ctx, n):
+    s, d = _clique_pairs(n)
+    g = Graph.from_pairs(n, s, d).oriented()
+    a, b, _ = _both(ctx, g)
+    assert a == b == math.comb(n, 3)
+
+
+def test_clique_beyond_lds_slot(ctx):
+    # d+(0) = 4199 > 4096: the pivot list is searched in HBM (global bin).
+    n = 4200
+    s, d = _clique_pairs(n)
+    g = Graph.from_pairs(n, s, d).oriented()
+    a, st = ctx.count(ctx.upload(g))
+    assert a == math.comb(n, 3)
+    assert st["bin_pivots"]["global"] > 0 and st["bin_pivots"]["workgroup"] > 0
+
+
+def test_wheel_star_cycle(ctx):
+    k = 500  # wheel: hub 0 + rim cycle 1..k -> k triangles
+    rim = np.arange(1, k + 1)
+    src = np.concatenate([np.zeros(k, np.int32), rim]).astype(np.int32)
+    dst = np.concatenate([rim, np.roll(rim, -1)]).astype(np.int32)
+    g = Graph.from_pairs(k + 1, src, dst).oriented()
+    assert _both(ctx, g)[:2] == (k, k)
+    star = Graph.from_pairs(1001, np.zeros(1000, np.int32), np.arange(1, 1001, dtype=np.int32)).oriented()
+    assert _both(ctx, star)[:2] == (0, 0)
+    cyc = Graph.from_pairs(1000, np.arange(1000, dtype=np.int32),
+                           ((np.arange(1000) + 1) % 1000).astype(np.int32)).oriented()
+    assert _both(ctx, cyc)[:2] == (0, 0)
+
+
+def test_empty_and_tiny(ctx):
+    for n in (1, 2, 5):
+        g = Graph.from_pairs(n, np.zeros(0, np.int32), np.zeros(0, np.int32)).oriented()
+        assert _both(ctx, g)[:2] == (0, 0)
+    tri = Graph.from_pairs(3, np.array([0, 1, 2], np.int32), np.array([1, 2, 0], np.int32)).oriented()
+    assert _both(ctx, tri)[:2] == (1, 1)
+
+
+def test_disjoint_cliques_all_bins(ctx):
+    sizes = [3, 5, 9, 17, 33, 65, 129, 260, 700, 1500]
+    src, dst, base = [], [], 0
+    for k in sizes:
+        s, d = _clique_pairs(k)
+        src.append(s + base)
+        dst.append(d + base)
+        base += k
+    g = Graph.from_pairs(base, np.concatenate(src), np.concatenate(dst)).oriented()
+    a, b, st = _both(ctx, g)
+    want = sum(math.comb(k, 3) for k in sizes)
+    assert a == b == want, st
+
+
+def test_repeat_is_stable(ctx):
+    g = Graph.rmat(1 << 15, 16, 9).oriented()
+    dg = ctx.upload(g)
+    vals = {ctx.count(dg)[0] for _ in range(5)} | {ctx.count_plain(dg)[0] for _ in range(3)}
+    dg.free()
+    assert vals == {oracle.tc_count(g.out_rowptr(), g.out_colidx())}
+
+
+def test_tune_changes_schedule_not_result(ctx):
+    g = Graph.rmat(1 << 16, 16, 21).oriented()
+    want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
+    dg = ctx.upload(g)
+    saved = ctx.tune
+    try:
+        for t in ({"t_g8": 0, "t_g16": 0, "t_g32": 0, "t_wave": 0},
+                  {"t_g8": 1 << 20, "t_g16": 1 << 20, "t_g32": 1 << 20, "t_wave": 1 << 20},
+                  {"t_g8": 64, "t_g16": 64, "t_g32": 512, "t_wave": 4096, "rounds": 1, "grid": 7}):
+            ctx.tune = t
+            assert ctx.count(dg)[0] == want, t
+    finally:
+        ctx.tune = saved
+    dg.free()
+
+
+def test_upload_rejects_malformed(ctx):
+    rp = np.array([0, 2, 3], np.int64)
+    with pytest.raises(TcError):
+        ctx.upload_arrays(rp, np.array([1, 5, 0], np.int32))  # id out of range
+    with pytest.raises(TcError):
+        ctx.upload_arrays(rp, np.array([1, 1, 0], np.int32), check=7)  # duplicate in row
+    with pytest.raises(TcError):
+        ctx.upload_arrays(np.array([0, 3, 2], np.int64), np.array([1, 1, 0], np.int32))

@@ -188,6 +188,9 @@ def main() -> int:
     ap.add_argument("--no-partition", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-rmat", action="store_true", help="skip the headline RMAT leg")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this triangle-counting build has no multi-GPU "
@@ -197,6 +200,13 @@ def main() -> int:
     from hetegeneous_parallel_programming_amd import Graph
     from hetegeneous_parallel_programming_amd.solver import Context
 
+    if args.copy_probe:
+        ctx = Context(0)
+        ms = ctx.probe_copy(args.copy_probe, max(1, args.steps))
+        ctx.close()
+        print(json.dumps({"copy_bytes": args.copy_probe, "iters": max(1, args.steps), "ms": ms,
+                          "GBps_read_plus_write": 2 * args.copy_probe / (ms / 1e3) / 1e9}), flush=True)
+        return 0
     t0 = time.perf_counter()
     g = Graph.rmat_scale(args.scale, 16, args.seed)
     g.orientation()

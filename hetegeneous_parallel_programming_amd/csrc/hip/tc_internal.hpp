@@ -38,13 +38,14 @@ void dev_free(void* p);
 struct PlanScratch {
   uint8_t* bin_of = nullptr;
   uint64_t* items = nullptr;
+  uint32_t* blk = nullptr;
   int64_t cap = 0;
   uint32_t* counters = nullptr;
   uint64_t* outs = nullptr;
   int64_t outs_cap = 0;
   tc_status ensure(int64_t pivots, int64_t nouts);
   void release();
-  tck::PlanBufs bufs() const { return tck::PlanBufs{bin_of, items, counters, outs}; }
+  tck::PlanBufs bufs() const { return tck::PlanBufs{bin_of, items, blk, counters, outs}; }
 };
 
 // Makes ctx's device current for the scope of one API call.

@@ -173,6 +173,7 @@ __device__ __forceinline__ int choose_bin(const int64_t* rp, const int32_t* ci, 
   return BIN_G8;
 }
 
+// Pass 1: bin of every pivot and, per planning block, the count per bin.
 __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
   __shared__ uint32_t s_cnt[kNumBins];
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -190,33 +191,77 @@ __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
     if (bin != kNoBin) atomicAdd(&s_cnt[bin], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < kNumBins && s_cnt[threadIdx.x])
-    atomicAdd(&a.bufs.counters[kCntBinCount + threadIdx.x], s_cnt[threadIdx.x]);
+  if (threadIdx.x < kNumBins) a.bufs.blk[(int64_t)blockIdx.x * kBlkStride + threadIdx.x] = s_cnt[threadIdx.x];
 }
 
+// Pass 2 (one workgroup of 1024): per-block counts -> item offsets, ordered
+// by (bin, block); bin totals into the counters.
+constexpr int kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(LaunchArgs a, int64_t nblk) {
+  __shared__ uint32_t s_wave[kScanThreads / 64];
+  __shared__ uint32_t s_bin_total[kNumBins];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t chunk = (nblk + kScanThreads - 1) / kScanThreads;
+  const int64_t b0 = min((int64_t)tid * chunk, nblk), b1 = min(b0 + chunk, nblk);
+  uint32_t* blk = a.bufs.blk;
+  uint32_t excl[kNumBins];
+  for (int b = 0; b < kNumBins; ++b) {
+    uint32_t local = 0;
+    for (int64_t k = b0; k < b1; ++k) local += blk[k * kBlkStride + b];
+    // inclusive scan over the 1024 threads
+    uint32_t incl = local;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63) s_wave[wid] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+      if (w < wid) before += s_wave[w];
+      all += s_wave[w];
+    }
+    excl[b] = before + incl - local;
+    if (tid == 0) s_bin_total[b] = all;
+    __syncthreads();
+  }
+  uint32_t base = 0;
+  for (int b = 0; b < kNumBins; ++b) {
+    uint32_t run = base + excl[b];
+    for (int64_t k = b0; k < b1; ++k) {
+      const uint32_t c = blk[k * kBlkStride + b];
+      blk[k * kBlkStride + b] = run;
+      run += c;
+    }
+    base += s_bin_total[b];
+  }
+  if (tid < kNumBins) a.bufs.counters[kCntBinCount + tid] = s_bin_total[tid];
+}
+
+// Pass 3: stable scatter of (graph, row) items into their bin segment.
 __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
-  __shared__ uint32_t s_cnt[kNumBins];
-  __shared__ uint32_t s_base[kNumBins];
+  __shared__ uint32_t s_wc[kWaves][kNumBins];
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (threadIdx.x < kNumBins) s_cnt[threadIdx.x] = 0;
-  __syncthreads();
-  int bin = kNoBin;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int bin = (p < a.P) ? (int)a.bufs.bin_of[p] : (int)kNoBin;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t rank = 0;
-  if (p < a.P) {
-    bin = a.bufs.bin_of[p];
-    if (bin != kNoBin) rank = atomicAdd(&s_cnt[bin], 1u);
+#pragma unroll
+  for (int b = 0; b < kNumBins; ++b) {
+    const uint64_t m = __ballot(bin == b);
+    if (bin == b) rank = (uint32_t)__popcll(m & lt);
+    if (lane == 0) s_wc[wid][b] = (uint32_t)__popcll(m);
   }
   __syncthreads();
-  if (threadIdx.x < kNumBins && s_cnt[threadIdx.x])
-    s_base[threadIdx.x] = atomicAdd(&a.bufs.counters[kCntBinCursor + threadIdx.x], s_cnt[threadIdx.x]);
-  __syncthreads();
   if (bin != kNoBin) {
-    int64_t seg = 0;
-    for (int b = 0; b < bin; ++b) seg += a.bufs.counters[kCntBinCount + b];
+    uint32_t off = a.bufs.blk[(int64_t)blockIdx.x * kBlkStride + bin] + rank;
+    for (int w = 0; w < wid; ++w) off += s_wc[w][bin];
     const int g = find_graph(a.descs, a.ngraphs, p);
     const GraphDesc d = a.descs[g];
     const int64_t u = d.row_begin + (p - d.plan_base);
-    a.bufs.items[seg + s_base[bin] + rank] = ((uint64_t)(uint32_t)g << 32) | (uint32_t)u;
+    a.bufs.items[off] = ((uint64_t)(uint32_t)g << 32) | (uint32_t)u;
   }
 }
 
@@ -338,6 +383,18 @@ __global__ __launch_bounds__(kBlock) void count_kernel(LaunchArgs a) {
       case BIN_G16: group_tile<16, kCapG16>(a, items, first, last, L, batch, acc); break;
       default: group_tile<8, kCapG8>(a, items, first, last, L, batch, acc); break;
     }
+    if (batch) {
+      // Items are ordered by graph, so a tile almost always belongs to one
+      // graph: then one block-reduced atomic; otherwise one per lane.
+      const int tg = (int)(items[first] >> 32);
+      if (!__syncthreads_or(acc.sum != 0 && acc.g != tg)) {
+        const uint64_t tot = block_sum(acc.sum, L.red);
+        if (threadIdx.x == 0 && tot) atomicAdd(&a.bufs.outs[tg], tot);
+      } else if (acc.sum) {
+        atomicAdd(&a.bufs.outs[acc.g], acc.sum);
+      }
+      acc.sum = 0;
+    }
     __syncthreads();
   }
   if (batch) {
@@ -384,6 +441,15 @@ __global__ __launch_bounds__(kBlock) void plain_kernel(const int64_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// plain streaming copy: known HBM bytes for calibrating traffic counters
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                      int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s) {
@@ -400,6 +466,9 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
   plan_bin_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  plan_scan_kernel<<<1, kScanThreads, 0, s>>>(a, blocks);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
   plan_scatter_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
@@ -412,6 +481,11 @@ hipError_t launch_count(const LaunchArgs& a, int grid, hipStream_t s) {
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
                         uint64_t* out, int grid, hipStream_t s) {
   plain_kernel<<<grid, kBlock, 0, s>>>(rp, ci, row_begin, row_end, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s) {
+  copy_kernel<<<grid, kBlock, 0, s>>>((const uint4*)src, (uint4*)dst, bytes / 16);
   return hipGetLastError();
 }
 

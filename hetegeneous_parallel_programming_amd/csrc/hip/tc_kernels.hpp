@@ -48,9 +48,14 @@ struct Tune {
   int32_t t_g8, t_g16, t_g32, t_wave, rounds;
 };
 
+// Per-planning-block bin counts, then their item offsets (kBlkStride words
+// per block of kBlock consecutive plan indices).
+constexpr int kBlkStride = 8;
+
 struct PlanBufs {
   uint8_t* bin_of;      // [P] bin of each pivot
-  uint64_t* items;      // [P] (graph << 32 | row), grouped by bin
+  uint64_t* items;      // [P] (graph << 32 | row), ordered by (bin, plan index)
+  uint32_t* blk;        // [ceil(P/kBlock) * kBlkStride]
   uint32_t* counters;   // [kNumCounters]
   uint64_t* outs;       // [ngraphs] u64 counts
 };
@@ -71,6 +76,7 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s);
 hipError_t launch_count(const LaunchArgs& a, int grid, hipStream_t s);
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
                         uint64_t* out, int grid, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
 // Resident workgroups per CU of the persistent and plain kernels.
 int count_blocks_per_cu();
 int plain_blocks_per_cu();

@@ -45,13 +45,18 @@ tc_status PlanScratch::ensure(int64_t pivots, int64_t nouts) {
   if (pivots > cap) {
     dev_free(bin_of);
     dev_free(items);
+    dev_free(blk);
     bin_of = nullptr;
     items = nullptr;
+    blk = nullptr;
     cap = 0;
     int64_t c = std::max<int64_t>(pivots, 1024);
     tc_status st = dev_alloc((void**)&bin_of, (size_t)c, "pivot bins");
     if (st != TC_OK) return st;
     st = dev_alloc((void**)&items, sizeof(uint64_t) * (size_t)c, "pivot items");
+    if (st != TC_OK) return st;
+    const int64_t nblk = (c + tck::kBlock - 1) / tck::kBlock;
+    st = dev_alloc((void**)&blk, sizeof(uint32_t) * tck::kBlkStride * (size_t)nblk, "plan blocks");
     if (st != TC_OK) return st;
     cap = c;
   }
@@ -70,6 +75,7 @@ tc_status PlanScratch::ensure(int64_t pivots, int64_t nouts) {
 void PlanScratch::release() {
   dev_free(bin_of);
   dev_free(items);
+  dev_free(blk);
   dev_free(counters);
   dev_free(outs);
   *this = PlanScratch();
@@ -332,6 +338,35 @@ tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* st
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
   return finish_stats(ctx, stats, t0, false, P);
+}
+
+tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter) {
+  if (!ctx || !ms_per_iter || bytes <= 0 || bytes % 16 != 0 || iters < 1)
+    return fail(TC_EINVAL, "need bytes > 0, a multiple of 16, and iters >= 1");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  void *src = nullptr, *dst = nullptr;
+  tc_status st = dev_alloc(&src, (size_t)bytes, "copy src");
+  if (st == TC_OK) st = dev_alloc(&dst, (size_t)bytes, "copy dst");
+  if (st != TC_OK) {
+    dev_free(src);
+    return st;
+  }
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemsetAsync(src, 1, (size_t)bytes, s);
+  const int grid = ctx->cus * 8;
+  if (e == hipSuccess) e = tck::launch_copy(src, dst, bytes, grid, s);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], s);
+  for (int i = 0; i < iters && e == hipSuccess; ++i) e = tck::launch_copy(src, dst, bytes, grid, s);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[2], s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]);
+  dev_free(src);
+  dev_free(dst);
+  if (e != hipSuccess) return fail(TC_EDEVICE, std::string("copy probe: ") + hipGetErrorString(e));
+  *ms_per_iter = ms / iters;
+  return TC_OK;
 }
 
 }  // extern "C"

@@ -80,6 +80,7 @@ def device_lib():
                                             ctypes.POINTER(Stats)])
     S(L, "tc_batch_free", nat.c_i32, [P, P])
     S(L, "tc_batch_bytes", nat.c_i64, [P])
+    S(L, "tc_probe_copy", nat.c_i32, [P, nat.c_i64, nat.c_i32, ctypes.POINTER(nat.c_dbl)])
     _dev = L
     return L
 
@@ -224,6 +225,12 @@ class Context:
         h = ctypes.c_void_p()
         _check(device_lib().tc_batch_create(self._h, views, len(graphs), check, ctypes.byref(h)))
         return Batch(self, h.value, len(graphs))
+
+    def probe_copy(self, nbytes: int, iters: int = 10) -> float:
+        """Mean ms of a plain device copy of nbytes (counter calibration)."""
+        ms = nat.c_dbl(0)
+        _check(device_lib().tc_probe_copy(self._h, nbytes, iters, ctypes.byref(ms)))
+        return ms.value
 
     def close(self) -> None:
         if self._h is not None and self._h.value:

@@ -108,6 +108,13 @@ tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double
 tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b);
 int64_t tc_batch_bytes(const tc_batch* b);
 
+/* ---- measurement utility ------------------------------------------------
+ * Copies `bytes` (a multiple of 16) between two device buffers with a plain
+ * streaming kernel `iters` times and reports the mean kernel time.  No
+ * counterpart in the reference: it gives a kernel of known HBM traffic
+ * (bytes read + bytes written) to calibrate profiler traffic counters. */
+tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter);
+
 #ifdef __cplusplus
 }
 #endif

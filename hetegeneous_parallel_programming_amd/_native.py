@@ -11,7 +11,8 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(_PKG, "lib")
+# HPP_TC_LIB_DIR selects another build of the two libraries (e.g. a tuning variant).
+LIB_DIR = os.environ.get("HPP_TC_LIB_DIR") or os.path.join(_PKG, "lib")
 HOST_LIB = os.path.join(LIB_DIR, "libtc_host.so")
 HIP_LIB = os.path.join(LIB_DIR, "libtc_hip.so")
 

@@ -62,29 +62,35 @@ def build_host() -> str:
     return out
 
 
-def build_hip(jobs: int = 8) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build_hip(jobs: int = 8, out_dir: str | None = None, defines: tuple = ()) -> str:
+    """Compile the HIP library (into out_dir, default lib/) with extra -D defines."""
+    lib_dir = out_dir or LIB
+    obj_dir = os.path.join(lib_dir, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    if lib_dir != LIB:
+        shutil.copy2(os.path.join(LIB, "libtc_host.so"), os.path.join(lib_dir, "libtc_host.so"))
     hipcc = _hipcc()
+    dflags = [f"-D{d}" for d in defines]
     srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     objs = []
 
     def compile_one(rel: str) -> str:
         src = os.path.join(CSRC, rel)
-        obj = os.path.join(OBJ, os.path.basename(rel) + ".o")
+        obj = os.path.join(obj_dir, os.path.basename(rel) + ".o")
         lang = ["-x", "hip"] if rel.endswith(".hip") else []
-        _run([hipcc, *HIP_FLAGS, *lang, "-c", src, "-o", obj])
+        _run([hipcc, *HIP_FLAGS, *dflags, *lang, "-c", src, "-o", obj])
         return obj
 
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(srcs)))) as ex:
         objs = list(ex.map(compile_one, srcs))
-    out = os.path.join(LIB, "libtc_hip.so")
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L" + LIB, "-ltc_host",
+    out = os.path.join(lib_dir, "libtc_hip.so")
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L" + lib_dir, "-ltc_host",
           "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     cli = os.path.join(CSRC, "hip", "tc_run.cpp")
     if os.path.exists(cli):
-        exe = os.path.join(LIB, "tc_run")
-        _run([hipcc, *HIP_FLAGS, cli, "-L" + LIB, "-ltc_hip", "-ltc_host",
+        exe = os.path.join(lib_dir, "tc_run")
+        _run([hipcc, *HIP_FLAGS, *dflags, cli, "-L" + lib_dir, "-ltc_hip", "-ltc_host",
               "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"])
         os.replace(exe + ".tmp", exe)
     return out

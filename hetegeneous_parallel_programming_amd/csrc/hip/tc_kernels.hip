@@ -49,42 +49,62 @@ __device__ __forceinline__ int group_excl_scan(int x, int gl, int& total) {
   return incl - x;
 }
 
-// Largest k in [0, G) with sP[k] <= t (sP non-decreasing, sP[0] = 0 <= t).
+// First k in [0, G) with sEnd[k] > t (sEnd non-decreasing, sEnd[G-1] > t).
 template <int G>
-__device__ __forceinline__ int find_edge(const int32_t* sP, int t) {
+__device__ __forceinline__ int find_edge(const int32_t* sEnd, int t) {
   int k = 0;
 #pragma unroll
   for (int step = G / 2; step >= 1; step >>= 1)
-    if (sP[k + step] <= t) k += step;
+    if (sEnd[k + step - 1] <= t) k += step;
   return k;
 }
 
-constexpr int kInFlight = 4;  // lookups per lane per iteration
+// LDS copies of pivot lists are padded with one word per 32 so that the
+// power-of-two probe strides of a binary search fall into different banks.
+__device__ __forceinline__ int pad_idx(int i) { return i + (i >> 5); }
+constexpr int padded(int cap) { return cap + cap / 32; }
+
+template <bool kPadded>
+__device__ __forceinline__ int a_at(const int32_t* A, int i) {
+  return A[kPadded ? pad_idx(i) : i];
+}
+
+#ifndef TC_IN_FLIGHT
+#define TC_IN_FLIGHT 4
+#endif
+constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
 
 // Counts, for the edges j = jfirst, jfirst + jstride, ... of pivot u (list
-// A[0, da), in LDS or HBM), the elements of N+(v) found in A.  sP/sOff are
-// this group's G-entry LDS scratch.  Returns this lane's share of the hits.
-template <int G>
+// A[0, da): padded in LDS, or plain in HBM), the elements of N+(v) found in
+// A.  sEnd/sBase are this group's G-entry LDS scratch.  Returns this lane's
+// share of the hits.
+//
+// Flat index t of a batch maps to edge k (sEnd[k-1] <= t < sEnd[k]) and to
+// colidx[sBase[k] + t]; each lane keeps its current edge's end and base in
+// registers and searches sEnd only when it steps past that edge.
+template <int G, bool kPadded>
 __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
                                                 const int32_t* __restrict__ ci, const int32_t* A,
-                                                int da, int jfirst, int jstride, int32_t* sP,
-                                                int64_t* sOff, int gl) {
+                                                int da, int jfirst, int jstride, int32_t* sEnd,
+                                                int64_t* sBase, int gl) {
   uint32_t hits = 0;
   for (int j0 = jfirst; j0 < da; j0 += jstride) {
     const int j = j0 + gl;
     int c = 0;
     int64_t off = 0;
     if (j < da) {
-      const int v = A[j];
+      const int v = a_at<kPadded>(A, j);
       const int64_t b = rp[v];
       c = (int)(rp[v + 1] - b);
       off = b;
     }
     int total;
     const int excl = group_excl_scan<G>(c, gl, total);
-    sP[gl] = excl;
-    sOff[gl] = off;
+    sEnd[gl] = excl + c;
+    sBase[gl] = off - excl;
     wave_sync();
+    int cur_end = 0;        // t < cur_end  <=>  t lies in the lane's current edge
+    int64_t cur_base = 0;
     for (int t0 = 0; t0 < total; t0 += G * kInFlight) {
       int w[kInFlight];
 #pragma unroll
@@ -92,8 +112,12 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
         const int t = t0 + q * G + gl;
         w[q] = INT_MIN;  // ids are >= 0: never found
         if (t < total) {
-          const int k = find_edge<G>(sP, t);
-          w[q] = ci[sOff[k] + (t - sP[k])];
+          if (t >= cur_end) {
+            const int k = find_edge<G>(sEnd, t);
+            cur_end = sEnd[k];
+            cur_base = sBase[k];
+          }
+          w[q] = ci[cur_base + t];
         }
       }
       int lo[kInFlight];
@@ -102,15 +126,22 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
       for (int len = da; len > 1;) {
         const int half = len >> 1;
 #pragma unroll
-        for (int q = 0; q < kInFlight; ++q) lo[q] = (A[lo[q] + half] <= w[q]) ? lo[q] + half : lo[q];
+        for (int q = 0; q < kInFlight; ++q)
+          lo[q] = (a_at<kPadded>(A, lo[q] + half) <= w[q]) ? lo[q] + half : lo[q];
         len -= half;
       }
 #pragma unroll
-      for (int q = 0; q < kInFlight; ++q) hits += (A[lo[q]] == w[q]) ? 1u : 0u;
+      for (int q = 0; q < kInFlight; ++q) hits += (a_at<kPadded>(A, lo[q]) == w[q]) ? 1u : 0u;
     }
     wave_sync();
   }
   return hits;
+}
+
+// Copies N+(u) into a padded LDS slot with `n` cooperating lanes.
+__device__ __forceinline__ void stage_pivot(int32_t* slot, const int32_t* __restrict__ src, int da,
+                                            int lane, int n) {
+  for (int x = lane; x < da; x += n) slot[pad_idx(x)] = src[x];
 }
 
 __device__ __forceinline__ uint64_t shfl_down_u64(uint64_t x, int d) {
@@ -269,12 +300,17 @@ __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
 // persistent count
 // ---------------------------------------------------------------------------
 struct __align__(16) CountLds {
-  int32_t A[kCapBlock];   // pivot-list slots (layout depends on the tile's bin)
-  int32_t P[kBlock];      // per-lane edge prefix
-  int64_t Off[kBlock];    // per-lane N+(v) offset
+  int32_t A[padded(kCapBlock)];  // pivot-list slots (layout depends on the tile's bin)
+  int32_t End[kBlock];           // per-lane edge end in the batch's flat index
+  int64_t Base[kBlock];          // per-lane colidx base of that edge
   uint64_t red[kWaves];
-  uint32_t tile;
+  int64_t tab_cnt[1];
+  int64_t cur_first, cur_seg;
 };
+static_assert(padded(kCapWave) * kWaves <= padded(kCapBlock), "wave slots exceed LDS region");
+static_assert(padded(kCapG32) * (kBlock / 32) <= padded(kCapBlock), "g32 slots exceed LDS region");
+static_assert(padded(kCapG16) * (kBlock / 16) <= padded(kCapBlock), "g16 slots exceed LDS region");
+static_assert(padded(kCapG8) * (kBlock / 8) <= padded(kCapBlock), "g8 slots exceed LDS region");
 
 // Per-lane running count; in batch mode it is flushed to outs[g] whenever
 // the lane moves to a pivot of another graph.
@@ -295,9 +331,9 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
                                            int64_t last, CountLds& L, bool batch, Acc& acc) {
   constexpr int NGRP = kBlock / G;
   const int grp = threadIdx.x / G, gl = threadIdx.x % G;
-  int32_t* slot = L.A + grp * CAP;
-  int32_t* sP = L.P + grp * G;
-  int64_t* sOff = L.Off + grp * G;
+  int32_t* slot = L.A + grp * padded(CAP);
+  int32_t* sEnd = L.End + grp * G;
+  int64_t* sBase = L.Base + grp * G;
   for (int64_t i = first + grp; i < last; i += NGRP) {
     const uint64_t item = items[i];
     const int g = (int)(item >> 32);
@@ -307,9 +343,9 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
     const int32_t* ci = a.ci + d.ci_off;
     const int64_t rb = rp[u];
     const int da = (int)(rp[u + 1] - rb);
-    for (int x = gl; x < da; x += G) slot[x] = ci[rb + x];
+    stage_pivot(slot, ci + rb, da, gl, G);
     wave_sync();
-    const uint32_t h = pivot_edges<G>(rp, ci, slot, da, 0, G, sP, sOff, gl);
+    const uint32_t h = pivot_edges<G, true>(rp, ci, slot, da, 0, G, sEnd, sBase, gl);
     acc.to_graph(g, batch, a.bufs.outs);
     acc.sum += h;
     wave_sync();
@@ -329,60 +365,67 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t h;
   if (kLdsPivot) {
-    for (int x = threadIdx.x; x < da; x += kBlock) L.A[x] = ci[rb + x];
+    stage_pivot(L.A, ci + rb, da, threadIdx.x, kBlock);
     __syncthreads();
-    h = pivot_edges<64>(rp, ci, L.A, da, wid * 64, kBlock, L.P + wid * 64, L.Off + wid * 64, lane);
+    h = pivot_edges<64, true>(rp, ci, L.A, da, wid * 64, kBlock, L.End + wid * 64, L.Base + wid * 64, lane);
   } else {
-    h = pivot_edges<64>(rp, ci, ci + rb, da, wid * 64, kBlock, L.P + wid * 64, L.Off + wid * 64, lane);
+    h = pivot_edges<64, false>(rp, ci, ci + rb, da, wid * 64, kBlock, L.End + wid * 64, L.Base + wid * 64,
+                               lane);
   }
   acc.to_graph(g, batch, a.bufs.outs);
   acc.sum += h;
 }
 
-__global__ __launch_bounds__(kBlock) void count_kernel(LaunchArgs a) {
+// Minimum resident waves per SIMD the compiler must allow for (caps VGPRs:
+// 8 waves = 64 VGPRs).  Latency hiding dominates this kernel's speed.
+#ifndef TC_COUNT_WAVES_PER_EU
+#define TC_COUNT_WAVES_PER_EU 8
+#endif
+
+// Pivots per tile of each bin (rounds = pivots per group in light bins).
+__device__ __forceinline__ int tile_pivots(int bin, int rounds) {
+  switch (bin) {
+    case BIN_GLOBAL:
+    case BIN_BLOCK: return 1;
+    case BIN_WAVE: return kWaves;
+    case BIN_G32: return (kBlock / 32) * rounds;
+    case BIN_G16: return (kBlock / 16) * rounds;
+    default: return (kBlock / 8) * rounds;
+  }
+}
+
+// One persistent kernel per bin, specialised so that each keeps only its own
+// working set in registers.  Blocks pull tiles from the bin's queue head.
+template <int BIN>
+__global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(LaunchArgs a) {
   __shared__ CountLds L;
   const bool batch = a.ngraphs > 1;
-  // Tile table: bins in order, each split into runs of tile_piv pivots.
-  int64_t cnt[kNumBins], seg[kNumBins], tiles_end[kNumBins];
-  int tile_piv[kNumBins];
-  const int r = a.tune.rounds > 0 ? a.tune.rounds : 1;
-  tile_piv[BIN_GLOBAL] = 1;
-  tile_piv[BIN_BLOCK] = 1;
-  tile_piv[BIN_WAVE] = kWaves;
-  tile_piv[BIN_G32] = (kBlock / 32) * r;
-  tile_piv[BIN_G16] = (kBlock / 16) * r;
-  tile_piv[BIN_G8] = (kBlock / 8) * r;
-  int64_t s = 0, tacc = 0;
-#pragma unroll
-  for (int b = 0; b < kNumBins; ++b) {
-    cnt[b] = a.bufs.counters[kCntBinCount + b];
-    seg[b] = s;
-    s += cnt[b];
-    tacc += (cnt[b] + tile_piv[b] - 1) / tile_piv[b];
-    tiles_end[b] = tacc;
+  if (threadIdx.x == 0) {
+    int64_t seg = 0;
+    for (int b = 0; b < BIN; ++b) seg += a.bufs.counters[kCntBinCount + b];
+    L.cur_seg = seg;
+    L.tab_cnt[0] = a.bufs.counters[kCntBinCount + BIN];
   }
-  const int64_t ntiles = tacc;
+  __syncthreads();
+  const int piv = tile_pivots(BIN, a.tune.rounds > 0 ? a.tune.rounds : 1);
+  const int64_t cnt = L.tab_cnt[0];
+  const int64_t ntiles = (cnt + piv - 1) / piv;
+  const uint64_t* items = a.bufs.items + L.cur_seg;
   Acc acc;
   acc.g = 0;
   for (;;) {
-    if (threadIdx.x == 0) L.tile = atomicAdd(&a.bufs.counters[kCntTileHead], 1u);
+    if (threadIdx.x == 0) L.cur_first = atomicAdd(&a.bufs.counters[kCntTileHead + BIN], 1u);
     __syncthreads();
-    const int64_t t = L.tile;
+    const int64_t t = L.cur_first;
     if (t >= ntiles) break;
-    int b = 0;
-    while (t >= tiles_end[b]) ++b;
-    const int64_t tfirst = (b == 0) ? 0 : tiles_end[b - 1];
-    const int64_t first = (t - tfirst) * tile_piv[b];
-    const int64_t last = min(first + (int64_t)tile_piv[b], cnt[b]);
-    const uint64_t* items = a.bufs.items + seg[b];
-    switch (b) {
-      case BIN_GLOBAL: block_pivot<false>(a, items[first], L, batch, acc); break;
-      case BIN_BLOCK: block_pivot<true>(a, items[first], L, batch, acc); break;
-      case BIN_WAVE: group_tile<64, kCapWave>(a, items, first, last, L, batch, acc); break;
-      case BIN_G32: group_tile<32, kCapG32>(a, items, first, last, L, batch, acc); break;
-      case BIN_G16: group_tile<16, kCapG16>(a, items, first, last, L, batch, acc); break;
-      default: group_tile<8, kCapG8>(a, items, first, last, L, batch, acc); break;
-    }
+    const int64_t first = t * piv;
+    const int64_t last = min(first + (int64_t)piv, cnt);
+    if (BIN == BIN_GLOBAL) block_pivot<false>(a, items[first], L, batch, acc);
+    else if (BIN == BIN_BLOCK) block_pivot<true>(a, items[first], L, batch, acc);
+    else if (BIN == BIN_WAVE) group_tile<64, kCapWave>(a, items, first, last, L, batch, acc);
+    else if (BIN == BIN_G32) group_tile<32, kCapG32>(a, items, first, last, L, batch, acc);
+    else if (BIN == BIN_G16) group_tile<16, kCapG16>(a, items, first, last, L, batch, acc);
+    else group_tile<8, kCapG8>(a, items, first, last, L, batch, acc);
     if (batch) {
       // Items are ordered by graph, so a tile almost always belongs to one
       // graph: then one block-reduced atomic; otherwise one per lane.
@@ -397,9 +440,7 @@ __global__ __launch_bounds__(kBlock) void count_kernel(LaunchArgs a) {
     }
     __syncthreads();
   }
-  if (batch) {
-    if (acc.sum) atomicAdd(&a.bufs.outs[acc.g], acc.sum);
-  } else {
+  if (!batch) {
     const uint64_t tot = block_sum(acc.sum, L.red);
     if (threadIdx.x == 0 && tot) atomicAdd(&a.bufs.outs[0], tot);
   }
@@ -409,13 +450,13 @@ __global__ __launch_bounds__(kBlock) void count_kernel(LaunchArgs a) {
 // plain kernel: one wave64 per pivot, grid-stride (bs_warp_vertex.cuh:2-35)
 // ---------------------------------------------------------------------------
 struct __align__(16) PlainLds {
-  int32_t A[kWaves][kCapWave];
-  int32_t P[kBlock];
-  int64_t Off[kBlock];
+  int32_t A[kWaves][padded(kCapWave)];
+  int32_t End[kBlock];
+  int64_t Base[kBlock];
   uint64_t red[kWaves];
 };
 
-__global__ __launch_bounds__(kBlock) void plain_kernel(const int64_t* __restrict__ rp,
+__global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(const int64_t* __restrict__ rp,
                                                        const int32_t* __restrict__ ci,
                                                        int32_t row_begin, int32_t row_end,
                                                        uint64_t* out) {
@@ -428,12 +469,12 @@ __global__ __launch_bounds__(kBlock) void plain_kernel(const int64_t* __restrict
     const int da = (int)(rp[u + 1] - rb);
     if (da < 2) continue;
     if (da <= kCapWave) {
-      for (int x = lane; x < da; x += 64) L.A[wid][x] = ci[rb + x];
+      stage_pivot(L.A[wid], ci + rb, da, lane, 64);
       wave_sync();
-      acc += pivot_edges<64>(rp, ci, L.A[wid], da, 0, 64, L.P + wid * 64, L.Off + wid * 64, lane);
+      acc += pivot_edges<64, true>(rp, ci, L.A[wid], da, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
       wave_sync();
     } else {
-      acc += pivot_edges<64>(rp, ci, ci + rb, da, 0, 64, L.P + wid * 64, L.Off + wid * 64, lane);
+      acc += pivot_edges<64, false>(rp, ci, ci + rb, da, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
     }
   }
   const uint64_t tot = block_sum(acc, L.red);
@@ -474,7 +515,12 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_count(const LaunchArgs& a, int grid, hipStream_t s) {
-  count_kernel<<<grid, kBlock, 0, s>>>(a);
+  count_kernel<BIN_GLOBAL><<<grid, kBlock, 0, s>>>(a);
+  count_kernel<BIN_BLOCK><<<grid, kBlock, 0, s>>>(a);
+  count_kernel<BIN_WAVE><<<grid, kBlock, 0, s>>>(a);
+  count_kernel<BIN_G32><<<grid, kBlock, 0, s>>>(a);
+  count_kernel<BIN_G16><<<grid, kBlock, 0, s>>>(a);
+  count_kernel<BIN_G8><<<grid, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 
@@ -491,7 +537,8 @@ hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipS
 
 int count_blocks_per_cu() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel, kBlock, 0) != hipSuccess) return 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel<BIN_WAVE>, kBlock, 0) != hipSuccess)
+    return 1;
   return n > 0 ? n : 1;
 }
 

@@ -29,8 +29,8 @@ constexpr int kCapG8 = 32, kCapG16 = 64, kCapG32 = 256, kCapWave = 1024, kCapBlo
 // Counter words of a plan (uint32).
 constexpr int kCntBinCount = 0;                   // [kNumBins]
 constexpr int kCntBinCursor = kNumBins;           // [kNumBins]
-constexpr int kCntTileHead = 2 * kNumBins;        // tile queue head
-constexpr int kNumCounters = 2 * kNumBins + 2;
+constexpr int kCntTileHead = 2 * kNumBins;        // [kNumBins] tile queue heads
+constexpr int kNumCounters = 3 * kNumBins + 2;
 
 // One graph of a (possibly single-graph) count.  Its CSR lives at
 // rowptr_pool + rp_off (rowptr values relative to colidx_pool + ci_off);

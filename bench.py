@@ -116,15 +116,16 @@ def leg_single(ctx, g, steps, warmup, name):
     return work, res
 
 
-def leg_batch(ctx, seed, n_graphs, reps):
-    from hetegeneous_parallel_programming_amd import Graph
+def golden_big() -> dict:
+    """Oracle counts at full size (tests/golden/big.json), when present."""
+    p = os.path.join(ROOT, "tests", "golden", "big.json")
+    return json.load(open(p)) if os.path.exists(p) else {}
 
-    t0 = time.perf_counter()
-    graphs = Graph.batch(seed, 0, n_graphs)
-    log(f"batch: generated {n_graphs} graphs in {time.perf_counter() - t0:.1f} s")
+
+def leg_batch(ctx, graphs, seed, reps):
+    n_graphs = len(graphs)
     edges = sum(g.E() for g in graphs)
     b = ctx.batch(graphs)
-    del graphs
     each, ms_each, st_each = b.count_each()
     each2, ms_each2, st_each2 = b.count_each()
     got, _ = b.count()  # capture + first replay
@@ -142,10 +143,79 @@ def leg_batch(ctx, seed, n_graphs, reps):
            "batched_graphs_per_s": n_graphs / batched_s,
            "batched_ms": batched_s * 1e3, "batched_p99_ms": float(np.percentile(kms, 99)),
            "speedup": base_s / batched_s}
+    gold = golden_big().get(f"batch_seed{seed}_{n_graphs}")
+    res["exact_vs_cpu_oracle"] = (bool(np.array_equal(each, np.array(gold["triangles"], np.uint64)))
+                                  if gold else None)
     log(f"batch: baseline {res['baseline_graphs_per_s']:.0f} graphs/s, batched "
-        f"{res['batched_graphs_per_s']:.0f} graphs/s, speedup {res['speedup']:.2f}x")
+        f"{res['batched_graphs_per_s']:.0f} graphs/s, speedup {res['speedup']:.2f}x, "
+        f"exact vs oracle: {res['exact_vs_cpu_oracle']}")
     b.free()
-    return res
+    return res, each
+
+
+def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed):
+    """Poisson TC requests from the batched mix plus 1% RMAT-20 through the
+    continuous-batching server: saturated capacity, then 50% and 90% load."""
+    from hetegeneous_parallel_programming_amd.solver import Server, clock_ms
+
+    rng = np.random.default_rng(seed)
+    picks = np.where(rng.random(n_requests) < 0.01, -1, rng.integers(0, len(graphs), n_requests))
+    want = np.array([heavy_count if k < 0 else counts[k] for k in picks], np.uint64)
+    req = [heavy if k < 0 else graphs[k] for k in picks]
+    out = {"requests": n_requests, "heavy_fraction": float((picks < 0).mean())}
+    with Server(ctx, hbm_budget=16 << 30, slots=3, max_batch_graphs=2048, max_batch_rows=16 << 20,
+                max_queue=1 << 20, host_staging=4 << 30) as srv:
+        for i in range(64):  # warm-up
+            srv.submit(req[i], i)
+        srv.drain()
+        srv.poll(max_results=4096)
+
+        def run(arrivals):
+            got = {}
+            t0 = clock_ms() + 20.0
+            sched = t0 + arrivals if arrivals is not None else None
+            start = clock_ms()
+            for i in range(n_requests):
+                if sched is not None:
+                    while clock_ms() < sched[i]:
+                        pass
+                srv.submit(req[i], i)
+                if (i & 255) == 0:
+                    for r in srv.poll():
+                        got[r["tag"]] = r
+            srv.drain()
+            while len(got) < n_requests:
+                rs = srv.poll(timeout_ms=1000)
+                if not rs:
+                    break
+                for r in rs:
+                    got[r["tag"]] = r
+            done = np.array([got[i]["t_done_ms"] for i in range(n_requests)])
+            base = sched if sched is not None else np.array([got[i]["t_submit_ms"] for i in range(n_requests)])
+            lat = done - base
+            cnt = np.array([got[i]["count"] for i in range(n_requests)], np.uint64)
+            ok = bool(np.array_equal(cnt, want)) and all(got[i]["status"] == 0 for i in range(n_requests))
+            span = (done.max() - (sched[0] if sched is not None else start)) / 1e3
+            return {"graphs_per_s": n_requests / span, "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)), "exact": ok,
+                    "mean_batch": float(np.mean([got[i]["batch_graphs"] for i in range(n_requests)]))}
+
+        sat = run(None)
+        out["saturated"] = sat
+        cap = sat["graphs_per_s"]
+        log(f"serving saturated: {cap:.0f} graphs/s, p99 {sat['p99_ms']:.1f} ms, exact={sat['exact']}")
+        for frac in (0.5, 0.9):
+            lam = frac * cap
+            arr = np.cumsum(rng.exponential(1.0 / lam, n_requests)) * 1e3
+            r = run(arr)
+            r["offered_per_s"] = lam
+            out[f"load_{int(frac * 100)}"] = r
+            log(f"serving {int(frac * 100)}% load ({lam:.0f}/s): {r['graphs_per_s']:.0f} graphs/s, "
+                f"p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, exact={r['exact']}, "
+                f"mean batch {r['mean_batch']:.1f}")
+        out["server_stats"] = srv.stats()
+    out["exact_all"] = all(out[k]["exact"] for k in ("saturated", "load_50", "load_90"))
+    return out
 
 
 def leg_partition(ctx, seed, steps, warmup):
@@ -188,7 +258,8 @@ def main() -> int:
     ap.add_argument("--no-partition", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-rmat", action="store_true", help="skip the headline RMAT leg")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=10000, help="requests per serving run")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -237,9 +308,29 @@ def main() -> int:
         line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
         log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} edges/s on "
             f"{line['cpu_baseline']['cores']} cores")
+    gold = golden_big().get(f"rmat{args.scale}_seed{args.seed}")
+    line["config"]["exact_vs_cpu_oracle"] = (gold["triangles"] == best["count"]) if gold else None
     del g
-    if not args.no_batch:
-        line["extra"]["batched"] = leg_batch(ctx, args.seed, args.batch_graphs, max(3, args.steps // 4))
+    if not args.no_batch or not args.no_serving:
+        t0 = time.perf_counter()
+        graphs = Graph.batch(args.seed, 0, args.batch_graphs)
+        log(f"batch: generated {len(graphs)} graphs in {time.perf_counter() - t0:.1f} s")
+        res, counts = leg_batch(ctx, graphs, args.seed, max(3, args.steps // 4))
+        if not args.no_batch:
+            line["extra"]["batched"] = res
+        if not args.no_serving:
+            heavy = Graph.rmat_scale(20, 16, args.seed)
+            heavy.orientation()
+            dh = ctx.upload(heavy)
+            heavy_count = ctx.count(dh)[0]
+            dh.free()
+            line["extra"]["serving"] = leg_serving(ctx, graphs, counts, heavy, heavy_count,
+                                                   args.requests, args.seed + 100)
+            gold20 = golden_big().get(f"rmat20_seed{args.seed}")
+            line["extra"]["serving"]["rmat20_exact_vs_cpu_oracle"] = (
+                gold20["triangles"] == heavy_count if gold20 else None)
+            del heavy
+        del graphs
     if not args.no_partition:
         line["extra"]["partition_rmat24_p0"] = leg_partition(ctx, args.seed, max(3, args.steps // 4),
                                                              args.warmup)

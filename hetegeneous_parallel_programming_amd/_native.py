@@ -60,6 +60,12 @@ class Work(ctypes.Structure):
                 ("alg_bytes", c_i64), ("max_out", c_i32)]
 
 
+class ArenaStats(ctypes.Structure):
+    _fields_ = [("capacity", c_u64), ("used", c_u64), ("peak_used", c_u64), ("largest_free", c_u64),
+                ("live_extents", c_i64), ("free_extents", c_i64), ("allocs", c_i64),
+                ("failed_allocs", c_i64)]
+
+
 _host = None
 
 
@@ -105,6 +111,12 @@ def host():
     _sig(L, "tch_work_of_partition", c_i32, [ctypes.POINTER(PartitionView), ctypes.POINTER(Work)])
     _sig(L, "tch_read_graphminer", c_i32, [ctypes.c_char_p, c_i32, ctypes.POINTER(P)])
     _sig(L, "tch_write_graphminer", c_i32, [ctypes.c_char_p, ctypes.POINTER(CsrView)])
+    _sig(L, "tch_partition_assemble", c_i32, [ctypes.POINTER(PartitionView), ctypes.POINTER(P)])
+    _sig(L, "tch_arena_create", c_i32, [c_u64, c_u64, ctypes.POINTER(P)])
+    _sig(L, "tch_arena_destroy", None, [P])
+    _sig(L, "tch_arena_alloc", c_i32, [P, c_u64, ctypes.POINTER(c_u64)])
+    _sig(L, "tch_arena_free", c_i32, [P, c_u64])
+    _sig(L, "tch_arena_stats_get", c_i32, [P, ctypes.POINTER(ArenaStats)])
     _host = L
     return L
 

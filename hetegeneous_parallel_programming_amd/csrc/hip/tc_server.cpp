@@ -1,0 +1,449 @@
+// Serving loop: continuous batching of triangle-count requests over a
+// device-resident CSR pool.  One server thread owns all device work:
+//   retire  -- finished launches: deliver counts, free their pool extents;
+//   admit   -- waiting requests whose CSR fits the pool (arrival order, with
+//              bounded look-ahead past a blocked head) are uploaded on a
+//              copy stream while earlier launches keep running;
+//   launch  -- any free slot takes every uploaded request (up to the batch
+//              limits) and runs zero -> plan -> per-bin count kernels and
+//              the copy-back on its own stream.
+// Launch slots run concurrently on separate streams, so requests arriving
+// while others are on the GPU overlap with them on the device.
+#include <algorithm>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <thread>
+
+#include "tc_internal.hpp"
+
+using namespace tcr;
+
+namespace {
+
+constexpr uint64_t kAlign = 256;
+inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct Req {
+  uint64_t tag = 0;
+  int32_t n = 0;
+  int64_t nnz = 0;
+  std::vector<int64_t> rp;  // pageable copy (used when the staging area is full)
+  std::vector<int32_t> ci;
+  char* staged = nullptr;   // or: copy in the pinned staging area
+  uint64_t stage_off = 0;
+  uint64_t off = 0, bytes = 0;
+  double t_submit = 0, t_admit = 0, t_launch = 0;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  void* plan = nullptr;              // device: items | bin_of | blk | counters | outs | descs
+  uint64_t* pinned = nullptr;        // host: outs[G] then descs staging
+  tck::PlanBufs bufs{};
+  tck::GraphDesc* d_descs = nullptr;
+  std::vector<Req*> reqs;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct tc_server {
+  tc_ctx* ctx = nullptr;
+  tc_server_config cfg{};
+  char* pool = nullptr;
+  tch_arena* arena = nullptr;
+  char* staging = nullptr;         // pinned host memory
+  tch_arena* stage_arena = nullptr;
+  hipStream_t copy_stream = nullptr;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req*> waiting, ready;
+  std::deque<tc_request_result> results;
+  int64_t outstanding = 0, submitted = 0, completed = 0, rejected = 0, launches = 0, launched_graphs = 0;
+  bool stop = false;
+  std::thread th;
+
+  void run();
+  bool retire();
+  bool admit();
+  bool launch();
+  void finish(Req* r, uint64_t count, tc_status st, int batch, double t_done);
+  void release();
+};
+
+void tc_server::finish(Req* r, uint64_t count, tc_status st, int batch, double t_done) {
+  tc_request_result res;
+  res.tag = r->tag;
+  res.count = count;
+  res.status = st;
+  res.batch_graphs = batch;
+  res.t_submit_ms = r->t_submit;
+  res.t_admit_ms = r->t_admit;
+  res.t_launch_ms = r->t_launch;
+  res.t_done_ms = t_done;
+  if (r->bytes) tch_arena_free(arena, r->off);
+  delete r;
+  std::lock_guard<std::mutex> lk(mu);
+  results.push_back(res);
+  --outstanding;
+  ++completed;
+  cv_done.notify_all();
+}
+
+bool tc_server::retire() {
+  bool any = false;
+  for (auto& s : slots) {
+    if (!s.busy) continue;
+    hipError_t e = hipEventQuery(s.done);
+    if (e == hipErrorNotReady) continue;
+    const double t = now_ms();
+    const int k = (int)s.reqs.size();
+    for (int i = 0; i < k; ++i)
+      finish(s.reqs[i], e == hipSuccess ? s.pinned[i] : 0, e == hipSuccess ? TC_OK : TC_EDEVICE, k, t);
+    if (e != hipSuccess) (void)hipGetLastError();
+    s.reqs.clear();
+    s.busy = false;
+    any = true;
+  }
+  return any;
+}
+
+bool tc_server::admit() {
+  std::vector<Req*> take;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    int passed = 0;
+    for (auto it = waiting.begin(); it != waiting.end() && passed <= cfg.lookahead;) {
+      Req* r = *it;
+      uint64_t off;
+      if (tch_arena_alloc(arena, r->bytes, &off) == TC_OK) {
+        r->off = off;
+        take.push_back(r);
+        it = waiting.erase(it);
+      } else {
+        ++passed;
+        ++it;
+      }
+    }
+  }
+  if (take.empty()) return false;
+  hipError_t e = hipSuccess;
+  for (Req* r : take) {
+    char* base = pool + r->off;
+    const uint64_t rp_bytes = sizeof(int64_t) * ((uint64_t)r->n + 1);
+    const void* src_rp = r->staged ? (const void*)r->staged : (const void*)r->rp.data();
+    const void* src_ci = r->staged ? (const void*)(r->staged + align_up(rp_bytes)) : (const void*)r->ci.data();
+    e = hipMemcpyAsync(base, src_rp, rp_bytes, hipMemcpyHostToDevice, copy_stream);
+    if (e == hipSuccess && r->nnz > 0)
+      e = hipMemcpyAsync(base + align_up(rp_bytes), src_ci, sizeof(int32_t) * (size_t)r->nnz,
+                         hipMemcpyHostToDevice, copy_stream);
+    if (e != hipSuccess) break;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(copy_stream);
+  const double t = now_ms();
+  for (Req* r : take) {
+    if (r->staged) {
+      tch_arena_free(stage_arena, r->stage_off);
+      r->staged = nullptr;
+    }
+    if (e != hipSuccess) {
+      finish(r, 0, TC_EDEVICE, 0, t);
+      continue;
+    }
+    std::vector<int64_t>().swap(r->rp);
+    std::vector<int32_t>().swap(r->ci);
+    r->t_admit = t;
+    ready.push_back(r);
+  }
+  if (e != hipSuccess) (void)hipGetLastError();
+  return true;
+}
+
+bool tc_server::launch() {
+  bool any = false;
+  for (auto& s : slots) {
+    if (s.busy || ready.empty()) continue;
+    int64_t rows = 0;
+    while (!ready.empty() && (int)s.reqs.size() < cfg.max_batch_graphs &&
+           rows + ready.front()->n <= cfg.max_batch_rows) {
+      rows += ready.front()->n;
+      s.reqs.push_back(ready.front());
+      ready.pop_front();
+    }
+    if (s.reqs.empty()) continue;
+    const int k = (int)s.reqs.size();
+    tck::GraphDesc* hd = (tck::GraphDesc*)(s.pinned + cfg.max_batch_graphs);
+    int64_t pb = 0;
+    for (int i = 0; i < k; ++i) {
+      Req* r = s.reqs[i];
+      hd[i] = tck::GraphDesc{(int64_t)(r->off / sizeof(int64_t)),
+                             (int64_t)((r->off + align_up(sizeof(int64_t) * ((uint64_t)r->n + 1))) /
+                                       sizeof(int32_t)),
+                             pb, 0, r->n};
+      pb += r->n;
+    }
+    tck::LaunchArgs a;
+    a.descs = s.d_descs;
+    a.ngraphs = k;
+    a.P = pb;
+    a.rp = (const int64_t*)pool;
+    a.ci = (const int32_t*)pool;
+    {
+      std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+      a.tune = to_kernel_tune(ctx->tune);
+    }
+    a.bufs = s.bufs;
+    const double t = now_ms();
+    for (Req* r : s.reqs) r->t_launch = t;
+    hipError_t e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
+    // A batch of one graph would take the single-graph reduction path.
+    if (e == hipSuccess) e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s.stream);
+    if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)k, s.stream);
+    if (e == hipSuccess) e = tck::launch_plan(a, s.stream);
+    if (e == hipSuccess) e = tck::launch_count(a, ctx->grid(), s.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.pinned, a.bufs.outs, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(s.stream);
+      const double tf = now_ms();
+      for (Req* r : s.reqs) finish(r, 0, TC_EDEVICE, k, tf);
+      s.reqs.clear();
+      continue;
+    }
+    s.busy = true;
+    any = true;
+    std::lock_guard<std::mutex> lk(mu);
+    ++launches;
+    launched_graphs += k;
+  }
+  return any;
+}
+
+void tc_server::run() {
+  (void)hipSetDevice(ctx->device);
+  for (;;) {
+    bool progressed = retire();
+    progressed |= admit();
+    progressed |= launch();
+    if (progressed) continue;
+    std::unique_lock<std::mutex> lk(mu);
+    const bool idle = waiting.empty() && ready.empty() &&
+                      std::none_of(slots.begin(), slots.end(), [](const Slot& s) { return s.busy; });
+    if (stop && idle) break;
+    if (idle) {
+      cv_work.wait_for(lk, std::chrono::milliseconds(50));
+    } else {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));  // launches in flight
+    }
+  }
+}
+
+void tc_server::release() {
+  for (auto& s : slots) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    dev_free(s.plan);
+    if (s.pinned) (void)hipHostFree(s.pinned);
+  }
+  slots.clear();
+  if (copy_stream) (void)hipStreamDestroy(copy_stream);
+  dev_free(pool);
+  if (arena) tch_arena_destroy(arena);
+  if (staging) (void)hipHostFree(staging);
+  if (stage_arena) tch_arena_destroy(stage_arena);
+  staging = nullptr;
+  stage_arena = nullptr;
+  for (Req* r : waiting) delete r;
+  for (Req* r : ready) delete r;
+  waiting.clear();
+  ready.clear();
+}
+
+extern "C" {
+
+double tc_clock_ms(void) { return now_ms(); }
+
+uint64_t tc_request_footprint(int32_t n_rows, int64_t nnz) {
+  if (n_rows < 0 || nnz < 0) return 0;
+  return align_up(sizeof(int64_t) * ((uint64_t)n_rows + 1)) + align_up(sizeof(int32_t) * (uint64_t)nnz);
+}
+
+tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out) {
+  if (!ctx || !cfg || !out) return fail(TC_EINVAL, "null argument");
+  if (cfg->slots < 1 || cfg->slots > 8) return fail(TC_EINVAL, "slots must be in [1, 8]");
+  if (cfg->max_batch_graphs < 1 || cfg->max_batch_rows < 1 || cfg->max_queue < 1 || cfg->lookahead < 0)
+    return fail(TC_EINVAL, "batch limits and queue size must be positive");
+  if (cfg->max_batch_rows >= ((int64_t)1 << 31)) return fail(TC_ERANGE, "max_batch_rows must be < 2^31");
+  if (cfg->hbm_budget < kAlign) return fail(TC_EINVAL, "hbm_budget too small");
+  std::unique_ptr<tc_server> s(new tc_server);
+  s->ctx = ctx;
+  s->cfg = *cfg;
+  DeviceGuard guard(ctx->device);
+  tc_status st = dev_alloc((void**)&s->pool, cfg->hbm_budget, "server pool");
+  if (st != TC_OK) return st;
+  st = tch_arena_create(cfg->hbm_budget, kAlign, &s->arena);
+  if (st != TC_OK) {
+    s->release();
+    return fail(st, tch_last_error());
+  }
+  if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    s->release();
+    return fail(TC_EDEVICE, "copy stream creation failed");
+  }
+  if (cfg->host_staging >= kAlign) {
+    if (hipHostMalloc((void**)&s->staging, cfg->host_staging, hipHostMallocDefault) != hipSuccess ||
+        tch_arena_create(cfg->host_staging, kAlign, &s->stage_arena) != TC_OK) {
+      (void)hipGetLastError();
+      s->release();
+      return fail(TC_ENOMEM, "pinned staging allocation failed");
+    }
+  }
+  const int64_t R = cfg->max_batch_rows, G = cfg->max_batch_graphs;
+  const int64_t nblk = (R + tck::kBlock - 1) / tck::kBlock;
+  const uint64_t o_items = 0, o_bin = o_items + align_up(8 * R), o_blk = o_bin + align_up(R),
+                 o_cnt = o_blk + align_up(4 * tck::kBlkStride * nblk), o_outs = o_cnt + align_up(4 * 64),
+                 o_desc = o_outs + align_up(8 * G), total = o_desc + align_up(sizeof(tck::GraphDesc) * G);
+  s->slots.resize(cfg->slots);
+  for (auto& sl : s->slots) {
+    hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&sl.pinned, 8 * G + sizeof(tck::GraphDesc) * G, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      s->release();
+      return fail(TC_EDEVICE, std::string("server slot: ") + hipGetErrorString(e));
+    }
+    st = dev_alloc(&sl.plan, total, "slot plan scratch");
+    if (st != TC_OK) {
+      s->release();
+      return st;
+    }
+    char* p = (char*)sl.plan;
+    sl.bufs = tck::PlanBufs{(uint8_t*)(p + o_bin), (uint64_t*)(p + o_items), (uint32_t*)(p + o_blk),
+                            (uint32_t*)(p + o_cnt), (uint64_t*)(p + o_outs)};
+    sl.d_descs = (tck::GraphDesc*)(p + o_desc);
+  }
+  tc_server* raw = s.release();
+  raw->th = std::thread([raw]() { raw->run(); });
+  *out = raw;
+  return TC_OK;
+}
+
+tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check) {
+  if (!s || !dag) return fail(TC_EINVAL, "null argument");
+  tc_status st = tch_validate(dag, check | TC_CHECK_SHAPE);
+  if (st != TC_OK) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->rejected;
+    return fail(st, tch_last_error());
+  }
+  const uint64_t fp = tc_request_footprint(dag->n_rows, dag->nnz);
+  auto reject = [&](tc_status code, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->rejected;
+    return fail(code, msg);
+  };
+  tch_arena_stats as;
+  tch_arena_stats_get(s->arena, &as);
+  if (fp > as.capacity) return reject(TC_ENOMEM, "request footprint exceeds the server's HBM pool");
+  if (dag->n_rows > s->cfg.max_batch_rows) return reject(TC_ERANGE, "request has more rows than max_batch_rows");
+  for (int64_t v = 0; v < dag->n_rows; ++v)
+    if ((dag->rowptr[v + 1] - dag->rowptr[v]) * 64 >= ((int64_t)1 << 31))
+      return reject(TC_ERANGE, "DAG out-degree too large");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return fail(TC_ESTATE, "server is shutting down");
+    if (s->outstanding >= s->cfg.max_queue) {
+      ++s->rejected;
+      return fail(TC_EBUSY, "server queue is full");
+    }
+    ++s->outstanding;  // reserve the queue place before the copy
+  }
+  std::unique_ptr<Req> r(new Req);
+  r->tag = tag;
+  r->n = dag->n_rows;
+  r->nnz = dag->nnz;
+  r->bytes = fp;
+  uint64_t so = 0;
+  if (s->stage_arena && tch_arena_alloc(s->stage_arena, fp, &so) == TC_OK) {
+    r->stage_off = so;
+    r->staged = s->staging + so;
+    const uint64_t rp_bytes = sizeof(int64_t) * ((uint64_t)dag->n_rows + 1);
+    std::memcpy(r->staged, dag->rowptr, rp_bytes);
+    if (dag->nnz) std::memcpy(r->staged + align_up(rp_bytes), dag->colidx, sizeof(int32_t) * (size_t)dag->nnz);
+  } else {
+    r->rp.assign(dag->rowptr, dag->rowptr + dag->n_rows + 1);
+    r->ci.assign(dag->colidx, dag->colidx + dag->nnz);
+  }
+  r->t_submit = now_ms();
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->waiting.push_back(r.release());
+  ++s->submitted;
+  s->cv_work.notify_one();
+  return TC_OK;
+}
+
+tc_status tc_server_poll(tc_server* s, tc_request_result* out, int32_t max, int32_t* n_out, int32_t timeout_ms) {
+  if (!s || !n_out || (max > 0 && !out)) return fail(TC_EINVAL, "null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->results.empty() && timeout_ms > 0)
+    s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !s->results.empty(); });
+  int32_t k = 0;
+  while (k < max && !s->results.empty()) {
+    out[k++] = s->results.front();
+    s->results.pop_front();
+  }
+  *n_out = k;
+  return TC_OK;
+}
+
+tc_status tc_server_drain(tc_server* s, int32_t timeout_ms) {
+  if (!s) return fail(TC_EINVAL, "null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_work.notify_one();
+  if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(std::max(timeout_ms, 0)),
+                           [&] { return s->outstanding == 0; }))
+    return fail(TC_ETIMEDOUT, "requests still unfinished after the drain timeout");
+  return TC_OK;
+}
+
+tc_status tc_server_get_stats(tc_server* s, tc_server_stats* o) {
+  if (!s || !o) return fail(TC_EINVAL, "null argument");
+  tch_arena_stats as;
+  tch_arena_stats_get(s->arena, &as);
+  std::lock_guard<std::mutex> lk(s->mu);
+  o->submitted = s->submitted;
+  o->completed = s->completed;
+  o->rejected = s->rejected;
+  o->launches = s->launches;
+  o->queued = (int64_t)s->waiting.size();
+  o->in_flight = s->outstanding - (int64_t)s->waiting.size();
+  o->mean_batch_graphs = s->launches ? (double)s->launched_graphs / s->launches : 0.0;
+  o->pool_capacity = as.capacity;
+  o->pool_used = as.used;
+  o->pool_peak = as.peak_used;
+  return TC_OK;
+}
+
+tc_status tc_server_destroy(tc_server* s) {
+  if (!s) return TC_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->th.joinable()) s->th.join();
+  DeviceGuard guard(s->ctx->device);
+  s->release();
+  delete s;
+  return TC_OK;
+}
+
+}  // extern "C"

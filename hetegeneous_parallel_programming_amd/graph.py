@@ -271,6 +271,37 @@ class PartitionedGraph:
         return HaloPartition(out.value)
 
 
+class Arena:
+    """First-fit extent bookkeeping (``tch_arena_*``) used for the server's HBM pool."""
+
+    def __init__(self, capacity: int, align: int = 256):
+        h = ctypes.c_void_p()
+        host_check(host().tch_arena_create(capacity, align, ctypes.byref(h)))
+        self._h = h
+
+    def alloc(self, nbytes: int) -> int:
+        off = nat.c_u64(0)
+        host_check(host().tch_arena_alloc(self._h, nbytes, ctypes.byref(off)))
+        return int(off.value)
+
+    def free(self, offset: int) -> None:
+        host_check(host().tch_arena_free(self._h, offset))
+
+    def stats(self) -> dict:
+        st = nat.ArenaStats()
+        host_check(host().tch_arena_stats_get(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in nat.ArenaStats._fields_}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                host().tch_arena_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 def batch_spec(batch_seed: int, index: int) -> dict:
     s = nat.BatchSpec()
     host_check(host().tch_batch_spec_of(batch_seed, index, ctypes.byref(s)))

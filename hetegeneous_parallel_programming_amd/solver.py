@@ -47,6 +47,25 @@ class Tune(ctypes.Structure):
         return t
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", nat.c_u64), ("slots", nat.c_i32), ("max_batch_graphs", nat.c_i32),
+                ("max_batch_rows", nat.c_i64), ("max_queue", nat.c_i32), ("lookahead", nat.c_i32),
+                ("host_staging", nat.c_u64)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("tag", nat.c_u64), ("count", nat.c_u64), ("status", nat.c_i32),
+                ("batch_graphs", nat.c_i32), ("t_submit_ms", nat.c_dbl), ("t_admit_ms", nat.c_dbl),
+                ("t_launch_ms", nat.c_dbl), ("t_done_ms", nat.c_dbl)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", nat.c_i64), ("completed", nat.c_i64), ("rejected", nat.c_i64),
+                ("launches", nat.c_i64), ("queued", nat.c_i64), ("in_flight", nat.c_i64),
+                ("mean_batch_graphs", nat.c_dbl), ("pool_capacity", nat.c_u64),
+                ("pool_used", nat.c_u64), ("pool_peak", nat.c_u64)]
+
+
 _dev = None
 
 
@@ -81,6 +100,15 @@ def device_lib():
     S(L, "tc_batch_free", nat.c_i32, [P, P])
     S(L, "tc_batch_bytes", nat.c_i64, [P])
     S(L, "tc_probe_copy", nat.c_i32, [P, nat.c_i64, nat.c_i32, ctypes.POINTER(nat.c_dbl)])
+    S(L, "tc_clock_ms", nat.c_dbl, [])
+    S(L, "tc_request_footprint", nat.c_u64, [nat.c_i32, nat.c_i64])
+    S(L, "tc_server_create", nat.c_i32, [P, ctypes.POINTER(ServerConfig), PP])
+    S(L, "tc_server_submit", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_u64, nat.c_u32])
+    S(L, "tc_server_poll", nat.c_i32, [P, ctypes.POINTER(RequestResult), nat.c_i32,
+                                       ctypes.POINTER(nat.c_i32), nat.c_i32])
+    S(L, "tc_server_drain", nat.c_i32, [P, nat.c_i32])
+    S(L, "tc_server_get_stats", nat.c_i32, [P, ctypes.POINTER(ServerStats)])
+    S(L, "tc_server_destroy", nat.c_i32, [P])
     _dev = L
     return L
 
@@ -235,6 +263,75 @@ class Context:
     def close(self) -> None:
         if self._h is not None and self._h.value:
             device_lib().tc_ctx_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def clock_ms() -> float:
+    """The serving loop's monotonic clock (ms)."""
+    return float(device_lib().tc_clock_ms())
+
+
+def request_footprint(n_rows: int, nnz: int) -> int:
+    return int(device_lib().tc_request_footprint(n_rows, nnz))
+
+
+class Server:
+    """Continuous-batching TC server on one device (``tc_server_*``)."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 8 << 30, slots: int = 3,
+                 max_batch_graphs: int = 1024, max_batch_rows: int = 8 << 20,
+                 max_queue: int = 1 << 16, lookahead: int = 64, host_staging: int = 1 << 30):
+        cfg = ServerConfig(hbm_budget, slots, max_batch_graphs, max_batch_rows, max_queue, lookahead,
+                           host_staging)
+        h = ctypes.c_void_p()
+        _check(device_lib().tc_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.ctx = ctx
+        self._h = h
+        self._buf = (RequestResult * 4096)()
+
+    def submit(self, g: Graph, tag: int, check: int = nat.TC_CHECK_SHAPE) -> None:
+        _check(device_lib().tc_server_submit(self._h, ctypes.byref(g.view()), tag, check))
+
+    def try_submit(self, g: Graph, tag: int, check: int = nat.TC_CHECK_SHAPE) -> int:
+        """Status code instead of an exception (TC_EBUSY when the queue is full)."""
+        return int(device_lib().tc_server_submit(self._h, ctypes.byref(g.view()), tag, check))
+
+    def poll(self, timeout_ms: int = 0, max_results: int = 4096) -> list[dict]:
+        n = nat.c_i32(0)
+        _check(device_lib().tc_server_poll(self._h, self._buf, min(max_results, 4096),
+                                           ctypes.byref(n), timeout_ms))
+        out = []
+        for i in range(n.value):
+            r = self._buf[i]
+            out.append({"tag": r.tag, "count": r.count, "status": r.status,
+                        "batch_graphs": r.batch_graphs, "t_submit_ms": r.t_submit_ms,
+                        "t_admit_ms": r.t_admit_ms, "t_launch_ms": r.t_launch_ms,
+                        "t_done_ms": r.t_done_ms})
+        return out
+
+    def drain(self, timeout_ms: int = 600_000) -> None:
+        _check(device_lib().tc_server_drain(self._h, timeout_ms))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(device_lib().tc_server_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            _check(device_lib().tc_server_destroy(self._h))
         self._h = None
 
     def __enter__(self):

@@ -108,6 +108,58 @@ tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double
 tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b);
 int64_t tc_batch_bytes(const tc_batch* b);
 
+/* ---- serving loop --------------------------------------------------------
+ * Continuous batching of TC requests.  No counterpart in the reference,
+ * which counts one pre-loaded graph per process (successful_push_pull/main.cc:11-36).
+ * A server thread admits submitted requests in arrival order when their CSR
+ * footprint fits the HBM pool (a request that cannot fit yet waits; up to
+ * `lookahead` later ones may pass it), uploads them on a copy stream while
+ * earlier launches run, and whenever one of `slots` launch slots is free it
+ * launches every admitted request (up to the batch limits) as one batched
+ * count on that slot's stream.  Results are delivered by tc_server_poll. */
+typedef struct tc_server_config {
+  uint64_t hbm_budget;       /* bytes of HBM for resident request CSRs            */
+  int32_t slots;             /* concurrent launches (streams), 1..8               */
+  int32_t max_batch_graphs;  /* requests per launch                               */
+  int64_t max_batch_rows;    /* pivot rows per launch (sizes per-slot scratch)    */
+  int32_t max_queue;         /* unfinished requests before submit returns TC_EBUSY */
+  int32_t lookahead;         /* requests that may be admitted past a blocked head */
+  uint64_t host_staging;     /* bytes of pinned host memory submit copies requests
+                                into (async DMA to HBM); 0 = pageable copies     */
+} tc_server_config;
+
+typedef struct tc_request_result {
+  uint64_t tag;
+  uint64_t count;            /* u64 triangle count of the request's DAG           */
+  int32_t status;            /* TC_OK or the failure of this request              */
+  int32_t batch_graphs;      /* requests in the launch that counted it            */
+  double t_submit_ms, t_admit_ms, t_launch_ms, t_done_ms; /* tc_clock_ms() times  */
+} tc_request_result;
+
+typedef struct tc_server_stats {
+  int64_t submitted, completed, rejected, launches, queued, in_flight;
+  double mean_batch_graphs;
+  uint64_t pool_capacity, pool_used, pool_peak;
+} tc_server_stats;
+
+/* Monotonic clock (ms) used for every timestamp of the serving loop. */
+double tc_clock_ms(void);
+/* HBM bytes a request of this shape holds while resident. */
+uint64_t tc_request_footprint(int32_t n_rows, int64_t nnz);
+tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out);
+/* Copies the DAG (caller keeps ownership).  TC_EBUSY: queue full;
+ * TC_ENOMEM: footprint exceeds the whole pool; TC_ERANGE: more rows than
+ * max_batch_rows. */
+tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check);
+/* Up to `max` finished results; waits up to timeout_ms for the first one. */
+tc_status tc_server_poll(tc_server* s, tc_request_result* out, int32_t max, int32_t* n_out,
+                         int32_t timeout_ms);
+/* Waits until every submitted request has finished (TC_ETIMEDOUT otherwise). */
+tc_status tc_server_drain(tc_server* s, int32_t timeout_ms);
+tc_status tc_server_get_stats(tc_server* s, tc_server_stats* out);
+/* Finishes all submitted work, then frees the server. */
+tc_status tc_server_destroy(tc_server* s);
+
 /* ---- measurement utility ------------------------------------------------
  * Copies `bytes` (a multiple of 16) between two device buffers with a plain
  * streaming kernel `iters` times and reports the mean kernel time.  No

@@ -147,6 +147,25 @@ tc_status tch_work_of(const tc_csr_view* dag, int32_t row_begin, int32_t row_end
 /* Same for a partition with its halo cache. */
 tc_status tch_work_of_partition(const tch_partition_view* p, tch_work* out);
 
+/* ---- arena bookkeeping for device memory pools ---------------------------
+ * First-fit placement with coalescing over [0, capacity); offsets are
+ * multiples of `align`.  Pure host bookkeeping (no device calls): the
+ * serving loop places device-resident CSRs in one HBM allocation with it.
+ * No counterpart in the reference, which allocates per graph
+ * (successful_push_pull/graph_nvshmem.h:38-43). */
+typedef struct tch_arena tch_arena;
+tc_status tch_arena_create(uint64_t capacity, uint64_t align, tch_arena** out);
+void tch_arena_destroy(tch_arena* a);
+/* TC_ENOMEM when no free extent is large enough. */
+tc_status tch_arena_alloc(tch_arena* a, uint64_t bytes, uint64_t* offset);
+/* TC_EINVAL when offset is not the start of a live extent. */
+tc_status tch_arena_free(tch_arena* a, uint64_t offset);
+typedef struct tch_arena_stats {
+  uint64_t capacity, used, peak_used, largest_free;
+  int64_t live_extents, free_extents, allocs, failed_allocs;
+} tch_arena_stats;
+tc_status tch_arena_stats_get(const tch_arena* a, tch_arena_stats* out);
+
 /* ---- GraphMiner on-disk format (graph.cc:31-55) -------------------------- */
 
 /* Reads prefix.meta.txt ("n m vid_size eid_size vlabel_size elabel_size

@@ -90,13 +90,12 @@ def cpu_baseline(g, target_s: float, seed: int = 7) -> dict:
                       f"same RMAT-22 DAG, OpenMP merge count, {t:.2f} s"}
 
 
-def leg_single(ctx, g, steps, warmup, name):
-    from hetegeneous_parallel_programming_amd.solver import Context  # noqa: F401
-
+def leg_single(ctx, g, steps, warmup, name, plain=True):
     work = g.work()
     dg = ctx.upload(g)
     res = {}
-    for kind, fn in (("binned", ctx.count), ("plain", ctx.count_plain)):
+    kinds = [("binned", ctx.count)] + ([("plain", ctx.count_plain)] if plain else [])
+    for kind, fn in kinds:
         ms_total, outs = time_steps(lambda: fn(dg), steps, warmup)
         counts = {o[0] for o in outs}
         if len(counts) != 1:
@@ -259,6 +258,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-rmat", action="store_true", help="skip the headline RMAT leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--requests", type=int, default=10000, help="requests per serving run")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
@@ -278,15 +279,40 @@ def main() -> int:
         print(json.dumps({"copy_bytes": args.copy_probe, "iters": max(1, args.steps), "ms": ms,
                           "GBps_read_plus_write": 2 * args.copy_probe / (ms / 1e3) / 1e9}), flush=True)
         return 0
+    ctx = Context(0)
+    log(f"device: {ctx.name}")
+    line = {"metric": "TC edges/s, %HBM peak; batched graphs/s + p99 vs one-launch-per-graph",
+            "value": None, "extra": {}}
+    if not args.no_rmat:
+        line = leg_headline(ctx, args)
+    run_side_legs(ctx, args, line)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def traffic_of(workload: str):
+    """Measured HBM bytes per count-kernel launch from the committed rocprofv3
+    summary (profiles/traffic_<workload>.json), when there is one."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(p):
+        return None, None
+    t = json.load(open(p))
+    return t.get("bytes_per_step"), os.path.relpath(p, ROOT)
+
+
+def leg_headline(ctx, args) -> dict:
+    from hetegeneous_parallel_programming_amd import Graph
+
     t0 = time.perf_counter()
     g = Graph.rmat_scale(args.scale, 16, args.seed)
     g.orientation()
     log(f"RMAT-{args.scale} generated+oriented in {time.perf_counter() - t0:.1f} s: "
         f"n={g.V()} DAG edges={g.E()} max out={g.get_max_degree()}")
-    ctx = Context(0)
-    log(f"device: {ctx.name}")
-    work, single = leg_single(ctx, g, args.steps, args.warmup, f"rmat{args.scale}")
+    work, single = leg_single(ctx, g, args.steps, args.warmup, f"rmat{args.scale}",
+                              plain=not args.no_plain)
     best = single["binned"]
+    traffic, traffic_src = traffic_of(f"rmat{args.scale}")
     line = {
         "metric": "TC edges/s, %HBM peak; batched graphs/s + p99 vs one-launch-per-graph",
         "value": best["edges_per_s"], "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -298,11 +324,12 @@ def main() -> int:
                    "vertices": g.V(), "dag_edges": work["edges"], "count": best["count"],
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": best["alg_GBps"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": best["alg_GBps"] / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "count_kernel", "alg_bytes": work["alg_bytes"],
-                     "kernel_ms_avg": best["kernel_ms_avg"]},
+                     "unit": "GB/s", "frac": best["alg_GBps"] / PEAK_HBM_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": "count_kernel<bin> (6 per-bin persistent launches per step)",
+                     "alg_bytes": work["alg_bytes"], "kernel_ms_avg": best["kernel_ms_avg"]},
         "cpu_baseline": None,
-        "extra": {"plain_kernel": single["plain"], "binned": best},
+        "extra": {"plain_kernel": single.get("plain"), "binned": best},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
@@ -310,7 +337,12 @@ def main() -> int:
             f"{line['cpu_baseline']['cores']} cores")
     gold = golden_big().get(f"rmat{args.scale}_seed{args.seed}")
     line["config"]["exact_vs_cpu_oracle"] = (gold["triangles"] == best["count"]) if gold else None
-    del g
+    return line
+
+
+def run_side_legs(ctx, args, line) -> None:
+    from hetegeneous_parallel_programming_amd import Graph
+
     if not args.no_batch or not args.no_serving:
         t0 = time.perf_counter()
         graphs = Graph.batch(args.seed, 0, args.batch_graphs)
@@ -334,9 +366,6 @@ def main() -> int:
     if not args.no_partition:
         line["extra"]["partition_rmat24_p0"] = leg_partition(ctx, args.seed, max(3, args.steps // 4),
                                                              args.warmup)
-    ctx.close()
-    print(json.dumps(line), flush=True)
-    return 0
 
 
 if __name__ == "__main__":

@@ -59,14 +59,29 @@ __device__ __forceinline__ int find_edge(const int32_t* sEnd, int t) {
   return k;
 }
 
-// LDS copies of pivot lists are padded with one word per 32 so that the
-// power-of-two probe strides of a binary search fall into different banks.
-__device__ __forceinline__ int pad_idx(int i) { return i + (i >> 5); }
-constexpr int padded(int cap) { return cap + cap / 32; }
+// Pivot lists in LDS are implicit complete binary search trees in BFS
+// ("Eytzinger") order: node k has children 2k and 2k+1, node 1 is the root,
+// word 0 holds -1 (never an id) and leaves past the list hold INT_MAX.  An
+// in-order walk gives the sorted list, so a search is still a binary search,
+// but one descent step is just k = 2k + (slot[k] < w) -- compare and
+// add-with-carry, no select, no index arithmetic -- and the upper levels are
+// consecutive words, read without bank conflicts.  A tree of h levels holds
+// up to 2^h - 1 ids in a slot of 2^h words.
 
-template <bool kPadded>
-__device__ __forceinline__ int a_at(const int32_t* A, int i) {
-  return A[kPadded ? pad_idx(i) : i];
+// Levels of the smallest complete tree holding n >= 1 ids.
+__device__ __forceinline__ int tree_levels(int n) { return 32 - __builtin_clz((unsigned)n); }
+
+// BFS position of in-order index x (0-based) in a complete tree of h levels.
+__device__ __forceinline__ int tree_pos(int x, int h) {
+  const int i1 = x + 1;
+  const int tz = __builtin_ctz((unsigned)i1);
+  return (1 << (h - 1 - tz)) + (i1 >> (tz + 1));
+}
+
+// j-th smallest id of a pivot list: from the LDS tree, or from the sorted HBM copy.
+template <bool kTree>
+__device__ __forceinline__ int list_at(const int32_t* A, int j, int h) {
+  return kTree ? A[tree_pos(j, h)] : A[j];
 }
 
 #ifndef TC_IN_FLIGHT
@@ -75,25 +90,26 @@ __device__ __forceinline__ int a_at(const int32_t* A, int i) {
 constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
 
 // Counts, for the edges j = jfirst, jfirst + jstride, ... of pivot u (list
-// A[0, da): padded in LDS, or plain in HBM), the elements of N+(v) found in
-// A.  sEnd/sBase are this group's G-entry LDS scratch.  Returns this lane's
-// share of the hits.
+// of da ids: an LDS tree, or the sorted list in HBM), the elements of N+(v)
+// found in that list.  sEnd/sBase are this group's G-entry LDS scratch.
+// Returns this lane's share of the hits.
 //
 // Flat index t of a batch maps to edge k (sEnd[k-1] <= t < sEnd[k]) and to
 // colidx[sBase[k] + t]; each lane keeps its current edge's end and base in
 // registers and searches sEnd only when it steps past that edge.
-template <int G, bool kPadded>
+template <int G, bool kTree>
 __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
                                                 const int32_t* __restrict__ ci, const int32_t* A,
                                                 int da, int jfirst, int jstride, int32_t* sEnd,
                                                 int64_t* sBase, int gl) {
   uint32_t hits = 0;
+  const int h = tree_levels(da);
   for (int j0 = jfirst; j0 < da; j0 += jstride) {
     const int j = j0 + gl;
     int c = 0;
     int64_t off = 0;
     if (j < da) {
-      const int v = a_at<kPadded>(A, j);
+      const int v = list_at<kTree>(A, j, h);
       const int64_t b = rp[v];
       c = (int)(rp[v + 1] - b);
       off = b;
@@ -120,28 +136,53 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
           w[q] = ci[cur_base + t];
         }
       }
-      int lo[kInFlight];
+      if (kTree) {
+        // h descent steps; then the lower bound is k with its trailing one
+        // bits and the following zero shifted out (0 = past the end -> -1).
+        // The node index is carried as a byte offset (4k): one compare, one
+        // select and one shift-add per step.
+        const char* Ab = (const char*)A;
+        int kb[kInFlight];
 #pragma unroll
-      for (int q = 0; q < kInFlight; ++q) lo[q] = 0;
-      for (int len = da; len > 1;) {
-        const int half = len >> 1;
+        for (int q = 0; q < kInFlight; ++q) kb[q] = 4;
+        for (int s = 0; s < h; ++s) {
+          int val[kInFlight];
 #pragma unroll
-        for (int q = 0; q < kInFlight; ++q)
-          lo[q] = (a_at<kPadded>(A, lo[q] + half) <= w[q]) ? lo[q] + half : lo[q];
-        len -= half;
+          for (int q = 0; q < kInFlight; ++q) val[q] = *(const int32_t*)(Ab + kb[q]);
+#pragma unroll
+          for (int q = 0; q < kInFlight; ++q) kb[q] = 2 * kb[q] + (val[q] < w[q] ? 4 : 0);
+        }
+#pragma unroll
+        for (int q = 0; q < kInFlight; ++q) {
+          const int k = kb[q] >> 2;
+          hits += (A[k >> __builtin_ffs(~k)] == w[q]) ? 1u : 0u;
+        }
+      } else {
+        int lo[kInFlight];
+#pragma unroll
+        for (int q = 0; q < kInFlight; ++q) lo[q] = 0;
+        for (int len = da; len > 1;) {
+          const int half = len >> 1;
+#pragma unroll
+          for (int q = 0; q < kInFlight; ++q) lo[q] = (A[lo[q] + half] <= w[q]) ? lo[q] + half : lo[q];
+          len -= half;
+        }
+#pragma unroll
+        for (int q = 0; q < kInFlight; ++q) hits += (A[lo[q]] == w[q]) ? 1u : 0u;
       }
-#pragma unroll
-      for (int q = 0; q < kInFlight; ++q) hits += (a_at<kPadded>(A, lo[q]) == w[q]) ? 1u : 0u;
     }
     wave_sync();
   }
   return hits;
 }
 
-// Copies N+(u) into a padded LDS slot with `n` cooperating lanes.
+// Lays N+(u) out as an LDS tree with `n` cooperating lanes (coalesced reads).
 __device__ __forceinline__ void stage_pivot(int32_t* slot, const int32_t* __restrict__ src, int da,
                                             int lane, int n) {
-  for (int x = lane; x < da; x += n) slot[pad_idx(x)] = src[x];
+  const int h = tree_levels(da);
+  const int nodes = (1 << h) - 1;
+  for (int x = lane; x < nodes; x += n) slot[tree_pos(x, h)] = x < da ? src[x] : INT_MAX;
+  if (lane == 0) slot[0] = -1;
 }
 
 __device__ __forceinline__ uint64_t shfl_down_u64(uint64_t x, int d) {
@@ -189,8 +230,8 @@ __device__ __forceinline__ int find_graph(const GraphDesc* descs, int ngraphs, i
 __device__ __forceinline__ int choose_bin(const int64_t* rp, const int32_t* ci, int64_t rb,
                                           int64_t da, const Tune& t) {
   if (da < 2) return kNoBin;
-  if (da > kCapBlock) return BIN_GLOBAL;
-  if (da > kCapWave) return BIN_BLOCK;
+  if (da > kCapBlock - 1) return BIN_GLOBAL;
+  if (da > kCapWave - 1) return BIN_BLOCK;
   int64_t work = 0;
   for (int64_t e = rb; e < rb + da; ++e) {
     const int v = ci[e];
@@ -198,9 +239,9 @@ __device__ __forceinline__ int choose_bin(const int64_t* rp, const int32_t* ci, 
     if (work > t.t_wave) return BIN_BLOCK;
   }
   if (work == 0) return kNoBin;
-  if (da > kCapG32 || work > t.t_g32) return BIN_WAVE;
-  if (da > kCapG16 || work > t.t_g16) return BIN_G32;
-  if (da > kCapG8 || work > t.t_g8) return BIN_G16;
+  if (da > kCapG32 - 1 || work > t.t_g32) return BIN_WAVE;
+  if (da > kCapG16 - 1 || work > t.t_g16) return BIN_G32;
+  if (da > kCapG8 - 1 || work > t.t_g8) return BIN_G16;
   return BIN_G8;
 }
 
@@ -300,17 +341,17 @@ __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
 // persistent count
 // ---------------------------------------------------------------------------
 struct __align__(16) CountLds {
-  int32_t A[padded(kCapBlock)];  // pivot-list slots (layout depends on the tile's bin)
+  int32_t A[kCapBlock];          // pivot-tree slots (layout depends on the tile's bin)
   int32_t End[kBlock];           // per-lane edge end in the batch's flat index
   int64_t Base[kBlock];          // per-lane colidx base of that edge
   uint64_t red[kWaves];
   int64_t tab_cnt[1];
   int64_t cur_first, cur_seg;
 };
-static_assert(padded(kCapWave) * kWaves <= padded(kCapBlock), "wave slots exceed LDS region");
-static_assert(padded(kCapG32) * (kBlock / 32) <= padded(kCapBlock), "g32 slots exceed LDS region");
-static_assert(padded(kCapG16) * (kBlock / 16) <= padded(kCapBlock), "g16 slots exceed LDS region");
-static_assert(padded(kCapG8) * (kBlock / 8) <= padded(kCapBlock), "g8 slots exceed LDS region");
+static_assert(kCapWave * kWaves <= kCapBlock, "wave slots exceed LDS region");
+static_assert(kCapG32 * (kBlock / 32) <= kCapBlock, "g32 slots exceed LDS region");
+static_assert(kCapG16 * (kBlock / 16) <= kCapBlock, "g16 slots exceed LDS region");
+static_assert(kCapG8 * (kBlock / 8) <= kCapBlock, "g8 slots exceed LDS region");
 
 // Per-lane running count; in batch mode it is flushed to outs[g] whenever
 // the lane moves to a pivot of another graph.
@@ -331,7 +372,7 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
                                            int64_t last, CountLds& L, bool batch, Acc& acc) {
   constexpr int NGRP = kBlock / G;
   const int grp = threadIdx.x / G, gl = threadIdx.x % G;
-  int32_t* slot = L.A + grp * padded(CAP);
+  int32_t* slot = L.A + grp * CAP;
   int32_t* sEnd = L.End + grp * G;
   int64_t* sBase = L.Base + grp * G;
   for (int64_t i = first + grp; i < last; i += NGRP) {
@@ -450,7 +491,7 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
 // plain kernel: one wave64 per pivot, grid-stride (bs_warp_vertex.cuh:2-35)
 // ---------------------------------------------------------------------------
 struct __align__(16) PlainLds {
-  int32_t A[kWaves][padded(kCapWave)];
+  int32_t A[kWaves][kCapWave];
   int32_t End[kBlock];
   int64_t Base[kBlock];
   uint64_t red[kWaves];
@@ -468,7 +509,7 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(co
     const int64_t rb = rp[u];
     const int da = (int)(rp[u + 1] - rb);
     if (da < 2) continue;
-    if (da <= kCapWave) {
+    if (da <= kCapWave - 1) {
       stage_pivot(L.A[wid], ci + rb, da, lane, 64);
       wave_sync();
       acc += pivot_edges<64, true>(rp, ci, L.A[wid], da, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);

@@ -85,7 +85,7 @@ __device__ __forceinline__ int list_at(const int32_t* A, int j, int h) {
 }
 
 #ifndef TC_IN_FLIGHT
-#define TC_IN_FLIGHT 4
+#define TC_IN_FLIGHT 6
 #endif
 constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
 

@@ -260,6 +260,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-rmat", action="store_true", help="skip the headline RMAT leg")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
+    ap.add_argument("--tune", default="", help="launch parameters, e.g. t_wave=8192,stage_ratio16=24")
     ap.add_argument("--requests", type=int, default=10000, help="requests per serving run")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
@@ -281,6 +282,9 @@ def main() -> int:
         return 0
     ctx = Context(0)
     log(f"device: {ctx.name}")
+    if args.tune:
+        ctx.tune = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))}
+    log(f"tune: {ctx.tune}")
     line = {"metric": "TC edges/s, %HBM peak; batched graphs/s + p99 vs one-launch-per-graph",
             "value": None, "extra": {}}
     if not args.no_rmat:

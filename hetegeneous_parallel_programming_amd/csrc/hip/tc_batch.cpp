@@ -60,7 +60,7 @@ static tc_status capture(tc_ctx* ctx, tc_batch* b) {
   hipError_t e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s);
   if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)b->n, s);
   if (e == hipSuccess) e = tck::launch_plan(a, s);
-  if (e == hipSuccess) e = tck::launch_count(a, ctx->grid(), s);
+  if (e == hipSuccess) e = tck::launch_count(a, ctx->cus, ctx->tune.grid, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->pinned, a.bufs.outs, sizeof(uint64_t) * (size_t)b->n, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)

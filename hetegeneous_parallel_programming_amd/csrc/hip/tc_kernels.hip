@@ -9,12 +9,12 @@
 // j of the batch reads rowptr[v], rowptr[v+1] for its edge, a group scan of
 // the d+(v) gives each edge's slice of a flat index space, and the group's
 // lanes then stride that space with coalesced int32 reads of the N+(v)
-// lists.  Each read neighbour w is binary-searched in the LDS copy of N+(u);
-// the shorter list of the pair (the streamed N+(v) when d+(v) <= d+(u)) is
-// the one whose elements are looked up, as in the reference's warp_intersect
-// (successful_push_pull/graph_gpu.h:228-252), and every element of N+(v) is
-// read exactly once per edge.  Four lookups per lane are kept in flight so
-// that the HBM/L2 reads and the LDS search chains overlap.
+// lists.  Each read neighbour w is binary-searched in the LDS copy of N+(u)
+// (the search of the reference's warp_intersect, successful_push_pull/
+// graph_gpu.h:228-252, with the pivot list as the searched side), and every
+// element of N+(v) is read exactly once per edge.  Several lookups per lane
+// (TC_IN_FLIGHT) are kept in flight so that the HBM/L2 reads and the LDS
+// search chains overlap.
 //
 // Scheduling.  A planning pass bins pivots by d+(u) and by the wedge work
 // T(u) = sum of d+(v) over v in N+(u); light pivots go to 8/16/32-lane
@@ -63,10 +63,10 @@ __device__ __forceinline__ int find_edge(const int32_t* sEnd, int t) {
 // ("Eytzinger") order: node k has children 2k and 2k+1, node 1 is the root,
 // word 0 holds -1 (never an id) and leaves past the list hold INT_MAX.  An
 // in-order walk gives the sorted list, so a search is still a binary search,
-// but one descent step is just k = 2k + (slot[k] < w) -- compare and
-// add-with-carry, no select, no index arithmetic -- and the upper levels are
-// consecutive words, read without bank conflicts.  A tree of h levels holds
-// up to 2^h - 1 ids in a slot of 2^h words.
+// but one descent step is just k = 2k + (slot[k] < w) -- no bounds, no
+// midpoint arithmetic -- and the upper levels are consecutive words, read
+// without bank conflicts.  A tree of h levels holds up to 2^h - 1 ids in a
+// slot of 2^h words.
 
 // Levels of the smallest complete tree holding n >= 1 ids.
 __device__ __forceinline__ int tree_levels(int n) { return 32 - __builtin_clz((unsigned)n); }
@@ -555,13 +555,29 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_count(const LaunchArgs& a, int grid, hipStream_t s) {
-  count_kernel<BIN_GLOBAL><<<grid, kBlock, 0, s>>>(a);
-  count_kernel<BIN_BLOCK><<<grid, kBlock, 0, s>>>(a);
-  count_kernel<BIN_WAVE><<<grid, kBlock, 0, s>>>(a);
-  count_kernel<BIN_G32><<<grid, kBlock, 0, s>>>(a);
-  count_kernel<BIN_G16><<<grid, kBlock, 0, s>>>(a);
-  count_kernel<BIN_G8><<<grid, kBlock, 0, s>>>(a);
+template <int BIN>
+static int bin_blocks_per_cu() {
+  static const int n = [] {
+    int k = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, count_kernel<BIN>, kBlock, 0) != hipSuccess) k = 1;
+    return k > 0 ? k : 1;
+  }();
+  return n;
+}
+
+template <int BIN>
+static void launch_bin(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
+  const int g = grid > 0 ? grid : bin_blocks_per_cu<BIN>() * cus;
+  count_kernel<BIN><<<g, kBlock, 0, s>>>(a);
+}
+
+hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
+  launch_bin<BIN_GLOBAL>(a, cus, grid, s);
+  launch_bin<BIN_BLOCK>(a, cus, grid, s);
+  launch_bin<BIN_WAVE>(a, cus, grid, s);
+  launch_bin<BIN_G32>(a, cus, grid, s);
+  launch_bin<BIN_G16>(a, cus, grid, s);
+  launch_bin<BIN_G8>(a, cus, grid, s);
   return hipGetLastError();
 }
 

@@ -73,7 +73,8 @@ struct LaunchArgs {
 // Launchers (stream-ordered; return the launch status).
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);
 hipError_t launch_plan(const LaunchArgs& a, hipStream_t s);
-hipError_t launch_count(const LaunchArgs& a, int grid, hipStream_t s);
+// One full-residency persistent launch per bin (grid > 0 overrides the size).
+hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s);
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
                         uint64_t* out, int grid, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);

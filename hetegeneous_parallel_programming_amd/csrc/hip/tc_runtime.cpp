@@ -308,7 +308,7 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  TC_HIP_TRY(tck::launch_count(a, ctx->grid(), s));
+  TC_HIP_TRY(tck::launch_count(a, ctx->cus, ctx->tune.grid, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,

@@ -203,7 +203,7 @@ bool tc_server::launch() {
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s.stream);
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)k, s.stream);
     if (e == hipSuccess) e = tck::launch_plan(a, s.stream);
-    if (e == hipSuccess) e = tck::launch_count(a, ctx->grid(), s.stream);
+    if (e == hipSuccess) e = tck::launch_count(a, ctx->cus, ctx->tune.grid, s.stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s.pinned, a.bufs.outs, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);

@@ -20,6 +20,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -296,13 +297,17 @@ def main() -> int:
 
 
 def traffic_of(workload: str):
-    """Measured HBM bytes per count-kernel launch from the committed rocprofv3
-    summary (profiles/traffic_<workload>.json), when there is one."""
-    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    """Measured HBM bytes per step of the six count kernels, from the committed
+    rocprofv3 summary profiles/count_kernel_<workload>.md (its line
+    `traffic_bytes_per_step = N`: FETCH_SIZE and WRITE_SIZE scaled by the
+    copy-kernel calibration), when there is one."""
+    p = os.path.join(ROOT, "profiles", f"count_kernel_{workload}.md")
     if not os.path.exists(p):
         return None, None
-    t = json.load(open(p))
-    return t.get("bytes_per_step"), os.path.relpath(p, ROOT)
+    m = re.search(r"^traffic_bytes_per_step = (\d+)\s*$", open(p).read(), re.M)
+    if not m:
+        return None, None
+    return int(m.group(1)), os.path.relpath(p, ROOT)
 
 
 def leg_headline(ctx, args) -> dict:

@@ -122,35 +122,70 @@ def golden_big() -> dict:
     return json.load(open(p)) if os.path.exists(p) else {}
 
 
-def leg_batch(ctx, graphs, seed, reps):
+def leg_batch(ctx, graphs, seed, reps, tuner=None):
+    """4096-graph mix: one plain launch per graph (baseline) vs one captured
+    persistent launch, first with the default launch setting, then with the
+    autotuned one (tuned on a different batch of the same mix)."""
     n_graphs = len(graphs)
     edges = sum(g.E() for g in graphs)
     b = ctx.batch(graphs)
     each, ms_each, st_each = b.count_each()
     each2, ms_each2, st_each2 = b.count_each()
-    got, _ = b.count()  # capture + first replay
-    ms_total, outs = time_steps(lambda: b.count(), reps, 1)
-    for o in outs:
-        if not np.array_equal(o[0], each):
-            raise RuntimeError("batched counts differ from per-graph counts")
     base_s = st_each2["total_ms"] / 1e3
-    batched_s = ms_total / reps / 1e3
-    kms = [o[1]["count_ms"] for o in outs]
+
+    def timed():
+        b.count()  # capture + first replay
+        ms_total, outs = time_steps(lambda: b.count(), reps, 1)
+        for o in outs:
+            if not np.array_equal(o[0], each):
+                raise RuntimeError("batched counts differ from per-graph counts")
+        return ms_total / reps / 1e3, [o[1]["count_ms"] for o in outs]
+
+    default_s, kms_default = timed()
     res = {"graphs": n_graphs, "edges": edges, "sum_count": int(each.sum()),
            "baseline_graphs_per_s": n_graphs / base_s,
            "baseline_p50_ms": float(np.percentile(ms_each2, 50)),
            "baseline_p99_ms": float(np.percentile(ms_each2, 99)),
-           "batched_graphs_per_s": n_graphs / batched_s,
-           "batched_ms": batched_s * 1e3, "batched_p99_ms": float(np.percentile(kms, 99)),
-           "speedup": base_s / batched_s}
+           "default_tune": ctx.tune, "default_graphs_per_s": n_graphs / default_s,
+           "default_speedup": base_s / default_s}
+    batched_s, kms = default_s, kms_default
+    if tuner is not None:
+        tune, report = tuner(ctx)
+        res["tune"], res["autotune"] = tune, report
+        batched_s, kms = timed()
+    res.update({"batched_graphs_per_s": n_graphs / batched_s, "batched_ms": batched_s * 1e3,
+                "batched_p99_ms": float(np.percentile(kms, 99)), "speedup": base_s / batched_s})
     gold = golden_big().get(f"batch_seed{seed}_{n_graphs}")
     res["exact_vs_cpu_oracle"] = (bool(np.array_equal(each, np.array(gold["triangles"], np.uint64)))
                                   if gold else None)
-    log(f"batch: baseline {res['baseline_graphs_per_s']:.0f} graphs/s, batched "
-        f"{res['batched_graphs_per_s']:.0f} graphs/s, speedup {res['speedup']:.2f}x, "
-        f"exact vs oracle: {res['exact_vs_cpu_oracle']}")
+    log(f"batch: baseline {res['baseline_graphs_per_s']:.0f} graphs/s, batched default "
+        f"{res['default_graphs_per_s']:.0f} graphs/s, batched tuned {res['batched_graphs_per_s']:.0f} "
+        f"graphs/s, speedup {res['speedup']:.2f}x, exact vs oracle: {res['exact_vs_cpu_oracle']}")
     b.free()
     return res, each
+
+
+def batch_tuner(args):
+    """Autotune on a 1024-graph batch of the same mix drawn with another seed
+    (or load the setting saved for this device from --tune-file)."""
+    def tune(ctx):
+        from hetegeneous_parallel_programming_amd import Graph
+
+        key = "batch_mix"
+        if args.tune_file and os.path.exists(args.tune_file) and ctx.load_tune(args.tune_file, key):
+            log(f"autotune: loaded {ctx.tune} from {args.tune_file}")
+            return ctx.tune, {"source": args.tune_file}
+        tg = Graph.batch(args.seed + 1000, 0, 1024)
+        tb = ctx.batch(tg)
+        t, rep = ctx.autotune(batch=tb, reps=3, budget_ms=args.tune_seconds * 1e3)
+        tb.free()
+        rep["tuned_on"] = f"1024 graphs of the same mix, seed {args.seed + 1000}"
+        log(f"autotune: {t} after {rep['trials']} settings in {rep['elapsed_ms'] / 1e3:.1f} s "
+            f"({rep['start_ms']:.2f} -> {rep['best_ms']:.2f} ms on the tuning batch)")
+        if args.tune_file:
+            ctx.save_tune(args.tune_file, key)
+        return t, rep
+    return tune
 
 
 def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed):
@@ -263,6 +298,9 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--tune", default="", help="launch parameters, e.g. t_wave=8192,stage_ratio16=24")
     ap.add_argument("--requests", type=int, default=10000, help="requests per serving run")
+    ap.add_argument("--no-autotune", action="store_true", help="time the batch with the default setting only")
+    ap.add_argument("--tune-seconds", type=float, default=12.0, help="autotuner wall-time budget")
+    ap.add_argument("--tune-file", default="", help="load the batch setting from / save it to this file")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -356,7 +394,9 @@ def run_side_legs(ctx, args, line) -> None:
         t0 = time.perf_counter()
         graphs = Graph.batch(args.seed, 0, args.batch_graphs)
         log(f"batch: generated {len(graphs)} graphs in {time.perf_counter() - t0:.1f} s")
-        res, counts = leg_batch(ctx, graphs, args.seed, max(3, args.steps // 4))
+        default = ctx.tune
+        res, counts = leg_batch(ctx, graphs, args.seed, max(3, args.steps // 4),
+                                None if args.no_autotune else batch_tuner(args))
         if not args.no_batch:
             line["extra"]["batched"] = res
         if not args.no_serving:
@@ -371,6 +411,7 @@ def run_side_legs(ctx, args, line) -> None:
             line["extra"]["serving"]["rmat20_exact_vs_cpu_oracle"] = (
                 gold20["triangles"] == heavy_count if gold20 else None)
             del heavy
+        ctx.tune = default
         del graphs
     if not args.no_partition:
         line["extra"]["partition_rmat24_p0"] = leg_partition(ctx, args.seed, max(3, args.steps // 4),

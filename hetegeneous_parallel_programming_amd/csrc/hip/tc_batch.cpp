@@ -266,5 +266,6 @@ tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b) {
 }
 
 int64_t tc_batch_bytes(const tc_batch* b) { return b ? b->bytes : 0; }
+int32_t tc_batch_graphs(const tc_batch* b) { return b ? b->n : 0; }
 
 }  // extern "C"

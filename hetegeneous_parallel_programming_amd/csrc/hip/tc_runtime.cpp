@@ -193,7 +193,8 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
   hipDeviceProp_t prop;
   TC_HIP_TRY(hipGetDeviceProperties(&prop, device));
   c->cus = prop.multiProcessorCount;
-  c->name = std::string(prop.name) + " (" + prop.gcnArchName + ", " + std::to_string(c->cus) + " CUs)";
+  const std::string marketing = prop.name[0] ? std::string(prop.name) : std::string("AMD GPU");
+  c->name = marketing + " (" + prop.gcnArchName + ", " + std::to_string(c->cus) + " CUs)";
   TC_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
   TC_HIP_TRY(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));

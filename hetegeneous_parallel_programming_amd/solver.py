@@ -47,6 +47,14 @@ class Tune(ctypes.Structure):
         return t
 
 
+class AutotuneReport(ctypes.Structure):
+    _fields_ = [("trials", nat.c_i32), ("start_ms", nat.c_dbl), ("best_ms", nat.c_dbl),
+                ("elapsed_ms", nat.c_dbl)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", nat.c_u64), ("slots", nat.c_i32), ("max_batch_graphs", nat.c_i32),
                 ("max_batch_rows", nat.c_i64), ("max_queue", nat.c_i32), ("lookahead", nat.c_i32),
@@ -99,6 +107,11 @@ def device_lib():
                                             ctypes.POINTER(Stats)])
     S(L, "tc_batch_free", nat.c_i32, [P, P])
     S(L, "tc_batch_bytes", nat.c_i64, [P])
+    S(L, "tc_batch_graphs", nat.c_i32, [P])
+    S(L, "tc_autotune", nat.c_i32, [P, P, P, nat.c_i32, nat.c_dbl, ctypes.POINTER(Tune),
+                                    ctypes.POINTER(AutotuneReport)])
+    S(L, "tc_tune_save", nat.c_i32, [P, ctypes.c_char_p, ctypes.c_char_p])
+    S(L, "tc_tune_load", nat.c_i32, [P, ctypes.c_char_p, ctypes.c_char_p])
     S(L, "tc_probe_copy", nat.c_i32, [P, nat.c_i64, nat.c_i32, ctypes.POINTER(nat.c_dbl)])
     S(L, "tc_clock_ms", nat.c_dbl, [])
     S(L, "tc_request_footprint", nat.c_u64, [nat.c_i32, nat.c_i64])
@@ -253,6 +266,31 @@ class Context:
         h = ctypes.c_void_p()
         _check(device_lib().tc_batch_create(self._h, views, len(graphs), check, ctypes.byref(h)))
         return Batch(self, h.value, len(graphs))
+
+    def autotune(self, graph: DeviceGraph | None = None, batch: Batch | None = None,
+                 reps: int = 3, budget_ms: float = 10000.0):
+        """Pick the launch setting (bin thresholds = group widths, tile
+        pivots, grid) by timing exactly one workload; installs it on this
+        context and returns (tune dict, report dict)."""
+        if (graph is None) == (batch is None):
+            raise ValueError("give exactly one of graph= or batch=")
+        t, rep = Tune(), AutotuneReport()
+        _check(device_lib().tc_autotune(self._h, graph._h if graph is not None else None,
+                                        batch._h if batch is not None else None, reps,
+                                        float(budget_ms), ctypes.byref(t), ctypes.byref(rep)))
+        return t.as_dict(), rep.as_dict()
+
+    def save_tune(self, path: str, key: str) -> None:
+        """Write this context's tune under (device name, key) into a tune file."""
+        _check(device_lib().tc_tune_save(self._h, os.fsencode(path), key.encode()))
+
+    def load_tune(self, path: str, key: str) -> bool:
+        """Install the tune saved under (device name, key); False if none."""
+        st = device_lib().tc_tune_load(self._h, os.fsencode(path), key.encode())
+        if st == nat.TC_EIO:
+            return False
+        _check(st)
+        return True
 
     def probe_copy(self, nbytes: int, iters: int = 10) -> float:
         """Mean ms of a plain device copy of nbytes (counter calibration)."""

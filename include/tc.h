@@ -30,7 +30,10 @@ typedef struct tc_graph tc_graph;
 typedef struct tc_batch tc_batch;
 typedef struct tc_server tc_server;
 
-/* Launch parameters of the degree-binned count (picked by tc_autotune). */
+/* Launch parameters of the degree-binned count (defaults, or picked by
+ * tc_autotune).  The thresholds choose each pivot's group width by its wedge
+ * work T(u) = sum of d+(v) over v in N+(u): at most t_g8 -> 8 lanes, ...,
+ * above t_wave -> a whole workgroup; 0 for a threshold empties its bin. */
 typedef struct tc_tune {
   int32_t t_g8;    /* max wedge elements of a pivot handled by an 8-lane group      */
   int32_t t_g16;   /* ... by a 16-lane group                                        */
@@ -107,6 +110,32 @@ tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double
                               tc_stats* stats);
 tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b);
 int64_t tc_batch_bytes(const tc_batch* b);
+int32_t tc_batch_graphs(const tc_batch* b);
+
+/* ---- autotuning ----------------------------------------------------------
+ * No counterpart in the reference, whose launch shape is fixed (one warp per
+ * vertex, maximum-residency grid: multigpu_nvshmem_pushpull.cu:41-51).
+ * Times the context's count on exactly one workload -- a resident graph `g`
+ * (tc_count) or a batch `b` (tc_batch_count, re-captured per setting) --
+ * under candidate tc_tune settings by coordinate descent over each field
+ * (minimum of `reps` timed calls after one untimed call per setting), stops
+ * after `budget_ms` of wall time, installs the fastest setting on the context
+ * and writes it to *best.  Every setting's counts must equal the first
+ * setting's (TC_EDEVICE otherwise).  best and report may be NULL. */
+typedef struct tc_autotune_report {
+  int32_t trials;       /* settings timed                                  */
+  double start_ms;      /* time of the starting setting                    */
+  double best_ms;       /* time of the installed setting                   */
+  double elapsed_ms;    /* wall time of the whole search                   */
+} tc_autotune_report;
+tc_status tc_autotune(tc_ctx* ctx, tc_graph* g, tc_batch* b, int32_t reps, double budget_ms,
+                      tc_tune* best, tc_autotune_report* report);
+/* Tune file: one line "<device>|<key> t_g8 t_g16 t_g32 t_wave rounds grid"
+ * per (device, workload key); <device> is tc_device_name with blanks and '|'
+ * replaced by '_'.  Save replaces the context's record for `key`
+ * (keeps other records); load installs it (TC_EIO if absent or malformed). */
+tc_status tc_tune_save(tc_ctx* ctx, const char* path, const char* key);
+tc_status tc_tune_load(tc_ctx* ctx, const char* path, const char* key);
 
 /* ---- serving loop --------------------------------------------------------
  * Continuous batching of TC requests.  No counterpart in the reference,

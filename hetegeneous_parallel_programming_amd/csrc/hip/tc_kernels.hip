@@ -78,49 +78,175 @@ __device__ __forceinline__ int tree_pos(int x, int h) {
   return (1 << (h - 1 - tz)) + (i1 >> (tz + 1));
 }
 
-// j-th smallest id of a pivot list: from the LDS tree, or from the sorted HBM copy.
-template <bool kTree>
-__device__ __forceinline__ int list_at(const int32_t* A, int j, int h) {
-  return kTree ? A[tree_pos(j, h)] : A[j];
-}
-
 #ifndef TC_IN_FLIGHT
 #define TC_IN_FLIGHT 6
 #endif
 constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
 
+// ---------------------------------------------------------------------------
+// Pivot sets: how a group holds N+(u) for membership tests
+// ---------------------------------------------------------------------------
+// kSetHash: a two-choice cuckoo hash in LDS.  Two tables of B buckets, two
+//   slots (8 bytes) per bucket; id w may sit only in bucket h0(w) of table 0
+//   or bucket h1(w) of table 1, so a lookup is two 8-byte LDS reads and four
+//   compares -- a fixed cost, independent of d+(u), with no data-dependent
+//   branch.  h0/h1 are the top log2(B) bits of w * A0 / w * A1 modulo 2^32
+//   with 24-bit odd multipliers (one full-rate v_mul_u32_u24 each).  Empty
+//   slots hold -1, which is never an id.
+// kSetTree: the Eytzinger tree above, used when the hash does not fit the
+//   group's LDS area or its construction did not converge.
+// kSetHbm: the sorted list in HBM, binary-searched (pivots whose list does
+//   not fit a workgroup's LDS).
+enum SetMode : int { kSetHash = 0, kSetTree = 1, kSetHbm = 2 };
+constexpr int kModeHash = 1 << kSetHash, kModeTree = 1 << kSetTree, kModeHbm = 1 << kSetHbm;
+
+struct PivotSet {
+  const int32_t* A;  // LDS slot area (hash tables or tree) or the HBM list
+  int mode;
+  int p0;            // hash: 32 - log2(B); tree: levels; hbm: list length
+  int p1;            // hash: byte offset of table 1 (8B)
+};
+
+constexpr int32_t kEmpty = -1;
+constexpr uint32_t kHashA0 = 0x9E3779u;  // odd 24-bit multipliers
+constexpr uint32_t kHashA1 = 0x7F4A7Du;
+constexpr int kMaxKicks = 128;           // evictions before giving up on a key
+
+__device__ __forceinline__ uint32_t hash_bucket(int w, uint32_t mul, int sh) {
+  return (uint32_t)__umul24((uint32_t)w, mul) >> sh;
+}
+
+// log2 of the buckets per table for a list of da ids in an LDS area of
+// `area` words (4B words for both tables): load <= 1/2 when the area allows,
+// at most 4/5; 0 when the list needs more than that.
+__device__ __forceinline__ int hash_log2_buckets(int da, int area) {
+  const int want = (da + 1) >> 1;  // B >= da/2
+  int lb = want <= 2 ? 1 : 32 - __builtin_clz((unsigned)(want - 1));
+  const int maxlb = 31 - __builtin_clz((unsigned)(area >> 2));
+  lb = lb < maxlb ? lb : maxlb;
+  return 5 * da <= (16 << lb) ? lb : 0;
+}
+
+// Inserts `key` into the two tables (B = 1 << lb buckets each, table 1 at
+// word 2B).  LDS atomics make concurrent inserts by the lanes of a group
+// safe: every id is always either in exactly one slot or carried by exactly
+// one lane.  Returns false when an evicted id could not be placed within
+// kMaxKicks moves (the set is then incomplete and must not be used).
+__device__ __forceinline__ bool hash_insert(int32_t* T, int lb, int key) {
+  const int sh = 32 - lb;
+  const int B = 1 << lb;
+  int t = 0;
+  for (int it = 0; it < kMaxKicks; ++it) {
+    int32_t* b = T + (t ? 2 * B : 0) + 2 * (int)hash_bucket(key, t ? kHashA1 : kHashA0, sh);
+    if (atomicCAS(&b[0], kEmpty, key) == kEmpty) return true;
+    if (atomicCAS(&b[1], kEmpty, key) == kEmpty) return true;
+    if (it == 0) {  // the other table's bucket before evicting anyone
+      int32_t* c = T + 2 * B + 2 * (int)hash_bucket(key, kHashA1, sh);
+      if (atomicCAS(&c[0], kEmpty, key) == kEmpty) return true;
+      if (atomicCAS(&c[1], kEmpty, key) == kEmpty) return true;
+    }
+    key = atomicExch(&b[(it ^ key) & 1], key);  // evicted id moves to its other table
+    if (key == kEmpty) return true;
+    t ^= 1;
+  }
+  return false;
+}
+
+// Lays N+(u) out as an LDS tree with `n` cooperating lanes (coalesced reads).
+__device__ __forceinline__ void stage_tree(int32_t* slot, const int32_t* __restrict__ src, int da,
+                                           int lane, int n) {
+  const int h = tree_levels(da);
+  const int nodes = (1 << h) - 1;
+  for (int x = lane; x < nodes; x += n) slot[tree_pos(x, h)] = x < da ? src[x] : INT_MAX;
+  if (lane == 0) slot[0] = -1;
+}
+
+// True on every lane of the group when `ok` holds on all of its G lanes
+// (G lanes of one wave, or the whole workgroup for G == kBlock).  Orders the
+// group's preceding LDS operations before its following ones.
+template <int G>
+__device__ __forceinline__ bool group_all(bool ok) {
+  if constexpr (G == kBlock) {
+    return __syncthreads_and(ok) != 0;
+  } else {
+    wave_sync();
+    const uint64_t bad = __ballot(!ok);
+    if constexpr (G == 64) {
+      return bad == 0;
+    } else {
+      const int first = (int)(threadIdx.x & 63) & ~(G - 1);
+      return ((bad >> first) & ((1ull << G) - 1)) == 0;
+    }
+  }
+}
+
+template <int G>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (G == kBlock) __syncthreads();
+  else wave_sync();
+}
+
+// Builds the set of N+(u) (src[0..da), sorted) in `slot` (area words) with
+// the G lanes of a group (gl = lane index in the group): a cuckoo hash when
+// it fits and converges, else the tree (da must then fit the tree cap).
+template <int G>
+__device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
+                                                    int da, int gl) {
+  PivotSet S{slot, kSetTree, 0, 0};
+  const int lb = hash_log2_buckets(da, area);
+  if (lb > 0) {
+    const int B = 1 << lb;
+    for (int x = gl; x < 4 * B; x += G) slot[x] = kEmpty;
+    group_sync<G>();
+    bool ok = true;
+    for (int j = gl; j < da; j += G) ok = hash_insert(slot, lb, src[j]) && ok;
+    if (group_all<G>(ok)) {
+      S.mode = kSetHash;
+      S.p0 = 32 - lb;
+      S.p1 = 8 * B;
+      return S;
+    }
+  }
+  stage_tree(slot, src, da, gl, G);
+  group_sync<G>();
+  S.p0 = tree_levels(da);
+  return S;
+}
+
 // Counts, for the edges j = jfirst, jfirst + jstride, ... of pivot u (list
-// of da ids: an LDS tree, or the sorted list in HBM), the elements of N+(v)
-// found in that list.  sEnd/sBase are this group's G-entry LDS scratch.
-// Returns this lane's share of the hits.
+// elist[0..da) in HBM, set S), the elements of N+(v) found in S.
+// sEnd/sBase are this group's G-entry LDS scratch.  kModes: the set modes
+// this instantiation handles.  Returns this lane's share of the hits.
 //
 // Flat index t of a batch maps to edge k (sEnd[k-1] <= t < sEnd[k]) and to
-// colidx[sBase[k] + t]; each lane keeps its current edge's end and base in
+// element t of the int32 array at sBase[k] (N+(v) shifted back by the edge's
+// first flat index); each lane keeps its current edge's end and base in
 // registers and searches sEnd only when it steps past that edge.
-template <int G, bool kTree>
+template <int G, int kModes>
 __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
-                                                const int32_t* __restrict__ ci, const int32_t* A,
-                                                int da, int jfirst, int jstride, int32_t* sEnd,
-                                                int64_t* sBase, int gl) {
+                                                const int32_t* __restrict__ ci,
+                                                const int32_t* __restrict__ elist, int da,
+                                                const PivotSet S, int jfirst, int jstride,
+                                                int32_t* sEnd, const int32_t** sBase, int gl) {
   uint32_t hits = 0;
-  const int h = tree_levels(da);
+  const int32_t* A = S.A;
   for (int j0 = jfirst; j0 < da; j0 += jstride) {
     const int j = j0 + gl;
     int c = 0;
-    int64_t off = 0;
+    const int32_t* base = ci;
     if (j < da) {
-      const int v = list_at<kTree>(A, j, h);
+      const int v = elist[j];
       const int64_t b = rp[v];
       c = (int)(rp[v + 1] - b);
-      off = b;
+      base = ci + b;
     }
     int total;
     const int excl = group_excl_scan<G>(c, gl, total);
     sEnd[gl] = excl + c;
-    sBase[gl] = off - excl;
+    sBase[gl] = base - excl;
     wave_sync();
-    int cur_end = 0;        // t < cur_end  <=>  t lies in the lane's current edge
-    int64_t cur_base = 0;
+    int cur_end = 0;  // t < cur_end  <=>  t lies in the lane's current edge
+    const int32_t* cur_base = ci;
     for (int t0 = 0; t0 < total; t0 += G * kInFlight) {
       int w[kInFlight];
 #pragma unroll
@@ -133,14 +259,24 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
             cur_end = sEnd[k];
             cur_base = sBase[k];
           }
-          w[q] = ci[cur_base + t];
+          w[q] = cur_base[t];
         }
       }
-      if (kTree) {
+      if ((kModes & kModeHash) && S.mode == kSetHash) {
+        const char* Tb = (const char*)A;
+        const int sh = S.p0;
+#pragma unroll
+        for (int q = 0; q < kInFlight; ++q) {
+          const int2 x = *(const int2*)(Tb + (hash_bucket(w[q], kHashA0, sh) << 3));
+          const int2 y = *(const int2*)(Tb + S.p1 + (hash_bucket(w[q], kHashA1, sh) << 3));
+          hits += ((x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q])) ? 1u : 0u;
+        }
+      } else if ((kModes & kModeTree) && ((kModes == kModeTree) || S.mode == kSetTree)) {
         // h descent steps; then the lower bound is k with its trailing one
         // bits and the following zero shifted out (0 = past the end -> -1).
         // The node index is carried as a byte offset (4k): one compare, one
-        // select and one shift-add per step.
+        // select and one shift-add per step.  ~k is never 0 (k < 2^h).
+        const int h = S.p0;
         const char* Ab = (const char*)A;
         int kb[kInFlight];
 #pragma unroll
@@ -154,14 +290,15 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
         }
 #pragma unroll
         for (int q = 0; q < kInFlight; ++q) {
-          const int k = kb[q] >> 2;
-          hits += (A[k >> __builtin_ffs(~k)] == w[q]) ? 1u : 0u;
+          const unsigned k = (unsigned)kb[q] >> 2;
+          const unsigned lo = (k >> __builtin_ctz(~k)) >> 1;
+          hits += (A[lo] == w[q]) ? 1u : 0u;
         }
-      } else {
+      } else if (kModes & kModeHbm) {
         int lo[kInFlight];
 #pragma unroll
         for (int q = 0; q < kInFlight; ++q) lo[q] = 0;
-        for (int len = da; len > 1;) {
+        for (int len = S.p0; len > 1;) {
           const int half = len >> 1;
 #pragma unroll
           for (int q = 0; q < kInFlight; ++q) lo[q] = (A[lo[q] + half] <= w[q]) ? lo[q] + half : lo[q];
@@ -175,16 +312,6 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
   }
   return hits;
 }
-
-// Lays N+(u) out as an LDS tree with `n` cooperating lanes (coalesced reads).
-__device__ __forceinline__ void stage_pivot(int32_t* slot, const int32_t* __restrict__ src, int da,
-                                            int lane, int n) {
-  const int h = tree_levels(da);
-  const int nodes = (1 << h) - 1;
-  for (int x = lane; x < nodes; x += n) slot[tree_pos(x, h)] = x < da ? src[x] : INT_MAX;
-  if (lane == 0) slot[0] = -1;
-}
-
 __device__ __forceinline__ uint64_t shfl_down_u64(uint64_t x, int d) {
   const uint32_t lo = __shfl_down((uint32_t)x, d, 64);
   const uint32_t hi = __shfl_down((uint32_t)(x >> 32), d, 64);
@@ -341,17 +468,19 @@ __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
 // persistent count
 // ---------------------------------------------------------------------------
 struct __align__(16) CountLds {
-  int32_t A[kCapBlock];          // pivot-tree slots (layout depends on the tile's bin)
+  int32_t A[kAreaBlock];         // pivot-set areas (one per group of the tile's bin)
   int32_t End[kBlock];           // per-lane edge end in the batch's flat index
-  int64_t Base[kBlock];          // per-lane colidx base of that edge
+  const int32_t* Base[kBlock];   // per-lane N+(v) base of that edge, shifted by its first index
   uint64_t red[kWaves];
   int64_t tab_cnt[1];
   int64_t cur_first, cur_seg;
 };
-static_assert(kCapWave * kWaves <= kCapBlock, "wave slots exceed LDS region");
-static_assert(kCapG32 * (kBlock / 32) <= kCapBlock, "g32 slots exceed LDS region");
-static_assert(kCapG16 * (kBlock / 16) <= kCapBlock, "g16 slots exceed LDS region");
-static_assert(kCapG8 * (kBlock / 8) <= kCapBlock, "g8 slots exceed LDS region");
+static_assert(kAreaWave * kWaves <= kAreaBlock, "wave areas exceed LDS region");
+static_assert(kAreaG32 * (kBlock / 32) <= kAreaBlock, "g32 areas exceed LDS region");
+static_assert(kAreaG16 * (kBlock / 16) <= kAreaBlock, "g16 areas exceed LDS region");
+static_assert(kAreaG8 * (kBlock / 8) <= kAreaBlock, "g8 areas exceed LDS region");
+static_assert(kCapG8 <= kAreaG8 && kCapG16 <= kAreaG16 && kCapG32 <= kAreaG32 &&
+              kCapWave <= kAreaWave && kCapBlock <= kAreaBlock, "a bin's tree must fit its area");
 
 // Per-lane running count; in batch mode it is flushed to outs[g] whenever
 // the lane moves to a pivot of another graph.
@@ -367,14 +496,14 @@ struct Acc {
   }
 };
 
-template <int G, int CAP>
+template <int G, int AREA>
 __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* items, int64_t first,
                                            int64_t last, CountLds& L, bool batch, Acc& acc) {
   constexpr int NGRP = kBlock / G;
   const int grp = threadIdx.x / G, gl = threadIdx.x % G;
-  int32_t* slot = L.A + grp * CAP;
+  int32_t* slot = L.A + grp * AREA;
   int32_t* sEnd = L.End + grp * G;
-  int64_t* sBase = L.Base + grp * G;
+  const int32_t** sBase = L.Base + grp * G;
   for (int64_t i = first + grp; i < last; i += NGRP) {
     const uint64_t item = items[i];
     const int g = (int)(item >> 32);
@@ -384,9 +513,8 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
     const int32_t* ci = a.ci + d.ci_off;
     const int64_t rb = rp[u];
     const int da = (int)(rp[u + 1] - rb);
-    stage_pivot(slot, ci + rb, da, gl, G);
-    wave_sync();
-    const uint32_t h = pivot_edges<G, true>(rp, ci, slot, da, 0, G, sEnd, sBase, gl);
+    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + rb, da, gl);
+    const uint32_t h = pivot_edges<G, kModeHash | kModeTree>(rp, ci, ci + rb, da, S, 0, G, sEnd, sBase, gl);
     acc.to_graph(g, batch, a.bufs.outs);
     acc.sum += h;
     wave_sync();
@@ -406,12 +534,13 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t h;
   if (kLdsPivot) {
-    stage_pivot(L.A, ci + rb, da, threadIdx.x, kBlock);
-    __syncthreads();
-    h = pivot_edges<64, true>(rp, ci, L.A, da, wid * 64, kBlock, L.End + wid * 64, L.Base + wid * 64, lane);
+    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, ci + rb, da, threadIdx.x);
+    h = pivot_edges<64, kModeHash | kModeTree>(rp, ci, ci + rb, da, S, wid * 64, kBlock, L.End + wid * 64,
+                                               L.Base + wid * 64, lane);
   } else {
-    h = pivot_edges<64, false>(rp, ci, ci + rb, da, wid * 64, kBlock, L.End + wid * 64, L.Base + wid * 64,
-                               lane);
+    const PivotSet S{ci + rb, kSetHbm, da, 0};
+    h = pivot_edges<64, kModeHbm>(rp, ci, ci + rb, da, S, wid * 64, kBlock, L.End + wid * 64,
+                                  L.Base + wid * 64, lane);
   }
   acc.to_graph(g, batch, a.bufs.outs);
   acc.sum += h;
@@ -463,10 +592,10 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
     const int64_t last = min(first + (int64_t)piv, cnt);
     if (BIN == BIN_GLOBAL) block_pivot<false>(a, items[first], L, batch, acc);
     else if (BIN == BIN_BLOCK) block_pivot<true>(a, items[first], L, batch, acc);
-    else if (BIN == BIN_WAVE) group_tile<64, kCapWave>(a, items, first, last, L, batch, acc);
-    else if (BIN == BIN_G32) group_tile<32, kCapG32>(a, items, first, last, L, batch, acc);
-    else if (BIN == BIN_G16) group_tile<16, kCapG16>(a, items, first, last, L, batch, acc);
-    else group_tile<8, kCapG8>(a, items, first, last, L, batch, acc);
+    else if (BIN == BIN_WAVE) group_tile<64, kAreaWave>(a, items, first, last, L, batch, acc);
+    else if (BIN == BIN_G32) group_tile<32, kAreaG32>(a, items, first, last, L, batch, acc);
+    else if (BIN == BIN_G16) group_tile<16, kAreaG16>(a, items, first, last, L, batch, acc);
+    else group_tile<8, kAreaG8>(a, items, first, last, L, batch, acc);
     if (batch) {
       // Items are ordered by graph, so a tile almost always belongs to one
       // graph: then one block-reduced atomic; otherwise one per lane.
@@ -493,7 +622,7 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
 struct __align__(16) PlainLds {
   int32_t A[kWaves][kCapWave];
   int32_t End[kBlock];
-  int64_t Base[kBlock];
+  const int32_t* Base[kBlock];
   uint64_t red[kWaves];
 };
 
@@ -509,13 +638,16 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(co
     const int64_t rb = rp[u];
     const int da = (int)(rp[u + 1] - rb);
     if (da < 2) continue;
+    constexpr int kModes = kModeTree | kModeHbm;
     if (da <= kCapWave - 1) {
-      stage_pivot(L.A[wid], ci + rb, da, lane, 64);
+      stage_tree(L.A[wid], ci + rb, da, lane, 64);
       wave_sync();
-      acc += pivot_edges<64, true>(rp, ci, L.A[wid], da, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
+      const PivotSet S{L.A[wid], kSetTree, tree_levels(da), 0};
+      acc += pivot_edges<64, kModes>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
       wave_sync();
     } else {
-      acc += pivot_edges<64, false>(rp, ci, ci + rb, da, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
+      const PivotSet S{ci + rb, kSetHbm, da, 0};
+      acc += pivot_edges<64, kModes>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
     }
   }
   const uint64_t tot = block_sum(acc, L.red);

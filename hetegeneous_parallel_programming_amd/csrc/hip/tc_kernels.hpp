@@ -23,8 +23,12 @@ enum Bin : int {
   kNoBin = 0xFF    // pivot cannot close a triangle (d+ < 2 or no wedge)
 };
 
-// LDS capacity (ids) of one pivot slot per bin.
+// Largest pivot list (ids + 1) of each bin: the tree of a bin's pivot fits
+// kCap words.
 constexpr int kCapG8 = 32, kCapG16 = 64, kCapG32 = 256, kCapWave = 1024, kCapBlock = 4096;
+// LDS words of one group's pivot-set area per bin (hash tables or tree);
+// every bin's areas fill the same 4096-word region of a workgroup.
+constexpr int kAreaG8 = 128, kAreaG16 = 256, kAreaG32 = 512, kAreaWave = 1024, kAreaBlock = 4096;
 
 // Counter words of a plan (uint32).
 constexpr int kCntBinCount = 0;                   // [kNumBins]

@@ -110,7 +110,6 @@ struct PivotSet {
 constexpr int32_t kEmpty = -1;
 constexpr uint32_t kHashA0 = 0x9E3779u;  // odd 24-bit multipliers
 constexpr uint32_t kHashA1 = 0x7F4A7Du;
-constexpr int kMaxKicks = 128;           // evictions before giving up on a key
 
 __device__ __forceinline__ uint32_t hash_bucket(int w, uint32_t mul, int sh) {
   return (uint32_t)__umul24((uint32_t)w, mul) >> sh;
@@ -131,12 +130,12 @@ __device__ __forceinline__ int hash_log2_buckets(int da, int area) {
 // word 2B).  LDS atomics make concurrent inserts by the lanes of a group
 // safe: every id is always either in exactly one slot or carried by exactly
 // one lane.  Returns false when an evicted id could not be placed within
-// kMaxKicks moves (the set is then incomplete and must not be used).
-__device__ __forceinline__ bool hash_insert(int32_t* T, int lb, int key) {
+// `kicks` moves (the set is then incomplete and must not be used).
+__device__ __forceinline__ bool hash_insert(int32_t* T, int lb, int key, int kicks) {
   const int sh = 32 - lb;
   const int B = 1 << lb;
   int t = 0;
-  for (int it = 0; it < kMaxKicks; ++it) {
+  for (int it = 0; it < kicks; ++it) {
     int32_t* b = T + (t ? 2 * B : 0) + 2 * (int)hash_bucket(key, t ? kHashA1 : kHashA0, sh);
     if (atomicCAS(&b[0], kEmpty, key) == kEmpty) return true;
     if (atomicCAS(&b[1], kEmpty, key) == kEmpty) return true;
@@ -188,18 +187,19 @@ __device__ __forceinline__ void group_sync() {
 
 // Builds the set of N+(u) (src[0..da), sorted) in `slot` (area words) with
 // the G lanes of a group (gl = lane index in the group): a cuckoo hash when
-// it fits and converges, else the tree (da must then fit the tree cap).
+// it fits and converges (and the tune allows it), else the tree (da must
+// then fit the tree cap).
 template <int G>
 __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
-                                                    int da, int gl) {
+                                                    int da, int gl, const Tune& tune) {
   PivotSet S{slot, kSetTree, 0, 0};
-  const int lb = hash_log2_buckets(da, area);
+  const int lb = tune.tree_only ? 0 : hash_log2_buckets(da, area);
   if (lb > 0) {
     const int B = 1 << lb;
     for (int x = gl; x < 4 * B; x += G) slot[x] = kEmpty;
     group_sync<G>();
     bool ok = true;
-    for (int j = gl; j < da; j += G) ok = hash_insert(slot, lb, src[j]) && ok;
+    for (int j = gl; j < da; j += G) ok = hash_insert(slot, lb, src[j], tune.hash_kicks) && ok;
     if (group_all<G>(ok)) {
       S.mode = kSetHash;
       S.p0 = 32 - lb;
@@ -513,7 +513,7 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
     const int32_t* ci = a.ci + d.ci_off;
     const int64_t rb = rp[u];
     const int da = (int)(rp[u + 1] - rb);
-    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + rb, da, gl);
+    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + rb, da, gl, a.tune);
     const uint32_t h = pivot_edges<G, kModeHash | kModeTree>(rp, ci, ci + rb, da, S, 0, G, sEnd, sBase, gl);
     acc.to_graph(g, batch, a.bufs.outs);
     acc.sum += h;
@@ -534,7 +534,7 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t h;
   if (kLdsPivot) {
-    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, ci + rb, da, threadIdx.x);
+    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, ci + rb, da, threadIdx.x, a.tune);
     h = pivot_edges<64, kModeHash | kModeTree>(rp, ci, ci + rb, da, S, wid * 64, kBlock, L.End + wid * 64,
                                                L.Base + wid * 64, lane);
   } else {

@@ -50,6 +50,8 @@ struct GraphDesc {
 
 struct Tune {
   int32_t t_g8, t_g16, t_g32, t_wave, rounds;
+  int32_t tree_only;   // 1: pivot sets are always LDS trees
+  int32_t hash_kicks;  // evictions per hash insert before falling back to the tree
 };
 
 // Per-planning-block bin counts, then their item offsets (kBlkStride words

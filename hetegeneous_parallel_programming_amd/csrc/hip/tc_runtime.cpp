@@ -89,6 +89,8 @@ tc_tune default_tune() {
   t.t_wave = 16384;
   t.rounds = 4;
   t.grid = 0;
+  t.pivot_set = 0;
+  t.hash_kicks = 128;
   return t;
 }
 
@@ -97,11 +99,13 @@ tc_status check_tune(const tc_tune& t) {
     return fail(TC_EINVAL, "tune thresholds must satisfy 0 <= t_g8 <= t_g16 <= t_g32 <= t_wave");
   if (t.rounds < 1 || t.rounds > 64) return fail(TC_EINVAL, "tune rounds must be in [1, 64]");
   if (t.grid < 0 || t.grid > (1 << 20)) return fail(TC_EINVAL, "tune grid must be in [0, 2^20]");
+  if (t.pivot_set < 0 || t.pivot_set > 1) return fail(TC_EINVAL, "tune pivot_set must be 0 or 1");
+  if (t.hash_kicks < 1 || t.hash_kicks > 1024) return fail(TC_EINVAL, "tune hash_kicks must be in [1, 1024]");
   return TC_OK;
 }
 
 tck::Tune to_kernel_tune(const tc_tune& t) {
-  return tck::Tune{t.t_g8, t.t_g16, t.t_g32, t.t_wave, t.rounds};
+  return tck::Tune{t.t_g8, t.t_g16, t.t_g32, t.t_wave, t.rounds, t.pivot_set, t.hash_kicks};
 }
 
 // Host-side checks shared by every upload: shape (and more when asked), and

@@ -4,7 +4,8 @@
 // maximum-residency grid, multigpu_nvshmem_pushpull.cu:41-51).  Here the
 // launch shape is a set of parameters -- the wedge-work thresholds that pick
 // each pivot's group width (8/16/32 lanes, a wave64, a workgroup), the pivots
-// per group in one light-bin tile, and the persistent grid -- and this file
+// per group in one light-bin tile, the persistent grid and the LDS layout of
+// the pivot sets (hash or tree) -- and this file
 // picks them by timing the real workload: coordinate descent over a fixed
 // candidate list per parameter, each candidate timed as the minimum of
 // `reps` calls after one untimed call (which also re-captures a batch's
@@ -101,6 +102,7 @@ tc_status tc_autotune(tc_ctx* ctx, tc_graph* g, tc_batch* b, int32_t reps, doubl
       {&tc_tune::t_g8, {0, 64, 128, 256, 512}},
       {&tc_tune::rounds, {1, 2, 4, 8, 16}},
       {&tc_tune::grid, {0, 2 * cus, 4 * cus, 8 * cus}},
+      {&tc_tune::pivot_set, {0, 1}},
   };
   bool out_of_time = false;
   for (int sweep = 0; sweep < 2 && !out_of_time; ++sweep) {
@@ -157,7 +159,7 @@ tc_status tc_tune_save(tc_ctx* ctx, const char* path, const char* key) {
   const tc_tune& t = ctx->tune;
   std::ostringstream rec;
   rec << id << " " << t.t_g8 << " " << t.t_g16 << " " << t.t_g32 << " " << t.t_wave << " "
-      << t.rounds << " " << t.grid;
+      << t.rounds << " " << t.grid << " " << t.pivot_set << " " << t.hash_kicks;
   keep.push_back(rec.str());
   const std::string tmp = std::string(path) + ".tmp";
   {
@@ -181,8 +183,8 @@ tc_status tc_tune_load(tc_ctx* ctx, const char* path, const char* key) {
     const std::string t = trim(line);
     if (t.compare(0, id.size() + 1, id + " ") != 0) continue;
     std::istringstream f(t.substr(id.size() + 1));
-    tc_tune v{};
-    if (!(f >> v.t_g8 >> v.t_g16 >> v.t_g32 >> v.t_wave >> v.rounds >> v.grid))
+    tc_tune v = tcr::default_tune();
+    if (!(f >> v.t_g8 >> v.t_g16 >> v.t_g32 >> v.t_wave >> v.rounds >> v.grid >> v.pivot_set >> v.hash_kicks))
       return fail(TC_EIO, std::string("malformed tune record in ") + path);
     tc_status s = tcr::check_tune(v);
     if (s != TC_OK) return s;

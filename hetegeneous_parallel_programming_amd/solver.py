@@ -34,7 +34,8 @@ class Stats(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("t_g8", nat.c_i32), ("t_g16", nat.c_i32), ("t_g32", nat.c_i32),
-                ("t_wave", nat.c_i32), ("rounds", nat.c_i32), ("grid", nat.c_i32)]
+                ("t_wave", nat.c_i32), ("rounds", nat.c_i32), ("grid", nat.c_i32),
+                ("pivot_set", nat.c_i32), ("hash_kicks", nat.c_i32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -41,6 +41,10 @@ typedef struct tc_tune {
   int32_t t_wave;  /* ... by one wave64; heavier pivots get a whole workgroup       */
   int32_t rounds;  /* pivots per group in one light-bin tile                          */
   int32_t grid;    /* persistent grid in workgroups (0 = full residency)             */
+  int32_t pivot_set;  /* LDS layout of N+(u): 0 = cuckoo hash where it fits, else the
+                         binary-search tree; 1 = always the tree                      */
+  int32_t hash_kicks; /* evictions allowed per insert before a hash build gives up and
+                         the pivot falls back to the tree (1..1024)                   */
 } tc_tune;
 
 #define TC_NUM_BINS 6
@@ -130,7 +134,8 @@ typedef struct tc_autotune_report {
 } tc_autotune_report;
 tc_status tc_autotune(tc_ctx* ctx, tc_graph* g, tc_batch* b, int32_t reps, double budget_ms,
                       tc_tune* best, tc_autotune_report* report);
-/* Tune file: one line "<device>|<key> t_g8 t_g16 t_g32 t_wave rounds grid"
+/* Tune file: one line "<device>|<key> t_g8 t_g16 t_g32 t_wave rounds grid
+ * pivot_set hash_kicks"
  * per (device, workload key); <device> is tc_device_name with blanks and '|'
  * replaced by '_'.  Save replaces the context's record for `key`
  * (keeps other records); load installs it (TC_EIO if absent or malformed). */

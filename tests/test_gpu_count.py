@@ -126,6 +126,56 @@ def test_tune_changes_schedule_not_result(ctx):
     dg.free()
 
 
+LAYOUTS = [{"pivot_set": 1},                   # every pivot set an LDS tree
+           {"pivot_set": 0, "hash_kicks": 1},   # most hash builds give up: tree fallback
+           {"pivot_set": 0, "hash_kicks": 3},
+           {"pivot_set": 0, "hash_kicks": 128},
+           # heavy wedge work stays in the wave bin: lists of 820..1023 ids
+           # exceed a wave's hash area and take its tree
+           {"t_wave": 1 << 20, "hash_kicks": 128},
+           {"t_wave": 1 << 20, "hash_kicks": 1}]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_pivot_set_layouts_same_count(ctx, layout):
+    """Hash sets, tree sets and hash builds that fall back to the tree give
+    the same counts (a clique of 3500 puts workgroup-bin pivots past the
+    workgroup's hash area; with t_wave = 2^20 the cliques of 850/1000 do the
+    same in the wave bin)."""
+    sizes = [5, 40, 300, 850, 1000, 3500]
+    src, dst, base = [], [], 0
+    for k in sizes:
+        s, d = _clique_pairs(k)
+        src.append(s + base)
+        dst.append(d + base)
+        base += k
+    cl = Graph.from_pairs(base, np.concatenate(src), np.concatenate(dst)).oriented()
+    rm = Graph.rmat(1 << 16, 16, 4).oriented()
+    saved = ctx.tune
+    try:
+        ctx.tune = layout
+        dcl, drm = ctx.upload(cl), ctx.upload(rm)
+        assert ctx.count(dcl)[0] == sum(math.comb(k, 3) for k in sizes)
+        assert ctx.count(drm)[0] == oracle.tc_count(rm.out_rowptr(), rm.out_colidx())
+        dcl.free()
+        drm.free()
+        graphs = Graph.batch(5, 0, 48)
+        want = np.array([oracle.tc_count(g.out_rowptr(), g.out_colidx()) for g in graphs], np.uint64)
+        b = ctx.batch(graphs)
+        assert np.array_equal(b.count()[0], want)
+        b.free()
+    finally:
+        ctx.tune = saved
+
+
+def test_tune_rejects_bad_layout(ctx):
+    saved = ctx.tune
+    for bad in ({"pivot_set": 2}, {"hash_kicks": 0}, {"hash_kicks": 5000}):
+        with pytest.raises(TcError):
+            ctx.tune = bad
+    assert ctx.tune == saved
+
+
 def test_upload_rejects_malformed(ctx):
     rp = np.array([0, 2, 3], np.int64)
     with pytest.raises(TcError):

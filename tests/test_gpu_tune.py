@@ -73,8 +73,10 @@ def test_autotune_arguments(ctx):
 def test_tune_file_round_trip(ctx, restore_tune, tmp_path):
     path = str(tmp_path / "tune.txt")
     assert not ctx.load_tune(path, "mix")  # no file yet
-    a = {"t_g8": 128, "t_g16": 512, "t_g32": 2048, "t_wave": 8192, "rounds": 2, "grid": 0}
-    b = {"t_g8": 0, "t_g16": 256, "t_g32": 1024, "t_wave": 32768, "rounds": 8, "grid": 512}
+    a = {"t_g8": 128, "t_g16": 512, "t_g32": 2048, "t_wave": 8192, "rounds": 2, "grid": 0,
+         "pivot_set": 0, "hash_kicks": 64}
+    b = {"t_g8": 0, "t_g16": 256, "t_g32": 1024, "t_wave": 32768, "rounds": 8, "grid": 512,
+         "pivot_set": 1, "hash_kicks": 128}
     ctx.tune = a
     ctx.save_tune(path, "mix")
     ctx.tune = b

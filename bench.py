@@ -92,10 +92,20 @@ def cpu_baseline(g, target_s: float, seed: int = 7) -> dict:
 
 
 def leg_single(ctx, g, steps, warmup, name, plain=True):
+    """Binned count (partner scheme: each edge streams the shorter list; its
+    algorithmic bytes are work['partner_bytes']) and the plain kernel (the
+    reference's edge iterator: N+(v) streamed for every edge, work['alg_bytes'])."""
     work = g.work()
+    t0 = time.perf_counter()
     dg = ctx.upload(g)
-    res = {}
+    upload_s = time.perf_counter() - t0
+    info = dg.info()
+    log(f"{name}: upload {upload_s * 1e3:.0f} ms incl. partner build {info['build_ms']:.2f} ms on the "
+        f"device; {info['partner_rows']} partner rows")
+    res = {"upload": {"ms": upload_s * 1e3, "partner_build_ms": info["build_ms"],
+                      "partner_rows": info["partner_rows"], "device_bytes": info["bytes"]}}
     kinds = [("binned", ctx.count)] + ([("plain", ctx.count_plain)] if plain else [])
+    alg = {"binned": work["partner_bytes"], "plain": work["alg_bytes"]}
     for kind, fn in kinds:
         ms_total, outs = time_steps(lambda: fn(dg), steps, warmup)
         counts = {o[0] for o in outs}
@@ -108,7 +118,8 @@ def leg_single(ctx, g, steps, warmup, name, plain=True):
                      "kernel_ms_avg": k_avg, "kernel_ms_min": kms[0],
                      "plan_ms_avg": float(np.mean(pms)),
                      "edges_per_s": work["edges"] / (ms_total / steps / 1e3),
-                     "alg_GBps": work["alg_bytes"] / (k_avg / 1e3) / 1e9,
+                     "alg_bytes": alg[kind], "alg_GBps": alg[kind] / (k_avg / 1e3) / 1e9,
+                     "edge_iterator_GBps": work["alg_bytes"] / (k_avg / 1e3) / 1e9,
                      "bins": outs[-1][1]["bin_pivots"]}
         log(f"{name} {kind}: count={res[kind]['count']} step={res[kind]['ms_per_step']:.3f} ms "
             f"kernel={k_avg:.3f} ms alg={res[kind]['alg_GBps']:.0f} GB/s")
@@ -265,17 +276,21 @@ def leg_partition(ctx, seed, steps, warmup):
         f"{work['edges']}, halo lists {part.halo_ids.size}")
     del g
     dg = ctx.upload_partition(part)
+    info = dg.info()
     ms_total, outs = time_steps(lambda: ctx.count(dg), steps, warmup)
     counts = {o[0] for o in outs}
     if len(counts) != 1:
         raise RuntimeError("partition counts differ across steps")
     k_avg = float(np.mean([o[1]["count_ms"] for o in outs]))
     dg.free()
-    res = {"count": counts.pop(), "owned_edges": work["edges"], "alg_bytes": work["alg_bytes"],
+    res = {"count": counts.pop(), "owned_edges": work["edges"], "alg_bytes": work["partner_bytes"],
+           "alg_bytes_edge_iterator": work["alg_bytes"],
            "halo_lists": int(part.halo_ids.size), "halo_ids": int(part.halo_colidx.size),
+           "partner_rows": info["partner_rows"], "partner_build_ms": info["build_ms"],
            "ms_per_step": ms_total / steps, "kernel_ms_avg": k_avg,
            "edges_per_s": work["edges"] / (ms_total / steps / 1e3),
-           "alg_GBps": work["alg_bytes"] / (k_avg / 1e3) / 1e9}
+           "alg_GBps": work["partner_bytes"] / (k_avg / 1e3) / 1e9,
+           "edge_iterator_GBps": work["alg_bytes"] / (k_avg / 1e3) / 1e9}
     res["roofline_frac"] = res["alg_GBps"] / PEAK_HBM_GBS
     log(f"partition: count={res['count']} kernel={k_avg:.3f} ms frac={res['roofline_frac']:.3f}")
     return res
@@ -374,9 +389,14 @@ def leg_headline(ctx, args) -> dict:
                      "unit": "GB/s", "frac": best["alg_GBps"] / PEAK_HBM_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "kernel": "count_kernel<bin> (6 per-bin persistent launches per step)",
-                     "alg_bytes": work["alg_bytes"], "kernel_ms_avg": best["kernel_ms_avg"]},
+                     "alg_bytes": work["partner_bytes"],
+                     "alg_bytes_note": "bytes the partner-scheme count reads with no reuse: per partner row "
+                                       "36 + 4 d+(x), per edge 20 + 4 min(d+(u), d+(v))",
+                     "alg_bytes_edge_iterator": work["alg_bytes"],
+                     "edge_iterator_GBps": best["edge_iterator_GBps"],
+                     "kernel_ms_avg": best["kernel_ms_avg"]},
         "cpu_baseline": None,
-        "extra": {"plain_kernel": single.get("plain"), "binned": best},
+        "extra": {"plain_kernel": single.get("plain"), "binned": best, "upload": single["upload"]},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)

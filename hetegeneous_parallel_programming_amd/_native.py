@@ -57,7 +57,11 @@ class PartitionView(ctypes.Structure):
 
 class Work(ctypes.Structure):
     _fields_ = [("rows", c_i64), ("edges", c_i64), ("wedge_elems", c_i64), ("min_elems", c_i64),
-                ("alg_bytes", c_i64), ("max_out", c_i32)]
+                ("alg_bytes", c_i64), ("max_out", c_i32), ("partner_rows", c_i64),
+                ("partner_bytes", c_i64)]
+
+
+PARTNERS_PER_ROW = 1024  # TCH_PARTNERS_PER_ROW
 
 
 class ArenaStats(ctypes.Structure):

@@ -19,7 +19,12 @@ struct tc_batch {
   tck::GraphDesc* d_descs = nullptr;
   std::vector<tck::GraphDesc> descs;
   std::vector<int32_t> rows;
-  int64_t P = 0;
+  int64_t P = 0;        // plan indices (partner-row slots over all graphs)
+  int64_t RP = 0;       // rowptr pool slots
+  int64_t cio = 0;      // colidx pool slots
+  Partners partners;
+  int64_t nv = 0;       // partner rows in use over all graphs
+  double build_ms = 0;
   PlanScratch scratch;
   uint64_t* pinned = nullptr;  // n counts, then counters
   hipGraph_t graph = nullptr;
@@ -35,6 +40,7 @@ struct tc_batch {
     dev_free(rp);
     dev_free(ci);
     dev_free(d_descs);
+    partners.release();
     scratch.release();
     if (pinned) (void)hipHostFree(pinned);
     pinned = nullptr;
@@ -46,12 +52,13 @@ static tc_status capture(tc_ctx* ctx, tc_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
-  tck::LaunchArgs a;
+  tck::LaunchArgs a{};
   a.descs = b->d_descs;
   a.ngraphs = b->n;
   a.P = b->P;
   a.rp = b->rp;
   a.ci = b->ci;
+  b->partners.into(a);
   a.tune = to_kernel_tune(ctx->tune);
   a.bufs = b->scratch.bufs();
   hipStream_t s = ctx->stream;
@@ -83,7 +90,6 @@ extern "C" {
 tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs, uint32_t check,
                           tc_batch** out) {
   if (!ctx || !out || n_graphs < 0 || (n_graphs > 0 && !dags)) return fail(TC_EINVAL, "bad arguments");
-  int64_t tot_rows = 0, tot_nnz = 0;
   for (int32_t i = 0; i < n_graphs; ++i) {
     const tc_csr_view& d = dags[i];
     tc_status st = tch_validate(&d, check | TC_CHECK_SHAPE);
@@ -91,31 +97,36 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
     for (int64_t v = 0; v < d.n_rows; ++v)
       if ((d.rowptr[v + 1] - d.rowptr[v]) * 64 >= ((int64_t)1 << 31))
         return fail(TC_ERANGE, "graph " + std::to_string(i) + ": DAG out-degree too large");
-    tot_rows += d.n_rows;
-    tot_nnz += d.nnz;
   }
-  if (tot_rows >= ((int64_t)1 << 31)) return fail(TC_ERANGE, "batch has 2^31 or more pivots");
+  int64_t tot_slots = 0;
+  for (int32_t i = 0; i < n_graphs; ++i) tot_slots += tck::partner_rows_max(dags[i].n_rows, dags[i].nnz);
+  if (tot_slots >= ((int64_t)1 << 31)) return fail(TC_ERANGE, "batch has 2^31 or more pivots");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
   std::unique_ptr<tc_batch> b(new tc_batch);
   b->ctx = ctx;
   b->n = n_graphs;
-  b->P = tot_rows;
   b->descs.resize(n_graphs);
   b->rows.resize(n_graphs);
-  int64_t rpo = 0, cio = 0, pb = 0;
+  int64_t rpo = 0, cio = 0, pro = 0, pb = 0;
   for (int32_t i = 0; i < n_graphs; ++i) {
-    b->descs[i] = tck::GraphDesc{rpo, cio, pb, 0, dags[i].n_rows};
+    const int32_t nvm = tck::partner_rows_max(dags[i].n_rows, dags[i].nnz);
+    b->descs[i] = tck::GraphDesc{rpo, cio, cio, pro, pro, rpo, pb, dags[i].n_rows, nvm, 0, dags[i].n_rows};
     b->rows[i] = dags[i].n_rows;
     rpo += dags[i].n_rows + 1;
     cio += dags[i].nnz;
-    pb += dags[i].n_rows;
+    pro += (int64_t)nvm + 1;
+    pb += nvm;
   }
+  b->P = pb;
+  b->RP = rpo;
+  b->cio = cio;
   tc_status st = dev_alloc((void**)&b->rp, sizeof(int64_t) * (size_t)std::max<int64_t>(rpo, 1), "batch rowptr");
   if (st == TC_OK) st = dev_alloc((void**)&b->ci, sizeof(int32_t) * (size_t)std::max<int64_t>(cio, 1), "batch colidx");
   if (st == TC_OK)
     st = dev_alloc((void**)&b->d_descs, sizeof(tck::GraphDesc) * (size_t)std::max(n_graphs, 1), "batch descs");
-  if (st == TC_OK) st = b->scratch.ensure(tot_rows, n_graphs);
+  if (st == TC_OK) st = b->partners.alloc(cio, pro, n_graphs);
+  if (st == TC_OK) st = b->scratch.ensure(b->P, n_graphs);
   if (st != TC_OK) {
     b->release();
     return st;
@@ -185,8 +196,37 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
     b->release();
     return fail(TC_EDEVICE, std::string("batch upload: ") + hipGetErrorString(e));
   }
+  // Partner structures of all graphs in one pass over the pools.
+  if (n_graphs > 0) {
+    tck::LaunchArgs a{};
+    a.descs = b->d_descs;
+    a.ngraphs = b->n;
+    a.rp = b->rp;
+    a.ci = b->ci;
+    b->partners.into(a);
+    BuildScratch bs;
+    st = TC_OK;
+    e = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (e == hipSuccess) st = launch_build(a, b->RP, bs, ctx->stream);
+    if (st == TC_OK && e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
+    std::vector<int32_t> nv((size_t)n_graphs);
+    if (st == TC_OK && e == hipSuccess)
+      e = hipMemcpyAsync(nv.data(), b->partners.nv, sizeof(int32_t) * nv.size(), hipMemcpyDeviceToHost,
+                         ctx->stream);
+    if (st == TC_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    bs.release();
+    if (st == TC_OK && e != hipSuccess) st = fail(TC_EDEVICE, std::string("partner build: ") + hipGetErrorString(e));
+    if (st != TC_OK) {
+      b->release();
+      return st;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    b->build_ms = ms;
+    for (int32_t v : nv) b->nv += v;
+  }
   b->bytes = sizeof(int64_t) * rpo + sizeof(int32_t) * cio + (int64_t)sizeof(tck::GraphDesc) * n_graphs +
-             9 * tot_rows + 8 * (int64_t)n_graphs;
+             b->partners.bytes + PlanScratch::bytes_for(b->P) + 8 * (int64_t)n_graphs;
   *out = b.release();
   return TC_OK;
 }
@@ -212,7 +252,7 @@ tc_status tc_batch_count(tc_ctx* ctx, tc_batch* b, uint64_t* counts, tc_stats* s
     float ms = 0;
     TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
     stats->count_ms = ms;
-    stats->pivots = b->P;
+    stats->pivots = b->nv;
     const uint32_t* c = (const uint32_t*)(b->pinned + b->n);
     for (int k = 0; k < TC_NUM_BINS; ++k) stats->bin_pivots[k] = c[tck::kCntBinCount + k];
   }
@@ -249,7 +289,7 @@ tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double
     float ms = 0;
     TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
     stats->count_ms = ms;
-    stats->pivots = b->P;
+    stats->pivots = (int64_t)b->RP - b->n;
   }
   return TC_OK;
 }
@@ -267,5 +307,19 @@ tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b) {
 
 int64_t tc_batch_bytes(const tc_batch* b) { return b ? b->bytes : 0; }
 int32_t tc_batch_graphs(const tc_batch* b) { return b ? b->n : 0; }
+
+tc_status tc_batch_get_info(const tc_batch* b, tc_graph_info* out) {
+  if (!b || !out) return fail(TC_EINVAL, "null argument");
+  out->graphs = b->n;
+  out->rows = b->RP - b->n;
+  int64_t nnz = 0;
+  for (int32_t i = 0; i < b->n; ++i)
+    nnz += (i + 1 < b->n ? b->descs[i + 1].ci_off : b->cio) - b->descs[i].ci_off;
+  out->nnz = nnz;
+  out->partner_rows = b->nv;
+  out->build_ms = b->build_ms;
+  out->bytes = b->bytes;
+  return TC_OK;
+}
 
 }  // extern "C"

@@ -39,14 +39,50 @@ struct PlanScratch {
   uint8_t* bin_of = nullptr;
   uint64_t* items = nullptr;
   uint32_t* blk = nullptr;
+  int64_t* part = nullptr;
   int64_t cap = 0;
   uint32_t* counters = nullptr;
   uint64_t* outs = nullptr;
   int64_t outs_cap = 0;
   tc_status ensure(int64_t pivots, int64_t nouts);
   void release();
-  tck::PlanBufs bufs() const { return tck::PlanBufs{bin_of, items, blk, counters, outs}; }
+  tck::PlanBufs bufs() const { return tck::PlanBufs{bin_of, items, blk, part, counters, outs}; }
+  // Device bytes per plan index, for footprint estimates.
+  static int64_t bytes_for(int64_t pivots);
 };
+
+// Partner structure of pooled graphs (see tck::GraphDesc): partner ids (as
+// many slots as the colidx pool), partner rows, rows in use per graph.
+struct Partners {
+  int32_t* pci = nullptr;
+  int64_t* vrp = nullptr;
+  int32_t* vrow = nullptr;
+  int32_t* nv = nullptr;
+  int64_t bytes = 0;
+  tc_status alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs);
+  void release();
+  void into(tck::LaunchArgs& a) const {
+    a.pci = pci;
+    a.vrp = vrp;
+    a.vrow = vrow;
+    a.nv = nv;
+  }
+  static int64_t bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs);
+};
+
+// Temporary arrays of a partner build over RP rowptr slots.
+struct BuildScratch {
+  char* mem = nullptr;
+  int64_t cap = 0;
+  tck::BuildBufs b{};
+  tc_status ensure(int64_t rp_slots);
+  void release();
+  static int64_t bytes_for(int64_t rp_slots);
+};
+
+// Builds the partners of the graphs of `a` (descs on the device, pools set)
+// on stream s, with `bs` sized for RP slots.  Stream-ordered, no sync.
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s);
 
 // Makes ctx's device current for the scope of one API call.
 struct DeviceGuard {
@@ -90,6 +126,10 @@ struct tc_graph {
   int64_t nnz = 0;
   int32_t row_begin = 0, row_end = 0;
   int32_t max_out = 0;
+  tcr::Partners partners;
+  int32_t nv_max = 0;   // partner-row slots (plan indices)
+  int32_t nv = 0;       // partner rows in use
+  double build_ms = 0;  // partner build time on the device
   tck::GraphDesc* d_desc = nullptr;
   int64_t bytes = 0;
 };

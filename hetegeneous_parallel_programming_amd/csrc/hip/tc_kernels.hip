@@ -82,6 +82,12 @@ __device__ __forceinline__ int tree_pos(int x, int h) {
 #define TC_IN_FLIGHT 6
 #endif
 constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
+#ifndef TC_IN_FLIGHT_GROUP
+#define TC_IN_FLIGHT_GROUP 4
+#endif
+// Sub-wave groups keep per-group state in vector registers; fewer lookups
+// in flight leave room for it.
+constexpr int kInFlightGroup = TC_IN_FLIGHT_GROUP;
 
 // ---------------------------------------------------------------------------
 // Pivot sets: how a group holds N+(u) for membership tests
@@ -106,6 +112,11 @@ struct PivotSet {
   int p0;            // hash: 32 - log2(B); tree: levels; hbm: list length
   int p1;            // hash: byte offset of table 1 (8B)
 };
+
+// An int32 in global memory.  Edge bases pass through LDS as plain pointer
+// values; typing them as global keeps the element loads global_load (a
+// generic pointer would make them flat loads, which also wait on LDS).
+typedef __attribute__((address_space(1))) const int32_t gint32;
 
 constexpr int32_t kEmpty = -1;
 constexpr uint32_t kHashA0 = 0x9E3779u;  // odd 24-bit multipliers
@@ -222,23 +233,23 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
 // element t of the int32 array at sBase[k] (N+(v) shifted back by the edge's
 // first flat index); each lane keeps its current edge's end and base in
 // registers and searches sEnd only when it steps past that edge.
-template <int G, int kModes>
+template <int G, int kModes, int NF = kInFlight>
 __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
                                                 const int32_t* __restrict__ ci,
                                                 const int32_t* __restrict__ elist, int da,
                                                 const PivotSet S, int jfirst, int jstride,
-                                                int32_t* sEnd, const int32_t** sBase, int gl) {
+                                                int32_t* sEnd, gint32** sBase, int gl) {
   uint32_t hits = 0;
   const int32_t* A = S.A;
   for (int j0 = jfirst; j0 < da; j0 += jstride) {
     const int j = j0 + gl;
     int c = 0;
-    const int32_t* base = ci;
+    gint32* base = (gint32*)ci;
     if (j < da) {
       const int v = elist[j];
       const int64_t b = rp[v];
       c = (int)(rp[v + 1] - b);
-      base = ci + b;
+      base = (gint32*)ci + b;
     }
     int total;
     const int excl = group_excl_scan<G>(c, gl, total);
@@ -246,11 +257,11 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
     sBase[gl] = base - excl;
     wave_sync();
     int cur_end = 0;  // t < cur_end  <=>  t lies in the lane's current edge
-    const int32_t* cur_base = ci;
-    for (int t0 = 0; t0 < total; t0 += G * kInFlight) {
-      int w[kInFlight];
+    gint32* cur_base = (gint32*)ci;
+    for (int t0 = 0; t0 < total; t0 += G * NF) {
+      int w[NF];
 #pragma unroll
-      for (int q = 0; q < kInFlight; ++q) {
+      for (int q = 0; q < NF; ++q) {
         const int t = t0 + q * G + gl;
         w[q] = INT_MIN;  // ids are >= 0: never found
         if (t < total) {
@@ -266,7 +277,7 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
         const char* Tb = (const char*)A;
         const int sh = S.p0;
 #pragma unroll
-        for (int q = 0; q < kInFlight; ++q) {
+        for (int q = 0; q < NF; ++q) {
           const int2 x = *(const int2*)(Tb + (hash_bucket(w[q], kHashA0, sh) << 3));
           const int2 y = *(const int2*)(Tb + S.p1 + (hash_bucket(w[q], kHashA1, sh) << 3));
           hits += ((x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q])) ? 1u : 0u;
@@ -278,40 +289,49 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
         // select and one shift-add per step.  ~k is never 0 (k < 2^h).
         const int h = S.p0;
         const char* Ab = (const char*)A;
-        int kb[kInFlight];
+        int kb[NF];
 #pragma unroll
-        for (int q = 0; q < kInFlight; ++q) kb[q] = 4;
+        for (int q = 0; q < NF; ++q) kb[q] = 4;
         for (int s = 0; s < h; ++s) {
-          int val[kInFlight];
+          int val[NF];
 #pragma unroll
-          for (int q = 0; q < kInFlight; ++q) val[q] = *(const int32_t*)(Ab + kb[q]);
+          for (int q = 0; q < NF; ++q) val[q] = *(const int32_t*)(Ab + kb[q]);
 #pragma unroll
-          for (int q = 0; q < kInFlight; ++q) kb[q] = 2 * kb[q] + (val[q] < w[q] ? 4 : 0);
+          for (int q = 0; q < NF; ++q) kb[q] = 2 * kb[q] + (val[q] < w[q] ? 4 : 0);
         }
 #pragma unroll
-        for (int q = 0; q < kInFlight; ++q) {
+        for (int q = 0; q < NF; ++q) {
           const unsigned k = (unsigned)kb[q] >> 2;
           const unsigned lo = (k >> __builtin_ctz(~k)) >> 1;
           hits += (A[lo] == w[q]) ? 1u : 0u;
         }
       } else if (kModes & kModeHbm) {
-        int lo[kInFlight];
+        int lo[NF];
 #pragma unroll
-        for (int q = 0; q < kInFlight; ++q) lo[q] = 0;
+        for (int q = 0; q < NF; ++q) lo[q] = 0;
         for (int len = S.p0; len > 1;) {
           const int half = len >> 1;
 #pragma unroll
-          for (int q = 0; q < kInFlight; ++q) lo[q] = (A[lo[q] + half] <= w[q]) ? lo[q] + half : lo[q];
+          for (int q = 0; q < NF; ++q) lo[q] = (A[lo[q] + half] <= w[q]) ? lo[q] + half : lo[q];
           len -= half;
         }
 #pragma unroll
-        for (int q = 0; q < kInFlight; ++q) hits += (A[lo[q]] == w[q]) ? 1u : 0u;
+        for (int q = 0; q < NF; ++q) hits += (A[lo[q]] == w[q]) ? 1u : 0u;
       }
     }
     wave_sync();
   }
   return hits;
 }
+// Values the compiler cannot prove wave-uniform (read from LDS, or derived
+// from threadIdx / 64) but that are: moved to scalar registers.
+__device__ __forceinline__ int uniform32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ uint64_t shfl_down_u64(uint64_t x, int d) {
   const uint32_t lo = __shfl_down((uint32_t)x, d, 64);
   const uint32_t hi = __shfl_down((uint32_t)(x >> 32), d, 64);
@@ -354,25 +374,289 @@ __device__ __forceinline__ int find_graph(const GraphDesc* descs, int ngraphs, i
   return lo;
 }
 
-__device__ __forceinline__ int choose_bin(const int64_t* rp, const int32_t* ci, int64_t rb,
-                                          int64_t da, const Tune& t) {
-  if (da < 2) return kNoBin;
-  if (da > kCapBlock - 1) return BIN_GLOBAL;
-  if (da > kCapWave - 1) return BIN_BLOCK;
-  int64_t work = 0;
-  for (int64_t e = rb; e < rb + da; ++e) {
+// Largest g with row_base <= q (build row slots are in graph order).
+__device__ __forceinline__ int find_graph_rows(const GraphDesc* descs, int ngraphs, int64_t q) {
+  int lo = 0, hi = ngraphs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].row_base <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan over n values (three passes: block sums, one-workgroup scan
+// of the sums, block scans); loads and stores go through functors so one
+// implementation serves uint32 plan counts and int64 partner offsets.
+// ---------------------------------------------------------------------------
+constexpr int kScanPer = 8;                        // values per thread
+constexpr int kScanChunk = kBlock * kScanPer;      // values per block
+constexpr int kScanTop = 1024;                     // threads of the top pass
+
+int64_t scan_partials(int64_t n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
+
+// Exclusive scan of x over the workgroup; *total = workgroup sum (all threads).
+__device__ __forceinline__ int64_t block_excl_scan(int64_t x, int64_t* s_wave, int64_t* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int64_t incl = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) s_wave[wid] = incl;
+  __syncthreads();
+  int64_t before = 0, all = 0;
+  for (int w = 0; w < (int)(blockDim.x / 64); ++w) {
+    if (w < wid) before += s_wave[w];
+    all += s_wave[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - x;
+}
+
+template <class Load>
+__global__ __launch_bounds__(kBlock) void scan_reduce_kernel(Load in, int64_t n, int64_t* part) {
+  __shared__ int64_t s_wave[kWaves];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+    if (i < n) s += in(i);
+  }
+  int64_t total;
+  block_excl_scan(s, s_wave, &total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// part[0..nb) -> exclusive offsets; part[nb] = grand total.
+__global__ __launch_bounds__(kScanTop) void scan_top_kernel(int64_t* part, int64_t nb) {
+  __shared__ int64_t s_wave[kScanTop / 64];
+  const int64_t per = (nb + kScanTop - 1) / kScanTop;
+  const int64_t b0 = min((int64_t)threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+  int64_t s = 0;
+  for (int64_t b = b0; b < b1; ++b) s += part[b];
+  int64_t total;
+  int64_t run = block_excl_scan(s, s_wave, &total);
+  for (int64_t b = b0; b < b1; ++b) {
+    const int64_t c = part[b];
+    part[b] = run;
+    run += c;
+  }
+  if (threadIdx.x == 0) part[nb] = total;
+}
+
+// Each thread scans kScanPer consecutive values of its block's chunk.
+template <class Load, class Store>
+__global__ __launch_bounds__(kBlock) void scan_down_kernel(Load in, Store out, int64_t n,
+                                                           const int64_t* part) {
+  __shared__ int64_t s_wave[kWaves];
+  const int64_t i0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  int64_t v[kScanPer];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = (i0 + k < n) ? in(i0 + k) : 0;
+    s += v[k];
+  }
+  int64_t total;
+  int64_t run = part[blockIdx.x] + block_excl_scan(s, s_wave, &total);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (i0 + k < n) out(i0 + k, run);
+    run += v[k];
+  }
+}
+
+template <class Load, class Store>
+static hipError_t launch_scan(Load in, Store out, int64_t n, int64_t* part, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
+  scan_reduce_kernel<Load><<<(unsigned)nb, kBlock, 0, s>>>(in, n, part);
+  scan_top_kernel<<<1, kScanTop, 0, s>>>(part, nb);
+  scan_down_kernel<Load, Store><<<(unsigned)nb, kBlock, 0, s>>>(in, out, n, part);
+  return hipGetLastError();
+}
+
+struct LoadU32 {
+  const uint32_t* p;
+  __device__ int64_t operator()(int64_t i) const { return p[i]; }
+};
+struct StoreU32 {
+  uint32_t* p;
+  __device__ void operator()(int64_t i, int64_t v) const { p[i] = (uint32_t)v; }
+};
+struct LoadSum2 {
+  const int32_t* a;
+  const int32_t* b;
+  __device__ int64_t operator()(int64_t i) const { return (int64_t)a[i] + b[i]; }
+};
+struct LoadI32 {
+  const int32_t* p;
+  __device__ int64_t operator()(int64_t i) const { return p[i]; }
+};
+struct StoreI64 {
+  int64_t* p;
+  __device__ void operator()(int64_t i, int64_t v) const { p[i] = v; }
+};
+
+// ---------------------------------------------------------------------------
+// partner build (one thread per DAG row slot of the rowptr pool)
+// ---------------------------------------------------------------------------
+// Pass 1: per row u (owned), partners through out-edges (c1, plain store)
+// and, through atomics, partners through in-edges of the edge's head (c2).
+__global__ __launch_bounds__(kBlock) void partner_count_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= RP) return;
+  const int g = find_graph_rows(a.descs, a.ngraphs, q);
+  const GraphDesc d = a.descs[g];
+  const int64_t u = q - d.row_base;
+  int32_t c1 = 0;
+  if (u >= d.own_begin && u < d.own_end) {
+    const int64_t* rp = a.rp + d.rp_off;
+    const int32_t* ci = a.ci + d.ci_off;
+    const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const int v = ci[e];
+      if (rp[v + 1] - rp[v] <= du) ++c1;
+      else atomicAdd(&b.c2[d.row_base + v], 1);
+    }
+  }
+  b.c1[q] = c1;
+}
+
+// Pass 2: partner ids.  Out-edge partners of u fill the front of u's range
+// in row order; in-edge partners take the rest in arrival order (c2 is
+// counted down).  Order inside a range does not change any count.
+__global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= RP) return;
+  const int g = find_graph_rows(a.descs, a.ngraphs, q);
+  const GraphDesc d = a.descs[g];
+  const int64_t u = q - d.row_base;
+  if (u < d.own_begin || u >= d.own_end) return;
+  const int64_t* rp = a.rp + d.rp_off;
+  const int32_t* ci = a.ci + d.ci_off;
+  int32_t* pci = (int32_t*)a.pci + d.pci_off;
+  const int64_t* off = b.off + d.row_base;
+  const int64_t base = off[0];
+  int64_t w = off[u] - base;
+  const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
+  for (int64_t e = e0; e < e1; ++e) {
     const int v = ci[e];
-    work += rp[v + 1] - rp[v];
+    if (rp[v + 1] - rp[v] <= du) {
+      pci[w++] = v;
+    } else {
+      const int64_t slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
+      pci[slot] = (int32_t)u;
+    }
+  }
+}
+
+// Pass 3: partner rows per DAG row.
+__global__ __launch_bounds__(kBlock) void partner_rows_count_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= RP) return;
+  const int g = find_graph_rows(a.descs, a.ngraphs, q);
+  const GraphDesc d = a.descs[g];
+  const int64_t u = q - d.row_base;
+  int32_t k = 0;
+  if (u < d.n_rows) {
+    const int64_t c = b.off[q + 1] - b.off[q];
+    k = (int32_t)((c + kPartnersPerRow - 1) / kPartnersPerRow);
+  }
+  b.kc[q] = k;
+}
+
+// Pass 4: partner rows (owner, partner range), split evenly by partner
+// count; the graph's row-count slot also writes nv[g] and the final end.
+__global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= RP) return;
+  const int g = find_graph_rows(a.descs, a.ngraphs, q);
+  const GraphDesc d = a.descs[g];
+  const int64_t u = q - d.row_base;
+  const int64_t pbase = b.off[d.row_base], vbase = b.voff[d.row_base];
+  int64_t* vrp = (int64_t*)a.vrp + d.vrp_off;
+  int32_t* vrow = (int32_t*)a.vrow + d.vrow_off;
+  if (u == d.n_rows) {
+    const int64_t nv = b.voff[q] - vbase;
+    ((int32_t*)a.nv)[g] = (int32_t)nv;
+    vrp[nv] = b.off[q] - pbase;
+    return;
+  }
+  const int32_t K = b.kc[q];
+  if (K == 0) return;
+  const int64_t v0 = b.voff[q] - vbase, p0 = b.off[q] - pbase, c = b.off[q + 1] - b.off[q];
+  for (int32_t k = 0; k < K; ++k) {
+    vrow[v0 + k] = (int32_t)u;
+    vrp[v0 + k] = p0 + (c * k) / K;
+  }
+}
+
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s) {
+  if (RP <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
+  hipError_t e = launch_zero(b.c2, RP, s);
+  if (e != hipSuccess) return e;
+  partner_count_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(LoadSum2{b.c1, b.c2}, StoreI64{b.off}, RP, b.part, s)) != hipSuccess) return e;
+  partner_fill_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  partner_rows_count_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(LoadI32{b.kc}, StoreI64{b.voff}, RP, b.part, s)) != hipSuccess) return e;
+  partner_rows_fill_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// planning: bin every partner row, then scatter (graph, row) items by bin
+// ---------------------------------------------------------------------------
+// Bin of a partner row: its set is N+(x) (s ids), its partners' lists hold
+// `work` ids in all.
+__device__ __forceinline__ int choose_bin(const int64_t* rp, const int32_t* part, int64_t np, int64_t s,
+                                          const Tune& t) {
+  if (s < 1 || np < 1) return kNoBin;
+  if (s > kCapBlock - 1) return BIN_GLOBAL;
+  if (s > kCapWave - 1) return BIN_BLOCK;
+  int64_t work = 0;
+  for (int64_t e = 0; e < np; ++e) {
+    const int p = part[e];
+    work += rp[p + 1] - rp[p];
     if (work > t.t_wave) return BIN_BLOCK;
   }
   if (work == 0) return kNoBin;
-  if (da > kCapG32 - 1 || work > t.t_g32) return BIN_WAVE;
-  if (da > kCapG16 - 1 || work > t.t_g16) return BIN_G32;
-  if (da > kCapG8 - 1 || work > t.t_g8) return BIN_G16;
+  if (s > kCapG32 - 1 || work > t.t_g32) return BIN_WAVE;
+  if (s > kCapG16 - 1 || work > t.t_g16) return BIN_G32;
+  if (s > kCapG8 - 1 || work > t.t_g8) return BIN_G16;
   return BIN_G8;
 }
 
-// Pass 1: bin of every pivot and, per planning block, the count per bin.
+// A plan index's partner row: graph g, row r, owner x, partner range.
+struct PRow {
+  int g;
+  int r;       // -1 when the slot is unused
+  int x;
+  int64_t pb, pe;
+};
+
+__device__ __forceinline__ PRow partner_row(const LaunchArgs& a, int g, int r) {
+  const GraphDesc d = a.descs[g];
+  PRow pr{g, r, 0, 0, 0};
+  if (r >= a.nv[g]) { pr.r = -1; return pr; }
+  pr.x = a.vrow[d.vrow_off + r];
+  pr.pb = a.vrp[d.vrp_off + r];
+  pr.pe = a.vrp[d.vrp_off + r + 1];
+  return pr;
+}
+
+// Pass 1: bin of every plan index and, per planning block, the count per
+// bin (bin-major: blk[bin * nblk + block]).
 __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
   __shared__ uint32_t s_cnt[kNumBins];
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -381,62 +665,25 @@ __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
   if (p < a.P) {
     const int g = find_graph(a.descs, a.ngraphs, p);
     const GraphDesc d = a.descs[g];
-    const int64_t u = d.row_begin + (p - d.plan_base);
-    const int64_t* rp = a.rp + d.rp_off;
-    const int32_t* ci = a.ci + d.ci_off;
-    const int64_t rb = rp[u];
-    const int bin = choose_bin(rp, ci, rb, rp[u + 1] - rb, a.tune);
+    const PRow pr = partner_row(a, g, (int)(p - d.plan_base));
+    int bin = kNoBin;
+    if (pr.r >= 0) {
+      const int64_t* rp = a.rp + d.rp_off;
+      bin = choose_bin(rp, a.pci + d.pci_off + pr.pb, pr.pe - pr.pb, rp[pr.x + 1] - rp[pr.x], a.tune);
+    }
     a.bufs.bin_of[p] = (uint8_t)bin;
     if (bin != kNoBin) atomicAdd(&s_cnt[bin], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < kNumBins) a.bufs.blk[(int64_t)blockIdx.x * kBlkStride + threadIdx.x] = s_cnt[threadIdx.x];
+  const int64_t nblk = gridDim.x;
+  if (threadIdx.x < kNumBins) a.bufs.blk[threadIdx.x * nblk + blockIdx.x] = s_cnt[threadIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.bufs.blk[kNumBins * nblk] = 0;  // scanned to the total
 }
 
-// Pass 2 (one workgroup of 1024): per-block counts -> item offsets, ordered
-// by (bin, block); bin totals into the counters.
-constexpr int kScanThreads = 1024;
-
-__global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(LaunchArgs a, int64_t nblk) {
-  __shared__ uint32_t s_wave[kScanThreads / 64];
-  __shared__ uint32_t s_bin_total[kNumBins];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int64_t chunk = (nblk + kScanThreads - 1) / kScanThreads;
-  const int64_t b0 = min((int64_t)tid * chunk, nblk), b1 = min(b0 + chunk, nblk);
-  uint32_t* blk = a.bufs.blk;
-  uint32_t excl[kNumBins];
-  for (int b = 0; b < kNumBins; ++b) {
-    uint32_t local = 0;
-    for (int64_t k = b0; k < b1; ++k) local += blk[k * kBlkStride + b];
-    // inclusive scan over the 1024 threads
-    uint32_t incl = local;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += y;
-    }
-    if (lane == 63) s_wave[wid] = incl;
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (int w = 0; w < kScanThreads / 64; ++w) {
-      if (w < wid) before += s_wave[w];
-      all += s_wave[w];
-    }
-    excl[b] = before + incl - local;
-    if (tid == 0) s_bin_total[b] = all;
-    __syncthreads();
-  }
-  uint32_t base = 0;
-  for (int b = 0; b < kNumBins; ++b) {
-    uint32_t run = base + excl[b];
-    for (int64_t k = b0; k < b1; ++k) {
-      const uint32_t c = blk[k * kBlkStride + b];
-      blk[k * kBlkStride + b] = run;
-      run += c;
-    }
-    base += s_bin_total[b];
-  }
-  if (tid < kNumBins) a.bufs.counters[kCntBinCount + tid] = s_bin_total[tid];
+// Pass 2 (after the scan of blk): bin totals into the counters.
+__global__ void plan_totals_kernel(LaunchArgs a, int64_t nblk) {
+  const int b = threadIdx.x;
+  if (b < kNumBins) a.bufs.counters[kCntBinCount + b] = a.bufs.blk[(b + 1) * nblk] - a.bufs.blk[b * nblk];
 }
 
 // Pass 3: stable scatter of (graph, row) items into their bin segment.
@@ -455,12 +702,12 @@ __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
   }
   __syncthreads();
   if (bin != kNoBin) {
-    uint32_t off = a.bufs.blk[(int64_t)blockIdx.x * kBlkStride + bin] + rank;
+    const int64_t nblk = gridDim.x;
+    uint32_t off = a.bufs.blk[bin * nblk + blockIdx.x] + rank;
     for (int w = 0; w < wid; ++w) off += s_wc[w][bin];
     const int g = find_graph(a.descs, a.ngraphs, p);
-    const GraphDesc d = a.descs[g];
-    const int64_t u = d.row_begin + (p - d.plan_base);
-    a.bufs.items[off] = ((uint64_t)(uint32_t)g << 32) | (uint32_t)u;
+    const int64_t r = p - a.descs[g].plan_base;
+    a.bufs.items[off] = ((uint64_t)(uint32_t)g << 32) | (uint32_t)r;
   }
 }
 
@@ -470,7 +717,7 @@ __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
 struct __align__(16) CountLds {
   int32_t A[kAreaBlock];         // pivot-set areas (one per group of the tile's bin)
   int32_t End[kBlock];           // per-lane edge end in the batch's flat index
-  const int32_t* Base[kBlock];   // per-lane N+(v) base of that edge, shifted by its first index
+  gint32* Base[kBlock];          // per-lane N+(v) base of that edge, shifted by its first index
   uint64_t red[kWaves];
   int64_t tab_cnt[1];
   int64_t cur_first, cur_seg;
@@ -500,21 +747,24 @@ template <int G, int AREA>
 __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* items, int64_t first,
                                            int64_t last, CountLds& L, bool batch, Acc& acc) {
   constexpr int NGRP = kBlock / G;
-  const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+  // With G = 64 a group is a wave: its pivot and everything derived from it
+  // is wave-uniform and lives in scalar registers.
+  const int grp = G == 64 ? uniform32(threadIdx.x / 64) : threadIdx.x / G, gl = threadIdx.x % G;
   int32_t* slot = L.A + grp * AREA;
   int32_t* sEnd = L.End + grp * G;
-  const int32_t** sBase = L.Base + grp * G;
+  gint32** sBase = L.Base + grp * G;
   for (int64_t i = first + grp; i < last; i += NGRP) {
     const uint64_t item = items[i];
     const int g = (int)(item >> 32);
-    const int u = (int)(uint32_t)item;
+    const PRow pr = partner_row(a, g, (int)(uint32_t)item);
     const GraphDesc d = a.descs[g];
     const int64_t* rp = a.rp + d.rp_off;
     const int32_t* ci = a.ci + d.ci_off;
-    const int64_t rb = rp[u];
-    const int da = (int)(rp[u + 1] - rb);
-    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + rb, da, gl, a.tune);
-    const uint32_t h = pivot_edges<G, kModeHash | kModeTree>(rp, ci, ci + rb, da, S, 0, G, sEnd, sBase, gl);
+    const int64_t rb = rp[pr.x];
+    const int ds = (int)(rp[pr.x + 1] - rb);
+    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + rb, ds, gl, a.tune);
+    const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup>(
+        rp, ci, a.pci + d.pci_off + pr.pb, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl);
     acc.to_graph(g, batch, a.bufs.outs);
     acc.sum += h;
     wave_sync();
@@ -525,21 +775,23 @@ template <bool kLdsPivot>
 __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, CountLds& L, bool batch,
                                             Acc& acc) {
   const int g = (int)(item >> 32);
-  const int u = (int)(uint32_t)item;
+  const PRow pr = partner_row(a, g, (int)(uint32_t)item);
   const GraphDesc d = a.descs[g];
   const int64_t* rp = a.rp + d.rp_off;
   const int32_t* ci = a.ci + d.ci_off;
-  const int64_t rb = rp[u];
-  const int da = (int)(rp[u + 1] - rb);
+  const int64_t rb = rp[pr.x];
+  const int ds = (int)(rp[pr.x + 1] - rb);
+  const int32_t* part = a.pci + d.pci_off + pr.pb;
+  const int np = (int)(pr.pe - pr.pb);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t h;
   if (kLdsPivot) {
-    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, ci + rb, da, threadIdx.x, a.tune);
-    h = pivot_edges<64, kModeHash | kModeTree>(rp, ci, ci + rb, da, S, wid * 64, kBlock, L.End + wid * 64,
+    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, ci + rb, ds, threadIdx.x, a.tune);
+    h = pivot_edges<64, kModeHash | kModeTree>(rp, ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
                                                L.Base + wid * 64, lane);
   } else {
-    const PivotSet S{ci + rb, kSetHbm, da, 0};
-    h = pivot_edges<64, kModeHbm>(rp, ci, ci + rb, da, S, wid * 64, kBlock, L.End + wid * 64,
+    const PivotSet S{ci + rb, kSetHbm, ds, 0};
+    h = pivot_edges<64, kModeHbm>(rp, ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
                                   L.Base + wid * 64, lane);
   }
   acc.to_graph(g, batch, a.bufs.outs);
@@ -586,7 +838,7 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
   for (;;) {
     if (threadIdx.x == 0) L.cur_first = atomicAdd(&a.bufs.counters[kCntTileHead + BIN], 1u);
     __syncthreads();
-    const int64_t t = L.cur_first;
+    const int64_t t = uniform64(L.cur_first);
     if (t >= ntiles) break;
     const int64_t first = t * piv;
     const int64_t last = min(first + (int64_t)piv, cnt);
@@ -622,7 +874,7 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
 struct __align__(16) PlainLds {
   int32_t A[kWaves][kCapWave];
   int32_t End[kBlock];
-  const int32_t* Base[kBlock];
+  gint32* Base[kBlock];
   uint64_t red[kWaves];
 };
 
@@ -676,14 +928,15 @@ hipError_t launch_zero(void* p, int64_t words32, hipStream_t s) {
 
 hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
   if (a.P <= 0) return hipSuccess;
-  const int64_t blocks = (a.P + kBlock - 1) / kBlock;
-  plan_bin_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(a);
+  const int64_t nblk = plan_blocks(a.P);
+  plan_bin_kernel<<<(unsigned)nblk, kBlock, 0, s>>>(a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  plan_scan_kernel<<<1, kScanThreads, 0, s>>>(a, blocks);
-  e = hipGetLastError();
+  e = launch_scan(LoadU32{a.bufs.blk}, StoreU32{a.bufs.blk}, kNumBins * nblk + 1, a.bufs.part, s);
   if (e != hipSuccess) return e;
-  plan_scatter_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(a);
+  plan_totals_kernel<<<1, 64, 0, s>>>(a, nblk);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  plan_scatter_kernel<<<(unsigned)nblk, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "tc_host.h"
+
 namespace tck {
 
 constexpr int kBlock = 256;   // threads per workgroup = 4 wave64s
@@ -36,17 +38,46 @@ constexpr int kCntBinCursor = kNumBins;           // [kNumBins]
 constexpr int kCntTileHead = 2 * kNumBins;        // [kNumBins] tile queue heads
 constexpr int kNumCounters = 3 * kNumBins + 2;
 
-// One graph of a (possibly single-graph) count.  Its CSR lives at
-// rowptr_pool + rp_off (rowptr values relative to colidx_pool + ci_off);
-// its pivots are rows [row_begin, row_end), numbered from plan_base in the
-// plan index space shared by all graphs of the launch.
+// Partner rows.  Every DAG edge (u, v) is counted from the endpoint whose
+// out-list is longer: its "owner" x keeps N+(x) as the LDS set and its
+// "partner" p has N+(p) streamed through it, so each edge streams
+// min(d+(u), d+(v)) ids instead of d+(v).  Owner of (u, v): u when
+// d+(v) <= d+(u) (partner v), else v (partner u).  The partners of one
+// owner x are consecutive in the partner-id array; x's list is cut into
+// ceil(count / kPartnersPerRow) "partner rows" (vrow[k] = x, partner ids
+// pci[vrp[k] .. vrp[k+1])), the unit of work of the count.
+constexpr int kPartnersPerRow = TCH_PARTNERS_PER_ROW;
+
+// One graph of a (possibly single-graph) count.
+//   DAG: rowptr at rp_off of the rowptr pool (values relative to ci_off of
+//        the colidx pool), n_rows rows; only rows [own_begin, own_end) have
+//        their out-edges counted (a partition's owned range), other rows are
+//        lists read through them (halo).
+//   Partners: ids at pci_off of the partner-id pool (as many slots as the
+//        DAG has ids); partner rows at vrp_off / vrow_off of their pools,
+//        nv_max + 1 slots, of which the first nv[g] are used (nv is written
+//        on the device; vrp values are relative to pci_off).
+//   Build: the graph's n_rows + 1 slots of a partner build's per-row
+//        scratch start at row_base (graphs in order, contiguous).
+//   Plan: its partner rows are plan indices [plan_base, plan_base + nv_max).
 struct GraphDesc {
   int64_t rp_off;
   int64_t ci_off;
+  int64_t pci_off;
+  int64_t vrp_off;
+  int64_t vrow_off;
+  int64_t row_base;
   int64_t plan_base;
-  int32_t row_begin;
-  int32_t row_end;
+  int32_t n_rows;
+  int32_t nv_max;
+  int32_t own_begin;
+  int32_t own_end;
 };
+
+// Partner-row slots a graph of n rows and m DAG ids may need.
+inline int32_t partner_rows_max(int64_t n, int64_t m) {
+  return (int32_t)(n + m / kPartnersPerRow + 1);
+}
 
 struct Tune {
   int32_t t_g8, t_g16, t_g32, t_wave, rounds;
@@ -54,14 +85,11 @@ struct Tune {
   int32_t hash_kicks;  // evictions per hash insert before falling back to the tree
 };
 
-// Per-planning-block bin counts, then their item offsets (kBlkStride words
-// per block of kBlock consecutive plan indices).
-constexpr int kBlkStride = 8;
-
 struct PlanBufs {
   uint8_t* bin_of;      // [P] bin of each pivot
-  uint64_t* items;      // [P] (graph << 32 | row), ordered by (bin, plan index)
-  uint32_t* blk;        // [ceil(P/kBlock) * kBlkStride]
+  uint64_t* items;      // [P] (graph << 32 | partner row), ordered by (bin, plan index)
+  uint32_t* blk;        // [kNumBins * nblk + 1] per-(bin, planning block) counts -> offsets
+  int64_t* part;        // [scan_partials(kNumBins * nblk + 1)] scan partials
   uint32_t* counters;   // [kNumCounters]
   uint64_t* outs;       // [ngraphs] u64 counts
 };
@@ -69,12 +97,36 @@ struct PlanBufs {
 struct LaunchArgs {
   const GraphDesc* descs;
   int32_t ngraphs;
-  int64_t P;            // total pivots over all graphs
-  const int64_t* rp;    // rowptr pool
-  const int32_t* ci;    // colidx pool
+  int64_t P;            // plan indices over all graphs (sum of nv_max)
+  const int64_t* rp;    // DAG rowptr pool
+  const int32_t* ci;    // DAG colidx pool
+  const int32_t* pci;   // partner-id pool
+  const int64_t* vrp;   // partner-row offsets pool
+  const int32_t* vrow;  // partner-row owners pool
+  const int32_t* nv;    // [ngraphs] partner rows in use
   Tune tune;
   PlanBufs bufs;
 };
+
+// Planning blocks of a launch over P plan indices.
+inline int64_t plan_blocks(int64_t P) { return (P + 255) / 256; }
+
+// Scratch of a partner build over RP row slots (the graphs' n_rows + 1
+// slots, concatenated in graph order from row_base 0).
+struct BuildBufs {
+  int32_t* c1;          // [RP] partners owned through out-edges
+  int32_t* c2;          // [RP] partners owned through in-edges (consumed by the fill)
+  int64_t* off;         // [RP] partner offsets (pool-global exclusive scan)
+  int32_t* kc;          // [RP] partner rows per DAG row
+  int64_t* voff;        // [RP] partner-row offsets (pool-global exclusive scan)
+  int64_t* part;        // [scan_partials(RP)] scan partials
+};
+int64_t scan_partials(int64_t n);
+
+// Builds the partner ids, rows and counts of every graph of `a` (descs,
+// rp, ci as for a count; pci/vrp/vrow/nv are written); RP = sum of
+// n_rows + 1 over the graphs.
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s);
 
 // Launchers (stream-ordered; return the launch status).
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);

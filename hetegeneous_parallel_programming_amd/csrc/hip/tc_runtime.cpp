@@ -46,17 +46,18 @@ tc_status PlanScratch::ensure(int64_t pivots, int64_t nouts) {
     dev_free(bin_of);
     dev_free(items);
     dev_free(blk);
+    dev_free(part);
     bin_of = nullptr;
     items = nullptr;
     blk = nullptr;
+    part = nullptr;
     cap = 0;
     int64_t c = std::max<int64_t>(pivots, 1024);
+    const int64_t nb = tck::kNumBins * tck::plan_blocks(c) + 1;
     tc_status st = dev_alloc((void**)&bin_of, (size_t)c, "pivot bins");
-    if (st != TC_OK) return st;
-    st = dev_alloc((void**)&items, sizeof(uint64_t) * (size_t)c, "pivot items");
-    if (st != TC_OK) return st;
-    const int64_t nblk = (c + tck::kBlock - 1) / tck::kBlock;
-    st = dev_alloc((void**)&blk, sizeof(uint32_t) * tck::kBlkStride * (size_t)nblk, "plan blocks");
+    if (st == TC_OK) st = dev_alloc((void**)&items, sizeof(uint64_t) * (size_t)c, "pivot items");
+    if (st == TC_OK) st = dev_alloc((void**)&blk, sizeof(uint32_t) * (size_t)nb, "plan blocks");
+    if (st == TC_OK) st = dev_alloc((void**)&part, sizeof(int64_t) * (size_t)tck::scan_partials(nb), "plan scan");
     if (st != TC_OK) return st;
     cap = c;
   }
@@ -76,9 +77,82 @@ void PlanScratch::release() {
   dev_free(bin_of);
   dev_free(items);
   dev_free(blk);
+  dev_free(part);
   dev_free(counters);
   dev_free(outs);
   *this = PlanScratch();
+}
+
+int64_t PlanScratch::bytes_for(int64_t pivots) {
+  const int64_t nb = tck::kNumBins * tck::plan_blocks(pivots) + 1;
+  return 9 * pivots + 4 * nb + 8 * tck::scan_partials(nb);
+}
+
+tc_status Partners::alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
+  tc_status st = dev_alloc((void**)&pci, sizeof(int32_t) * (size_t)std::max<int64_t>(ci_slots, 1), "partner ids");
+  if (st == TC_OK) st = dev_alloc((void**)&vrp, sizeof(int64_t) * (size_t)std::max<int64_t>(row_slots, 1), "partner rows");
+  if (st == TC_OK) st = dev_alloc((void**)&vrow, sizeof(int32_t) * (size_t)std::max<int64_t>(row_slots, 1), "partner owners");
+  if (st == TC_OK) st = dev_alloc((void**)&nv, sizeof(int32_t) * (size_t)std::max(ngraphs, 1), "partner row counts");
+  if (st != TC_OK) {
+    release();
+    return st;
+  }
+  bytes = bytes_for(ci_slots, row_slots, ngraphs);
+  return TC_OK;
+}
+
+void Partners::release() {
+  dev_free(pci);
+  dev_free(vrp);
+  dev_free(vrow);
+  dev_free(nv);
+  *this = Partners();
+}
+
+int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
+  return 4 * ci_slots + 12 * row_slots + 4 * (int64_t)ngraphs;
+}
+
+static int64_t build_words(int64_t rp_slots) {
+  // c1, c2, kc (int32) + off, voff (int64) + scan partials, 256-byte pieces
+  auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+  return up(4 * rp_slots) * 3 + up(8 * rp_slots) * 2 + up(8 * tck::scan_partials(rp_slots));
+}
+
+int64_t BuildScratch::bytes_for(int64_t rp_slots) { return build_words(rp_slots); }
+
+tc_status BuildScratch::ensure(int64_t rp_slots) {
+  const int64_t need = build_words(std::max<int64_t>(rp_slots, 1));
+  if (need > cap) {
+    dev_free(mem);
+    mem = nullptr;
+    cap = 0;
+    tc_status st = dev_alloc((void**)&mem, (size_t)need, "partner build scratch");
+    if (st != TC_OK) return st;
+    cap = need;
+  }
+  auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+  const int64_t n = std::max<int64_t>(rp_slots, 1);
+  char* p = mem;
+  b.c1 = (int32_t*)p;  p += up(4 * n);
+  b.c2 = (int32_t*)p;  p += up(4 * n);
+  b.kc = (int32_t*)p;  p += up(4 * n);
+  b.off = (int64_t*)p; p += up(8 * n);
+  b.voff = (int64_t*)p; p += up(8 * n);
+  b.part = (int64_t*)p;
+  return TC_OK;
+}
+
+void BuildScratch::release() {
+  dev_free(mem);
+  *this = BuildScratch();
+}
+
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s) {
+  tc_status st = bs.ensure(RP);
+  if (st != TC_OK) return st;
+  TC_HIP_TRY(tck::launch_partners(a, RP, bs.b, s));
+  return TC_OK;
 }
 
 tc_tune default_tune() {
@@ -131,19 +205,22 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   g->row_begin = row_begin;
   g->row_end = row_end;
   g->max_out = max_out;
+  g->nv_max = tck::partner_rows_max(g->n, g->nnz);
   auto cleanup = [&]() {
     dev_free(g->rp);
     dev_free(g->ci);
     dev_free(g->d_desc);
+    g->partners.release();
   };
   tc_status st = dev_alloc((void**)&g->rp, sizeof(int64_t) * ((size_t)g->n + 1), "rowptr");
   if (st == TC_OK) st = dev_alloc((void**)&g->ci, sizeof(int32_t) * (size_t)std::max<int64_t>(g->nnz, 1), "colidx");
   if (st == TC_OK) st = dev_alloc((void**)&g->d_desc, sizeof(tck::GraphDesc), "graph desc");
+  if (st == TC_OK) st = g->partners.alloc(g->nnz, (int64_t)g->nv_max + 1, 1);
   if (st != TC_OK) {
     cleanup();
     return st;
   }
-  tck::GraphDesc d{0, 0, 0, row_begin, row_end};
+  const tck::GraphDesc d{0, 0, 0, 0, 0, 0, 0, g->n, g->nv_max, row_begin, row_end};
   hipError_t e = hipMemcpy(g->rp, dag->rowptr, sizeof(int64_t) * ((size_t)g->n + 1), hipMemcpyHostToDevice);
   if (e == hipSuccess && g->nnz > 0)
     e = hipMemcpy(g->ci, dag->colidx, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice);
@@ -152,7 +229,35 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
     cleanup();
     return fail(TC_EDEVICE, std::string("graph upload: ") + hipGetErrorString(e));
   }
-  g->bytes = sizeof(int64_t) * ((int64_t)g->n + 1) + sizeof(int32_t) * g->nnz + sizeof(tck::GraphDesc);
+  // Partner structure, built on the device once per upload.
+  tck::LaunchArgs a{};
+  a.descs = g->d_desc;
+  a.ngraphs = 1;
+  a.rp = g->rp;
+  a.ci = g->ci;
+  g->partners.into(a);
+  BuildScratch bs;
+  hipStream_t s = ctx->stream;
+  e = hipEventRecord(ctx->ev[0], s);
+  if (e == hipSuccess) {
+    st = launch_build(a, (int64_t)g->n + 1, bs, s);
+    if (st == TC_OK) e = hipEventRecord(ctx->ev[1], s);
+    if (st == TC_OK && e == hipSuccess)
+      e = hipMemcpyAsync(ctx->pinned, g->partners.nv, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (st == TC_OK && e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  bs.release();
+  if (st == TC_OK && e != hipSuccess) st = fail(TC_EDEVICE, std::string("partner build: ") + hipGetErrorString(e));
+  if (st != TC_OK) {
+    cleanup();
+    return st;
+  }
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+  g->build_ms = ms;
+  g->nv = *(const int32_t*)ctx->pinned;
+  g->bytes = sizeof(int64_t) * ((int64_t)g->n + 1) + sizeof(int32_t) * g->nnz + sizeof(tck::GraphDesc) +
+             g->partners.bytes;
   *out = g.release();
   return TC_OK;
 }
@@ -284,11 +389,23 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) {
   dev_free(g->rp);
   dev_free(g->ci);
   dev_free(g->d_desc);
+  g->partners.release();
   delete g;
   return TC_OK;
 }
 
 int64_t tc_graph_bytes(const tc_graph* g) { return g ? g->bytes : 0; }
+
+tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) {
+  if (!g || !out) return fail(TC_EINVAL, "null argument");
+  out->graphs = 1;
+  out->rows = g->n;
+  out->nnz = g->nnz;
+  out->partner_rows = g->nv;
+  out->build_ms = g->build_ms;
+  out->bytes = g->bytes;
+  return TC_OK;
+}
 
 tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
@@ -296,15 +413,16 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
   const double t0 = now_ms();
-  const int64_t P = (int64_t)g->row_end - g->row_begin;
+  const int64_t P = g->nv_max;
   tc_status st = ctx->scratch.ensure(P, 1);
   if (st != TC_OK) return st;
-  tck::LaunchArgs a;
+  tck::LaunchArgs a{};
   a.descs = g->d_desc;
   a.ngraphs = 1;
   a.P = P;
   a.rp = g->rp;
   a.ci = g->ci;
+  g->partners.into(a);
   a.tune = to_kernel_tune(ctx->tune);
   a.bufs = ctx->scratch.bufs();
   hipStream_t s = ctx->stream;
@@ -320,7 +438,7 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
                             hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
-  return finish_stats(ctx, stats, t0, true, P);
+  return finish_stats(ctx, stats, t0, true, g->nv);
 }
 
 tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {

@@ -5,8 +5,8 @@
 //              bounded look-ahead past a blocked head) are uploaded on a
 //              copy stream while earlier launches keep running;
 //   launch  -- any free slot takes every uploaded request (up to the batch
-//              limits) and runs zero -> plan -> per-bin count kernels and
-//              the copy-back on its own stream.
+//              limits) and runs partner build -> zero -> plan -> per-bin
+//              count kernels and the copy-back on its own stream.
 // Launch slots run concurrently on separate streams, so requests arriving
 // while others are on the GPU overlap with them on the device.
 #include <algorithm>
@@ -24,9 +24,24 @@ namespace {
 constexpr uint64_t kAlign = 256;
 inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
+// A request's extent in the HBM pool: rowptr | colidx | partner ids |
+// partner-row offsets | partner-row owners (each 256-byte aligned).
+struct Layout {
+  uint64_t ci, pci, vrp, vrow, total;
+  Layout(int32_t n, int64_t nnz) {
+    const int64_t nvm = tck::partner_rows_max(n, nnz);
+    ci = align_up(sizeof(int64_t) * ((uint64_t)n + 1));
+    pci = ci + align_up(sizeof(int32_t) * (uint64_t)nnz);
+    vrp = pci + align_up(sizeof(int32_t) * (uint64_t)nnz);
+    vrow = vrp + align_up(sizeof(int64_t) * ((uint64_t)nvm + 1));
+    total = vrow + align_up(sizeof(int32_t) * ((uint64_t)nvm + 1));
+  }
+};
+
 struct Req {
   uint64_t tag = 0;
   int32_t n = 0;
+  int32_t nv_max = 0;
   int64_t nnz = 0;
   std::vector<int64_t> rp;  // pageable copy (used when the staging area is full)
   std::vector<int32_t> ci;
@@ -39,10 +54,12 @@ struct Req {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  void* plan = nullptr;              // device: items | bin_of | blk | counters | outs | descs
+  void* plan = nullptr;              // device: items | bin_of | blk | part | counters | outs | descs | nv
   uint64_t* pinned = nullptr;        // host: outs[G] then descs staging
   tck::PlanBufs bufs{};
   tck::GraphDesc* d_descs = nullptr;
+  int32_t* d_nv = nullptr;
+  BuildScratch build;
   std::vector<Req*> reqs;
   bool busy = false;
 };
@@ -138,7 +155,7 @@ bool tc_server::admit() {
     const void* src_ci = r->staged ? (const void*)(r->staged + align_up(rp_bytes)) : (const void*)r->ci.data();
     e = hipMemcpyAsync(base, src_rp, rp_bytes, hipMemcpyHostToDevice, copy_stream);
     if (e == hipSuccess && r->nnz > 0)
-      e = hipMemcpyAsync(base + align_up(rp_bytes), src_ci, sizeof(int32_t) * (size_t)r->nnz,
+      e = hipMemcpyAsync(base + Layout(r->n, r->nnz).ci, src_ci, sizeof(int32_t) * (size_t)r->nnz,
                          hipMemcpyHostToDevice, copy_stream);
     if (e != hipSuccess) break;
   }
@@ -168,29 +185,35 @@ bool tc_server::launch() {
     if (s.busy || ready.empty()) continue;
     int64_t rows = 0;
     while (!ready.empty() && (int)s.reqs.size() < cfg.max_batch_graphs &&
-           rows + ready.front()->n <= cfg.max_batch_rows) {
-      rows += ready.front()->n;
+           rows + ready.front()->nv_max <= cfg.max_batch_rows) {
+      rows += ready.front()->nv_max;
       s.reqs.push_back(ready.front());
       ready.pop_front();
     }
     if (s.reqs.empty()) continue;
     const int k = (int)s.reqs.size();
     tck::GraphDesc* hd = (tck::GraphDesc*)(s.pinned + cfg.max_batch_graphs);
-    int64_t pb = 0;
+    int64_t pb = 0, rb = 0;
     for (int i = 0; i < k; ++i) {
       Req* r = s.reqs[i];
-      hd[i] = tck::GraphDesc{(int64_t)(r->off / sizeof(int64_t)),
-                             (int64_t)((r->off + align_up(sizeof(int64_t) * ((uint64_t)r->n + 1))) /
-                                       sizeof(int32_t)),
-                             pb, 0, r->n};
-      pb += r->n;
+      const Layout L(r->n, r->nnz);
+      hd[i] = tck::GraphDesc{(int64_t)(r->off / sizeof(int64_t)), (int64_t)((r->off + L.ci) / sizeof(int32_t)),
+                             (int64_t)((r->off + L.pci) / sizeof(int32_t)),
+                             (int64_t)((r->off + L.vrp) / sizeof(int64_t)),
+                             (int64_t)((r->off + L.vrow) / sizeof(int32_t)), rb, pb, r->n, r->nv_max, 0, r->n};
+      pb += r->nv_max;
+      rb += (int64_t)r->n + 1;
     }
-    tck::LaunchArgs a;
+    tck::LaunchArgs a{};
     a.descs = s.d_descs;
     a.ngraphs = k;
     a.P = pb;
     a.rp = (const int64_t*)pool;
     a.ci = (const int32_t*)pool;
+    a.pci = (const int32_t*)pool;
+    a.vrp = (const int64_t*)pool;
+    a.vrow = (const int32_t*)pool;
+    a.nv = s.d_nv;
     {
       std::lock_guard<std::recursive_mutex> lk(ctx->mu);
       a.tune = to_kernel_tune(ctx->tune);
@@ -199,6 +222,9 @@ bool tc_server::launch() {
     const double t = now_ms();
     for (Req* r : s.reqs) r->t_launch = t;
     hipError_t e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
+    // Partner structures of this launch's graphs, then plan and count.
+    if (e == hipSuccess) e = s.build.ensure(rb) == TC_OK ? tck::launch_partners(a, rb, s.build.b, s.stream)
+                                                         : hipErrorOutOfMemory;
     // A batch of one graph would take the single-graph reduction path.
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s.stream);
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)k, s.stream);
@@ -250,6 +276,7 @@ void tc_server::release() {
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     dev_free(s.plan);
+    s.build.release();
     if (s.pinned) (void)hipHostFree(s.pinned);
   }
   slots.clear();
@@ -272,7 +299,7 @@ double tc_clock_ms(void) { return now_ms(); }
 
 uint64_t tc_request_footprint(int32_t n_rows, int64_t nnz) {
   if (n_rows < 0 || nnz < 0) return 0;
-  return align_up(sizeof(int64_t) * ((uint64_t)n_rows + 1)) + align_up(sizeof(int32_t) * (uint64_t)nnz);
+  return Layout(n_rows, nnz).total;
 }
 
 tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out) {
@@ -306,10 +333,11 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
     }
   }
   const int64_t R = cfg->max_batch_rows, G = cfg->max_batch_graphs;
-  const int64_t nblk = (R + tck::kBlock - 1) / tck::kBlock;
+  const int64_t nb = tck::kNumBins * tck::plan_blocks(R) + 1;
   const uint64_t o_items = 0, o_bin = o_items + align_up(8 * R), o_blk = o_bin + align_up(R),
-                 o_cnt = o_blk + align_up(4 * tck::kBlkStride * nblk), o_outs = o_cnt + align_up(4 * 64),
-                 o_desc = o_outs + align_up(8 * G), total = o_desc + align_up(sizeof(tck::GraphDesc) * G);
+                 o_part = o_blk + align_up(4 * nb), o_cnt = o_part + align_up(8 * tck::scan_partials(nb)),
+                 o_outs = o_cnt + align_up(4 * 64), o_desc = o_outs + align_up(8 * G),
+                 o_nv = o_desc + align_up(sizeof(tck::GraphDesc) * G), total = o_nv + align_up(4 * G);
   s->slots.resize(cfg->slots);
   for (auto& sl : s->slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
@@ -321,14 +349,16 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
       return fail(TC_EDEVICE, std::string("server slot: ") + hipGetErrorString(e));
     }
     st = dev_alloc(&sl.plan, total, "slot plan scratch");
+    if (st == TC_OK) st = sl.build.ensure(R);
     if (st != TC_OK) {
       s->release();
       return st;
     }
     char* p = (char*)sl.plan;
     sl.bufs = tck::PlanBufs{(uint8_t*)(p + o_bin), (uint64_t*)(p + o_items), (uint32_t*)(p + o_blk),
-                            (uint32_t*)(p + o_cnt), (uint64_t*)(p + o_outs)};
+                            (int64_t*)(p + o_part), (uint32_t*)(p + o_cnt), (uint64_t*)(p + o_outs)};
     sl.d_descs = (tck::GraphDesc*)(p + o_desc);
+    sl.d_nv = (int32_t*)(p + o_nv);
   }
   tc_server* raw = s.release();
   raw->th = std::thread([raw]() { raw->run(); });
@@ -353,7 +383,9 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
   tch_arena_stats as;
   tch_arena_stats_get(s->arena, &as);
   if (fp > as.capacity) return reject(TC_ENOMEM, "request footprint exceeds the server's HBM pool");
-  if (dag->n_rows > s->cfg.max_batch_rows) return reject(TC_ERANGE, "request has more rows than max_batch_rows");
+  const int32_t nvm = tck::partner_rows_max(dag->n_rows, dag->nnz);
+  if (nvm > s->cfg.max_batch_rows)
+    return reject(TC_ERANGE, "request needs more plan rows than max_batch_rows (rows + nnz/1024 + 1)");
   for (int64_t v = 0; v < dag->n_rows; ++v)
     if ((dag->rowptr[v + 1] - dag->rowptr[v]) * 64 >= ((int64_t)1 << 31))
       return reject(TC_ERANGE, "DAG out-degree too large");
@@ -369,10 +401,11 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
   std::unique_ptr<Req> r(new Req);
   r->tag = tag;
   r->n = dag->n_rows;
+  r->nv_max = nvm;
   r->nnz = dag->nnz;
   r->bytes = fp;
   uint64_t so = 0;
-  if (s->stage_arena && tch_arena_alloc(s->stage_arena, fp, &so) == TC_OK) {
+  if (s->stage_arena && tch_arena_alloc(s->stage_arena, Layout(dag->n_rows, dag->nnz).pci, &so) == TC_OK) {
     r->stage_off = so;
     r->staged = s->staging + so;
     const uint64_t rp_bytes = sizeof(int64_t) * ((uint64_t)dag->n_rows + 1);

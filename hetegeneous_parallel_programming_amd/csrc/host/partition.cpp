@@ -209,6 +209,49 @@ tc_status tch_partition_induced(const tc_csr_view* g, int32_t k, int32_t part, t
   return TC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Owner counts of the partner scheme over the edges of `rows`; deg(v) and
+// slot(v) give d+ and the owner-count slot of any vertex the rows reach.
+template <class Deg, class Slot>
+static void partner_units(int64_t nl, const int64_t* rowptr, const int32_t* colidx, int64_t first_row,
+                          std::vector<int32_t>& cnt, Deg deg, Slot slot, int64_t* units, int64_t* unit_bytes) {
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t i = 0; i < nl; ++i) {
+    const int64_t du = rowptr[i + 1] - rowptr[i];
+    int32_t own = 0;
+    for (int64_t x = rowptr[i]; x < rowptr[i + 1]; ++x) {
+      const int32_t v = colidx[x];
+      if (deg(v) <= du) {
+        ++own;
+      } else {
+#pragma omp atomic
+        cnt[slot(v)] += 1;
+      }
+    }
+#pragma omp atomic
+    cnt[slot((int32_t)(first_row + i))] += own;
+  }
+  int64_t u = 0, b = 0;
+  const int64_t ns = (int64_t)cnt.size();
+#pragma omp parallel for reduction(+ : u, b)
+  for (int64_t k = 0; k < ns; ++k) {
+    const int64_t c = cnt[k];
+    if (!c) continue;
+    const int64_t K = (c + TCH_PARTNERS_PER_ROW - 1) / TCH_PARTNERS_PER_ROW;
+    u += K;
+    b += K * 36;
+  }
+  *units = u;
+  *unit_bytes = b;
+}
+
+}  // namespace
+
+extern "C" {
+
 tc_status tch_work_of(const tc_csr_view* dag, int32_t row_begin, int32_t row_end, tch_work* out) {
   if (!dag || !out) return fail(TC_EINVAL, "null argument");
   if (row_begin < 0 || row_end > dag->n_rows || row_begin > row_end)
@@ -234,6 +277,19 @@ tc_status tch_work_of(const tc_csr_view* dag, int32_t row_begin, int32_t row_end
   out->min_elems = mins;
   out->alg_bytes = 8 * (R + 1) + 20 * m + 4 * wedge;
   out->max_out = mo;
+  std::vector<int32_t> cnt((size_t)dag->n_rows, 0);
+  const int64_t* rp = dag->rowptr;
+  int64_t units = 0, ub = 0;
+  partner_units(R, rp + row_begin, dag->colidx, row_begin, cnt,
+                [rp](int32_t v) { return rp[v + 1] - rp[v]; }, [](int32_t v) { return (int64_t)v; }, &units, &ub);
+  // the owners' lists: 4 d+(x) per unit
+  int64_t setb = 0;
+  const int64_t n = dag->n_rows;
+#pragma omp parallel for reduction(+ : setb)
+  for (int64_t x = 0; x < n; ++x)
+    if (cnt[x]) setb += 4 * ((cnt[x] + TCH_PARTNERS_PER_ROW - 1) / TCH_PARTNERS_PER_ROW) * (rp[x + 1] - rp[x]);
+  out->partner_rows = units;
+  out->partner_bytes = ub + setb + 20 * m + 4 * mins;
   return TC_OK;
 }
 
@@ -243,22 +299,23 @@ tc_status tch_work_of_partition(const tch_partition_view* p, tch_work* out) {
   int64_t m = 0, wedge = 0, mins = 0;
   int32_t mo = 0;
   int64_t missing = 0;
+  auto halo_of = [p](int32_t v) -> int64_t {
+    const int32_t* it = std::lower_bound(p->halo_ids, p->halo_ids + p->n_halo, v);
+    return (it == p->halo_ids + p->n_halo || *it != v) ? -1 : (int64_t)(it - p->halo_ids);
+  };
+  auto deg = [p, b, halo_of](int32_t v) -> int64_t {
+    if (v >= b && v < p->row_end) return p->local_rowptr[v - b + 1] - p->local_rowptr[v - b];
+    const int64_t h = halo_of(v);
+    return h < 0 ? -1 : p->halo_rowptr[h + 1] - p->halo_rowptr[h];
+  };
 #pragma omp parallel for schedule(dynamic, 1024) reduction(+ : m, wedge, mins, missing) reduction(max : mo)
   for (int64_t i = 0; i < nl; ++i) {
     const int64_t du = p->local_rowptr[i + 1] - p->local_rowptr[i];
     m += du;
     if (du > mo) mo = (int32_t)du;
     for (int64_t x = p->local_rowptr[i]; x < p->local_rowptr[i + 1]; ++x) {
-      const int32_t v = p->local_colidx[x];
-      int64_t dv;
-      if (v >= b && v < p->row_end) {
-        dv = p->local_rowptr[v - b + 1] - p->local_rowptr[v - b];
-      } else {
-        const int32_t* it = std::lower_bound(p->halo_ids, p->halo_ids + p->n_halo, v);
-        if (it == p->halo_ids + p->n_halo || *it != v) { ++missing; continue; }
-        const int64_t h = it - p->halo_ids;
-        dv = p->halo_rowptr[h + 1] - p->halo_rowptr[h];
-      }
+      const int64_t dv = deg(p->local_colidx[x]);
+      if (dv < 0) { ++missing; continue; }
       wedge += dv;
       mins += std::min(du, dv);
     }
@@ -270,6 +327,25 @@ tc_status tch_work_of_partition(const tch_partition_view* p, tch_work* out) {
   out->min_elems = mins;
   out->alg_bytes = 8 * ((int64_t)nl + 1) + 20 * m + 4 * wedge;
   out->max_out = mo;
+  // owner slots: local rows [0, nl), then halo lists [nl, nl + n_halo)
+  std::vector<int32_t> cnt((size_t)nl + (size_t)p->n_halo, 0);
+  auto slot = [p, b, nl, halo_of](int32_t v) -> int64_t {
+    if (v >= b && v < p->row_end) return v - b;
+    return nl + halo_of(v);
+  };
+  int64_t units = 0, ub = 0;
+  partner_units(nl, p->local_rowptr, p->local_colidx, b, cnt, deg, slot, &units, &ub);
+  int64_t setb = 0;
+  const int64_t ns = (int64_t)cnt.size();
+#pragma omp parallel for reduction(+ : setb)
+  for (int64_t k = 0; k < ns; ++k) {
+    if (!cnt[k]) continue;
+    const int64_t d = k < nl ? p->local_rowptr[k + 1] - p->local_rowptr[k]
+                             : p->halo_rowptr[k - nl + 1] - p->halo_rowptr[k - nl];
+    setb += 4 * ((cnt[k] + TCH_PARTNERS_PER_ROW - 1) / TCH_PARTNERS_PER_ROW) * d;
+  }
+  out->partner_rows = units;
+  out->partner_bytes = ub + setb + 20 * m + 4 * mins;
   return TC_OK;
 }
 

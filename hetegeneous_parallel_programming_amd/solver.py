@@ -48,6 +48,14 @@ class Tune(ctypes.Structure):
         return t
 
 
+class GraphInfo(ctypes.Structure):
+    _fields_ = [("graphs", nat.c_i32), ("rows", nat.c_i64), ("nnz", nat.c_i64),
+                ("partner_rows", nat.c_i64), ("build_ms", nat.c_dbl), ("bytes", nat.c_i64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class AutotuneReport(ctypes.Structure):
     _fields_ = [("trials", nat.c_i32), ("start_ms", nat.c_dbl), ("best_ms", nat.c_dbl),
                 ("elapsed_ms", nat.c_dbl)]
@@ -109,6 +117,8 @@ def device_lib():
     S(L, "tc_batch_free", nat.c_i32, [P, P])
     S(L, "tc_batch_bytes", nat.c_i64, [P])
     S(L, "tc_batch_graphs", nat.c_i32, [P])
+    S(L, "tc_graph_get_info", nat.c_i32, [P, ctypes.POINTER(GraphInfo)])
+    S(L, "tc_batch_get_info", nat.c_i32, [P, ctypes.POINTER(GraphInfo)])
     S(L, "tc_autotune", nat.c_i32, [P, P, P, nat.c_i32, nat.c_dbl, ctypes.POINTER(Tune),
                                     ctypes.POINTER(AutotuneReport)])
     S(L, "tc_tune_save", nat.c_i32, [P, ctypes.c_char_p, ctypes.c_char_p])
@@ -144,6 +154,12 @@ class DeviceGraph:
     @property
     def bytes(self) -> int:
         return int(device_lib().tc_graph_bytes(self._h))
+
+    def info(self) -> dict:
+        """What the upload built (partner rows, build time, device bytes)."""
+        gi = GraphInfo()
+        _check(device_lib().tc_graph_get_info(self._h, ctypes.byref(gi)))
+        return gi.as_dict()
 
     def free(self) -> None:
         if self._h is not None and self._h.value and self.ctx._h is not None:
@@ -186,6 +202,11 @@ class Batch:
     @property
     def bytes(self) -> int:
         return int(device_lib().tc_batch_bytes(self._h))
+
+    def info(self) -> dict:
+        gi = GraphInfo()
+        _check(device_lib().tc_batch_get_info(self._h, ctypes.byref(gi)))
+        return gi.as_dict()
 
     def free(self) -> None:
         if self._h is not None and self._h.value and self.ctx._h is not None:

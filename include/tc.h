@@ -88,10 +88,29 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g);
 /* Device bytes held by a graph. */
 int64_t tc_graph_bytes(const tc_graph* g);
 
+/* What an upload built.  Besides the DAG, every upload builds on the device
+ * the partner structure the count works from: each DAG edge (u, v) is owned
+ * by u when d+(v) <= d+(u), else by v; the owner's list N+(x) becomes the
+ * LDS set and the other endpoint's list is streamed through it, so an edge
+ * streams min(d+(u), d+(v)) ids (the reference streams d+(v) for every edge,
+ * bs_warp_vertex.cuh:13-22).  An owner's edges are cut into units of at most
+ * TCH_PARTNERS_PER_ROW ("partner rows").  No counterpart in the reference. */
+typedef struct tc_graph_info {
+  int32_t graphs;         /* 1 for a graph, the graph count for a batch          */
+  int64_t rows;           /* DAG rows                                           */
+  int64_t nnz;            /* DAG ids                                            */
+  int64_t partner_rows;   /* partner rows built (units of work of the count)    */
+  double build_ms;        /* device time of the partner build                   */
+  int64_t bytes;          /* device bytes held                                  */
+} tc_graph_info;
+tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out);
+
 /* ---- counting -----------------------------------------------------------
- * tc_count: degree-binned persistent count (8/16/32-lane groups for light
- * pivots, one wave per medium pivot, one workgroup per heavy pivot; pivot
- * list staged in LDS).  Replaces TCSolver's launch + copy-back
+ * tc_count: degree-binned persistent count over the graph's partner rows
+ * (8/16/32-lane groups for light rows, one wave per medium row, one
+ * workgroup per heavy row; the owner's list held in LDS as a hash or a
+ * search tree, the partners' lists streamed through it).  Same total as the
+ * reference's loop nest.  Replaces TCSolver's launch + copy-back
  * (successful_push_pull/multigpu_nvshmem_pushpull.cu:69-83) around
  * warp_vertex (bs_warp_vertex.cuh:2-35). */
 tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
@@ -115,6 +134,7 @@ tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double
 tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b);
 int64_t tc_batch_bytes(const tc_batch* b);
 int32_t tc_batch_graphs(const tc_batch* b);
+tc_status tc_batch_get_info(const tc_batch* b, tc_graph_info* out);
 
 /* ---- autotuning ----------------------------------------------------------
  * No counterpart in the reference, whose launch shape is fixed (one warp per

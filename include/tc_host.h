@@ -136,9 +136,20 @@ typedef struct tch_work {
   int64_t edges;         /* m_R: DAG edges of those rows                       */
   int64_t wedge_elems;   /* sum over (u,v), u in R, of d+(v)                    */
   int64_t min_elems;     /* sum over (u,v) of min(d+(u), d+(v))                 */
-  int64_t alg_bytes;     /* 8*(R+1) + 20*m_R + 4*wedge_elems                     */
+  int64_t alg_bytes;     /* edge-iterator bytes (every N+(v) streamed per edge):
+                            8*(R+1) + 20*m_R + 4*wedge_elems                     */
   int32_t max_out;       /* max d+ over pivot rows                               */
+  int64_t partner_rows;  /* work units of the partner scheme: per owner x,
+                            ceil(c(x) / TCH_PARTNERS_PER_ROW), c(x) = edges it owns */
+  int64_t partner_bytes; /* bytes the partner scheme's count reads with no reuse:
+                            sum over units (36 + 4*d+(x)) + sum over edges
+                            (20 + 4*min(d+(u), d+(v)))                           */
 } tch_work;
+
+/* Partner scheme: edge (u, v) is owned by u when d+(v) <= d+(u), else by v;
+ * the owner's list is the set, the other endpoint's list is streamed.  An
+ * owner's edges are cut into units of at most this many. */
+#define TCH_PARTNERS_PER_ROW 1024
 
 /* Work of counting rows [row_begin, row_end) of a DAG (rows outside use the
  * lists in `dag` directly). */

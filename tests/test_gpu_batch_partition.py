@@ -21,6 +21,16 @@ def test_batch_matches_oracle(ctx):
     b.free()
 
 
+def test_batch_partner_rows_match_host_model(ctx):
+    graphs = Graph.batch(2, 0, 40)
+    b = ctx.batch(graphs)
+    info = b.info()
+    assert info["graphs"] == 40
+    assert info["partner_rows"] == sum(g.work()["partner_rows"] for g in graphs)
+    assert info["nnz"] == sum(g.E() for g in graphs)
+    b.free()
+
+
 def test_batch_with_empty_and_tiny_graphs(ctx):
     gs = [Graph.from_pairs(1, np.zeros(0, np.int32), np.zeros(0, np.int32)).oriented(),
           Graph.from_pairs(3, np.array([0, 1, 2], np.int32), np.array([1, 2, 0], np.int32)).oriented(),
@@ -59,6 +69,7 @@ def test_partition_halo_counts(ctx, k):
                                     part.local_colidx, part.halo_ids, part.halo_rowptr,
                                     part.halo_colidx)
         dg = ctx.upload_partition(part)
+        assert dg.info()["partner_rows"] == part.work()["partner_rows"]
         got, st = ctx.count(dg)
         assert got == want, (p, got, want)
         assert ctx.count_plain(dg)[0] == want

@@ -176,6 +176,17 @@ def test_tune_rejects_bad_layout(ctx):
     assert ctx.tune == saved
 
 
+def test_partner_rows_match_host_model(ctx):
+    """The device-built partner structure has as many rows as the host work
+    model predicts (same owner rule, same cut into units)."""
+    g = Graph.rmat(1 << 16, 16, 5).oriented()
+    dg = ctx.upload(g)
+    info = dg.info()
+    assert info["partner_rows"] == g.work()["partner_rows"]
+    assert info["rows"] == g.V() and info["nnz"] == g.E() and info["build_ms"] >= 0
+    dg.free()
+
+
 def test_upload_rejects_malformed(ctx):
     rp = np.array([0, 2, 3], np.int64)
     with pytest.raises(TcError):

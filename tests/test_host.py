@@ -114,6 +114,44 @@ def test_work_model_matches_numpy():
     assert d.work(lo, hi)["wedge_elems"] == int(deg[ci[sel]].sum())
 
 
+def _partner_model(rp, ci, lo, hi, per_row=1024):
+    """Partner scheme restated in numpy: edge (u, v) is owned by u when
+    d+(v) <= d+(u), else by v; an owner's edges form ceil(c / per_row) units."""
+    deg = np.diff(rp).astype(np.int64)
+    src = np.repeat(np.arange(deg.size), deg)
+    sel = (src >= lo) & (src < hi)
+    s, t = src[sel], ci[sel].astype(np.int64)
+    du, dv = deg[s], deg[t]
+    owner = np.where(dv <= du, s, t)
+    cnt = np.bincount(owner, minlength=deg.size)
+    units = -(-cnt // per_row)
+    byts = 36 * units.sum() + 4 * (units * deg).sum() + 20 * s.size + 4 * np.minimum(du, dv).sum()
+    return int(units.sum()), int(byts), int(cnt.max(initial=0))
+
+
+@pytest.mark.parametrize("lo,hi", [(0, None), (1000, 30000), (0, 1)])
+def test_partner_work_model_matches_numpy(lo, hi):
+    d = Graph.rmat(1 << 16, 16, 5).oriented()
+    rp, ci = d.out_rowptr(), d.out_colidx()
+    hi = d.V() if hi is None else hi
+    units, byts, cmax = _partner_model(rp, ci, lo, hi)
+    w = d.work(lo, hi)
+    assert (w["partner_rows"], w["partner_bytes"]) == (units, byts)
+    if (lo, hi) == (0, d.V()):
+        assert cmax > 1024  # some owners are cut into several units
+        # every edge streams the shorter list: never more than the reference's
+        assert w["min_elems"] <= w["wedge_elems"]
+
+
+def test_partner_work_model_of_partition_matches_rows_of_dag():
+    d = Graph.rmat(1 << 15, 16, 3).oriented()
+    pg = PartitionedGraph(d, 3)
+    for p in range(3):
+        hp = pg.halo_partition(p)
+        a, b = hp.work(), d.work(hp.row_begin, hp.row_end)
+        assert (a["partner_rows"], a["partner_bytes"]) == (b["partner_rows"], b["partner_bytes"])
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 7])
 def test_halo_partition_counts_sum_to_total(k):
     d = Graph.rmat(1 << 14, 16, 7).oriented()

@@ -207,7 +207,7 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
     BuildScratch bs;
     st = TC_OK;
     e = hipEventRecord(ctx->ev[0], ctx->stream);
-    if (e == hipSuccess) st = launch_build(a, b->RP, bs, ctx->stream);
+    if (e == hipSuccess) st = launch_build(a, b->RP, b->P, bs, ctx->stream);
     if (st == TC_OK && e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
     std::vector<int32_t> nv((size_t)n_graphs);
     if (st == TC_OK && e == hipSuccess)

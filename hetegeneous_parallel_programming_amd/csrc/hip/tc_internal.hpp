@@ -56,7 +56,7 @@ struct PlanScratch {
 struct Partners {
   int32_t* pci = nullptr;
   int64_t* vrp = nullptr;
-  int32_t* vrow = nullptr;
+  tck::VRow* vrow = nullptr;
   int32_t* nv = nullptr;
   int64_t bytes = 0;
   tc_status alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs);
@@ -80,9 +80,9 @@ struct BuildScratch {
   static int64_t bytes_for(int64_t rp_slots);
 };
 
-// Builds the partners of the graphs of `a` (descs on the device, pools set)
-// on stream s, with `bs` sized for RP slots.  Stream-ordered, no sync.
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s);
+// Builds the partners of the graphs of `a` (descs on the device, pools set;
+// RP row slots, P = sum of nv_max) on stream s.  Stream-ordered, no sync.
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t P, BuildScratch& bs, hipStream_t s);
 
 // Makes ctx's device current for the scope of one API call.
 struct DeviceGuard {

@@ -79,11 +79,11 @@ __device__ __forceinline__ int tree_pos(int x, int h) {
 }
 
 #ifndef TC_IN_FLIGHT
-#define TC_IN_FLIGHT 6
+#define TC_IN_FLIGHT 8
 #endif
 constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
 #ifndef TC_IN_FLIGHT_GROUP
-#define TC_IN_FLIGHT_GROUP 4
+#define TC_IN_FLIGHT_GROUP 6
 #endif
 // Sub-wave groups keep per-group state in vector registers; fewer lookups
 // in flight leave room for it.
@@ -506,11 +506,17 @@ struct StoreI64 {
 // ---------------------------------------------------------------------------
 // partner build (one thread per DAG row slot of the rowptr pool)
 // ---------------------------------------------------------------------------
-// Pass 1: per row u (owned), partners through out-edges (c1, plain store)
-// and, through atomics, partners through in-edges of the edge's head (c2).
+// Passes 1 and 2 give each DAG row a group of kBuildG lanes that stride its
+// edges (rows average a few tens of ids; a thread per row would serialise
+// the long ones).
+constexpr int kBuildG = 16;
+
+// Pass 1: per row u (owned), partners through out-edges (c1) and, through
+// atomics, partners through in-edges of each edge's head (c2).
 __global__ __launch_bounds__(kBlock) void partner_count_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (q >= RP) return;
+  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
+  const int gl = threadIdx.x % kBuildG;
+  if (q >= RP) return;  // the whole group
   const int g = find_graph_rows(a.descs, a.ngraphs, q);
   const GraphDesc d = a.descs[g];
   const int64_t u = q - d.row_base;
@@ -519,20 +525,23 @@ __global__ __launch_bounds__(kBlock) void partner_count_kernel(LaunchArgs a, int
     const int64_t* rp = a.rp + d.rp_off;
     const int32_t* ci = a.ci + d.ci_off;
     const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
-    for (int64_t e = e0; e < e1; ++e) {
+    for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
       const int v = ci[e];
       if (rp[v + 1] - rp[v] <= du) ++c1;
       else atomicAdd(&b.c2[d.row_base + v], 1);
     }
   }
-  b.c1[q] = c1;
+#pragma unroll
+  for (int o = kBuildG / 2; o >= 1; o >>= 1) c1 += __shfl_xor(c1, o, kBuildG);
+  if (gl == 0) b.c1[q] = c1;
 }
 
 // Pass 2: partner ids.  Out-edge partners of u fill the front of u's range
-// in row order; in-edge partners take the rest in arrival order (c2 is
-// counted down).  Order inside a range does not change any count.
+// in row order (ballot ranks); in-edge partners take the rest in arrival
+// order (c2 is counted down).  Order inside a range does not change any count.
 __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
+  const int gl = threadIdx.x % kBuildG;
   if (q >= RP) return;
   const int g = find_graph_rows(a.descs, a.ngraphs, q);
   const GraphDesc d = a.descs[g];
@@ -545,14 +554,26 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
   const int64_t base = off[0];
   int64_t w = off[u] - base;
   const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
-  for (int64_t e = e0; e < e1; ++e) {
-    const int v = ci[e];
-    if (rp[v + 1] - rp[v] <= du) {
-      pci[w++] = v;
-    } else {
-      const int64_t slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
-      pci[slot] = (int32_t)u;
+  const int shift = (int)(threadIdx.x & 63) & ~(kBuildG - 1);
+  const uint32_t below = (1u << gl) - 1u;
+  for (int64_t eb = e0; eb < e1; eb += kBuildG) {
+    const int64_t e = eb + gl;
+    int v = 0;
+    bool own = false;
+    if (e < e1) {
+      v = ci[e];
+      own = rp[v + 1] - rp[v] <= du;
     }
+    const uint32_t gm = (uint32_t)(__ballot(own) >> shift) & ((1u << kBuildG) - 1u);
+    if (e < e1) {
+      if (own) {
+        pci[w + __popc(gm & below)] = v;
+      } else {
+        const int64_t slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
+        pci[slot] = (int32_t)u;
+      }
+    }
+    w += __popc(gm);
   }
 }
 
@@ -581,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a,
   const int64_t u = q - d.row_base;
   const int64_t pbase = b.off[d.row_base], vbase = b.voff[d.row_base];
   int64_t* vrp = (int64_t*)a.vrp + d.vrp_off;
-  int32_t* vrow = (int32_t*)a.vrow + d.vrow_off;
+  VRow* vrow = (VRow*)a.vrow + d.vrow_off;
   if (u == d.n_rows) {
     const int64_t nv = b.voff[q] - vbase;
     ((int32_t*)a.nv)[g] = (int32_t)nv;
@@ -592,25 +613,53 @@ __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a,
   if (K == 0) return;
   const int64_t v0 = b.voff[q] - vbase, p0 = b.off[q] - pbase, c = b.off[q + 1] - b.off[q];
   for (int32_t k = 0; k < K; ++k) {
-    vrow[v0 + k] = (int32_t)u;
+    vrow[v0 + k] = VRow{(int32_t)u, 0u};
     vrp[v0 + k] = p0 + (c * k) / K;
   }
 }
 
-hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s) {
+// Pass 5: work of every partner row (the sum of its partners' d+), with a
+// group of kBuildG lanes per plan slot.
+__global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
+  const int64_t p = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
+  const int gl = threadIdx.x % kBuildG;
+  if (p >= a.P) return;  // the whole group
+  const int g = find_graph(a.descs, a.ngraphs, p);
+  const GraphDesc d = a.descs[g];
+  const int64_t r = p - d.plan_base;
+  if (r >= a.nv[g]) return;
+  const int64_t* rp = a.rp + d.rp_off;
+  const int32_t* part = a.pci + d.pci_off;
+  const int64_t pb = a.vrp[d.vrp_off + r], pe = a.vrp[d.vrp_off + r + 1];
+  int64_t w = 0;
+  for (int64_t e = pb + gl; e < pe; e += kBuildG) {
+    const int v = part[e];
+    w += rp[v + 1] - rp[v];
+  }
+#pragma unroll
+  for (int o = kBuildG / 2; o >= 1; o >>= 1) w += __shfl_xor(w, o, kBuildG);
+  if (gl == 0) ((VRow*)a.vrow)[d.vrow_off + r].work = (uint32_t)min(w, (int64_t)0xFFFFFFFF);
+}
+
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t P, const BuildBufs& b, hipStream_t s) {
   if (RP <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
+  const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
   hipError_t e = launch_zero(b.c2, RP, s);
   if (e != hipSuccess) return e;
-  partner_count_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  partner_count_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(LoadSum2{b.c1, b.c2}, StoreI64{b.off}, RP, b.part, s)) != hipSuccess) return e;
-  partner_fill_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  partner_fill_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   partner_rows_count_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(LoadI32{b.kc}, StoreI64{b.voff}, RP, b.part, s)) != hipSuccess) return e;
   partner_rows_fill_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  LaunchArgs w = a;
+  w.P = P;
+  if (P > 0) partner_work_kernel<<<(unsigned)((P * kBuildG + kBlock - 1) / kBlock), kBlock, 0, s>>>(w);
   return hipGetLastError();
 }
 
@@ -619,18 +668,10 @@ hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, 
 // ---------------------------------------------------------------------------
 // Bin of a partner row: its set is N+(x) (s ids), its partners' lists hold
 // `work` ids in all.
-__device__ __forceinline__ int choose_bin(const int64_t* rp, const int32_t* part, int64_t np, int64_t s,
-                                          const Tune& t) {
-  if (s < 1 || np < 1) return kNoBin;
+__device__ __forceinline__ int choose_bin(int64_t np, int64_t s, int64_t work, const Tune& t) {
+  if (s < 1 || np < 1 || work == 0) return kNoBin;
   if (s > kCapBlock - 1) return BIN_GLOBAL;
-  if (s > kCapWave - 1) return BIN_BLOCK;
-  int64_t work = 0;
-  for (int64_t e = 0; e < np; ++e) {
-    const int p = part[e];
-    work += rp[p + 1] - rp[p];
-    if (work > t.t_wave) return BIN_BLOCK;
-  }
-  if (work == 0) return kNoBin;
+  if (s > kCapWave - 1 || work > t.t_wave) return BIN_BLOCK;
   if (s > kCapG32 - 1 || work > t.t_g32) return BIN_WAVE;
   if (s > kCapG16 - 1 || work > t.t_g16) return BIN_G32;
   if (s > kCapG8 - 1 || work > t.t_g8) return BIN_G16;
@@ -642,14 +683,17 @@ struct PRow {
   int g;
   int r;       // -1 when the slot is unused
   int x;
+  uint32_t work;
   int64_t pb, pe;
 };
 
 __device__ __forceinline__ PRow partner_row(const LaunchArgs& a, int g, int r) {
   const GraphDesc d = a.descs[g];
-  PRow pr{g, r, 0, 0, 0};
+  PRow pr{g, r, 0, 0u, 0, 0};
   if (r >= a.nv[g]) { pr.r = -1; return pr; }
-  pr.x = a.vrow[d.vrow_off + r];
+  const VRow vr = a.vrow[d.vrow_off + r];
+  pr.x = vr.x;
+  pr.work = vr.work;
   pr.pb = a.vrp[d.vrp_off + r];
   pr.pe = a.vrp[d.vrp_off + r + 1];
   return pr;
@@ -669,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
     int bin = kNoBin;
     if (pr.r >= 0) {
       const int64_t* rp = a.rp + d.rp_off;
-      bin = choose_bin(rp, a.pci + d.pci_off + pr.pb, pr.pe - pr.pb, rp[pr.x + 1] - rp[pr.x], a.tune);
+      bin = choose_bin(pr.pe - pr.pb, rp[pr.x + 1] - rp[pr.x], pr.work, a.tune);
     }
     a.bufs.bin_of[p] = (uint8_t)bin;
     if (bin != kNoBin) atomicAdd(&s_cnt[bin], 1u);
@@ -891,15 +935,18 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(co
     const int da = (int)(rp[u + 1] - rb);
     if (da < 2) continue;
     constexpr int kModes = kModeTree | kModeHbm;
+    constexpr int kPlainInFlight = 6;
     if (da <= kCapWave - 1) {
       stage_tree(L.A[wid], ci + rb, da, lane, 64);
       wave_sync();
       const PivotSet S{L.A[wid], kSetTree, tree_levels(da), 0};
-      acc += pivot_edges<64, kModes>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
+      acc += pivot_edges<64, kModes, kPlainInFlight>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64,
+                                                     L.Base + wid * 64, lane);
       wave_sync();
     } else {
       const PivotSet S{ci + rb, kSetHbm, da, 0};
-      acc += pivot_edges<64, kModes>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64, L.Base + wid * 64, lane);
+      acc += pivot_edges<64, kModes, kPlainInFlight>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64,
+                                                     L.Base + wid * 64, lane);
     }
   }
   const uint64_t tot = block_sum(acc, L.red);

@@ -54,7 +54,8 @@ constexpr int kPartnersPerRow = TCH_PARTNERS_PER_ROW;
 //        their out-edges counted (a partition's owned range), other rows are
 //        lists read through them (halo).
 //   Partners: ids at pci_off of the partner-id pool (as many slots as the
-//        DAG has ids); partner rows at vrp_off / vrow_off of their pools,
+//        DAG has ids); partner rows at vrp_off / vrow_off of their pools
+//        (vrow_off in VRow units),
 //        nv_max + 1 slots, of which the first nv[g] are used (nv is written
 //        on the device; vrp values are relative to pci_off).
 //   Build: the graph's n_rows + 1 slots of a partner build's per-row
@@ -94,6 +95,13 @@ struct PlanBufs {
   uint64_t* outs;       // [ngraphs] u64 counts
 };
 
+// A partner row: its owner and the total length of its partners' lists
+// (saturated at 2^32 - 1; only compared with the bin thresholds).
+struct VRow {
+  int32_t x;
+  uint32_t work;
+};
+
 struct LaunchArgs {
   const GraphDesc* descs;
   int32_t ngraphs;
@@ -102,7 +110,7 @@ struct LaunchArgs {
   const int32_t* ci;    // DAG colidx pool
   const int32_t* pci;   // partner-id pool
   const int64_t* vrp;   // partner-row offsets pool
-  const int32_t* vrow;  // partner-row owners pool
+  const VRow* vrow;     // partner rows pool
   const int32_t* nv;    // [ngraphs] partner rows in use
   Tune tune;
   PlanBufs bufs;
@@ -125,8 +133,8 @@ int64_t scan_partials(int64_t n);
 
 // Builds the partner ids, rows and counts of every graph of `a` (descs,
 // rp, ci as for a count; pci/vrp/vrow/nv are written); RP = sum of
-// n_rows + 1 over the graphs.
-hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s);
+// n_rows + 1 over the graphs, P = sum of nv_max.
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t P, const BuildBufs& b, hipStream_t s);
 
 // Launchers (stream-ordered; return the launch status).
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);

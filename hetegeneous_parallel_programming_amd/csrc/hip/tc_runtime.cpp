@@ -91,7 +91,7 @@ int64_t PlanScratch::bytes_for(int64_t pivots) {
 tc_status Partners::alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
   tc_status st = dev_alloc((void**)&pci, sizeof(int32_t) * (size_t)std::max<int64_t>(ci_slots, 1), "partner ids");
   if (st == TC_OK) st = dev_alloc((void**)&vrp, sizeof(int64_t) * (size_t)std::max<int64_t>(row_slots, 1), "partner rows");
-  if (st == TC_OK) st = dev_alloc((void**)&vrow, sizeof(int32_t) * (size_t)std::max<int64_t>(row_slots, 1), "partner owners");
+  if (st == TC_OK) st = dev_alloc((void**)&vrow, sizeof(tck::VRow) * (size_t)std::max<int64_t>(row_slots, 1), "partner owners");
   if (st == TC_OK) st = dev_alloc((void**)&nv, sizeof(int32_t) * (size_t)std::max(ngraphs, 1), "partner row counts");
   if (st != TC_OK) {
     release();
@@ -110,7 +110,7 @@ void Partners::release() {
 }
 
 int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
-  return 4 * ci_slots + 12 * row_slots + 4 * (int64_t)ngraphs;
+  return 4 * ci_slots + (8 + (int64_t)sizeof(tck::VRow)) * row_slots + 4 * (int64_t)ngraphs;
 }
 
 static int64_t build_words(int64_t rp_slots) {
@@ -148,10 +148,10 @@ void BuildScratch::release() {
   *this = BuildScratch();
 }
 
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s) {
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t P, BuildScratch& bs, hipStream_t s) {
   tc_status st = bs.ensure(RP);
   if (st != TC_OK) return st;
-  TC_HIP_TRY(tck::launch_partners(a, RP, bs.b, s));
+  TC_HIP_TRY(tck::launch_partners(a, RP, P, bs.b, s));
   return TC_OK;
 }
 
@@ -240,7 +240,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   hipStream_t s = ctx->stream;
   e = hipEventRecord(ctx->ev[0], s);
   if (e == hipSuccess) {
-    st = launch_build(a, (int64_t)g->n + 1, bs, s);
+    st = launch_build(a, (int64_t)g->n + 1, g->nv_max, bs, s);
     if (st == TC_OK) e = hipEventRecord(ctx->ev[1], s);
     if (st == TC_OK && e == hipSuccess)
       e = hipMemcpyAsync(ctx->pinned, g->partners.nv, sizeof(int32_t), hipMemcpyDeviceToHost, s);

@@ -34,7 +34,7 @@ struct Layout {
     pci = ci + align_up(sizeof(int32_t) * (uint64_t)nnz);
     vrp = pci + align_up(sizeof(int32_t) * (uint64_t)nnz);
     vrow = vrp + align_up(sizeof(int64_t) * ((uint64_t)nvm + 1));
-    total = vrow + align_up(sizeof(int32_t) * ((uint64_t)nvm + 1));
+    total = vrow + align_up(sizeof(tck::VRow) * ((uint64_t)nvm + 1));
   }
 };
 
@@ -200,7 +200,7 @@ bool tc_server::launch() {
       hd[i] = tck::GraphDesc{(int64_t)(r->off / sizeof(int64_t)), (int64_t)((r->off + L.ci) / sizeof(int32_t)),
                              (int64_t)((r->off + L.pci) / sizeof(int32_t)),
                              (int64_t)((r->off + L.vrp) / sizeof(int64_t)),
-                             (int64_t)((r->off + L.vrow) / sizeof(int32_t)), rb, pb, r->n, r->nv_max, 0, r->n};
+                             (int64_t)((r->off + L.vrow) / sizeof(tck::VRow)), rb, pb, r->n, r->nv_max, 0, r->n};
       pb += r->nv_max;
       rb += (int64_t)r->n + 1;
     }
@@ -212,7 +212,7 @@ bool tc_server::launch() {
     a.ci = (const int32_t*)pool;
     a.pci = (const int32_t*)pool;
     a.vrp = (const int64_t*)pool;
-    a.vrow = (const int32_t*)pool;
+    a.vrow = (const tck::VRow*)pool;
     a.nv = s.d_nv;
     {
       std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -223,7 +223,7 @@ bool tc_server::launch() {
     for (Req* r : s.reqs) r->t_launch = t;
     hipError_t e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
     // Partner structures of this launch's graphs, then plan and count.
-    if (e == hipSuccess) e = s.build.ensure(rb) == TC_OK ? tck::launch_partners(a, rb, s.build.b, s.stream)
+    if (e == hipSuccess) e = s.build.ensure(rb) == TC_OK ? tck::launch_partners(a, rb, pb, s.build.b, s.stream)
                                                          : hipErrorOutOfMemory;
     // A batch of one graph would take the single-graph reduction path.
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s.stream);

@@ -391,7 +391,7 @@ def leg_headline(ctx, args) -> dict:
                      "kernel": "count_kernel<bin> (6 per-bin persistent launches per step)",
                      "alg_bytes": work["partner_bytes"],
                      "alg_bytes_note": "bytes the partner-scheme count reads with no reuse: per partner row "
-                                       "36 + 4 d+(x), per edge 20 + 4 min(d+(u), d+(v))",
+                                       "32 + 4 d+(x), per edge 8 + 4 min(d+(u), d+(v))",
                      "alg_bytes_edge_iterator": work["alg_bytes"],
                      "edge_iterator_GBps": best["edge_iterator_GBps"],
                      "kernel_ms_avg": best["kernel_ms_avg"]},

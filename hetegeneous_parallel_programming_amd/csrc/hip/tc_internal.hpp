@@ -54,7 +54,7 @@ struct PlanScratch {
 // Partner structure of pooled graphs (see tck::GraphDesc): partner ids (as
 // many slots as the colidx pool), partner rows, rows in use per graph.
 struct Partners {
-  int32_t* pci = nullptr;
+  tck::ListRef* pci = nullptr;
   int64_t* vrp = nullptr;
   tck::VRow* vrow = nullptr;
   int32_t* nv = nullptr;

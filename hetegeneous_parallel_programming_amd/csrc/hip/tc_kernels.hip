@@ -204,7 +204,9 @@ template <int G>
 __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
                                                     int da, int gl, const Tune& tune) {
   PivotSet S{slot, kSetTree, 0, 0};
-  const int lb = tune.tree_only ? 0 : hash_log2_buckets(da, area);
+  // tree_only: 1 = every set a tree; 2 = trees for sub-wave groups only
+  const bool tree = tune.tree_only == 1 || (tune.tree_only == 2 && G < 64);
+  const int lb = tree ? 0 : hash_log2_buckets(da, area);
   if (lb > 0) {
     const int B = 1 << lb;
     for (int x = gl; x < 4 * B; x += G) slot[x] = kEmpty;
@@ -233,10 +235,28 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
 // element t of the int32 array at sBase[k] (N+(v) shifted back by the edge's
 // first flat index); each lane keeps its current edge's end and base in
 // registers and searches sEnd only when it steps past that edge.
-template <int G, int kModes, int NF = kInFlight>
-__device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
-                                                const int32_t* __restrict__ ci,
-                                                const int32_t* __restrict__ elist, int da,
+// Where the streamed lists of a pivot's edges come from: ids of a DAG row
+// (one rowptr lookup each), or partner list references.
+struct DagEdges {
+  const int64_t* rp;
+  const int32_t* ids;
+  __device__ __forceinline__ void operator()(int j, int64_t& b, int& c) const {
+    const int v = ids[j];
+    b = rp[v];
+    c = (int)(rp[v + 1] - b);
+  }
+};
+struct RefEdges {
+  const ListRef* refs;
+  __device__ __forceinline__ void operator()(int j, int64_t& b, int& c) const {
+    const ListRef r = refs[j];
+    b = list_start(r);
+    c = list_len(r);
+  }
+};
+
+template <int G, int kModes, int NF = kInFlight, class Src>
+__device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, const Src src, int da,
                                                 const PivotSet S, int jfirst, int jstride,
                                                 int32_t* sEnd, gint32** sBase, int gl) {
   uint32_t hits = 0;
@@ -246,9 +266,8 @@ __device__ __forceinline__ uint32_t pivot_edges(const int64_t* __restrict__ rp,
     int c = 0;
     gint32* base = (gint32*)ci;
     if (j < da) {
-      const int v = elist[j];
-      const int64_t b = rp[v];
-      c = (int)(rp[v + 1] - b);
+      int64_t b;
+      src(j, b, c);
       base = (gint32*)ci + b;
     }
     int total;
@@ -549,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
   if (u < d.own_begin || u >= d.own_end) return;
   const int64_t* rp = a.rp + d.rp_off;
   const int32_t* ci = a.ci + d.ci_off;
-  int32_t* pci = (int32_t*)a.pci + d.pci_off;
+  ListRef* pci = (ListRef*)a.pci + d.pci_off;
   const int64_t* off = b.off + d.row_base;
   const int64_t base = off[0];
   int64_t w = off[u] - base;
@@ -559,18 +578,21 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
   for (int64_t eb = e0; eb < e1; eb += kBuildG) {
     const int64_t e = eb + gl;
     int v = 0;
+    int64_t vb = 0, dv = 0;
     bool own = false;
     if (e < e1) {
       v = ci[e];
-      own = rp[v + 1] - rp[v] <= du;
+      vb = rp[v];
+      dv = rp[v + 1] - vb;
+      own = dv <= du;
     }
     const uint32_t gm = (uint32_t)(__ballot(own) >> shift) & ((1u << kBuildG) - 1u);
     if (e < e1) {
       if (own) {
-        pci[w + __popc(gm & below)] = v;
+        pci[w + __popc(gm & below)] = list_ref(vb, dv);
       } else {
         const int64_t slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
-        pci[slot] = (int32_t)u;
+        pci[slot] = list_ref(e0, du);
       }
     }
     w += __popc(gm);
@@ -612,8 +634,10 @@ __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a,
   const int32_t K = b.kc[q];
   if (K == 0) return;
   const int64_t v0 = b.voff[q] - vbase, p0 = b.off[q] - pbase, c = b.off[q + 1] - b.off[q];
+  const int64_t* rp = a.rp + d.rp_off;
+  const ListRef set = list_ref(rp[u], rp[u + 1] - rp[u]);
   for (int32_t k = 0; k < K; ++k) {
-    vrow[v0 + k] = VRow{(int32_t)u, 0u};
+    vrow[v0 + k] = VRow{set, 0u, (int32_t)u};
     vrp[v0 + k] = p0 + (c * k) / K;
   }
 }
@@ -628,14 +652,10 @@ __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
   const GraphDesc d = a.descs[g];
   const int64_t r = p - d.plan_base;
   if (r >= a.nv[g]) return;
-  const int64_t* rp = a.rp + d.rp_off;
-  const int32_t* part = a.pci + d.pci_off;
+  const ListRef* part = a.pci + d.pci_off;
   const int64_t pb = a.vrp[d.vrp_off + r], pe = a.vrp[d.vrp_off + r + 1];
   int64_t w = 0;
-  for (int64_t e = pb + gl; e < pe; e += kBuildG) {
-    const int v = part[e];
-    w += rp[v + 1] - rp[v];
-  }
+  for (int64_t e = pb + gl; e < pe; e += kBuildG) w += list_len(part[e]);
 #pragma unroll
   for (int o = kBuildG / 2; o >= 1; o >>= 1) w += __shfl_xor(w, o, kBuildG);
   if (gl == 0) ((VRow*)a.vrow)[d.vrow_off + r].work = (uint32_t)min(w, (int64_t)0xFFFFFFFF);
@@ -682,7 +702,7 @@ __device__ __forceinline__ int choose_bin(int64_t np, int64_t s, int64_t work, c
 struct PRow {
   int g;
   int r;       // -1 when the slot is unused
-  int x;
+  ListRef set;
   uint32_t work;
   int64_t pb, pe;
 };
@@ -692,7 +712,7 @@ __device__ __forceinline__ PRow partner_row(const LaunchArgs& a, int g, int r) {
   PRow pr{g, r, 0, 0u, 0, 0};
   if (r >= a.nv[g]) { pr.r = -1; return pr; }
   const VRow vr = a.vrow[d.vrow_off + r];
-  pr.x = vr.x;
+  pr.set = vr.set;
   pr.work = vr.work;
   pr.pb = a.vrp[d.vrp_off + r];
   pr.pe = a.vrp[d.vrp_off + r + 1];
@@ -711,10 +731,7 @@ __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
     const GraphDesc d = a.descs[g];
     const PRow pr = partner_row(a, g, (int)(p - d.plan_base));
     int bin = kNoBin;
-    if (pr.r >= 0) {
-      const int64_t* rp = a.rp + d.rp_off;
-      bin = choose_bin(pr.pe - pr.pb, rp[pr.x + 1] - rp[pr.x], pr.work, a.tune);
-    }
+    if (pr.r >= 0) bin = choose_bin(pr.pe - pr.pb, list_len(pr.set), pr.work, a.tune);
     a.bufs.bin_of[p] = (uint8_t)bin;
     if (bin != kNoBin) atomicAdd(&s_cnt[bin], 1u);
   }
@@ -802,13 +819,10 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
     const int g = (int)(item >> 32);
     const PRow pr = partner_row(a, g, (int)(uint32_t)item);
     const GraphDesc d = a.descs[g];
-    const int64_t* rp = a.rp + d.rp_off;
     const int32_t* ci = a.ci + d.ci_off;
-    const int64_t rb = rp[pr.x];
-    const int ds = (int)(rp[pr.x + 1] - rb);
-    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + rb, ds, gl, a.tune);
+    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + list_start(pr.set), list_len(pr.set), gl, a.tune);
     const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup>(
-        rp, ci, a.pci + d.pci_off + pr.pb, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl);
+        ci, RefEdges{a.pci + d.pci_off + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl);
     acc.to_graph(g, batch, a.bufs.outs);
     acc.sum += h;
     wave_sync();
@@ -821,22 +835,21 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int g = (int)(item >> 32);
   const PRow pr = partner_row(a, g, (int)(uint32_t)item);
   const GraphDesc d = a.descs[g];
-  const int64_t* rp = a.rp + d.rp_off;
   const int32_t* ci = a.ci + d.ci_off;
-  const int64_t rb = rp[pr.x];
-  const int ds = (int)(rp[pr.x + 1] - rb);
-  const int32_t* part = a.pci + d.pci_off + pr.pb;
+  const int32_t* set = ci + list_start(pr.set);
+  const int ds = list_len(pr.set);
+  const RefEdges part{a.pci + d.pci_off + pr.pb};
   const int np = (int)(pr.pe - pr.pb);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t h;
   if (kLdsPivot) {
-    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, ci + rb, ds, threadIdx.x, a.tune);
-    h = pivot_edges<64, kModeHash | kModeTree>(rp, ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
+    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune);
+    h = pivot_edges<64, kModeHash | kModeTree>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
                                                L.Base + wid * 64, lane);
   } else {
-    const PivotSet S{ci + rb, kSetHbm, ds, 0};
-    h = pivot_edges<64, kModeHbm>(rp, ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
-                                  L.Base + wid * 64, lane);
+    const PivotSet S{set, kSetHbm, ds, 0};
+    h = pivot_edges<64, kModeHbm>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64, L.Base + wid * 64,
+                                  lane);
   }
   acc.to_graph(g, batch, a.bufs.outs);
   acc.sum += h;
@@ -940,12 +953,12 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(co
       stage_tree(L.A[wid], ci + rb, da, lane, 64);
       wave_sync();
       const PivotSet S{L.A[wid], kSetTree, tree_levels(da), 0};
-      acc += pivot_edges<64, kModes, kPlainInFlight>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64,
+      acc += pivot_edges<64, kModes, kPlainInFlight>(ci, DagEdges{rp, ci + rb}, da, S, 0, 64, L.End + wid * 64,
                                                      L.Base + wid * 64, lane);
       wave_sync();
     } else {
       const PivotSet S{ci + rb, kSetHbm, da, 0};
-      acc += pivot_edges<64, kModes, kPlainInFlight>(rp, ci, ci + rb, da, S, 0, 64, L.End + wid * 64,
+      acc += pivot_edges<64, kModes, kPlainInFlight>(ci, DagEdges{rp, ci + rb}, da, S, 0, 64, L.End + wid * 64,
                                                      L.Base + wid * 64, lane);
     }
   }

@@ -53,8 +53,9 @@ constexpr int kPartnersPerRow = TCH_PARTNERS_PER_ROW;
 //        the colidx pool), n_rows rows; only rows [own_begin, own_end) have
 //        their out-edges counted (a partition's owned range), other rows are
 //        lists read through them (halo).
-//   Partners: ids at pci_off of the partner-id pool (as many slots as the
-//        DAG has ids); partner rows at vrp_off / vrow_off of their pools
+//   Partners: the partners' lists (ListRef) at pci_off of the partner pool
+//        (as many slots as the DAG has ids); partner rows at vrp_off /
+//        vrow_off of their pools
 //        (vrow_off in VRow units),
 //        nv_max + 1 slots, of which the first nv[g] are used (nv is written
 //        on the device; vrp values are relative to pci_off).
@@ -82,7 +83,7 @@ inline int32_t partner_rows_max(int64_t n, int64_t m) {
 
 struct Tune {
   int32_t t_g8, t_g16, t_g32, t_wave, rounds;
-  int32_t tree_only;   // 1: pivot sets are always LDS trees
+  int32_t tree_only;   // 1: pivot sets are always LDS trees; 2: for sub-wave groups
   int32_t hash_kicks;  // evictions per hash insert before falling back to the tree
 };
 
@@ -95,11 +96,24 @@ struct PlanBufs {
   uint64_t* outs;       // [ngraphs] u64 counts
 };
 
-// A partner row: its owner and the total length of its partners' lists
-// (saturated at 2^32 - 1; only compared with the bin thresholds).
+// A list of the DAG as one word: colidx offset (relative to the graph's
+// colidx) in the high 39 bits, length in the low 25 (out-degrees are below
+// 2^25 by the upload checks).
+typedef uint64_t ListRef;
+constexpr int kLenBits = 25;
+__host__ __device__ inline ListRef list_ref(int64_t start, int64_t len) {
+  return ((uint64_t)start << kLenBits) | (uint64_t)len;
+}
+__host__ __device__ inline int64_t list_start(ListRef r) { return (int64_t)(r >> kLenBits); }
+__host__ __device__ inline int32_t list_len(ListRef r) { return (int32_t)(r & ((1u << kLenBits) - 1)); }
+
+// A partner row: the owner's list (the set), the total length of its
+// partners' lists (saturated at 2^32 - 1; only compared with the bin
+// thresholds) and the owner's id.
 struct VRow {
-  int32_t x;
+  ListRef set;
   uint32_t work;
+  int32_t x;
 };
 
 struct LaunchArgs {
@@ -108,7 +122,7 @@ struct LaunchArgs {
   int64_t P;            // plan indices over all graphs (sum of nv_max)
   const int64_t* rp;    // DAG rowptr pool
   const int32_t* ci;    // DAG colidx pool
-  const int32_t* pci;   // partner-id pool
+  const ListRef* pci;   // partner pool: each partner's list
   const int64_t* vrp;   // partner-row offsets pool
   const VRow* vrow;     // partner rows pool
   const int32_t* nv;    // [ngraphs] partner rows in use

@@ -89,7 +89,7 @@ int64_t PlanScratch::bytes_for(int64_t pivots) {
 }
 
 tc_status Partners::alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
-  tc_status st = dev_alloc((void**)&pci, sizeof(int32_t) * (size_t)std::max<int64_t>(ci_slots, 1), "partner ids");
+  tc_status st = dev_alloc((void**)&pci, sizeof(tck::ListRef) * (size_t)std::max<int64_t>(ci_slots, 1), "partner lists");
   if (st == TC_OK) st = dev_alloc((void**)&vrp, sizeof(int64_t) * (size_t)std::max<int64_t>(row_slots, 1), "partner rows");
   if (st == TC_OK) st = dev_alloc((void**)&vrow, sizeof(tck::VRow) * (size_t)std::max<int64_t>(row_slots, 1), "partner owners");
   if (st == TC_OK) st = dev_alloc((void**)&nv, sizeof(int32_t) * (size_t)std::max(ngraphs, 1), "partner row counts");
@@ -110,7 +110,8 @@ void Partners::release() {
 }
 
 int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
-  return 4 * ci_slots + (8 + (int64_t)sizeof(tck::VRow)) * row_slots + 4 * (int64_t)ngraphs;
+  return (int64_t)sizeof(tck::ListRef) * ci_slots + (8 + (int64_t)sizeof(tck::VRow)) * row_slots +
+         4 * (int64_t)ngraphs;
 }
 
 static int64_t build_words(int64_t rp_slots) {
@@ -173,7 +174,7 @@ tc_status check_tune(const tc_tune& t) {
     return fail(TC_EINVAL, "tune thresholds must satisfy 0 <= t_g8 <= t_g16 <= t_g32 <= t_wave");
   if (t.rounds < 1 || t.rounds > 64) return fail(TC_EINVAL, "tune rounds must be in [1, 64]");
   if (t.grid < 0 || t.grid > (1 << 20)) return fail(TC_EINVAL, "tune grid must be in [0, 2^20]");
-  if (t.pivot_set < 0 || t.pivot_set > 1) return fail(TC_EINVAL, "tune pivot_set must be 0 or 1");
+  if (t.pivot_set < 0 || t.pivot_set > 2) return fail(TC_EINVAL, "tune pivot_set must be 0, 1 or 2");
   if (t.hash_kicks < 1 || t.hash_kicks > 1024) return fail(TC_EINVAL, "tune hash_kicks must be in [1, 1024]");
   return TC_OK;
 }

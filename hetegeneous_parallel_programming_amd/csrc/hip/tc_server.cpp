@@ -32,7 +32,7 @@ struct Layout {
     const int64_t nvm = tck::partner_rows_max(n, nnz);
     ci = align_up(sizeof(int64_t) * ((uint64_t)n + 1));
     pci = ci + align_up(sizeof(int32_t) * (uint64_t)nnz);
-    vrp = pci + align_up(sizeof(int32_t) * (uint64_t)nnz);
+    vrp = pci + align_up(sizeof(tck::ListRef) * (uint64_t)nnz);
     vrow = vrp + align_up(sizeof(int64_t) * ((uint64_t)nvm + 1));
     total = vrow + align_up(sizeof(tck::VRow) * ((uint64_t)nvm + 1));
   }
@@ -198,7 +198,7 @@ bool tc_server::launch() {
       Req* r = s.reqs[i];
       const Layout L(r->n, r->nnz);
       hd[i] = tck::GraphDesc{(int64_t)(r->off / sizeof(int64_t)), (int64_t)((r->off + L.ci) / sizeof(int32_t)),
-                             (int64_t)((r->off + L.pci) / sizeof(int32_t)),
+                             (int64_t)((r->off + L.pci) / sizeof(tck::ListRef)),
                              (int64_t)((r->off + L.vrp) / sizeof(int64_t)),
                              (int64_t)((r->off + L.vrow) / sizeof(tck::VRow)), rb, pb, r->n, r->nv_max, 0, r->n};
       pb += r->nv_max;
@@ -210,7 +210,7 @@ bool tc_server::launch() {
     a.P = pb;
     a.rp = (const int64_t*)pool;
     a.ci = (const int32_t*)pool;
-    a.pci = (const int32_t*)pool;
+    a.pci = (const tck::ListRef*)pool;
     a.vrp = (const int64_t*)pool;
     a.vrow = (const tck::VRow*)pool;
     a.nv = s.d_nv;

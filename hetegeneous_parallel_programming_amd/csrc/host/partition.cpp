@@ -242,7 +242,7 @@ static void partner_units(int64_t nl, const int64_t* rowptr, const int32_t* coli
     if (!c) continue;
     const int64_t K = (c + TCH_PARTNERS_PER_ROW - 1) / TCH_PARTNERS_PER_ROW;
     u += K;
-    b += K * 36;
+    b += K * 32;
   }
   *units = u;
   *unit_bytes = b;
@@ -289,7 +289,7 @@ tc_status tch_work_of(const tc_csr_view* dag, int32_t row_begin, int32_t row_end
   for (int64_t x = 0; x < n; ++x)
     if (cnt[x]) setb += 4 * ((cnt[x] + TCH_PARTNERS_PER_ROW - 1) / TCH_PARTNERS_PER_ROW) * (rp[x + 1] - rp[x]);
   out->partner_rows = units;
-  out->partner_bytes = ub + setb + 20 * m + 4 * mins;
+  out->partner_bytes = ub + setb + 8 * m + 4 * mins;
   return TC_OK;
 }
 
@@ -345,7 +345,7 @@ tc_status tch_work_of_partition(const tch_partition_view* p, tch_work* out) {
     setb += 4 * ((cnt[k] + TCH_PARTNERS_PER_ROW - 1) / TCH_PARTNERS_PER_ROW) * d;
   }
   out->partner_rows = units;
-  out->partner_bytes = ub + setb + 20 * m + 4 * mins;
+  out->partner_bytes = ub + setb + 8 * m + 4 * mins;
   return TC_OK;
 }
 

@@ -42,7 +42,8 @@ typedef struct tc_tune {
   int32_t rounds;  /* pivots per group in one light-bin tile                          */
   int32_t grid;    /* persistent grid in workgroups (0 = full residency)             */
   int32_t pivot_set;  /* LDS layout of N+(u): 0 = cuckoo hash where it fits, else the
-                         binary-search tree; 1 = always the tree                      */
+                         binary-search tree; 1 = always the tree; 2 = the tree for
+                         8/16/32-lane groups, the hash for waves and workgroups      */
   int32_t hash_kicks; /* evictions allowed per insert before a hash build gives up and
                          the pivot falls back to the tree (1..1024)                   */
 } tc_tune;

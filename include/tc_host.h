@@ -142,8 +142,9 @@ typedef struct tch_work {
   int64_t partner_rows;  /* work units of the partner scheme: per owner x,
                             ceil(c(x) / TCH_PARTNERS_PER_ROW), c(x) = edges it owns */
   int64_t partner_bytes; /* bytes the partner scheme's count reads with no reuse:
-                            sum over units (36 + 4*d+(x)) + sum over edges
-                            (20 + 4*min(d+(u), d+(v)))                           */
+                            per unit its 16-byte row record, 16 bytes of row
+                            offsets and the owner's list (4*d+(x)); per edge an
+                            8-byte list reference and 4*min(d+(u), d+(v))        */
 } tch_work;
 
 /* Partner scheme: edge (u, v) is owned by u when d+(v) <= d+(u), else by v;

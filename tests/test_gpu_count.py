@@ -127,6 +127,7 @@ def test_tune_changes_schedule_not_result(ctx):
 
 
 LAYOUTS = [{"pivot_set": 1},                   # every pivot set an LDS tree
+           {"pivot_set": 2},                   # trees for sub-wave groups, hashes above
            {"pivot_set": 0, "hash_kicks": 1},   # most hash builds give up: tree fallback
            {"pivot_set": 0, "hash_kicks": 3},
            {"pivot_set": 0, "hash_kicks": 128},
@@ -170,7 +171,7 @@ def test_pivot_set_layouts_same_count(ctx, layout):
 
 def test_tune_rejects_bad_layout(ctx):
     saved = ctx.tune
-    for bad in ({"pivot_set": 2}, {"hash_kicks": 0}, {"hash_kicks": 5000}):
+    for bad in ({"pivot_set": 3}, {"pivot_set": -1}, {"hash_kicks": 0}, {"hash_kicks": 5000}):
         with pytest.raises(TcError):
             ctx.tune = bad
     assert ctx.tune == saved

@@ -125,7 +125,7 @@ def _partner_model(rp, ci, lo, hi, per_row=1024):
     owner = np.where(dv <= du, s, t)
     cnt = np.bincount(owner, minlength=deg.size)
     units = -(-cnt // per_row)
-    byts = 36 * units.sum() + 4 * (units * deg).sum() + 20 * s.size + 4 * np.minimum(du, dv).sum()
+    byts = 32 * units.sum() + 4 * (units * deg).sum() + 8 * s.size + 4 * np.minimum(du, dv).sum()
     return int(units.sum()), int(byts), int(cnt.max(initial=0))
 
 

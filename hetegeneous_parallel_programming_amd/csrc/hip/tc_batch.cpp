@@ -99,7 +99,7 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
         return fail(TC_ERANGE, "graph " + std::to_string(i) + ": DAG out-degree too large");
   }
   int64_t tot_slots = 0;
-  for (int32_t i = 0; i < n_graphs; ++i) tot_slots += tck::partner_rows_max(dags[i].n_rows, dags[i].nnz);
+  for (int32_t i = 0; i < n_graphs; ++i) tot_slots += (int64_t)tck::partner_rows_max(dags[i].n_rows, dags[i].nnz) + 1;
   if (tot_slots >= ((int64_t)1 << 31)) return fail(TC_ERANGE, "batch has 2^31 or more pivots");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
@@ -108,17 +108,16 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
   b->n = n_graphs;
   b->descs.resize(n_graphs);
   b->rows.resize(n_graphs);
-  int64_t rpo = 0, cio = 0, pro = 0, pb = 0;
+  int64_t rpo = 0, cio = 0, pro = 0;
   for (int32_t i = 0; i < n_graphs; ++i) {
     const int32_t nvm = tck::partner_rows_max(dags[i].n_rows, dags[i].nnz);
-    b->descs[i] = tck::GraphDesc{rpo, cio, cio, pro, pro, rpo, pb, dags[i].n_rows, nvm, 0, dags[i].n_rows};
+    b->descs[i] = tck::GraphDesc{rpo, cio, cio, rpo, pro, dags[i].n_rows, nvm, 0, dags[i].n_rows};
     b->rows[i] = dags[i].n_rows;
     rpo += dags[i].n_rows + 1;
     cio += dags[i].nnz;
     pro += (int64_t)nvm + 1;
-    pb += nvm;
   }
-  b->P = pb;
+  b->P = pro;
   b->RP = rpo;
   b->cio = cio;
   tc_status st = dev_alloc((void**)&b->rp, sizeof(int64_t) * (size_t)std::max<int64_t>(rpo, 1), "batch rowptr");
@@ -203,11 +202,12 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
     a.ngraphs = b->n;
     a.rp = b->rp;
     a.ci = b->ci;
+    a.P = b->P;
     b->partners.into(a);
     BuildScratch bs;
     st = TC_OK;
     e = hipEventRecord(ctx->ev[0], ctx->stream);
-    if (e == hipSuccess) st = launch_build(a, b->RP, b->P, bs, ctx->stream);
+    if (e == hipSuccess) st = launch_build(a, b->RP, bs, ctx->stream);
     if (st == TC_OK && e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
     std::vector<int32_t> nv((size_t)n_graphs);
     if (st == TC_OK && e == hipSuccess)

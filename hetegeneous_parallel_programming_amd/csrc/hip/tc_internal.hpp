@@ -51,8 +51,9 @@ struct PlanScratch {
   static int64_t bytes_for(int64_t pivots);
 };
 
-// Partner structure of pooled graphs (see tck::GraphDesc): partner ids (as
-// many slots as the colidx pool), partner rows, rows in use per graph.
+// Partner structure of pooled graphs (see tck::GraphDesc): partner list
+// references (as many as the colidx pool has ids), partner-row slots (and
+// one more row offset), rows in use per graph.
 struct Partners {
   tck::ListRef* pci = nullptr;
   int64_t* vrp = nullptr;
@@ -60,6 +61,7 @@ struct Partners {
   int32_t* nv = nullptr;
   int64_t bytes = 0;
   tc_status alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs);
+  static int64_t vrow_bytes(int64_t row_slots) { return (int64_t)sizeof(tck::VRow) * row_slots; }
   void release();
   void into(tck::LaunchArgs& a) const {
     a.pci = pci;
@@ -80,9 +82,9 @@ struct BuildScratch {
   static int64_t bytes_for(int64_t rp_slots);
 };
 
-// Builds the partners of the graphs of `a` (descs on the device, pools set;
-// RP row slots, P = sum of nv_max) on stream s.  Stream-ordered, no sync.
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t P, BuildScratch& bs, hipStream_t s);
+// Builds the partners of the graphs of `a` (descs on the device, pools and
+// P set; RP row slots) on stream s.  Stream-ordered, no sync.
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s);
 
 // Makes ctx's device current for the scope of one API call.
 struct DeviceGuard {
@@ -127,7 +129,7 @@ struct tc_graph {
   int32_t row_begin = 0, row_end = 0;
   int32_t max_out = 0;
   tcr::Partners partners;
-  int32_t nv_max = 0;   // partner-row slots (plan indices)
+  int32_t nv_max = 0;   // partner rows it may need (nv_max + 1 slots = plan indices)
   int32_t nv = 0;       // partner rows in use
   double build_ms = 0;  // partner build time on the device
   tck::GraphDesc* d_desc = nullptr;

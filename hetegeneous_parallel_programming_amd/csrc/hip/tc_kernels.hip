@@ -383,16 +383,6 @@ __global__ void zero_kernel(uint32_t* p, int64_t n) {
 // ---------------------------------------------------------------------------
 // planning: bin every pivot, then scatter (graph, row) items grouped by bin
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int find_graph(const GraphDesc* descs, int ngraphs, int64_t p) {
-  int lo = 0, hi = ngraphs - 1;  // largest g with plan_base <= p
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].plan_base <= p) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // Largest g with row_base <= q (build row slots are in graph order).
 __device__ __forceinline__ int find_graph_rows(const GraphDesc* descs, int ngraphs, int64_t q) {
   int lo = 0, hi = ngraphs - 1;
@@ -589,10 +579,10 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
     const uint32_t gm = (uint32_t)(__ballot(own) >> shift) & ((1u << kBuildG) - 1u);
     if (e < e1) {
       if (own) {
-        pci[w + __popc(gm & below)] = list_ref(vb, dv);
+        pci[w + __popc(gm & below)] = list_ref(d.ci_off + vb, dv);
       } else {
         const int64_t slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
-        pci[slot] = list_ref(e0, du);
+        pci[slot] = list_ref(d.ci_off + e0, du);
       }
     }
     w += __popc(gm);
@@ -614,8 +604,9 @@ __global__ __launch_bounds__(kBlock) void partner_rows_count_kernel(LaunchArgs a
   b.kc[q] = k;
 }
 
-// Pass 4: partner rows (owner, partner range), split evenly by partner
-// count; the graph's row-count slot also writes nv[g] and the final end.
+// Pass 4: partner rows (owner's list, partner range), split evenly by
+// partner count; the graph's row-count slot also writes nv[g] and the end
+// of its last row.  Row offsets are absolute in the partner pool.
 __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= RP) return;
@@ -623,49 +614,47 @@ __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a,
   const GraphDesc d = a.descs[g];
   const int64_t u = q - d.row_base;
   const int64_t pbase = b.off[d.row_base], vbase = b.voff[d.row_base];
-  int64_t* vrp = (int64_t*)a.vrp + d.vrp_off;
-  VRow* vrow = (VRow*)a.vrow + d.vrow_off;
+  int64_t* vrp = (int64_t*)a.vrp + d.pr_base;
+  VRow* vrow = (VRow*)a.vrow + d.pr_base;
   if (u == d.n_rows) {
     const int64_t nv = b.voff[q] - vbase;
     ((int32_t*)a.nv)[g] = (int32_t)nv;
-    vrp[nv] = b.off[q] - pbase;
+    vrp[nv] = d.pci_off + b.off[q] - pbase;
     return;
   }
   const int32_t K = b.kc[q];
   if (K == 0) return;
-  const int64_t v0 = b.voff[q] - vbase, p0 = b.off[q] - pbase, c = b.off[q + 1] - b.off[q];
+  const int64_t v0 = b.voff[q] - vbase, p0 = d.pci_off + b.off[q] - pbase, c = b.off[q + 1] - b.off[q];
   const int64_t* rp = a.rp + d.rp_off;
-  const ListRef set = list_ref(rp[u], rp[u + 1] - rp[u]);
+  const ListRef set = list_ref(d.ci_off + rp[u], rp[u + 1] - rp[u]);
   for (int32_t k = 0; k < K; ++k) {
-    vrow[v0 + k] = VRow{set, 0u, (int32_t)u};
+    vrow[v0 + k] = VRow{set, 0u, g};
     vrp[v0 + k] = p0 + (c * k) / K;
   }
 }
 
-// Pass 5: work of every partner row (the sum of its partners' d+), with a
-// group of kBuildG lanes per plan slot.
+// Pass 5: work of every partner row (the sum of its partners' list
+// lengths), with a group of kBuildG lanes per slot.
 __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
   const int64_t p = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
   const int gl = threadIdx.x % kBuildG;
   if (p >= a.P) return;  // the whole group
-  const int g = find_graph(a.descs, a.ngraphs, p);
-  const GraphDesc d = a.descs[g];
-  const int64_t r = p - d.plan_base;
-  if (r >= a.nv[g]) return;
-  const ListRef* part = a.pci + d.pci_off;
-  const int64_t pb = a.vrp[d.vrp_off + r], pe = a.vrp[d.vrp_off + r + 1];
+  VRow* vr = (VRow*)a.vrow + p;
+  if (vr->set == 0) return;  // unused slot
+  const int64_t pb = a.vrp[p], pe = a.vrp[p + 1];
   int64_t w = 0;
-  for (int64_t e = pb + gl; e < pe; e += kBuildG) w += list_len(part[e]);
+  for (int64_t e = pb + gl; e < pe; e += kBuildG) w += list_len(a.pci[e]);
 #pragma unroll
   for (int o = kBuildG / 2; o >= 1; o >>= 1) w += __shfl_xor(w, o, kBuildG);
-  if (gl == 0) ((VRow*)a.vrow)[d.vrow_off + r].work = (uint32_t)min(w, (int64_t)0xFFFFFFFF);
+  if (gl == 0) vr->work = (uint32_t)min(w, (int64_t)0xFFFFFFFF);
 }
 
-hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t P, const BuildBufs& b, hipStream_t s) {
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s) {
   if (RP <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
   const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
   hipError_t e = launch_zero(b.c2, RP, s);
+  if (e == hipSuccess) e = launch_zero((void*)a.vrow, a.P * (int64_t)(sizeof(VRow) / 4), s);  // slots unused
   if (e != hipSuccess) return e;
   partner_count_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -677,9 +666,7 @@ hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t P, const Bui
   if ((e = launch_scan(LoadI32{b.kc}, StoreI64{b.voff}, RP, b.part, s)) != hipSuccess) return e;
   partner_rows_fill_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  LaunchArgs w = a;
-  w.P = P;
-  if (P > 0) partner_work_kernel<<<(unsigned)((P * kBuildG + kBlock - 1) / kBlock), kBlock, 0, s>>>(w);
+  if (a.P > 0) partner_work_kernel<<<(unsigned)((a.P * kBuildG + kBlock - 1) / kBlock), kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 
@@ -698,24 +685,21 @@ __device__ __forceinline__ int choose_bin(int64_t np, int64_t s, int64_t work, c
   return BIN_G8;
 }
 
-// A plan index's partner row: graph g, row r, owner x, partner range.
+// A plan index's partner row: graph, owner's list, work, partner range.
 struct PRow {
   int g;
-  int r;       // -1 when the slot is unused
-  ListRef set;
+  ListRef set;  // 0: unused slot
   uint32_t work;
   int64_t pb, pe;
 };
 
-__device__ __forceinline__ PRow partner_row(const LaunchArgs& a, int g, int r) {
-  const GraphDesc d = a.descs[g];
-  PRow pr{g, r, 0, 0u, 0, 0};
-  if (r >= a.nv[g]) { pr.r = -1; return pr; }
-  const VRow vr = a.vrow[d.vrow_off + r];
-  pr.set = vr.set;
-  pr.work = vr.work;
-  pr.pb = a.vrp[d.vrp_off + r];
-  pr.pe = a.vrp[d.vrp_off + r + 1];
+__device__ __forceinline__ PRow partner_row(const LaunchArgs& a, int64_t k) {
+  const VRow vr = a.vrow[k];
+  PRow pr{vr.g, vr.set, vr.work, 0, 0};
+  if (vr.set != 0) {
+    pr.pb = a.vrp[k];
+    pr.pe = a.vrp[k + 1];
+  }
   return pr;
 }
 
@@ -727,11 +711,8 @@ __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
   if (threadIdx.x < kNumBins) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   if (p < a.P) {
-    const int g = find_graph(a.descs, a.ngraphs, p);
-    const GraphDesc d = a.descs[g];
-    const PRow pr = partner_row(a, g, (int)(p - d.plan_base));
-    int bin = kNoBin;
-    if (pr.r >= 0) bin = choose_bin(pr.pe - pr.pb, list_len(pr.set), pr.work, a.tune);
+    const PRow pr = partner_row(a, p);
+    const int bin = pr.set ? choose_bin(pr.pe - pr.pb, list_len(pr.set), pr.work, a.tune) : (int)kNoBin;
     a.bufs.bin_of[p] = (uint8_t)bin;
     if (bin != kNoBin) atomicAdd(&s_cnt[bin], 1u);
   }
@@ -766,9 +747,7 @@ __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
     const int64_t nblk = gridDim.x;
     uint32_t off = a.bufs.blk[bin * nblk + blockIdx.x] + rank;
     for (int w = 0; w < wid; ++w) off += s_wc[w][bin];
-    const int g = find_graph(a.descs, a.ngraphs, p);
-    const int64_t r = p - a.descs[g].plan_base;
-    a.bufs.items[off] = ((uint64_t)(uint32_t)g << 32) | (uint32_t)r;
+    a.bufs.items[off] = (uint64_t)p;
   }
 }
 
@@ -815,15 +794,11 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   int32_t* sEnd = L.End + grp * G;
   gint32** sBase = L.Base + grp * G;
   for (int64_t i = first + grp; i < last; i += NGRP) {
-    const uint64_t item = items[i];
-    const int g = (int)(item >> 32);
-    const PRow pr = partner_row(a, g, (int)(uint32_t)item);
-    const GraphDesc d = a.descs[g];
-    const int32_t* ci = a.ci + d.ci_off;
-    const PivotSet S = build_pivot_set<G>(slot, AREA, ci + list_start(pr.set), list_len(pr.set), gl, a.tune);
+    const PRow pr = partner_row(a, (int64_t)items[i]);
+    const PivotSet S = build_pivot_set<G>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set), gl, a.tune);
     const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup>(
-        ci, RefEdges{a.pci + d.pci_off + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl);
-    acc.to_graph(g, batch, a.bufs.outs);
+        a.ci, RefEdges{a.pci + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl);
+    acc.to_graph(pr.g, batch, a.bufs.outs);
     acc.sum += h;
     wave_sync();
   }
@@ -832,13 +807,11 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
 template <bool kLdsPivot>
 __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, CountLds& L, bool batch,
                                             Acc& acc) {
-  const int g = (int)(item >> 32);
-  const PRow pr = partner_row(a, g, (int)(uint32_t)item);
-  const GraphDesc d = a.descs[g];
-  const int32_t* ci = a.ci + d.ci_off;
+  const PRow pr = partner_row(a, (int64_t)item);
+  const int32_t* ci = a.ci;
   const int32_t* set = ci + list_start(pr.set);
   const int ds = list_len(pr.set);
-  const RefEdges part{a.pci + d.pci_off + pr.pb};
+  const RefEdges part{a.pci + pr.pb};
   const int np = (int)(pr.pe - pr.pb);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t h;
@@ -851,7 +824,7 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
     h = pivot_edges<64, kModeHbm>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64, L.Base + wid * 64,
                                   lane);
   }
-  acc.to_graph(g, batch, a.bufs.outs);
+  acc.to_graph(pr.g, batch, a.bufs.outs);
   acc.sum += h;
 }
 
@@ -908,7 +881,7 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
     if (batch) {
       // Items are ordered by graph, so a tile almost always belongs to one
       // graph: then one block-reduced atomic; otherwise one per lane.
-      const int tg = (int)(items[first] >> 32);
+      const int tg = a.vrow[items[first]].g;
       if (!__syncthreads_or(acc.sum != 0 && acc.g != tg)) {
         const uint64_t tot = block_sum(acc.sum, L.red);
         if (threadIdx.x == 0 && tot) atomicAdd(&a.bufs.outs[tg], tot);

@@ -53,30 +53,28 @@ constexpr int kPartnersPerRow = TCH_PARTNERS_PER_ROW;
 //        the colidx pool), n_rows rows; only rows [own_begin, own_end) have
 //        their out-edges counted (a partition's owned range), other rows are
 //        lists read through them (halo).
-//   Partners: the partners' lists (ListRef) at pci_off of the partner pool
-//        (as many slots as the DAG has ids); partner rows at vrp_off /
-//        vrow_off of their pools
-//        (vrow_off in VRow units),
-//        nv_max + 1 slots, of which the first nv[g] are used (nv is written
-//        on the device; vrp values are relative to pci_off).
+//   Partners: the graph's partner list references at pci_off of the
+//        partner pool (as many slots as the DAG has ids).  Its partner rows
+//        take slots [pr_base, pr_base + nv_max + 1) of the launch's VRow and
+//        row-offset arrays; those slots are also the plan indices.
 //   Build: the graph's n_rows + 1 slots of a partner build's per-row
 //        scratch start at row_base (graphs in order, contiguous).
-//   Plan: its partner rows are plan indices [plan_base, plan_base + nv_max).
+// List references (set and partners) and row offsets are absolute in their
+// pools, so planning and counting never read this descriptor.
 struct GraphDesc {
   int64_t rp_off;
   int64_t ci_off;
   int64_t pci_off;
-  int64_t vrp_off;
-  int64_t vrow_off;
   int64_t row_base;
-  int64_t plan_base;
+  int64_t pr_base;
   int32_t n_rows;
   int32_t nv_max;
   int32_t own_begin;
   int32_t own_end;
 };
 
-// Partner-row slots a graph of n rows and m DAG ids may need.
+// Partner rows a graph of n rows and m DAG ids may need (it takes this
+// many slots plus one).
 inline int32_t partner_rows_max(int64_t n, int64_t m) {
   return (int32_t)(n + m / kPartnersPerRow + 1);
 }
@@ -96,9 +94,9 @@ struct PlanBufs {
   uint64_t* outs;       // [ngraphs] u64 counts
 };
 
-// A list of the DAG as one word: colidx offset (relative to the graph's
-// colidx) in the high 39 bits, length in the low 25 (out-degrees are below
-// 2^25 by the upload checks).
+// A list of the DAG as one word: offset in the colidx pool in the high 39
+// bits, length in the low 25 (out-degrees are below 2^25 by the upload
+// checks).
 typedef uint64_t ListRef;
 constexpr int kLenBits = 25;
 __host__ __device__ inline ListRef list_ref(int64_t start, int64_t len) {
@@ -107,24 +105,24 @@ __host__ __device__ inline ListRef list_ref(int64_t start, int64_t len) {
 __host__ __device__ inline int64_t list_start(ListRef r) { return (int64_t)(r >> kLenBits); }
 __host__ __device__ inline int32_t list_len(ListRef r) { return (int32_t)(r & ((1u << kLenBits) - 1)); }
 
-// A partner row: the owner's list (the set), the total length of its
-// partners' lists (saturated at 2^32 - 1; only compared with the bin
-// thresholds) and the owner's id.
+// A partner row: the owner's list (the set; 0 for an unused slot), the
+// total length of its partners' lists (saturated at 2^32 - 1; only compared
+// with the bin thresholds) and its graph.
 struct VRow {
   ListRef set;
   uint32_t work;
-  int32_t x;
+  int32_t g;
 };
 
 struct LaunchArgs {
   const GraphDesc* descs;
   int32_t ngraphs;
-  int64_t P;            // plan indices over all graphs (sum of nv_max)
+  int64_t P;            // partner-row slots = plan indices (sum of nv_max + 1)
   const int64_t* rp;    // DAG rowptr pool
   const int32_t* ci;    // DAG colidx pool
   const ListRef* pci;   // partner pool: each partner's list
-  const int64_t* vrp;   // partner-row offsets pool
-  const VRow* vrow;     // partner rows pool
+  const int64_t* vrp;   // [P + 1] partner-row offsets into the partner pool
+  const VRow* vrow;     // [P] partner rows
   const int32_t* nv;    // [ngraphs] partner rows in use
   Tune tune;
   PlanBufs bufs;
@@ -145,10 +143,10 @@ struct BuildBufs {
 };
 int64_t scan_partials(int64_t n);
 
-// Builds the partner ids, rows and counts of every graph of `a` (descs,
-// rp, ci as for a count; pci/vrp/vrow/nv are written); RP = sum of
-// n_rows + 1 over the graphs, P = sum of nv_max.
-hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t P, const BuildBufs& b, hipStream_t s);
+// Builds the partner lists, rows and counts of every graph of `a` (descs,
+// rp, ci, P as for a count; pci/vrp/vrow/nv are written); RP = sum of
+// n_rows + 1 over the graphs.
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s);
 
 // Launchers (stream-ordered; return the launch status).
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);

@@ -90,7 +90,7 @@ int64_t PlanScratch::bytes_for(int64_t pivots) {
 
 tc_status Partners::alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
   tc_status st = dev_alloc((void**)&pci, sizeof(tck::ListRef) * (size_t)std::max<int64_t>(ci_slots, 1), "partner lists");
-  if (st == TC_OK) st = dev_alloc((void**)&vrp, sizeof(int64_t) * (size_t)std::max<int64_t>(row_slots, 1), "partner rows");
+  if (st == TC_OK) st = dev_alloc((void**)&vrp, sizeof(int64_t) * (size_t)(row_slots + 1), "partner rows");
   if (st == TC_OK) st = dev_alloc((void**)&vrow, sizeof(tck::VRow) * (size_t)std::max<int64_t>(row_slots, 1), "partner owners");
   if (st == TC_OK) st = dev_alloc((void**)&nv, sizeof(int32_t) * (size_t)std::max(ngraphs, 1), "partner row counts");
   if (st != TC_OK) {
@@ -110,7 +110,7 @@ void Partners::release() {
 }
 
 int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {
-  return (int64_t)sizeof(tck::ListRef) * ci_slots + (8 + (int64_t)sizeof(tck::VRow)) * row_slots +
+  return (int64_t)sizeof(tck::ListRef) * ci_slots + (8 + (int64_t)sizeof(tck::VRow)) * row_slots + 8 +
          4 * (int64_t)ngraphs;
 }
 
@@ -149,10 +149,10 @@ void BuildScratch::release() {
   *this = BuildScratch();
 }
 
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t P, BuildScratch& bs, hipStream_t s) {
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s) {
   tc_status st = bs.ensure(RP);
   if (st != TC_OK) return st;
-  TC_HIP_TRY(tck::launch_partners(a, RP, P, bs.b, s));
+  TC_HIP_TRY(tck::launch_partners(a, RP, bs.b, s));
   return TC_OK;
 }
 
@@ -221,7 +221,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
     cleanup();
     return st;
   }
-  const tck::GraphDesc d{0, 0, 0, 0, 0, 0, 0, g->n, g->nv_max, row_begin, row_end};
+  const tck::GraphDesc d{0, 0, 0, 0, 0, g->n, g->nv_max, row_begin, row_end};
   hipError_t e = hipMemcpy(g->rp, dag->rowptr, sizeof(int64_t) * ((size_t)g->n + 1), hipMemcpyHostToDevice);
   if (e == hipSuccess && g->nnz > 0)
     e = hipMemcpy(g->ci, dag->colidx, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice);
@@ -234,6 +234,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   tck::LaunchArgs a{};
   a.descs = g->d_desc;
   a.ngraphs = 1;
+  a.P = (int64_t)g->nv_max + 1;
   a.rp = g->rp;
   a.ci = g->ci;
   g->partners.into(a);
@@ -241,7 +242,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   hipStream_t s = ctx->stream;
   e = hipEventRecord(ctx->ev[0], s);
   if (e == hipSuccess) {
-    st = launch_build(a, (int64_t)g->n + 1, g->nv_max, bs, s);
+    st = launch_build(a, (int64_t)g->n + 1, bs, s);
     if (st == TC_OK) e = hipEventRecord(ctx->ev[1], s);
     if (st == TC_OK && e == hipSuccess)
       e = hipMemcpyAsync(ctx->pinned, g->partners.nv, sizeof(int32_t), hipMemcpyDeviceToHost, s);
@@ -414,7 +415,7 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
   const double t0 = now_ms();
-  const int64_t P = g->nv_max;
+  const int64_t P = (int64_t)g->nv_max + 1;
   tc_status st = ctx->scratch.ensure(P, 1);
   if (st != TC_OK) return st;
   tck::LaunchArgs a{};

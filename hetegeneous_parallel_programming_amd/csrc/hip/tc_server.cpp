@@ -24,17 +24,15 @@ namespace {
 constexpr uint64_t kAlign = 256;
 inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-// A request's extent in the HBM pool: rowptr | colidx | partner ids |
-// partner-row offsets | partner-row owners (each 256-byte aligned).
+// A request's extent in the HBM pool: rowptr | colidx | partner list
+// references (each 256-byte aligned).  Its partner rows live in the
+// launching slot's scratch.
 struct Layout {
-  uint64_t ci, pci, vrp, vrow, total;
+  uint64_t ci, pci, total;
   Layout(int32_t n, int64_t nnz) {
-    const int64_t nvm = tck::partner_rows_max(n, nnz);
     ci = align_up(sizeof(int64_t) * ((uint64_t)n + 1));
     pci = ci + align_up(sizeof(int32_t) * (uint64_t)nnz);
-    vrp = pci + align_up(sizeof(tck::ListRef) * (uint64_t)nnz);
-    vrow = vrp + align_up(sizeof(int64_t) * ((uint64_t)nvm + 1));
-    total = vrow + align_up(sizeof(tck::VRow) * ((uint64_t)nvm + 1));
+    total = pci + align_up(sizeof(tck::ListRef) * (uint64_t)nnz);
   }
 };
 
@@ -54,11 +52,14 @@ struct Req {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  void* plan = nullptr;              // device: items | bin_of | blk | part | counters | outs | descs | nv
+  void* plan = nullptr;              // device: items | bin_of | blk | part | counters | outs | descs | nv |
+                                     //         partner rows | row offsets
   uint64_t* pinned = nullptr;        // host: outs[G] then descs staging
   tck::PlanBufs bufs{};
   tck::GraphDesc* d_descs = nullptr;
   int32_t* d_nv = nullptr;
+  tck::VRow* d_vrow = nullptr;
+  int64_t* d_vrp = nullptr;
   BuildScratch build;
   std::vector<Req*> reqs;
   bool busy = false;
@@ -185,8 +186,8 @@ bool tc_server::launch() {
     if (s.busy || ready.empty()) continue;
     int64_t rows = 0;
     while (!ready.empty() && (int)s.reqs.size() < cfg.max_batch_graphs &&
-           rows + ready.front()->nv_max <= cfg.max_batch_rows) {
-      rows += ready.front()->nv_max;
+           rows + ready.front()->nv_max + 1 <= cfg.max_batch_rows) {
+      rows += ready.front()->nv_max + 1;
       s.reqs.push_back(ready.front());
       ready.pop_front();
     }
@@ -198,10 +199,8 @@ bool tc_server::launch() {
       Req* r = s.reqs[i];
       const Layout L(r->n, r->nnz);
       hd[i] = tck::GraphDesc{(int64_t)(r->off / sizeof(int64_t)), (int64_t)((r->off + L.ci) / sizeof(int32_t)),
-                             (int64_t)((r->off + L.pci) / sizeof(tck::ListRef)),
-                             (int64_t)((r->off + L.vrp) / sizeof(int64_t)),
-                             (int64_t)((r->off + L.vrow) / sizeof(tck::VRow)), rb, pb, r->n, r->nv_max, 0, r->n};
-      pb += r->nv_max;
+                             (int64_t)((r->off + L.pci) / sizeof(tck::ListRef)), rb, pb, r->n, r->nv_max, 0, r->n};
+      pb += (int64_t)r->nv_max + 1;
       rb += (int64_t)r->n + 1;
     }
     tck::LaunchArgs a{};
@@ -211,8 +210,8 @@ bool tc_server::launch() {
     a.rp = (const int64_t*)pool;
     a.ci = (const int32_t*)pool;
     a.pci = (const tck::ListRef*)pool;
-    a.vrp = (const int64_t*)pool;
-    a.vrow = (const tck::VRow*)pool;
+    a.vrp = s.d_vrp;
+    a.vrow = s.d_vrow;
     a.nv = s.d_nv;
     {
       std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -223,7 +222,7 @@ bool tc_server::launch() {
     for (Req* r : s.reqs) r->t_launch = t;
     hipError_t e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
     // Partner structures of this launch's graphs, then plan and count.
-    if (e == hipSuccess) e = s.build.ensure(rb) == TC_OK ? tck::launch_partners(a, rb, pb, s.build.b, s.stream)
+    if (e == hipSuccess) e = s.build.ensure(rb) == TC_OK ? tck::launch_partners(a, rb, s.build.b, s.stream)
                                                          : hipErrorOutOfMemory;
     // A batch of one graph would take the single-graph reduction path.
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s.stream);
@@ -337,7 +336,8 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
   const uint64_t o_items = 0, o_bin = o_items + align_up(8 * R), o_blk = o_bin + align_up(R),
                  o_part = o_blk + align_up(4 * nb), o_cnt = o_part + align_up(8 * tck::scan_partials(nb)),
                  o_outs = o_cnt + align_up(4 * 64), o_desc = o_outs + align_up(8 * G),
-                 o_nv = o_desc + align_up(sizeof(tck::GraphDesc) * G), total = o_nv + align_up(4 * G);
+                 o_nv = o_desc + align_up(sizeof(tck::GraphDesc) * G), o_vrow = o_nv + align_up(4 * G),
+                 o_vrp = o_vrow + align_up(sizeof(tck::VRow) * R), total = o_vrp + align_up(8 * (R + 1));
   s->slots.resize(cfg->slots);
   for (auto& sl : s->slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
@@ -359,6 +359,8 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
                             (int64_t*)(p + o_part), (uint32_t*)(p + o_cnt), (uint64_t*)(p + o_outs)};
     sl.d_descs = (tck::GraphDesc*)(p + o_desc);
     sl.d_nv = (int32_t*)(p + o_nv);
+    sl.d_vrow = (tck::VRow*)(p + o_vrow);
+    sl.d_vrp = (int64_t*)(p + o_vrp);
   }
   tc_server* raw = s.release();
   raw->th = std::thread([raw]() { raw->run(); });
@@ -384,8 +386,8 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
   tch_arena_stats_get(s->arena, &as);
   if (fp > as.capacity) return reject(TC_ENOMEM, "request footprint exceeds the server's HBM pool");
   const int32_t nvm = tck::partner_rows_max(dag->n_rows, dag->nnz);
-  if (nvm > s->cfg.max_batch_rows)
-    return reject(TC_ERANGE, "request needs more plan rows than max_batch_rows (rows + nnz/1024 + 1)");
+  if ((int64_t)nvm + 1 > s->cfg.max_batch_rows)
+    return reject(TC_ERANGE, "request needs more plan rows than max_batch_rows (rows + nnz/1024 + 2)");
   for (int64_t v = 0; v < dag->n_rows; ++v)
     if ((dag->rowptr[v + 1] - dag->rowptr[v]) * 64 >= ((int64_t)1 << 31))
       return reject(TC_ERANGE, "DAG out-degree too large");

@@ -88,6 +88,12 @@ constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
 // Sub-wave groups keep per-group state in vector registers; fewer lookups
 // in flight leave room for it.
 constexpr int kInFlightGroup = TC_IN_FLIGHT_GROUP;
+// Steps whose every slot holds an id skip the per-slot bounds checks: bit 0
+// for wave-wide groups, bit 1 for sub-wave groups (code size and registers
+// grow with the extra copy of the step).
+#ifndef TC_SPLIT_FULL
+#define TC_SPLIT_FULL 1
+#endif
 
 // ---------------------------------------------------------------------------
 // Pivot sets: how a group holds N+(u) for membership tests
@@ -275,21 +281,41 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
     sEnd[gl] = excl + c;
     sBase[gl] = base - excl;
     wave_sync();
+    int cur_k = -1;   // the lane's current edge
     int cur_end = 0;  // t < cur_end  <=>  t lies in the lane's current edge
     gint32* cur_base = (gint32*)ci;
+    // A lane's flat index grows by G per id, so its next edge is almost
+    // always the following one: check that first, search only otherwise.
+    // (t < total = sEnd[G - 1] keeps cur_k + 1 inside the batch.)
+    auto step_to = [&](int t) {
+      int k = cur_k + 1;
+      int e = sEnd[k];
+      if (e <= t) {
+        k = find_edge<G>(sEnd, t);
+        e = sEnd[k];
+      }
+      cur_k = k;
+      cur_end = e;
+      cur_base = sBase[k];
+    };
     for (int t0 = 0; t0 < total; t0 += G * NF) {
       int w[NF];
+      if ((TC_SPLIT_FULL & (G == 64 ? 1 : 2)) && t0 + G * NF <= total) {  // every slot holds an id
 #pragma unroll
-      for (int q = 0; q < NF; ++q) {
-        const int t = t0 + q * G + gl;
-        w[q] = INT_MIN;  // ids are >= 0: never found
-        if (t < total) {
-          if (t >= cur_end) {
-            const int k = find_edge<G>(sEnd, t);
-            cur_end = sEnd[k];
-            cur_base = sBase[k];
-          }
+        for (int q = 0; q < NF; ++q) {
+          const int t = t0 + q * G + gl;
+          if (t >= cur_end) step_to(t);
           w[q] = cur_base[t];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          const int t = t0 + q * G + gl;
+          w[q] = INT_MIN;  // ids are >= 0: never found
+          if (t < total) {
+            if (t >= cur_end) step_to(t);
+            w[q] = cur_base[t];
+          }
         }
       }
       if ((kModes & kModeHash) && S.mode == kSetHash) {

@@ -22,6 +22,7 @@ import json
 import os
 import re
 import sys
+import threading
 import time
 
 import numpy as np
@@ -199,7 +200,7 @@ def batch_tuner(args):
     return tune
 
 
-def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed):
+def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clients=8):
     """Poisson TC requests from the batched mix plus 1% RMAT-20 through the
     continuous-batching server: saturated capacity, then 50% and 90% load."""
     from hetegeneous_parallel_programming_amd.solver import Server, clock_ms
@@ -216,26 +217,45 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed):
         srv.drain()
         srv.poll(max_results=4096)
 
-        def run(arrivals):
+        def run(arrivals, clients):
+            """Requests i = c, c + clients, ... go out from client thread c,
+            each at its scheduled arrival (or back to back when saturating);
+            the main thread collects results.  Latency = completion - arrival."""
             got = {}
-            t0 = clock_ms() + 20.0
+            t0 = clock_ms() + 50.0
             sched = t0 + arrivals if arrivals is not None else None
+            errors = []
+
+            def client(c):
+                try:
+                    for i in range(c, n_requests, clients):
+                        if sched is not None:
+                            srv.submit_at(req[i], i, sched[i])
+                        else:
+                            srv.submit(req[i], i)
+                except Exception as e:  # surfaced after the run
+                    errors.append(e)
+
             start = clock_ms()
-            for i in range(n_requests):
-                if sched is not None:
-                    while clock_ms() < sched[i]:
-                        pass
-                srv.submit(req[i], i)
-                if (i & 255) == 0:
-                    for r in srv.poll():
-                        got[r["tag"]] = r
-            srv.drain()
+            ths = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
+            for t in ths:
+                t.start()
             while len(got) < n_requests:
-                rs = srv.poll(timeout_ms=1000)
-                if not rs:
-                    break
+                rs = srv.poll(timeout_ms=200)
                 for r in rs:
                     got[r["tag"]] = r
+                if not rs and not any(t.is_alive() for t in ths):
+                    srv.drain()  # everything submitted: collect the rest
+                    rs = srv.poll()
+                    while rs:
+                        for r in rs:
+                            got[r["tag"]] = r
+                        rs = srv.poll()
+                    break
+            for t in ths:
+                t.join()
+            if errors:
+                raise errors[0]
             done = np.array([got[i]["t_done_ms"] for i in range(n_requests)])
             base = sched if sched is not None else np.array([got[i]["t_submit_ms"] for i in range(n_requests)])
             lat = done - base
@@ -243,17 +263,17 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed):
             ok = bool(np.array_equal(cnt, want)) and all(got[i]["status"] == 0 for i in range(n_requests))
             span = (done.max() - (sched[0] if sched is not None else start)) / 1e3
             return {"graphs_per_s": n_requests / span, "p50_ms": float(np.percentile(lat, 50)),
-                    "p99_ms": float(np.percentile(lat, 99)), "exact": ok,
+                    "p99_ms": float(np.percentile(lat, 99)), "exact": ok, "clients": clients,
                     "mean_batch": float(np.mean([got[i]["batch_graphs"] for i in range(n_requests)]))}
 
-        sat = run(None)
+        sat = run(None, clients)
         out["saturated"] = sat
         cap = sat["graphs_per_s"]
         log(f"serving saturated: {cap:.0f} graphs/s, p99 {sat['p99_ms']:.1f} ms, exact={sat['exact']}")
         for frac in (0.5, 0.9):
             lam = frac * cap
             arr = np.cumsum(rng.exponential(1.0 / lam, n_requests)) * 1e3
-            r = run(arr)
+            r = run(arr, clients)
             r["offered_per_s"] = lam
             out[f"load_{int(frac * 100)}"] = r
             log(f"serving {int(frac * 100)}% load ({lam:.0f}/s): {r['graphs_per_s']:.0f} graphs/s, "
@@ -313,6 +333,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--tune", default="", help="launch parameters, e.g. t_wave=8192,stage_ratio16=24")
     ap.add_argument("--requests", type=int, default=10000, help="requests per serving run")
+    ap.add_argument("--clients", type=int, default=8, help="client threads submitting requests")
     ap.add_argument("--no-autotune", action="store_true", help="time the batch with the default setting only")
     ap.add_argument("--tune-seconds", type=float, default=12.0, help="autotuner wall-time budget")
     ap.add_argument("--tune-file", default="", help="load the batch setting from / save it to this file")
@@ -426,7 +447,7 @@ def run_side_legs(ctx, args, line) -> None:
             heavy_count = ctx.count(dh)[0]
             dh.free()
             line["extra"]["serving"] = leg_serving(ctx, graphs, counts, heavy, heavy_count,
-                                                   args.requests, args.seed + 100)
+                                                   args.requests, args.seed + 100, args.clients)
             gold20 = golden_big().get(f"rmat20_seed{args.seed}")
             line["extra"]["serving"]["rmat20_exact_vs_cpu_oracle"] = (
                 gold20["triangles"] == heavy_count if gold20 else None)

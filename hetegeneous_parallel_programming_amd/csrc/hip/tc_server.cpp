@@ -36,6 +36,13 @@ struct Layout {
   }
 };
 
+// One admission's host-to-device copies: launches of its requests wait on
+// the event; the last request to finish returns the event to the pool.
+struct Upload {
+  hipEvent_t ev = nullptr;
+  int refs = 0;
+};
+
 struct Req {
   uint64_t tag = 0;
   int32_t n = 0;
@@ -46,6 +53,7 @@ struct Req {
   char* staged = nullptr;   // or: copy in the pinned staging area
   uint64_t stage_off = 0;
   uint64_t off = 0, bytes = 0;
+  Upload* upload = nullptr;
   double t_submit = 0, t_admit = 0, t_launch = 0;
 };
 
@@ -75,6 +83,7 @@ struct tc_server {
   char* staging = nullptr;         // pinned host memory
   tch_arena* stage_arena = nullptr;
   hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> free_events;
   std::vector<Slot> slots;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -93,6 +102,13 @@ struct tc_server {
 };
 
 void tc_server::finish(Req* r, uint64_t count, tc_status st, int batch, double t_done) {
+  // The request's copies are complete by now (its launch waited on them, or
+  // they failed): its staging and the upload event can be reused.
+  if (r->staged) tch_arena_free(stage_arena, r->stage_off);
+  if (r->upload && --r->upload->refs == 0) {
+    free_events.push_back(r->upload->ev);
+    delete r->upload;
+  }
   tc_request_result res;
   res.tag = r->tag;
   res.count = count;
@@ -148,35 +164,54 @@ bool tc_server::admit() {
     }
   }
   if (take.empty()) return false;
+  // Copies go out on the copy stream without waiting: one event marks their
+  // end, and the launches of these requests wait on it on the device.
   hipError_t e = hipSuccess;
   for (Req* r : take) {
+    const Layout L(r->n, r->nnz);
     char* base = pool + r->off;
-    const uint64_t rp_bytes = sizeof(int64_t) * ((uint64_t)r->n + 1);
-    const void* src_rp = r->staged ? (const void*)r->staged : (const void*)r->rp.data();
-    const void* src_ci = r->staged ? (const void*)(r->staged + align_up(rp_bytes)) : (const void*)r->ci.data();
-    e = hipMemcpyAsync(base, src_rp, rp_bytes, hipMemcpyHostToDevice, copy_stream);
-    if (e == hipSuccess && r->nnz > 0)
-      e = hipMemcpyAsync(base + Layout(r->n, r->nnz).ci, src_ci, sizeof(int32_t) * (size_t)r->nnz,
-                         hipMemcpyHostToDevice, copy_stream);
+    if (r->staged) {  // rowptr | pad | colidx, laid out as in the pool
+      e = hipMemcpyAsync(base, r->staged, L.pci, hipMemcpyHostToDevice, copy_stream);
+    } else {
+      e = hipMemcpyAsync(base, r->rp.data(), sizeof(int64_t) * ((size_t)r->n + 1), hipMemcpyHostToDevice,
+                         copy_stream);
+      if (e == hipSuccess && r->nnz > 0)
+        e = hipMemcpyAsync(base + L.ci, r->ci.data(), sizeof(int32_t) * (size_t)r->nnz, hipMemcpyHostToDevice,
+                           copy_stream);
+    }
     if (e != hipSuccess) break;
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(copy_stream);
+  Upload* up = nullptr;
+  if (e == hipSuccess) {
+    up = new Upload;
+    if (!free_events.empty()) {
+      up->ev = free_events.back();
+      free_events.pop_back();
+    } else {
+      e = hipEventCreateWithFlags(&up->ev, hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventRecord(up->ev, copy_stream);
+    if (e != hipSuccess) {
+      if (up->ev) free_events.push_back(up->ev);
+      delete up;
+      up = nullptr;
+    }
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(copy_stream);  // nothing may still read the staging
+  }
   const double t = now_ms();
   for (Req* r : take) {
-    if (r->staged) {
-      tch_arena_free(stage_arena, r->stage_off);
-      r->staged = nullptr;
-    }
     if (e != hipSuccess) {
       finish(r, 0, TC_EDEVICE, 0, t);
       continue;
     }
-    std::vector<int64_t>().swap(r->rp);
-    std::vector<int32_t>().swap(r->ci);
+    r->upload = up;
+    ++up->refs;
     r->t_admit = t;
     ready.push_back(r);
   }
-  if (e != hipSuccess) (void)hipGetLastError();
   return true;
 }
 
@@ -220,7 +255,15 @@ bool tc_server::launch() {
     a.bufs = s.bufs;
     const double t = now_ms();
     for (Req* r : s.reqs) r->t_launch = t;
-    hipError_t e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < k && e == hipSuccess; ++i) {  // this launch's uploads must have landed
+      Upload* up = s.reqs[i]->upload;
+      bool seen = false;
+      for (int j = 0; j < i && !seen; ++j) seen = s.reqs[j]->upload == up;
+      if (up && !seen) e = hipStreamWaitEvent(s.stream, up->ev, 0);
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
     // Partner structures of this launch's graphs, then plan and count.
     if (e == hipSuccess) e = s.build.ensure(rb) == TC_OK ? tck::launch_partners(a, rb, s.build.b, s.stream)
                                                          : hipErrorOutOfMemory;
@@ -235,6 +278,7 @@ bool tc_server::launch() {
     if (e != hipSuccess) {
       (void)hipGetLastError();
       (void)hipStreamSynchronize(s.stream);
+      (void)hipStreamSynchronize(copy_stream);  // staging is released below
       const double tf = now_ms();
       for (Req* r : s.reqs) finish(r, 0, TC_EDEVICE, k, tf);
       s.reqs.clear();
@@ -279,6 +323,9 @@ void tc_server::release() {
     if (s.pinned) (void)hipHostFree(s.pinned);
   }
   slots.clear();
+  if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+  for (hipEvent_t ev : free_events) (void)hipEventDestroy(ev);
+  free_events.clear();
   if (copy_stream) (void)hipStreamDestroy(copy_stream);
   dev_free(pool);
   if (arena) tch_arena_destroy(arena);
@@ -287,7 +334,13 @@ void tc_server::release() {
   staging = nullptr;
   stage_arena = nullptr;
   for (Req* r : waiting) delete r;
-  for (Req* r : ready) delete r;
+  for (Req* r : ready) {
+    if (r->upload && --r->upload->refs == 0) {
+      (void)hipEventDestroy(r->upload->ev);
+      delete r->upload;
+    }
+    delete r;
+  }
   waiting.clear();
   ready.clear();
 }
@@ -423,6 +476,17 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
   ++s->submitted;
   s->cv_work.notify_one();
   return TC_OK;
+}
+
+tc_status tc_server_submit_at(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check,
+                              double t_ms) {
+  for (;;) {
+    const double left = t_ms - now_ms();
+    if (left <= 0) break;
+    if (left > 2.0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)((left - 1.5) * 1000)));
+    else std::this_thread::yield();
+  }
+  return tc_server_submit(s, dag, tag, check);
 }
 
 tc_status tc_server_poll(tc_server* s, tc_request_result* out, int32_t max, int32_t* n_out, int32_t timeout_ms) {

@@ -128,6 +128,7 @@ def device_lib():
     S(L, "tc_request_footprint", nat.c_u64, [nat.c_i32, nat.c_i64])
     S(L, "tc_server_create", nat.c_i32, [P, ctypes.POINTER(ServerConfig), PP])
     S(L, "tc_server_submit", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_u64, nat.c_u32])
+    S(L, "tc_server_submit_at", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_u64, nat.c_u32, nat.c_dbl])
     S(L, "tc_server_poll", nat.c_i32, [P, ctypes.POINTER(RequestResult), nat.c_i32,
                                        ctypes.POINTER(nat.c_i32), nat.c_i32])
     S(L, "tc_server_drain", nat.c_i32, [P, nat.c_i32])
@@ -367,6 +368,10 @@ class Server:
     def try_submit(self, g: Graph, tag: int, check: int = nat.TC_CHECK_SHAPE) -> int:
         """Status code instead of an exception (TC_EBUSY when the queue is full)."""
         return int(device_lib().tc_server_submit(self._h, ctypes.byref(g.view()), tag, check))
+
+    def submit_at(self, g: Graph, tag: int, t_ms: float, check: int = nat.TC_CHECK_SHAPE) -> None:
+        """Submit at a scheduled time of clock_ms() (waits without holding the GIL)."""
+        _check(device_lib().tc_server_submit_at(self._h, ctypes.byref(g.view()), tag, check, float(t_ms)))
 
     def poll(self, timeout_ms: int = 0, max_results: int = 4096) -> list[dict]:
         n = nat.c_i32(0)

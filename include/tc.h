@@ -206,6 +206,11 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
  * TC_ENOMEM: footprint exceeds the whole pool; TC_ERANGE: more rows than
  * max_batch_rows. */
 tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check);
+/* tc_server_submit at a scheduled time: waits (sleeping, then spinning for
+ * the last ~2 ms) until tc_clock_ms() >= t_ms, then submits.  For clients
+ * and load generators replaying an arrival schedule from several threads. */
+tc_status tc_server_submit_at(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check,
+                              double t_ms);
 /* Up to `max` finished results; waits up to timeout_ms for the first one. */
 tc_status tc_server_poll(tc_server* s, tc_request_result* out, int32_t max, int32_t* n_out,
                          int32_t timeout_ms);

@@ -5,14 +5,20 @@ Headline (``value``): TC edges/s on RMAT scale 22, edge factor 16
 tc_count call (degree binning + persistent count kernel + u64 copy back) on
 the DAG already resident in HBM.  Beside it:
 
-* roofline: the count kernel's algorithmic bytes B_alg = 8(R+1) + 20 m_R +
-  4 sum_{(u,v)} d+(v) divided by its average duration from HIP events on the
-  library's stream, against 8 TB/s;
+* roofline: the count kernels' algorithmic bytes -- what the partner-scheme
+  count reads with no reuse: per work unit its 16-byte row record, 16 bytes of
+  row offsets and the owner's list, per DAG edge an 8-byte list reference and
+  the shorter list 4 min(d+(u), d+(v)) -- divided by their average duration
+  from HIP events on the library's stream, against 8 TB/s; the reference
+  algorithm's edge-iterator bytes 8(R+1) + 20 m_R + 4 sum d+(v) over the same
+  time are reported beside it;
 * cpu_baseline: the CPU oracle (OpenMP merge count, the reference's CPU
   algorithm) on a bounded random sample of pivot rows of the same graph;
 * extra legs: the plain one-wave-per-pivot kernel, the 4096-graph batched
-  workload (one captured persistent launch vs one plain launch per graph),
-  and partition 0 of RMAT scale 24 split 4 ways with its halo cache.
+  workload (one captured persistent launch, default and autotuned, vs one
+  plain launch per graph), the serving loop under Poisson arrivals, and
+  partition 0 of RMAT scale 24 split 4 ways with its halo cache, with every
+  part of the reference's vertex split and of a partner-balanced split.
 """
 from __future__ import annotations
 
@@ -284,7 +290,7 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
     return out
 
 
-def leg_partition(ctx, seed, steps, warmup):
+def leg_partition(ctx, seed, steps, warmup, split=True):
     from hetegeneous_parallel_programming_amd import Graph, PartitionedGraph
 
     t0 = time.perf_counter()
@@ -294,7 +300,8 @@ def leg_partition(ctx, seed, steps, warmup):
     work = part.work()
     log(f"partition: RMAT-24 + halo built in {time.perf_counter() - t0:.1f} s; owned edges "
         f"{work['edges']}, halo lists {part.halo_ids.size}")
-    del g
+    if not split:
+        del g
     dg = ctx.upload_partition(part)
     info = dg.info()
     ms_total, outs = time_steps(lambda: ctx.count(dg), steps, warmup)
@@ -313,7 +320,50 @@ def leg_partition(ctx, seed, steps, warmup):
            "edge_iterator_GBps": work["alg_bytes"] / (k_avg / 1e3) / 1e9}
     res["roofline_frac"] = res["alg_GBps"] / PEAK_HBM_GBS
     log(f"partition: count={res['count']} kernel={k_avg:.3f} ms frac={res['roofline_frac']:.3f}")
+    if split:
+        res["split4"] = split_parts(ctx, g, res, steps=3, warmup=1)
     return res
+
+
+def split_parts(ctx, g, p0, steps, warmup):
+    """Every part of the 4-way split, under the reference's vertex split and
+    the partner-balanced split: per-part count time, and the slowest part
+    (what a 4-GPU run would wait for).  Both splits must sum to the same total."""
+    from hetegeneous_parallel_programming_amd import PartitionedGraph
+
+    out = {}
+    for balance in ("vertices", "partner"):
+        pg = PartitionedGraph(g, 4, balance)
+        parts = []
+        for i in range(4):
+            if balance == "vertices" and i == 0:
+                parts.append({"rows": [0, p0_rows(pg)], "edges": p0["owned_edges"],
+                              "count": p0["count"], "kernel_ms": p0["kernel_ms_avg"]})
+                continue
+            part = pg.halo_partition(i)
+            dg = ctx.upload_partition(part)
+            _, outs = time_steps(lambda: ctx.count(dg), steps, warmup)
+            dg.free()
+            counts = {o[0] for o in outs}
+            if len(counts) != 1:
+                raise RuntimeError("partition counts differ across steps")
+            parts.append({"rows": [part.row_begin, part.row_end], "edges": part.work()["edges"],
+                          "count": counts.pop(),
+                          "kernel_ms": float(np.mean([o[1]["count_ms"] for o in outs]))})
+            del part
+        out[balance] = {"parts": parts, "total": sum(x["count"] for x in parts),
+                        "max_kernel_ms": max(x["kernel_ms"] for x in parts),
+                        "sum_kernel_ms": sum(x["kernel_ms"] for x in parts)}
+        log(f"split4 {balance}: kernel ms per part "
+            f"{[round(x['kernel_ms'], 2) for x in parts]}, total {out[balance]['total']}")
+    out["exact"] = out["vertices"]["total"] == out["partner"]["total"]
+    out["critical_path_speedup"] = (out["vertices"]["max_kernel_ms"]
+                                    / out["partner"]["max_kernel_ms"])
+    return out
+
+
+def p0_rows(pg):
+    return int(pg.bounds()[1])
 
 
 def main() -> int:
@@ -326,6 +376,8 @@ def main() -> int:
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-partition", action="store_true")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip timing the other parts of the RMAT-24 4-way splits")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serving", action="store_true")
@@ -456,7 +508,7 @@ def run_side_legs(ctx, args, line) -> None:
         del graphs
     if not args.no_partition:
         line["extra"]["partition_rmat24_p0"] = leg_partition(ctx, args.seed, max(3, args.steps // 4),
-                                                             args.warmup)
+                                                             args.warmup, split=not args.no_split)
 
 
 if __name__ == "__main__":

@@ -62,6 +62,8 @@ class Work(ctypes.Structure):
 
 
 PARTNERS_PER_ROW = 1024  # TCH_PARTNERS_PER_ROW
+# tch_partition_bounds criteria
+BALANCE = {"vertices": 0, "edges": 1, "wedges": 2, "partner": 3}
 
 
 class ArenaStats(ctypes.Structure):
@@ -105,6 +107,8 @@ def host():
     _sig(L, "tch_orient", c_i32, [ctypes.POINTER(CsrView), ctypes.POINTER(P)])
     _sig(L, "tch_sort_neighbors", c_i32, [P])
     _sig(L, "tch_partition_range", c_i32, [ctypes.POINTER(CsrView), c_i32, c_i32, ctypes.POINTER(P)])
+    _sig(L, "tch_partition_bounds", c_i32, [ctypes.POINTER(CsrView), c_i32, c_i32, i32p])
+    _sig(L, "tch_partition_span", c_i32, [ctypes.POINTER(CsrView), c_i32, c_i32, ctypes.POINTER(P)])
     _sig(L, "tch_partition_get", c_i32, [P, ctypes.POINTER(PartitionView)])
     _sig(L, "tch_partition_free", None, [P])
     _sig(L, "tch_partition_induced", c_i32, [ctypes.POINTER(CsrView), c_i32, c_i32,

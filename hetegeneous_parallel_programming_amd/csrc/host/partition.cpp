@@ -54,10 +54,63 @@ extern "C" {
 tc_status tch_partition_range(const tc_csr_view* dag, int32_t k, int32_t part, tch_partition** out) {
   if (!dag || !out) return fail(TC_EINVAL, "null argument");
   if (k < 1 || part < 0 || part >= k) return fail(TC_EINVAL, "need k >= 1 and 0 <= part < k");
+  int32_t b, e;
+  range_of(dag->n_rows, k, part, b, e);
+  return tch_partition_span(dag, b, e, out);
+}
+
+tc_status tch_partition_bounds(const tc_csr_view* dag, int32_t k, int32_t balance, int32_t* bounds) {
+  if (!dag || !bounds) return fail(TC_EINVAL, "null argument");
+  if (k < 1) return fail(TC_EINVAL, "need k >= 1");
+  if (balance < TCH_BALANCE_VERTICES || balance > TCH_BALANCE_PARTNER)
+    return fail(TC_EINVAL, "unknown balance criterion");
   const int32_t n = dag->n_rows;
+  if (balance == TCH_BALANCE_VERTICES) {
+    for (int32_t i = 0; i < k; ++i) range_of(n, k, i, bounds[i], bounds[i + 1]);
+    return TC_OK;
+  }
+  const int64_t* rp = dag->rowptr;
+  std::unique_ptr<int64_t[]> w(new int64_t[(size_t)n + 1]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t u = 0; u < n; ++u) {
+    const int64_t du = rp[u + 1] - rp[u];
+    int64_t c = du;
+    if (balance != TCH_BALANCE_EDGES) {
+      c = (balance == TCH_BALANCE_WEDGES) ? du : 2 * du;
+      for (int64_t x = rp[u]; x < rp[u + 1]; ++x) {
+        const int32_t v = dag->colidx[x];
+        const int64_t dv = rp[v + 1] - rp[v];
+        c += (balance == TCH_BALANCE_WEDGES) ? dv : std::min(du, dv);
+      }
+    }
+    w[u] = c;
+  }
+  std::unique_ptr<int64_t[]> pre(new int64_t[(size_t)n + 1]);
+  exclusive_scan(w.get(), n, pre.get());
+  w.reset();
+  const int64_t total = pre[n];
+  bounds[0] = 0;
+  bounds[k] = n;
+  for (int32_t i = 1; i < k; ++i) {
+    // split where the prefix is nearest to i/k of the total
+    const int64_t t = (int64_t)((__int128)total * i / k);
+    int64_t r = std::lower_bound(pre.get(), pre.get() + n + 1, t) - pre.get();
+    if (r > 0 && t - pre[r - 1] < pre[r] - t) --r;
+    bounds[i] = std::max(bounds[i - 1], (int32_t)r);
+  }
+  return TC_OK;
+}
+
+tc_status tch_partition_span(const tc_csr_view* dag, int32_t row_begin, int32_t row_end,
+                             tch_partition** out) {
+  if (!dag || !out) return fail(TC_EINVAL, "null argument");
+  const int32_t n = dag->n_rows;
+  if (row_begin < 0 || row_end > n || row_begin > row_end)
+    return fail(TC_EINVAL, "owned range outside [0, n]");
   std::unique_ptr<tch_partition> p(new tch_partition);
   p->n_global = n;
-  range_of(n, k, part, p->row_begin, p->row_end);
+  p->row_begin = row_begin;
+  p->row_end = row_end;
   const int32_t b = p->row_begin, e = p->row_end, nl = e - b;
   const int64_t e0 = dag->rowptr[b];
   p->local.allocate(nl, dag->rowptr[e] - e0);

@@ -221,13 +221,24 @@ class HaloPartition:
 
 
 class PartitionedGraph:
-    """Contiguous-range partitioner (graph_partition.h:11-68)."""
+    """Contiguous-range partitioner (graph_partition.h:11-68).
 
-    def __init__(self, g: Graph, nc: int):
+    ``balance`` picks the split points of :meth:`halo_partition`: "vertices"
+    is the reference's ceil(n/k) split (graph_partition.cc:88-107); "edges",
+    "wedges" and "partner" balance d+(u), the edge-iterator elements or the
+    partner-scheme ints of the owned rows (``tch_partition_bounds``).  The
+    vertex-induced subgraphs of :meth:`edgecut_induced_partition1D` always
+    use the reference split."""
+
+    def __init__(self, g: Graph, nc: int, balance: str = "vertices"):
         if nc < 1:
             raise ValueError("need at least one vertex chunk")
+        if balance not in nat.BALANCE:
+            raise ValueError(f"balance must be one of {sorted(nat.BALANCE)}")
         self.g = g
         self.num_vertex_chunks = nc
+        self.balance = balance
+        self._bounds: np.ndarray | None = None
         self._subgraphs: list[Graph] = []
         self._local_begin: list[int] = []
         self._local_end: list[int] = []
@@ -264,10 +275,28 @@ class PartitionedGraph:
     def get_local_end(self, i: int) -> int:
         return self._local_end[i]
 
+    def bounds(self) -> np.ndarray:
+        """Split points b[0..k]: part i owns global ids [b[i], b[i+1])."""
+        if self._bounds is None:
+            b = np.zeros(self.num_vertex_chunks + 1, np.int32)
+            host_check(host().tch_partition_bounds(
+                ctypes.byref(self.g.view()), self.num_vertex_chunks, nat.BALANCE[self.balance],
+                b.ctypes.data_as(nat.i32p)))
+            self._bounds = b
+        return self._bounds
+
     def halo_partition(self, i: int) -> HaloPartition:
+        """Owned rows of part ``i`` plus the halo cache of the remote lists they reach."""
+        if not 0 <= i < self.num_vertex_chunks:
+            raise IndexError(f"part {i} outside [0, {self.num_vertex_chunks})")
         out = ctypes.c_void_p()
-        host_check(host().tch_partition_range(ctypes.byref(self.g.view()), self.num_vertex_chunks,
-                                              i, ctypes.byref(out)))
+        if self.balance == "vertices":
+            host_check(host().tch_partition_range(ctypes.byref(self.g.view()),
+                                                  self.num_vertex_chunks, i, ctypes.byref(out)))
+        else:
+            b = self.bounds()
+            host_check(host().tch_partition_span(ctypes.byref(self.g.view()), int(b[i]),
+                                                 int(b[i + 1]), ctypes.byref(out)))
         return HaloPartition(out.value)
 
 

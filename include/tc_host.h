@@ -109,6 +109,30 @@ typedef struct tch_partition_view {
 
 tc_status tch_partition_range(const tc_csr_view* dag, int32_t k, int32_t part,
                               tch_partition** out);
+
+/* Balanced contiguous ranges: bounds[0..k] with bounds[0] = 0, bounds[k] = n,
+ * part p owning [bounds[p], bounds[p+1]).  The criterion weighs each row u:
+ *   VERTICES  1 per row, exactly the reference split (graph_partition.cc:88-107)
+ *   EDGES     d+(u)
+ *   WEDGES    d+(u) + sum over v in N+(u) of d+(v)   (the elements an
+ *             edge-iterator count streams, bs_warp_vertex.cuh:13-22)
+ *   PARTNER   sum over v in N+(u) of 2 + min(d+(u), d+(v))  (the ints the
+ *             partner-scheme count reads for the row's edges)
+ * and each split point is the row where the prefix of weights is nearest to
+ * p/k of the total.  No counterpart in the reference, whose vertex split puts
+ * about half of an RMAT graph's edges into part 0. */
+enum {
+  TCH_BALANCE_VERTICES = 0,
+  TCH_BALANCE_EDGES = 1,
+  TCH_BALANCE_WEDGES = 2,
+  TCH_BALANCE_PARTNER = 3
+};
+tc_status tch_partition_bounds(const tc_csr_view* dag, int32_t k, int32_t balance,
+                               int32_t* bounds);
+/* The partition owning rows [row_begin, row_end) with its halo cache (the
+ * same structure tch_partition_range builds for a reference range). */
+tc_status tch_partition_span(const tc_csr_view* dag, int32_t row_begin, int32_t row_end,
+                             tch_partition** out);
 tc_status tch_partition_get(const tch_partition* p, tch_partition_view* out);
 void tch_partition_free(tch_partition* p);
 

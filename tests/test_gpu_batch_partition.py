@@ -57,11 +57,13 @@ def test_batch_interleaved_with_single_counts(ctx):
     b.free()
 
 
-@pytest.mark.parametrize("k", [2, 4, 5])
-def test_partition_halo_counts(ctx, k):
+@pytest.mark.parametrize("k,balance", [(2, "vertices"), (4, "vertices"), (5, "vertices"),
+                                       (4, "edges"), (4, "wedges"), (4, "partner"),
+                                       (7, "partner")])
+def test_partition_halo_counts(ctx, k, balance):
     g = Graph.rmat(1 << 16, 16, 3).oriented()
     total = oracle.tc_count(g.out_rowptr(), g.out_colidx())
-    pg = PartitionedGraph(g, k)
+    pg = PartitionedGraph(g, k, balance)
     got_sum = 0
     for p in range(k):
         part = pg.halo_partition(p)

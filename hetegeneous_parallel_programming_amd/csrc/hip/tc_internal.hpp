@@ -72,6 +72,19 @@ struct Partners {
   static int64_t bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs);
 };
 
+// Buffers of a local count of one graph: the partner entries' DAG edge
+// positions, per-edge and per-vertex counts, and per-part sums with their
+// split points.
+struct LocalBufs {
+  int64_t* pedge = nullptr;
+  uint32_t* ecount = nullptr;
+  unsigned long long* vcount = nullptr;
+  unsigned long long* pe = nullptr;  // [tck::kMaxParts]
+  int32_t* bounds = nullptr;         // [tck::kMaxParts + 1]
+  int64_t bytes = 0;
+  void release();
+};
+
 // Temporary arrays of a partner build over RP rowptr slots.
 struct BuildScratch {
   char* mem = nullptr;
@@ -84,7 +97,9 @@ struct BuildScratch {
 
 // Builds the partners of the graphs of `a` (descs on the device, pools and
 // P set; RP row slots) on stream s.  Stream-ordered, no sync.
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s);
+// pedge: null, or where to write each partner entry's DAG edge position.
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s,
+                       int64_t* pedge = nullptr);
 
 // Makes ctx's device current for the scope of one API call.
 struct DeviceGuard {
@@ -127,8 +142,10 @@ struct tc_graph {
   int32_t n = 0;
   int64_t nnz = 0;
   int32_t row_begin = 0, row_end = 0;
+  int64_t edge_begin = 0, edge_end = 0;  // DAG edge positions of rows [row_begin, row_end)
   int32_t max_out = 0;
   tcr::Partners partners;
+  tcr::LocalBufs local;
   int32_t nv_max = 0;   // partner rows it may need (nv_max + 1 slots = plan indices)
   int32_t nv = 0;       // partner rows in use
   double build_ms = 0;  // partner build time on the device

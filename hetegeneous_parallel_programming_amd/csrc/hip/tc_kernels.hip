@@ -206,16 +206,21 @@ __device__ __forceinline__ void group_sync() {
 // the G lanes of a group (gl = lane index in the group): a cuckoo hash when
 // it fits and converges (and the tune allows it), else the tree (da must
 // then fit the tree cap).
-template <int G>
+// kLocal: C (area words, parallel to slot) gets zeroed over the words the
+// set uses, for per-slot hit counters.
+template <int G, bool kLocal = false>
 __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
-                                                    int da, int gl, const Tune& tune) {
+                                                    int da, int gl, const Tune& tune, uint32_t* C = nullptr) {
   PivotSet S{slot, kSetTree, 0, 0};
   // tree_only: 1 = every set a tree; 2 = trees for sub-wave groups only
   const bool tree = tune.tree_only == 1 || (tune.tree_only == 2 && G < 64);
   const int lb = tree ? 0 : hash_log2_buckets(da, area);
   if (lb > 0) {
     const int B = 1 << lb;
-    for (int x = gl; x < 4 * B; x += G) slot[x] = kEmpty;
+    for (int x = gl; x < 4 * B; x += G) {
+      slot[x] = kEmpty;
+      if constexpr (kLocal) C[x] = 0;
+    }
     group_sync<G>();
     bool ok = true;
     for (int j = gl; j < da; j += G) ok = hash_insert(slot, lb, src[j], tune.hash_kicks) && ok;
@@ -227,9 +232,36 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
     }
   }
   stage_tree(slot, src, da, gl, G);
-  group_sync<G>();
   S.p0 = tree_levels(da);
+  if constexpr (kLocal)
+    for (int x = gl; x < (1 << S.p0); x += G) C[x] = 0;
+  group_sync<G>();
   return S;
+}
+
+// Per-hit attribution of a local count.  C: LDS hit counters parallel to
+// the set's slots (null for an HBM set, whose hits go to vcount directly);
+// EC: LDS hit counters of the G edges of the current batch; pedge: DAG edge
+// position of each of the row's partners; ecount/vcount: the outputs.
+struct LocalSink {
+  uint32_t* C;
+  uint32_t* EC;
+  const int64_t* pedge;
+  uint32_t* ecount;
+  unsigned long long* vcount;
+};
+
+// Adds the set's per-slot hit counters to vcount of the ids in those slots
+// (empty hash slots hold -1, tree padding INT_MAX, tree word 0 -1).
+template <int G>
+__device__ __forceinline__ void flush_set_hits(const int32_t* slot, const uint32_t* C, const PivotSet& S, int gl,
+                                               unsigned long long* vcount) {
+  const int used = S.mode == kSetHash ? (4 << (32 - S.p0)) : (1 << S.p0);
+  for (int x = gl; x < used; x += G) {
+    const uint32_t c = C[x];
+    const int key = slot[x];
+    if (c != 0 && key >= 0 && key != INT_MAX) atomicAdd(&vcount[key], (unsigned long long)c);
+  }
 }
 
 // Counts, for the edges j = jfirst, jfirst + jstride, ... of pivot u (list
@@ -261,10 +293,11 @@ struct RefEdges {
   }
 };
 
-template <int G, int kModes, int NF = kInFlight, class Src>
+template <int G, int kModes, int NF = kInFlight, bool kLocal = false, class Src>
 __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, const Src src, int da,
                                                 const PivotSet S, int jfirst, int jstride,
-                                                int32_t* sEnd, gint32** sBase, int gl) {
+                                                int32_t* sEnd, gint32** sBase, int gl,
+                                                const LocalSink& ls = LocalSink{}) {
   uint32_t hits = 0;
   const int32_t* A = S.A;
   for (int j0 = jfirst; j0 < da; j0 += jstride) {
@@ -300,21 +333,25 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
     };
     for (int t0 = 0; t0 < total; t0 += G * NF) {
       int w[NF];
+      int kq[NF];  // kLocal: batch edge of each id
       if ((TC_SPLIT_FULL & (G == 64 ? 1 : 2)) && t0 + G * NF <= total) {  // every slot holds an id
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
           const int t = t0 + q * G + gl;
           if (t >= cur_end) step_to(t);
           w[q] = cur_base[t];
+          if constexpr (kLocal) kq[q] = cur_k;
         }
       } else {
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
           const int t = t0 + q * G + gl;
           w[q] = INT_MIN;  // ids are >= 0: never found
+          if constexpr (kLocal) kq[q] = 0;
           if (t < total) {
             if (t >= cur_end) step_to(t);
             w[q] = cur_base[t];
+            if constexpr (kLocal) kq[q] = cur_k;
           }
         }
       }
@@ -323,9 +360,21 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
         const int sh = S.p0;
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
-          const int2 x = *(const int2*)(Tb + (hash_bucket(w[q], kHashA0, sh) << 3));
-          const int2 y = *(const int2*)(Tb + S.p1 + (hash_bucket(w[q], kHashA1, sh) << 3));
-          hits += ((x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q])) ? 1u : 0u;
+          const uint32_t b0 = hash_bucket(w[q], kHashA0, sh), b1 = hash_bucket(w[q], kHashA1, sh);
+          const int2 x = *(const int2*)(Tb + (b0 << 3));
+          const int2 y = *(const int2*)(Tb + S.p1 + (b1 << 3));
+          const bool f = (x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q]);
+          hits += f ? 1u : 0u;
+          if constexpr (kLocal) {
+            if (f) {
+              const int t1 = S.p1 >> 2;  // word offset of table 1
+              const int idx = x.x == w[q] ? 2 * (int)b0
+                            : x.y == w[q] ? 2 * (int)b0 + 1
+                            : y.x == w[q] ? t1 + 2 * (int)b1 : t1 + 2 * (int)b1 + 1;
+              atomicAdd(&ls.C[idx], 1u);
+              atomicAdd(&ls.EC[kq[q]], 1u);
+            }
+          }
         }
       } else if ((kModes & kModeTree) && ((kModes == kModeTree) || S.mode == kSetTree)) {
         // h descent steps; then the lower bound is k with its trailing one
@@ -348,7 +397,14 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
         for (int q = 0; q < NF; ++q) {
           const unsigned k = (unsigned)kb[q] >> 2;
           const unsigned lo = (k >> __builtin_ctz(~k)) >> 1;
-          hits += (A[lo] == w[q]) ? 1u : 0u;
+          const bool f = A[lo] == w[q];
+          hits += f ? 1u : 0u;
+          if constexpr (kLocal) {
+            if (f) {
+              atomicAdd(&ls.C[lo], 1u);
+              atomicAdd(&ls.EC[kq[q]], 1u);
+            }
+          }
         }
       } else if (kModes & kModeHbm) {
         int lo[NF];
@@ -361,8 +417,23 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
           len -= half;
         }
 #pragma unroll
-        for (int q = 0; q < NF; ++q) hits += (A[lo[q]] == w[q]) ? 1u : 0u;
+        for (int q = 0; q < NF; ++q) {
+          const bool f = A[lo[q]] == w[q];
+          hits += f ? 1u : 0u;
+          if constexpr (kLocal) {
+            if (f) {
+              atomicAdd(&ls.vcount[w[q]], 1ull);
+              atomicAdd(&ls.EC[kq[q]], 1u);
+            }
+          }
+        }
       }
+    }
+    if constexpr (kLocal) {
+      // every edge of the batch is counted in this batch only: store it
+      wave_sync();
+      if (j < da) ls.ecount[ls.pedge[j]] = ls.EC[gl];
+      ls.EC[gl] = 0;
     }
     wave_sync();
   }
@@ -604,12 +675,15 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
     }
     const uint32_t gm = (uint32_t)(__ballot(own) >> shift) & ((1u << kBuildG) - 1u);
     if (e < e1) {
+      int64_t slot;
       if (own) {
-        pci[w + __popc(gm & below)] = list_ref(d.ci_off + vb, dv);
+        slot = w + __popc(gm & below);
+        pci[slot] = list_ref(d.ci_off + vb, dv);
       } else {
-        const int64_t slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
+        slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
         pci[slot] = list_ref(d.ci_off + e0, du);
       }
+      if (b.pedge) b.pedge[d.pci_off + slot] = d.ci_off + e;  // both ways it is edge (u, v)
     }
     w += __popc(gm);
   }
@@ -809,9 +883,25 @@ struct Acc {
   }
 };
 
-template <int G, int AREA>
+// LDS of a local count beside CountLds: per-slot hit counters parallel to
+// the set areas, per-batch-edge hit counters parallel to End.
+template <bool kLocal>
+struct LocalLds {
+  __device__ uint32_t* c() { return nullptr; }
+  __device__ uint32_t* ec() { return nullptr; }
+};
+template <>
+struct __align__(16) LocalLds<true> {
+  uint32_t C[kAreaBlock];
+  uint32_t EC[kBlock];
+  __device__ uint32_t* c() { return C; }
+  __device__ uint32_t* ec() { return EC; }
+};
+
+template <int G, int AREA, bool kLocal>
 __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* items, int64_t first,
-                                           int64_t last, CountLds& L, bool batch, Acc& acc) {
+                                           int64_t last, CountLds& L, LocalLds<kLocal>& LL, bool batch,
+                                           Acc& acc) {
   constexpr int NGRP = kBlock / G;
   // With G = 64 a group is a wave: its pivot and everything derived from it
   // is wave-uniform and lives in scalar registers.
@@ -819,20 +909,28 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   int32_t* slot = L.A + grp * AREA;
   int32_t* sEnd = L.End + grp * G;
   gint32** sBase = L.Base + grp * G;
+  uint32_t* C = kLocal ? LL.c() + grp * AREA : nullptr;
   for (int64_t i = first + grp; i < last; i += NGRP) {
     const PRow pr = partner_row(a, (int64_t)items[i]);
-    const PivotSet S = build_pivot_set<G>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set), gl, a.tune);
-    const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup>(
-        a.ci, RefEdges{a.pci + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl);
+    const PivotSet S = build_pivot_set<G, kLocal>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set), gl,
+                                                  a.tune, C);
+    const LocalSink ls{C, kLocal ? LL.ec() + grp * G : nullptr, kLocal ? a.local.pedge + pr.pb : nullptr,
+                       a.local.ecount, a.local.vcount};
+    const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kLocal>(
+        a.ci, RefEdges{a.pci + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl, ls);
+    if constexpr (kLocal) {
+      group_sync<G>();
+      flush_set_hits<G>(slot, C, S, gl, a.local.vcount);
+    }
     acc.to_graph(pr.g, batch, a.bufs.outs);
     acc.sum += h;
     wave_sync();
   }
 }
 
-template <bool kLdsPivot>
-__device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, CountLds& L, bool batch,
-                                            Acc& acc) {
+template <bool kLdsPivot, bool kLocal>
+__device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, CountLds& L,
+                                            LocalLds<kLocal>& LL, bool batch, Acc& acc) {
   const PRow pr = partner_row(a, (int64_t)item);
   const int32_t* ci = a.ci;
   const int32_t* set = ci + list_start(pr.set);
@@ -840,15 +938,22 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const RefEdges part{a.pci + pr.pb};
   const int np = (int)(pr.pe - pr.pb);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t* EC = kLocal ? LL.ec() + wid * 64 : nullptr;
   uint32_t h;
   if (kLdsPivot) {
-    const PivotSet S = build_pivot_set<kBlock>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune);
-    h = pivot_edges<64, kModeHash | kModeTree>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
-                                               L.Base + wid * 64, lane);
+    const PivotSet S = build_pivot_set<kBlock, kLocal>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune, LL.c());
+    const LocalSink ls{LL.c(), EC, kLocal ? a.local.pedge + pr.pb : nullptr, a.local.ecount, a.local.vcount};
+    h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock,
+                                                                   L.End + wid * 64, L.Base + wid * 64, lane, ls);
+    if constexpr (kLocal) {
+      __syncthreads();
+      flush_set_hits<kBlock>(L.A, LL.c(), S, threadIdx.x, a.local.vcount);
+    }
   } else {
     const PivotSet S{set, kSetHbm, ds, 0};
-    h = pivot_edges<64, kModeHbm>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64, L.Base + wid * 64,
-                                  lane);
+    const LocalSink ls{nullptr, EC, kLocal ? a.local.pedge + pr.pb : nullptr, a.local.ecount, a.local.vcount};
+    h = pivot_edges<64, kModeHbm, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
+                                                     L.Base + wid * 64, lane, ls);
   }
   acc.to_graph(pr.g, batch, a.bufs.outs);
   acc.sum += h;
@@ -872,12 +977,20 @@ __device__ __forceinline__ int tile_pivots(int bin, int rounds) {
   }
 }
 
+// Local variants hold twice the LDS of a plain count: half the residency.
+constexpr int kLocalWavesPerEU = 4;
+
 // One persistent kernel per bin, specialised so that each keeps only its own
 // working set in registers.  Blocks pull tiles from the bin's queue head.
-template <int BIN>
-__global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(LaunchArgs a) {
+// kLocal: also the per-edge and per-vertex attribution of a.local (one
+// graph only).
+template <int BIN, bool kLocal>
+__global__ __launch_bounds__(kBlock, kLocal ? kLocalWavesPerEU : TC_COUNT_WAVES_PER_EU) void count_kernel(
+    LaunchArgs a) {
   __shared__ CountLds L;
-  const bool batch = a.ngraphs > 1;
+  __shared__ LocalLds<kLocal> LL;
+  const bool batch = !kLocal && a.ngraphs > 1;
+  if constexpr (kLocal) LL.EC[threadIdx.x] = 0;  // ordered by the barrier below
   if (threadIdx.x == 0) {
     int64_t seg = 0;
     for (int b = 0; b < BIN; ++b) seg += a.bufs.counters[kCntBinCount + b];
@@ -898,12 +1011,12 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
     if (t >= ntiles) break;
     const int64_t first = t * piv;
     const int64_t last = min(first + (int64_t)piv, cnt);
-    if (BIN == BIN_GLOBAL) block_pivot<false>(a, items[first], L, batch, acc);
-    else if (BIN == BIN_BLOCK) block_pivot<true>(a, items[first], L, batch, acc);
-    else if (BIN == BIN_WAVE) group_tile<64, kAreaWave>(a, items, first, last, L, batch, acc);
-    else if (BIN == BIN_G32) group_tile<32, kAreaG32>(a, items, first, last, L, batch, acc);
-    else if (BIN == BIN_G16) group_tile<16, kAreaG16>(a, items, first, last, L, batch, acc);
-    else group_tile<8, kAreaG8>(a, items, first, last, L, batch, acc);
+    if (BIN == BIN_GLOBAL) block_pivot<false, kLocal>(a, items[first], L, LL, batch, acc);
+    else if (BIN == BIN_BLOCK) block_pivot<true, kLocal>(a, items[first], L, LL, batch, acc);
+    else if (BIN == BIN_WAVE) group_tile<64, kAreaWave, kLocal>(a, items, first, last, L, LL, batch, acc);
+    else if (BIN == BIN_G32) group_tile<32, kAreaG32, kLocal>(a, items, first, last, L, LL, batch, acc);
+    else if (BIN == BIN_G16) group_tile<16, kAreaG16, kLocal>(a, items, first, last, L, LL, batch, acc);
+    else group_tile<8, kAreaG8, kLocal>(a, items, first, last, L, LL, batch, acc);
     if (batch) {
       // Items are ordered by graph, so a tile almost always belongs to one
       // graph: then one block-reduced atomic; otherwise one per lane.
@@ -922,6 +1035,62 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void count_kernel(La
     const uint64_t tot = block_sum(acc.sum, L.red);
     if (threadIdx.x == 0 && tot) atomicAdd(&a.bufs.outs[0], tot);
   }
+}
+
+// ---------------------------------------------------------------------------
+// local count, second pass: per-edge counts to vertices and parts
+// ---------------------------------------------------------------------------
+// Part of vertex u under split points b[0..k] (b[0] = 0 <= u < b[k]): the
+// last p with b[p] <= u, which skips empty parts.
+__device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
+  int lo = 0, hi = k;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (b[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// A group of kBuildG lanes per counted row u.  Edge (u, v) with count c
+// (its triangles with u and v as the two lowest corners) adds c to
+// vcount[u] and vcount[v], and to the part that computes it in the
+// reference's push/pull kernel (successful_push_pull/
+// bs_warp_vertex_nvshmem.cuh:127-136): u's part when v is in the same part
+// or d+(u) >= d+(v) (pull), else v's part (push).
+__global__ __launch_bounds__(kBlock) void local_post_kernel(const int64_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci, int32_t row_begin,
+                                                            int32_t row_end, const uint32_t* __restrict__ ecount,
+                                                            unsigned long long* vcount, const int32_t* bounds,
+                                                            int32_t parts, unsigned long long* pe) {
+  __shared__ unsigned long long s_pe[kMaxParts];
+  for (int i = threadIdx.x; i < parts; i += kBlock) s_pe[i] = 0;
+  __syncthreads();
+  const int64_t u = row_begin + ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
+  const int gl = threadIdx.x % kBuildG;
+  if (u < row_end) {  // whole groups
+    const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
+    const int pu = parts > 0 ? part_of(bounds, parts, u) : 0;
+    unsigned long long rs = 0;
+    for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
+      const uint32_t c = ecount[e];
+      if (c == 0) continue;
+      const int v = ci[e];
+      rs += c;
+      if (vcount) atomicAdd(&vcount[v], (unsigned long long)c);
+      if (parts > 0) {
+        const int pv = part_of(bounds, parts, v);
+        const int owner = (pv != pu && du < rp[v + 1] - rp[v]) ? pv : pu;
+        atomicAdd(&s_pe[owner], (unsigned long long)c);
+      }
+    }
+#pragma unroll
+    for (int o = kBuildG / 2; o >= 1; o >>= 1) rs += __shfl_xor(rs, o, kBuildG);
+    if (gl == 0 && rs && vcount) atomicAdd(&vcount[u], rs);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < parts; i += kBlock)
+    if (s_pe[i]) atomicAdd(&pe[i], s_pe[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -999,20 +1168,21 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int BIN>
+template <int BIN, bool kLocal>
 static int bin_blocks_per_cu() {
   static const int n = [] {
     int k = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, count_kernel<BIN>, kBlock, 0) != hipSuccess) k = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, count_kernel<BIN, kLocal>, kBlock, 0) != hipSuccess)
+      k = 1;
     return k > 0 ? k : 1;
   }();
   return n;
 }
 
-template <int BIN>
+template <int BIN, bool kLocal = false>
 static void launch_bin(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
-  const int g = grid > 0 ? grid : bin_blocks_per_cu<BIN>() * cus;
-  count_kernel<BIN><<<g, kBlock, 0, s>>>(a);
+  const int g = grid > 0 ? grid : bin_blocks_per_cu<BIN, kLocal>() * cus;
+  count_kernel<BIN, kLocal><<<g, kBlock, 0, s>>>(a);
 }
 
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
@@ -1022,6 +1192,28 @@ hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
   launch_bin<BIN_G32>(a, cus, grid, s);
   launch_bin<BIN_G16>(a, cus, grid, s);
   launch_bin<BIN_G8>(a, cus, grid, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s) {
+  if (a.ngraphs != 1 || !a.local.pedge || !a.local.ecount || !a.local.vcount) return hipErrorInvalidValue;
+  launch_bin<BIN_GLOBAL, true>(a, cus, 0, s);
+  launch_bin<BIN_BLOCK, true>(a, cus, 0, s);
+  launch_bin<BIN_WAVE, true>(a, cus, 0, s);
+  launch_bin<BIN_G32, true>(a, cus, 0, s);
+  launch_bin<BIN_G16, true>(a, cus, 0, s);
+  launch_bin<BIN_G8, true>(a, cus, 0, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_local_post(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
+                             const uint32_t* ecount, unsigned long long* vcount, const int32_t* bounds,
+                             int32_t parts, unsigned long long* pe, hipStream_t s) {
+  if (parts < 0 || parts > kMaxParts || (parts > 0 && (!bounds || !pe))) return hipErrorInvalidValue;
+  const int64_t rows = (int64_t)row_end - row_begin;
+  if (rows <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((rows * kBuildG + kBlock - 1) / kBlock);
+  local_post_kernel<<<blocks, kBlock, 0, s>>>(rp, ci, row_begin, row_end, ecount, vcount, bounds, parts, pe);
   return hipGetLastError();
 }
 
@@ -1038,7 +1230,7 @@ hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipS
 
 int count_blocks_per_cu() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel<BIN_WAVE>, kBlock, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel<BIN_WAVE, false>, kBlock, 0) != hipSuccess)
     return 1;
   return n > 0 ? n : 1;
 }

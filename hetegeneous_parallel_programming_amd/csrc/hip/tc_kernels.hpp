@@ -114,6 +114,16 @@ struct VRow {
   int32_t g;
 };
 
+// Outputs of a local count (one graph only).  pedge: DAG edge position of
+// every partner entry (parallel to the partner pool); ecount: per DAG edge
+// position, |N+(u) ∩ N+(v)| of edge (u, v); vcount: per vertex, the
+// triangles with a corner there.  All null in a plain count.
+struct LocalOut {
+  const int64_t* pedge;
+  uint32_t* ecount;
+  unsigned long long* vcount;
+};
+
 struct LaunchArgs {
   const GraphDesc* descs;
   int32_t ngraphs;
@@ -126,6 +136,7 @@ struct LaunchArgs {
   const int32_t* nv;    // [ngraphs] partner rows in use
   Tune tune;
   PlanBufs bufs;
+  LocalOut local;
 };
 
 // Planning blocks of a launch over P plan indices.
@@ -140,6 +151,7 @@ struct BuildBufs {
   int32_t* kc;          // [RP] partner rows per DAG row
   int64_t* voff;        // [RP] partner-row offsets (pool-global exclusive scan)
   int64_t* part;        // [scan_partials(RP)] scan partials
+  int64_t* pedge;       // null, or [partner pool]: DAG edge position of each partner entry
 };
 int64_t scan_partials(int64_t n);
 
@@ -155,6 +167,17 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s);
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s);
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
                         uint64_t* out, int grid, hipStream_t s);
+// Count of one graph that also fills a.local (ecount and vcount zeroed by
+// the caller): same plan, local variants of the bin kernels.
+hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s);
+// After launch_count_local: adds each counted edge's ecount to vcount of
+// both endpoints (if vcount) and sums it into pe[owner] by the reference
+// push/pull rule over the split bounds[0..parts] (if parts > 0, at most
+// kMaxParts; pe zeroed by the caller).
+constexpr int kMaxParts = 1024;
+hipError_t launch_local_post(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
+                             const uint32_t* ecount, unsigned long long* vcount, const int32_t* bounds,
+                             int32_t parts, unsigned long long* pe, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
 // Resident workgroups per CU of the persistent and plain kernels.
 int count_blocks_per_cu();

@@ -149,11 +149,22 @@ void BuildScratch::release() {
   *this = BuildScratch();
 }
 
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s) {
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s,
+                       int64_t* pedge) {
   tc_status st = bs.ensure(RP);
   if (st != TC_OK) return st;
+  bs.b.pedge = pedge;
   TC_HIP_TRY(tck::launch_partners(a, RP, bs.b, s));
   return TC_OK;
+}
+
+void LocalBufs::release() {
+  dev_free(pedge);
+  dev_free(ecount);
+  dev_free(vcount);
+  dev_free(pe);
+  dev_free(bounds);
+  *this = LocalBufs();
 }
 
 tc_tune default_tune() {
@@ -205,6 +216,8 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   g->nnz = dag->nnz;
   g->row_begin = row_begin;
   g->row_end = row_end;
+  g->edge_begin = dag->rowptr[row_begin];
+  g->edge_end = dag->rowptr[row_end];
   g->max_out = max_out;
   g->nv_max = tck::partner_rows_max(g->n, g->nnz);
   auto cleanup = [&]() {
@@ -392,6 +405,7 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) {
   dev_free(g->ci);
   dev_free(g->d_desc);
   g->partners.release();
+  g->local.release();
   delete g;
   return TC_OK;
 }
@@ -405,7 +419,7 @@ tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) {
   out->nnz = g->nnz;
   out->partner_rows = g->nv;
   out->build_ms = g->build_ms;
-  out->bytes = g->bytes;
+  out->bytes = g->bytes + g->local.bytes;
   return TC_OK;
 }
 
@@ -463,6 +477,103 @@ tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* st
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
   return finish_stats(ctx, stats, t0, false, P);
+}
+
+// Fills the LaunchArgs of a count of g (descs, pools, partners, tune, plan
+// scratch).
+static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
+  tck::LaunchArgs a{};
+  a.descs = g->d_desc;
+  a.ngraphs = 1;
+  a.P = (int64_t)g->nv_max + 1;
+  a.rp = g->rp;
+  a.ci = g->ci;
+  g->partners.into(a);
+  a.tune = to_kernel_tune(ctx->tune);
+  a.bufs = ctx->scratch.bufs();
+  return a;
+}
+
+// First local count of g: its buffers, and a partner rebuild that also
+// records every partner entry's DAG edge position.
+static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
+  LocalBufs& L = g->local;
+  if (L.pedge) return TC_OK;
+  const int64_t m = std::max<int64_t>(g->nnz, 1);
+  tc_status st = dev_alloc((void**)&L.pedge, sizeof(int64_t) * (size_t)m, "partner edge positions");
+  if (st == TC_OK) st = dev_alloc((void**)&L.ecount, sizeof(uint32_t) * (size_t)m, "edge counts");
+  if (st == TC_OK)
+    st = dev_alloc((void**)&L.vcount, sizeof(unsigned long long) * ((size_t)g->n + 1), "vertex counts");
+  if (st == TC_OK) st = dev_alloc((void**)&L.pe, sizeof(unsigned long long) * tck::kMaxParts, "part counts");
+  if (st == TC_OK) st = dev_alloc((void**)&L.bounds, sizeof(int32_t) * (tck::kMaxParts + 1), "split points");
+  if (st == TC_OK) {
+    tck::LaunchArgs a = count_args(ctx, g);
+    BuildScratch bs;
+    st = launch_build(a, (int64_t)g->n + 1, bs, ctx->stream, L.pedge);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    bs.release();
+    if (st == TC_OK && e != hipSuccess)
+      st = fail(TC_EDEVICE, std::string("partner rebuild: ") + hipGetErrorString(e));
+  }
+  if (st != TC_OK) {
+    L.release();
+    return st;
+  }
+  L.bytes = (12 * m + 8 * ((int64_t)g->n + 1) + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1));
+  return TC_OK;
+}
+
+tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* out, tc_stats* stats) {
+  if (!ctx || !g || !total || !out) return fail(TC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  const int32_t k = out->parts;
+  if (k < 0 || k > tck::kMaxParts) return fail(TC_EINVAL, "parts must be in [0, 1024]");
+  if (k > 0) {
+    if (!out->bounds || !out->part_counts) return fail(TC_EINVAL, "parts > 0 needs bounds and part_counts");
+    if (out->bounds[0] != 0 || out->bounds[k] != g->n)
+      return fail(TC_EINVAL, "split points must start at 0 and end at the graph's row count");
+    for (int32_t p = 0; p < k; ++p)
+      if (out->bounds[p] > out->bounds[p + 1]) return fail(TC_EINVAL, "split points must not decrease");
+  }
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  const double t0 = now_ms();
+  const int64_t P = (int64_t)g->nv_max + 1;
+  tc_status st = ctx->scratch.ensure(P, 1);
+  if (st == TC_OK) st = ensure_local(ctx, g);
+  if (st != TC_OK) return st;
+  LocalBufs& L = g->local;
+  tck::LaunchArgs a = count_args(ctx, g);
+  a.local = tck::LocalOut{L.pedge, L.ecount, L.vcount};
+  hipStream_t s = ctx->stream;
+  if (k > 0)
+    TC_HIP_TRY(hipMemcpyAsync(L.bounds, out->bounds, sizeof(int32_t) * ((size_t)k + 1), hipMemcpyHostToDevice, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
+  TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
+  TC_HIP_TRY(tck::launch_zero(L.ecount, std::max<int64_t>(g->nnz, 1), s));
+  TC_HIP_TRY(tck::launch_zero(L.vcount, 2 * ((int64_t)g->n + 1), s));
+  TC_HIP_TRY(tck::launch_zero(L.pe, 2 * (int64_t)tck::kMaxParts, s));
+  TC_HIP_TRY(tck::launch_plan(a, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, s));
+  TC_HIP_TRY(tck::launch_local_post(g->rp, g->ci, g->row_begin, g->row_end, L.ecount, L.vcount,
+                                    k > 0 ? L.bounds : nullptr, k, L.pe, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,
+                            hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipStreamSynchronize(s));
+  *total = ctx->pinned[0];
+  // pageable destinations: synchronous copies after the stream is done
+  if (out->edge_counts && g->edge_end > g->edge_begin)
+    TC_HIP_TRY(hipMemcpy(out->edge_counts, L.ecount + g->edge_begin,
+                         sizeof(uint32_t) * (size_t)(g->edge_end - g->edge_begin), hipMemcpyDeviceToHost));
+  if (out->vertex_counts && g->n > 0)
+    TC_HIP_TRY(hipMemcpy(out->vertex_counts, L.vcount, sizeof(uint64_t) * (size_t)g->n, hipMemcpyDeviceToHost));
+  if (k > 0)
+    TC_HIP_TRY(hipMemcpy(out->part_counts, L.pe, sizeof(uint64_t) * (size_t)k, hipMemcpyDeviceToHost));
+  return finish_stats(ctx, stats, t0, true, g->nv);
 }
 
 tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter) {

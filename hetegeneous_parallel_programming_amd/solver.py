@@ -48,6 +48,11 @@ class Tune(ctypes.Structure):
         return t
 
 
+class Local(ctypes.Structure):
+    _fields_ = [("edge_counts", ctypes.POINTER(ctypes.c_uint32)), ("vertex_counts", nat.u64p),
+                ("parts", nat.c_i32), ("bounds", nat.i32p), ("part_counts", nat.u64p)]
+
+
 class GraphInfo(ctypes.Structure):
     _fields_ = [("graphs", nat.c_i32), ("rows", nat.c_i64), ("nnz", nat.c_i64),
                 ("partner_rows", nat.c_i64), ("build_ms", nat.c_dbl), ("bytes", nat.c_i64)]
@@ -110,6 +115,7 @@ def device_lib():
     S(L, "tc_graph_bytes", nat.c_i64, [P])
     S(L, "tc_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_plain", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
+    S(L, "tc_count_local", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Local), ctypes.POINTER(Stats)])
     S(L, "tc_batch_create", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_i32, nat.c_u32, PP])
     S(L, "tc_batch_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_batch_count_each", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(nat.c_dbl),
@@ -281,6 +287,37 @@ class Context:
         st = Stats()
         _check(device_lib().tc_count_plain(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
         return int(total.value), st.as_dict()
+
+    def count_local(self, dg: DeviceGraph, edges: bool = True, vertices: bool = True,
+                    bounds=None):
+        """Count with per-edge, per-vertex and per-part attribution (``tc_count_local``).
+
+        Returns (total, dict) with ``edge_counts`` (uint32 per DAG edge of the
+        counted rows, CSR order), ``vertex_counts`` (uint64 per row of the
+        uploaded graph), ``part_counts`` (uint64 per part of the split
+        ``bounds``, the reference push/pull kernel's per-PE sums) and
+        ``stats``; outputs not asked for are None."""
+        n = int(dg.info()["rows"])
+        lo = Local()
+        ec = vc = pc = None
+        if edges:
+            ec = np.zeros(dg.edges, np.uint32)
+            lo.edge_counts = ec.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        if vertices:
+            vc = np.zeros(n, np.uint64)
+            lo.vertex_counts = vc.ctypes.data_as(nat.u64p)
+        if bounds is not None:
+            b = np.ascontiguousarray(bounds, dtype=np.int32)
+            pc = np.zeros(b.size - 1, np.uint64)
+            lo.parts = b.size - 1
+            lo.bounds = b.ctypes.data_as(nat.i32p)
+            lo.part_counts = pc.ctypes.data_as(nat.u64p)
+        total = nat.c_u64(0)
+        st = Stats()
+        _check(device_lib().tc_count_local(self._h, dg._h, ctypes.byref(total), ctypes.byref(lo),
+                                           ctypes.byref(st)))
+        return int(total.value), {"edge_counts": ec, "vertex_counts": vc, "part_counts": pc,
+                                  "stats": st.as_dict()}
 
     def batch(self, graphs: list, check: int = nat.TC_CHECK_SHAPE) -> Batch:
         views = (CsrView * max(len(graphs), 1))()

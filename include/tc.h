@@ -121,6 +121,38 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
  * multigpu_nvshmem_pushpull.cu:41-51). */
 tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
 
+/* tc_count_local: the same count, also attributing every triangle to its
+ * edges, its corners and the parts of a split.  Outputs (host memory, each
+ * optional):
+ *   edge_counts[e]   for each DAG edge e = (u, v) of the counted rows, in
+ *                    CSR order from the first counted row (nnz entries for a
+ *                    whole graph, local_nnz for a partition):
+ *                    |N+(u) ∩ N+(v)|, the per-edge term the reference sums
+ *                    (bs_warp_vertex.cuh:20);
+ *   vertex_counts[v] for each of the graph's n rows: the counted triangles
+ *                    with a corner at v (for a whole graph the local
+ *                    clustering numerator; for a partition, the share of the
+ *                    triangles whose lowest vertex it owns);
+ *   part_counts[p]   for p < parts (1..1024) under split points
+ *                    bounds[0..parts] (0 = bounds[0] <= ... <= bounds[parts]
+ *                    = n): the sum each PE of the reference's push/pull
+ *                    kernel reports (successful_push_pull/
+ *                    bs_warp_vertex_nvshmem.cuh:127-136): an edge's count
+ *                    goes to u's part when v is in the same part or
+ *                    d+(u) >= d+(v), else to v's part.
+ * The reference emits none of these; they make a split's attribution
+ * checkable.  One graph only (not batches); the first call adds the partner
+ * entries' edge positions and per-edge and per-vertex buffers to the graph
+ * (tc_graph_get_info reports the bytes). */
+typedef struct tc_local {
+  uint32_t* edge_counts;
+  uint64_t* vertex_counts;
+  int32_t parts;
+  const int32_t* bounds;
+  uint64_t* part_counts;
+} tc_local;
+tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* out, tc_stats* stats);
+
 /* ---- batches of graphs --------------------------------------------------
  * Many graphs resident at once; tc_batch_count runs one hipGraph-captured
  * sequence (binning + one persistent kernel pulling (graph, degree-bin)

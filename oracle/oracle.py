@@ -76,6 +76,12 @@ def lib():
                                            _i64p]
         L.oracle_tc_per_vertex.restype = None
         L.oracle_tc_per_vertex.argtypes = [ctypes.c_int32, _i64p, _i32p, _u64p]
+        L.oracle_tc_edge_counts.restype = None
+        L.oracle_tc_edge_counts.argtypes = [ctypes.c_int32, _i64p, _i32p,
+                                            ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_tc_pushpull_bounds.restype = None
+        L.oracle_tc_pushpull_bounds.argtypes = [ctypes.c_int32, _i64p, _i32p, ctypes.c_int32,
+                                                _i32p, _u64p]
         L.oracle_tc_pushpull_pe.restype = ctypes.c_uint64
         L.oracle_tc_pushpull_pe.argtypes = [ctypes.c_int32, _i64p, _i32p, ctypes.c_int32,
                                             ctypes.c_int32]
@@ -161,6 +167,25 @@ def tc_per_vertex(rowptr, colidx):
     n = rowptr.size - 1
     out = np.zeros(n, dtype=np.uint64)
     lib().oracle_tc_per_vertex(n, _p(rowptr, _i64p), _p(colidx, _i32p), _p(out, _u64p))
+    return out
+
+
+def tc_edge_counts(rowptr, colidx):
+    """|N+(u) ∩ N+(v)| for every DAG edge (u, v), in CSR order."""
+    rowptr, colidx = _c64(rowptr), _c32(colidx)
+    out = np.zeros(colidx.size, dtype=np.uint32)
+    lib().oracle_tc_edge_counts(rowptr.size - 1, _p(rowptr, _i64p), _p(colidx, _i32p),
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out
+
+
+def tc_pushpull_bounds(rowptr, colidx, bounds):
+    """Per-PE push/pull sums when PE p owns ids [bounds[p], bounds[p+1])."""
+    rowptr, colidx = _c64(rowptr), _c32(colidx)
+    b = _c32(bounds)
+    out = np.zeros(b.size - 1, dtype=np.uint64)
+    lib().oracle_tc_pushpull_bounds(rowptr.size - 1, _p(rowptr, _i64p), _p(colidx, _i32p),
+                                    b.size - 1, _p(b, _i32p), _p(out, _u64p))
     return out
 
 

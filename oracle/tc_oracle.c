@@ -22,6 +22,9 @@
  *   oracle_tc_pushpull_pe     the per-PE attribution of the push/pull kernel:
  *                             local edges (:36-38), pulled edges with d+(u) >= d+(v)
  *                             (:127-136), pushed-in edges with d+(u) < d+(v) (:228-232)
+ *   oracle_tc_pushpull_bounds the same attribution under any split points
+ *   oracle_tc_edge_counts     per DAG edge (u, v), the term bs_warp_vertex.cuh:20 adds:
+ *                             |N+(u) ∩ N+(v)| by the graph.cc:376-391 merge
  *
  * Pinning: see the header of oracle.py and the tests under tests/ -- closed
  * forms, brute force, networkx/scipy golden fixtures and invariants.
@@ -247,4 +250,41 @@ uint64_t oracle_tc_pushpull_pe(int32_t n, const int64_t* rowptr, const int32_t* 
     }
   }
   return total;
+}
+
+/* Per DAG edge e = (u, v), in CSR order: |N+(u) ∩ N+(v)| (the term the
+ * loop nest of bs_warp_vertex.cuh:13-22 adds for that edge). */
+void oracle_tc_edge_counts(int32_t n, const int64_t* rowptr, const int32_t* colidx, uint32_t* out) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t u = 0; u < n; u++) {
+    const int32_t* a = colidx + rowptr[u];
+    int64_t da = rowptr[u + 1] - rowptr[u];
+    for (int64_t j = 0; j < da; j++) {
+      int32_t v = a[j];
+      out[rowptr[u] + j] = (uint32_t)oracle_intersect_merge(a, da, colidx + rowptr[v],
+                                                            rowptr[v + 1] - rowptr[v]);
+    }
+  }
+}
+
+/* Per-PE sums of the push/pull kernel (rule of oracle_tc_pushpull_pe) when
+ * PE p owns ids [bounds[p], bounds[p+1]). */
+void oracle_tc_pushpull_bounds(int32_t n, const int64_t* rowptr, const int32_t* colidx, int32_t k,
+                               const int32_t* bounds, uint64_t* out) {
+  int32_t* part = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  for (int32_t p = 0; p < k; p++)
+    for (int32_t v = bounds[p]; v < bounds[p + 1]; v++) part[v] = p;
+  memset(out, 0, sizeof(uint64_t) * (size_t)k);
+  for (int64_t u = 0; u < n; u++) {
+    const int32_t* a = colidx + rowptr[u];
+    int64_t da = rowptr[u + 1] - rowptr[u];
+    for (int64_t j = 0; j < da; j++) {
+      int32_t v = a[j];
+      int64_t db = rowptr[v + 1] - rowptr[v];
+      int32_t owner = part[u];
+      if (part[v] != part[u] && da < db) owner = part[v];
+      out[owner] += (uint64_t)oracle_intersect_merge(a, da, colidx + rowptr[v], db);
+    }
+  }
+  free(part);
 }

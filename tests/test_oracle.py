@@ -173,6 +173,64 @@ def test_pushpull_attribution_partitions_the_total(k):
     assert sum(oracle.tc_pushpull_pe(drp, dci, k, pe) for pe in range(k)) == total
 
 
+# Six vertices, ten edges, six triangles ({0,2,4} {0,2,5} {0,4,5} {0,1,5}
+# {2,4,5} {2,3,4}).  Undirected degrees 4 2 4 2 4 4 orient it to
+# N+ = 0:[2,4,5] 1:[0,5] 2:[4,5] 3:[2,4] 4:[5] 5:[], d+ = 3 2 2 2 1 0.
+# Per edge |N+(u) ∩ N+(v)| in CSR order: (0,2) 2 (0,4) 1 (0,5) 0 (1,0) 1
+# (1,5) 0 (2,4) 1 (2,5) 0 (3,2) 1 (3,4) 0 (4,5) 0.  Split {0} | {1..5}:
+# (0,2) and (0,4) cross with d+(0) >= d+(v): pulled by part 0 (3); (1,0)
+# crosses with d+(1) = 2 < d+(0) = 3: pushed to 0's part (1); (2,4), (3,2)
+# stay in part 1 (2).
+PUSH_EDGES = [(0, 5), (3, 4), (0, 2), (0, 1), (2, 3), (1, 5), (2, 4), (0, 4), (2, 5), (4, 5)]
+
+
+def test_edge_counts_and_push_attribution_by_hand():
+    s, d = zip(*PUSH_EDGES)
+    rp, ci = dag_of_pairs(6, s, d)
+    assert ci.tolist() == [2, 4, 5, 0, 5, 4, 5, 2, 4, 5]
+    assert oracle.tc_edge_counts(rp, ci).tolist() == [2, 1, 0, 1, 0, 1, 0, 1, 0, 0]
+    assert oracle.tc_pushpull_bounds(rp, ci, [0, 1, 6]).tolist() == [4, 2]
+    assert oracle.tc_per_vertex(rp, ci).tolist() == [4, 1, 4, 1, 4, 4]
+
+
+@pytest.mark.parametrize("n,d,seed", [(300, 12, 1), (2000, 20, 2)])
+def test_edge_counts_match_set_intersection(n, d, seed):
+    rp, ci = oracle.orient(*oracle.gen_er(n, d, seed))
+    got = oracle.tc_edge_counts(rp, ci)
+    want = []
+    for u in range(n):
+        a = ci[rp[u]:rp[u + 1]]
+        for v in a:
+            want.append(np.intersect1d(a, ci[rp[v]:rp[v + 1]]).size)
+    assert got.tolist() == want
+    assert int(got.sum()) == oracle.tc_count(rp, ci)
+
+
+def test_pushpull_bounds_on_disjoint_cliques():
+    # cut at the clique boundaries every edge is local: part p sees its own clique
+    sizes = [5, 7, 4, 9]
+    src, dst, base = [], [], 0
+    for k in sizes:
+        s, d = clique(k, base)
+        src += s
+        dst += d
+        base += k
+    rp, ci = dag_of_pairs(base, src, dst)
+    bounds = np.concatenate([[0], np.cumsum(sizes)])
+    assert oracle.tc_pushpull_bounds(rp, ci, bounds).tolist() == [math.comb(k, 3) for k in sizes]
+    assert oracle.tc_pushpull_bounds(rp, ci, [0, base]).tolist() == [sum(math.comb(k, 3) for k in sizes)]
+
+
+@pytest.mark.parametrize("k", [2, 3, 5])
+def test_pushpull_bounds_matches_the_chunked_split(k):
+    rp, ci = oracle.orient(*oracle.gen_rmat(4096, 16, 12))
+    n = rp.size - 1
+    chunk = -(-n // k)
+    bounds = [min(p * chunk, n) for p in range(k + 1)]
+    got = oracle.tc_pushpull_bounds(rp, ci, bounds)
+    assert got.tolist() == [oracle.tc_pushpull_pe(rp, ci, k, p) for p in range(k)]
+
+
 def test_planted_mistake_is_caught():
     """A plausible orientation bug (>= instead of > on equal degrees keeps both
     directions of a tie) must fail the closed form the real rule passes."""

@@ -73,10 +73,12 @@ struct Partners {
 };
 
 // Buffers of a local count of one graph: the partner entries' DAG edge
-// positions, per-edge and per-vertex counts, and per-part sums with their
-// split points.
+// positions and partner vertices, the partner rows' owners, per-edge and
+// per-vertex counts, and per-part sums with their split points.
 struct LocalBufs {
   int64_t* pedge = nullptr;
+  int32_t* pvid = nullptr;
+  int32_t* pown = nullptr;
   uint32_t* ecount = nullptr;
   unsigned long long* vcount = nullptr;
   unsigned long long* pe = nullptr;  // [tck::kMaxParts]
@@ -97,9 +99,10 @@ struct BuildScratch {
 
 // Builds the partners of the graphs of `a` (descs on the device, pools and
 // P set; RP row slots) on stream s.  Stream-ordered, no sync.
-// pedge: null, or where to write each partner entry's DAG edge position.
+// loc: null, or where to write each partner entry's DAG edge position and
+// partner vertex and each partner row's owner (pedge, pvid, pown).
 tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s,
-                       int64_t* pedge = nullptr);
+                       const LocalBufs* loc = nullptr);
 
 // Makes ctx's device current for the scope of one API call.
 struct DeviceGuard {

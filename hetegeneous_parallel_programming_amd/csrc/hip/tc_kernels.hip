@@ -206,8 +206,8 @@ __device__ __forceinline__ void group_sync() {
 // the G lanes of a group (gl = lane index in the group): a cuckoo hash when
 // it fits and converges (and the tune allows it), else the tree (da must
 // then fit the tree cap).
-// kLocal: C (area words, parallel to slot) gets zeroed over the words the
-// set uses, for per-slot hit counters.
+// kLocal: C (area / 2 words: a 16-bit hit counter per slot word, two per
+// word) gets zeroed over the slots the set uses.
 template <int G, bool kLocal = false>
 __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
                                                     int da, int gl, const Tune& tune, uint32_t* C = nullptr) {
@@ -219,7 +219,8 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
     const int B = 1 << lb;
     for (int x = gl; x < 4 * B; x += G) {
       slot[x] = kEmpty;
-      if constexpr (kLocal) C[x] = 0;
+      if constexpr (kLocal)
+        if ((x & 1) == 0) C[x >> 1] = 0;
     }
     group_sync<G>();
     bool ok = true;
@@ -234,22 +235,30 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
   stage_tree(slot, src, da, gl, G);
   S.p0 = tree_levels(da);
   if constexpr (kLocal)
-    for (int x = gl; x < (1 << S.p0); x += G) C[x] = 0;
+    for (int x = gl; x < (1 << S.p0) / 2; x += G) C[x] = 0;
   group_sync<G>();
   return S;
 }
 
-// Per-hit attribution of a local count.  C: LDS hit counters parallel to
-// the set's slots (null for an HBM set, whose hits go to vcount directly);
-// EC: LDS hit counters of the G edges of the current batch; pedge: DAG edge
-// position of each of the row's partners; ecount/vcount: the outputs.
+// Per-hit attribution of a local count.  C: LDS hit counters of the set's
+// slots, 16 bits each (a slot is hit at most once per partner, and a row has
+// at most kPartnersPerRow partners); null for an HBM set, whose hits go to
+// vcount directly.  EC: LDS hit counters of the G edges of the current
+// batch.  pedge / pvid: DAG edge position and partner vertex of each of the
+// row's partners.  ecount / vcount: the outputs.
 struct LocalSink {
   uint32_t* C;
   uint32_t* EC;
   const int64_t* pedge;
+  const int32_t* pvid;
   uint32_t* ecount;
   unsigned long long* vcount;
 };
+static_assert(kPartnersPerRow < 65536, "16-bit slot hit counters");
+
+__device__ __forceinline__ void count_slot_hit(uint32_t* C, int idx) {
+  atomicAdd(&C[idx >> 1], 1u << ((idx & 1) << 4));
+}
 
 // Adds the set's per-slot hit counters to vcount of the ids in those slots
 // (empty hash slots hold -1, tree padding INT_MAX, tree word 0 -1).
@@ -258,10 +267,18 @@ __device__ __forceinline__ void flush_set_hits(const int32_t* slot, const uint32
                                                unsigned long long* vcount) {
   const int used = S.mode == kSetHash ? (4 << (32 - S.p0)) : (1 << S.p0);
   for (int x = gl; x < used; x += G) {
-    const uint32_t c = C[x];
+    const uint32_t c = (C[x >> 1] >> ((x & 1) << 4)) & 0xFFFFu;
     const int key = slot[x];
     if (c != 0 && key >= 0 && key != INT_MAX) atomicAdd(&vcount[key], (unsigned long long)c);
   }
+}
+
+// Sum of x over the G lanes of a group (G <= 64), on every lane.
+template <int G>
+__device__ __forceinline__ uint32_t group_sum(uint32_t x) {
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1) x += __shfl_xor(x, o, G);
+  return x;
 }
 
 // Counts, for the edges j = jfirst, jfirst + jstride, ... of pivot u (list
@@ -371,7 +388,7 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
               const int idx = x.x == w[q] ? 2 * (int)b0
                             : x.y == w[q] ? 2 * (int)b0 + 1
                             : y.x == w[q] ? t1 + 2 * (int)b1 : t1 + 2 * (int)b1 + 1;
-              atomicAdd(&ls.C[idx], 1u);
+              count_slot_hit(ls.C, idx);
               atomicAdd(&ls.EC[kq[q]], 1u);
             }
           }
@@ -401,7 +418,7 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
           hits += f ? 1u : 0u;
           if constexpr (kLocal) {
             if (f) {
-              atomicAdd(&ls.C[lo], 1u);
+              count_slot_hit(ls.C, (int)lo);
               atomicAdd(&ls.EC[kq[q]], 1u);
             }
           }
@@ -430,9 +447,14 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
       }
     }
     if constexpr (kLocal) {
-      // every edge of the batch is counted in this batch only: store it
+      // every edge of the batch is counted in this batch only: store it,
+      // and credit its partner endpoint
       wave_sync();
-      if (j < da) ls.ecount[ls.pedge[j]] = ls.EC[gl];
+      if (j < da) {
+        const uint32_t c = ls.EC[gl];
+        ls.ecount[ls.pedge[j]] = c;
+        if (c) atomicAdd(&ls.vcount[ls.pvid[j]], (unsigned long long)c);
+      }
       ls.EC[gl] = 0;
     }
     wave_sync();
@@ -683,7 +705,10 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
         slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
         pci[slot] = list_ref(d.ci_off + e0, du);
       }
-      if (b.pedge) b.pedge[d.pci_off + slot] = d.ci_off + e;  // both ways it is edge (u, v)
+      if (b.pedge) {  // both ways it is edge (u, v)
+        b.pedge[d.pci_off + slot] = d.ci_off + e;
+        b.pvid[d.pci_off + slot] = own ? v : (int32_t)u;
+      }
     }
     w += __popc(gm);
   }
@@ -730,6 +755,7 @@ __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a,
   for (int32_t k = 0; k < K; ++k) {
     vrow[v0 + k] = VRow{set, 0u, g};
     vrp[v0 + k] = p0 + (c * k) / K;
+    if (b.pown) b.pown[d.pr_base + v0 + k] = (int32_t)u;
   }
 }
 
@@ -892,7 +918,7 @@ struct LocalLds {
 };
 template <>
 struct __align__(16) LocalLds<true> {
-  uint32_t C[kAreaBlock];
+  uint32_t C[kAreaBlock / 2];
   uint32_t EC[kBlock];
   __device__ uint32_t* c() { return C; }
   __device__ uint32_t* ec() { return EC; }
@@ -909,18 +935,21 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   int32_t* slot = L.A + grp * AREA;
   int32_t* sEnd = L.End + grp * G;
   gint32** sBase = L.Base + grp * G;
-  uint32_t* C = kLocal ? LL.c() + grp * AREA : nullptr;
+  uint32_t* C = kLocal ? LL.c() + grp * (AREA / 2) : nullptr;
   for (int64_t i = first + grp; i < last; i += NGRP) {
-    const PRow pr = partner_row(a, (int64_t)items[i]);
+    const int64_t item = (int64_t)items[i];
+    const PRow pr = partner_row(a, item);
     const PivotSet S = build_pivot_set<G, kLocal>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set), gl,
                                                   a.tune, C);
     const LocalSink ls{C, kLocal ? LL.ec() + grp * G : nullptr, kLocal ? a.local.pedge + pr.pb : nullptr,
-                       a.local.ecount, a.local.vcount};
+                       kLocal ? a.local.pvid + pr.pb : nullptr, a.local.ecount, a.local.vcount};
     const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kLocal>(
         a.ci, RefEdges{a.pci + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl, ls);
     if constexpr (kLocal) {
       group_sync<G>();
       flush_set_hits<G>(slot, C, S, gl, a.local.vcount);
+      const uint32_t hr = group_sum<G>(h);  // the row's triangles: each has the owner as a corner
+      if (gl == 0 && hr) atomicAdd(&a.local.vcount[a.local.pown[item]], (unsigned long long)hr);
     }
     acc.to_graph(pr.g, batch, a.bufs.outs);
     acc.sum += h;
@@ -939,10 +968,12 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int np = (int)(pr.pe - pr.pb);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t* EC = kLocal ? LL.ec() + wid * 64 : nullptr;
+  const int64_t* pedge = kLocal ? a.local.pedge + pr.pb : nullptr;
+  const int32_t* pvid = kLocal ? a.local.pvid + pr.pb : nullptr;
   uint32_t h;
   if (kLdsPivot) {
     const PivotSet S = build_pivot_set<kBlock, kLocal>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune, LL.c());
-    const LocalSink ls{LL.c(), EC, kLocal ? a.local.pedge + pr.pb : nullptr, a.local.ecount, a.local.vcount};
+    const LocalSink ls{LL.c(), EC, pedge, pvid, a.local.ecount, a.local.vcount};
     h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock,
                                                                    L.End + wid * 64, L.Base + wid * 64, lane, ls);
     if constexpr (kLocal) {
@@ -951,9 +982,13 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
     }
   } else {
     const PivotSet S{set, kSetHbm, ds, 0};
-    const LocalSink ls{nullptr, EC, kLocal ? a.local.pedge + pr.pb : nullptr, a.local.ecount, a.local.vcount};
+    const LocalSink ls{nullptr, EC, pedge, pvid, a.local.ecount, a.local.vcount};
     h = pivot_edges<64, kModeHbm, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
                                                      L.Base + wid * 64, lane, ls);
+  }
+  if constexpr (kLocal) {  // the owner's share, one atomic per wave
+    const uint32_t hw = group_sum<64>(h);
+    if (lane == 0 && hw) atomicAdd(&a.local.vcount[a.local.pown[item]], (unsigned long long)hw);
   }
   acc.to_graph(pr.g, batch, a.bufs.outs);
   acc.sum += h;
@@ -977,8 +1012,8 @@ __device__ __forceinline__ int tile_pivots(int bin, int rounds) {
   }
 }
 
-// Local variants hold twice the LDS of a plain count: half the residency.
-constexpr int kLocalWavesPerEU = 4;
+// Local variants hold 1.5x the LDS of a plain count: 5 workgroups per CU.
+constexpr int kLocalWavesPerEU = 5;
 
 // One persistent kernel per bin, specialised so that each keeps only its own
 // working set in registers.  Blocks pull tiles from the bin's queue head.
@@ -1052,41 +1087,31 @@ __device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
   return lo;
 }
 
-// A group of kBuildG lanes per counted row u.  Edge (u, v) with count c
-// (its triangles with u and v as the two lowest corners) adds c to
-// vcount[u] and vcount[v], and to the part that computes it in the
-// reference's push/pull kernel (successful_push_pull/
-// bs_warp_vertex_nvshmem.cuh:127-136): u's part when v is in the same part
-// or d+(u) >= d+(v) (pull), else v's part (push).
-__global__ __launch_bounds__(kBlock) void local_post_kernel(const int64_t* __restrict__ rp,
-                                                            const int32_t* __restrict__ ci, int32_t row_begin,
-                                                            int32_t row_end, const uint32_t* __restrict__ ecount,
-                                                            unsigned long long* vcount, const int32_t* bounds,
-                                                            int32_t parts, unsigned long long* pe) {
+// A thread per counted DAG edge slot of a row group: edge (u, v) with count
+// c goes to the part that computes it in the reference's push/pull kernel
+// (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136): u's part when
+// v is in the same part or d+(u) >= d+(v) (pull), else v's part (push).
+__global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __restrict__ rp,
+                                                             const int32_t* __restrict__ ci, int32_t row_begin,
+                                                             int32_t row_end, const uint32_t* __restrict__ ecount,
+                                                             const int32_t* bounds, int32_t parts,
+                                                             unsigned long long* pe) {
   __shared__ unsigned long long s_pe[kMaxParts];
   for (int i = threadIdx.x; i < parts; i += kBlock) s_pe[i] = 0;
   __syncthreads();
   const int64_t u = row_begin + ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
   const int gl = threadIdx.x % kBuildG;
-  if (u < row_end) {  // whole groups
+  if (u < row_end) {
     const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
-    const int pu = parts > 0 ? part_of(bounds, parts, u) : 0;
-    unsigned long long rs = 0;
+    const int pu = part_of(bounds, parts, u);
     for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
       const uint32_t c = ecount[e];
       if (c == 0) continue;
       const int v = ci[e];
-      rs += c;
-      if (vcount) atomicAdd(&vcount[v], (unsigned long long)c);
-      if (parts > 0) {
-        const int pv = part_of(bounds, parts, v);
-        const int owner = (pv != pu && du < rp[v + 1] - rp[v]) ? pv : pu;
-        atomicAdd(&s_pe[owner], (unsigned long long)c);
-      }
+      const int pv = part_of(bounds, parts, v);
+      const int owner = (pv != pu && du < rp[v + 1] - rp[v]) ? pv : pu;
+      atomicAdd(&s_pe[owner], (unsigned long long)c);
     }
-#pragma unroll
-    for (int o = kBuildG / 2; o >= 1; o >>= 1) rs += __shfl_xor(rs, o, kBuildG);
-    if (gl == 0 && rs && vcount) atomicAdd(&vcount[u], rs);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < parts; i += kBlock)
@@ -1196,7 +1221,8 @@ hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
 }
 
 hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s) {
-  if (a.ngraphs != 1 || !a.local.pedge || !a.local.ecount || !a.local.vcount) return hipErrorInvalidValue;
+  if (a.ngraphs != 1 || !a.local.pedge || !a.local.pvid || !a.local.pown || !a.local.ecount || !a.local.vcount)
+    return hipErrorInvalidValue;
   launch_bin<BIN_GLOBAL, true>(a, cus, 0, s);
   launch_bin<BIN_BLOCK, true>(a, cus, 0, s);
   launch_bin<BIN_WAVE, true>(a, cus, 0, s);
@@ -1206,14 +1232,14 @@ hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_local_post(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                             const uint32_t* ecount, unsigned long long* vcount, const int32_t* bounds,
-                             int32_t parts, unsigned long long* pe, hipStream_t s) {
-  if (parts < 0 || parts > kMaxParts || (parts > 0 && (!bounds || !pe))) return hipErrorInvalidValue;
+hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
+                              const uint32_t* ecount, const int32_t* bounds, int32_t parts,
+                              unsigned long long* pe, hipStream_t s) {
+  if (parts < 1 || parts > kMaxParts || !bounds || !pe) return hipErrorInvalidValue;
   const int64_t rows = (int64_t)row_end - row_begin;
   if (rows <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((rows * kBuildG + kBlock - 1) / kBlock);
-  local_post_kernel<<<blocks, kBlock, 0, s>>>(rp, ci, row_begin, row_end, ecount, vcount, bounds, parts, pe);
+  local_parts_kernel<<<blocks, kBlock, 0, s>>>(rp, ci, row_begin, row_end, ecount, bounds, parts, pe);
   return hipGetLastError();
 }
 

@@ -114,12 +114,15 @@ struct VRow {
   int32_t g;
 };
 
-// Outputs of a local count (one graph only).  pedge: DAG edge position of
-// every partner entry (parallel to the partner pool); ecount: per DAG edge
-// position, |N+(u) ∩ N+(v)| of edge (u, v); vcount: per vertex, the
-// triangles with a corner there.  All null in a plain count.
+// Outputs of a local count (one graph only).  pedge / pvid: DAG edge
+// position and partner vertex of every partner entry (parallel to the
+// partner pool); pown: owner vertex of every partner row (per plan index);
+// ecount: per DAG edge position, |N+(u) ∩ N+(v)| of edge (u, v); vcount: per
+// vertex, the triangles with a corner there.  All null in a plain count.
 struct LocalOut {
   const int64_t* pedge;
+  const int32_t* pvid;
+  const int32_t* pown;
   uint32_t* ecount;
   unsigned long long* vcount;
 };
@@ -152,6 +155,8 @@ struct BuildBufs {
   int64_t* voff;        // [RP] partner-row offsets (pool-global exclusive scan)
   int64_t* part;        // [scan_partials(RP)] scan partials
   int64_t* pedge;       // null, or [partner pool]: DAG edge position of each partner entry
+  int32_t* pvid;        // with pedge: [partner pool] partner vertex of each entry
+  int32_t* pown;        // with pedge: [P] owner vertex of each partner row
 };
 int64_t scan_partials(int64_t n);
 
@@ -170,14 +175,13 @@ hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin,
 // Count of one graph that also fills a.local (ecount and vcount zeroed by
 // the caller): same plan, local variants of the bin kernels.
 hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s);
-// After launch_count_local: adds each counted edge's ecount to vcount of
-// both endpoints (if vcount) and sums it into pe[owner] by the reference
-// push/pull rule over the split bounds[0..parts] (if parts > 0, at most
-// kMaxParts; pe zeroed by the caller).
+// After launch_count_local: sums each counted edge's ecount into pe[owner]
+// by the reference push/pull rule over the split bounds[0..parts]
+// (1 <= parts <= kMaxParts; pe zeroed by the caller).
 constexpr int kMaxParts = 1024;
-hipError_t launch_local_post(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                             const uint32_t* ecount, unsigned long long* vcount, const int32_t* bounds,
-                             int32_t parts, unsigned long long* pe, hipStream_t s);
+hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
+                              const uint32_t* ecount, const int32_t* bounds, int32_t parts,
+                              unsigned long long* pe, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
 // Resident workgroups per CU of the persistent and plain kernels.
 int count_blocks_per_cu();

@@ -150,16 +150,20 @@ void BuildScratch::release() {
 }
 
 tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s,
-                       int64_t* pedge) {
+                       const LocalBufs* loc) {
   tc_status st = bs.ensure(RP);
   if (st != TC_OK) return st;
-  bs.b.pedge = pedge;
+  bs.b.pedge = loc ? loc->pedge : nullptr;
+  bs.b.pvid = loc ? loc->pvid : nullptr;
+  bs.b.pown = loc ? loc->pown : nullptr;
   TC_HIP_TRY(tck::launch_partners(a, RP, bs.b, s));
   return TC_OK;
 }
 
 void LocalBufs::release() {
   dev_free(pedge);
+  dev_free(pvid);
+  dev_free(pown);
   dev_free(ecount);
   dev_free(vcount);
   dev_free(pe);
@@ -500,7 +504,10 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
   LocalBufs& L = g->local;
   if (L.pedge) return TC_OK;
   const int64_t m = std::max<int64_t>(g->nnz, 1);
+  const int64_t P = (int64_t)g->nv_max + 1;
   tc_status st = dev_alloc((void**)&L.pedge, sizeof(int64_t) * (size_t)m, "partner edge positions");
+  if (st == TC_OK) st = dev_alloc((void**)&L.pvid, sizeof(int32_t) * (size_t)m, "partner vertices");
+  if (st == TC_OK) st = dev_alloc((void**)&L.pown, sizeof(int32_t) * (size_t)P, "partner row owners");
   if (st == TC_OK) st = dev_alloc((void**)&L.ecount, sizeof(uint32_t) * (size_t)m, "edge counts");
   if (st == TC_OK)
     st = dev_alloc((void**)&L.vcount, sizeof(unsigned long long) * ((size_t)g->n + 1), "vertex counts");
@@ -509,7 +516,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
   if (st == TC_OK) {
     tck::LaunchArgs a = count_args(ctx, g);
     BuildScratch bs;
-    st = launch_build(a, (int64_t)g->n + 1, bs, ctx->stream, L.pedge);
+    st = launch_build(a, (int64_t)g->n + 1, bs, ctx->stream, &L);
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     bs.release();
     if (st == TC_OK && e != hipSuccess)
@@ -519,7 +526,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
     L.release();
     return st;
   }
-  L.bytes = (12 * m + 8 * ((int64_t)g->n + 1) + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1));
+  L.bytes = 16 * m + 4 * P + 8 * ((int64_t)g->n + 1) + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
   return TC_OK;
 }
 
@@ -544,7 +551,7 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   if (st != TC_OK) return st;
   LocalBufs& L = g->local;
   tck::LaunchArgs a = count_args(ctx, g);
-  a.local = tck::LocalOut{L.pedge, L.ecount, L.vcount};
+  a.local = tck::LocalOut{L.pedge, L.pvid, L.pown, L.ecount, L.vcount};
   hipStream_t s = ctx->stream;
   if (k > 0)
     TC_HIP_TRY(hipMemcpyAsync(L.bounds, out->bounds, sizeof(int32_t) * ((size_t)k + 1), hipMemcpyHostToDevice, s));
@@ -557,8 +564,7 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
   TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, s));
-  TC_HIP_TRY(tck::launch_local_post(g->rp, g->ci, g->row_begin, g->row_end, L.ecount, L.vcount,
-                                    k > 0 ? L.bounds : nullptr, k, L.pe, s));
+  if (k > 0) TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, g->row_begin, g->row_end, L.ecount, L.bounds, k, L.pe, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,

@@ -104,11 +104,19 @@ class Graph {
 };
 
 // One contiguous-range part of a DAG with its halo cache, or the reference's
-// vertex-induced subgraphs (graph_partition.cc:85-135).
+// vertex-induced subgraphs (graph_partition.cc:85-135).  `balance`
+// (TCH_BALANCE_*) picks the halo partitions' split points; the induced
+// subgraphs always use the reference's ceil(n/k) split.
 class PartitionedGraph {
  public:
-  PartitionedGraph(Graph* g, int nc) : g_(g), nc_(nc) {
+  PartitionedGraph(Graph* g, int nc, int balance = TCH_BALANCE_VERTICES) : g_(g), nc_(nc), balance_(balance) {
     if (nc < 1) throw Error(TC_EINVAL, "need at least one vertex chunk");
+  }
+  // Split points b[0..nc]: part i owns ids [b[i], b[i+1]).
+  std::vector<int32_t> bounds() const {
+    std::vector<int32_t> b((size_t)nc_ + 1);
+    host_check(tch_partition_bounds(&g_->view(), nc_, balance_, b.data()));
+    return b;
   }
   void edgecut_induced_partition1D() {
     subs_.clear();
@@ -136,13 +144,20 @@ class PartitionedGraph {
   // Owned rows of part i plus every remote list they reach (caller frees).
   tch_partition* halo_partition(int i) const {
     tch_partition* p = nullptr;
-    host_check(tch_partition_range(&g_->view(), nc_, i, &p));
+    if (balance_ == TCH_BALANCE_VERTICES) {
+      host_check(tch_partition_range(&g_->view(), nc_, i, &p));
+    } else {
+      const std::vector<int32_t> b = bounds();
+      if (i < 0 || i >= nc_) throw Error(TC_EINVAL, "part index out of range");
+      host_check(tch_partition_span(&g_->view(), b[(size_t)i], b[(size_t)i + 1], &p));
+    }
     return p;
   }
 
  private:
   Graph* g_;
   int nc_;
+  int balance_;
   std::vector<std::unique_ptr<Graph>> subs_;
   std::vector<vidType> lb_, le_;
   std::vector<std::vector<vidType>> idx_map_;
@@ -163,6 +178,36 @@ class Device {
     dev_check(tc_graph_upload(ctx_, &g.view(), TC_CHECK_SHAPE, &dg));
     uint64_t total = 0;
     tc_status s = tc_count(ctx_, dg, &total, st);
+    tc_graph_free(ctx_, dg);
+    dev_check(s);
+    return total;
+  }
+
+  // Count `g` and attribute its triangles (tc_count_local): per DAG edge
+  // |N+(u) ∩ N+(v)|, per vertex the triangles at it, and, given split points
+  // (bounds.size() - 1 parts), the reference push/pull kernel's per-PE sums.
+  // Null outputs are skipped.
+  AccType count_local(const Graph& g, std::vector<uint32_t>* edge_counts, std::vector<uint64_t>* vertex_counts,
+                      const std::vector<int32_t>* bounds = nullptr, std::vector<uint64_t>* part_counts = nullptr) {
+    tc_local lo{};
+    if (edge_counts) {
+      edge_counts->assign((size_t)g.E(), 0);
+      lo.edge_counts = edge_counts->data();
+    }
+    if (vertex_counts) {
+      vertex_counts->assign((size_t)g.V(), 0);
+      lo.vertex_counts = vertex_counts->data();
+    }
+    if (bounds && part_counts && bounds->size() > 1) {
+      lo.parts = (int32_t)bounds->size() - 1;
+      lo.bounds = bounds->data();
+      part_counts->assign(bounds->size() - 1, 0);
+      lo.part_counts = part_counts->data();
+    }
+    tc_graph* dg = nullptr;
+    dev_check(tc_graph_upload(ctx_, &g.view(), TC_CHECK_SHAPE, &dg));
+    uint64_t total = 0;
+    tc_status s = tc_count_local(ctx_, dg, &total, &lo, nullptr);
     tc_graph_free(ctx_, dg);
     dev_check(s);
     return total;

@@ -46,3 +46,21 @@ def test_counts_graphminer_file_and_generated_graph(tmp_path):
         assert m and int(m.group(1)) == want, p.stdout
     p = run(prefix, "2")
     assert p.returncode == 2 and "exactly one GPU" in p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("balance", ["vertices", "partner"])
+def test_per_pe_attribution_lines(balance):
+    from hetegeneous_parallel_programming_amd import PartitionedGraph
+
+    d = Graph.rmat(1 << 14, 16, 5).oriented()
+    rp, ci = d.out_rowptr(), d.out_colidx()
+    bounds = PartitionedGraph(d, 3, balance).bounds()
+    want = oracle.tc_pushpull_bounds(rp, ci, bounds)
+    p = run("rmat:14:16:5", "--parts", "3", "--balance", balance)
+    assert p.returncode == 0, p.stderr
+    got = re.findall(r"pe\[(\d+)\] ids \[(\d+), (\d+)\) count = (\d+)", p.stdout)
+    assert [(int(a), int(b), int(c), int(x)) for a, b, c, x in got] == [
+        (i, int(bounds[i]), int(bounds[i + 1]), int(want[i])) for i in range(3)]
+    m = re.search(r"total_num_triangles = (\d+)", p.stdout)
+    assert int(m.group(1)) == int(want.sum())

@@ -5,6 +5,6 @@ Host side: :mod:`.graph` (CSR, generators, orientation, partitioning) over
 batched persistent kernel, serving loop) over ``libtc_hip.so`` -- hand-written
 HIP kernels for gfx950 behind a C ABI (``include/tc.h``).
 """
-from .graph import Graph, HaloPartition, PartitionedGraph, batch_spec  # noqa: F401
+from .graph import Graph, HaloPartition, MetisPart, PartitionedGraph, assemble_metis, batch_spec  # noqa: F401
 
-__all__ = ["Graph", "HaloPartition", "PartitionedGraph", "batch_spec"]
+__all__ = ["Graph", "HaloPartition", "MetisPart", "PartitionedGraph", "assemble_metis", "batch_spec"]

@@ -55,6 +55,12 @@ class PartitionView(ctypes.Structure):
                 ("halo_colidx", i32p), ("halo_nnz", c_i64)]
 
 
+class MetisView(ctypes.Structure):
+    _fields_ = [("n", c_i32), ("n_real", c_i32), ("nnz", c_i64), ("max_subg_nv", c_i64),
+                ("max_subg_ne", c_i64), ("max_degree", c_i64), ("rowptr", i64p), ("colidx", i32p),
+                ("vertex_list", i32p), ("part", i64p)]
+
+
 class Work(ctypes.Structure):
     _fields_ = [("rows", c_i64), ("edges", c_i64), ("wedge_elems", c_i64), ("min_elems", c_i64),
                 ("alg_bytes", c_i64), ("max_out", c_i32), ("partner_rows", c_i64),
@@ -120,6 +126,12 @@ def host():
     _sig(L, "tch_read_graphminer", c_i32, [ctypes.c_char_p, c_i32, ctypes.POINTER(P)])
     _sig(L, "tch_write_graphminer", c_i32, [ctypes.c_char_p, ctypes.POINTER(CsrView)])
     _sig(L, "tch_partition_assemble", c_i32, [ctypes.POINTER(PartitionView), ctypes.POINTER(P)])
+    _sig(L, "tch_metis_write", c_i32, [ctypes.c_char_p, ctypes.POINTER(CsrView), i64p, c_i32])
+    _sig(L, "tch_metis_read", c_i32, [ctypes.c_char_p, c_i32, ctypes.POINTER(P)])
+    _sig(L, "tch_metis_get", c_i32, [P, ctypes.POINTER(MetisView)])
+    _sig(L, "tch_metis_free", None, [P])
+    _sig(L, "tch_metis_assemble", c_i32, [ctypes.POINTER(P), c_i32, c_i32, ctypes.POINTER(P),
+                                          ctypes.POINTER(ctypes.c_uint8)])
     _sig(L, "tch_arena_create", c_i32, [c_u64, c_u64, ctypes.POINTER(P)])
     _sig(L, "tch_arena_destroy", None, [P])
     _sig(L, "tch_arena_alloc", c_i32, [P, c_u64, ctypes.POINTER(c_u64)])

@@ -26,7 +26,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "lib")
 OBJ = os.path.join(PKG, "lib", "obj")
 
-HOST_SOURCES = ["host/csr.cpp", "host/gen.cpp", "host/partition.cpp", "host/arena.cpp"]
+HOST_SOURCES = ["host/csr.cpp", "host/gen.cpp", "host/partition.cpp", "host/metis.cpp", "host/arena.cpp"]
 HIP_SOURCES = ["hip/tc_kernels.hip", "hip/tc_runtime.cpp", "hip/tc_batch.cpp",
                "hip/tc_server.cpp", "hip/tc_tune.cpp"]
 

@@ -275,7 +275,7 @@ tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double
     const int64_t P = b->rows[i];
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->plain_grid, (P + tck::kWaves - 1) / tck::kWaves));
     TC_HIP_TRY(hipMemsetAsync(d_out + i, 0, sizeof(uint64_t), s));
-    TC_HIP_TRY(tck::launch_plain(b->rp + d.rp_off, b->ci + d.ci_off, 0, b->rows[i], d_out + i, grid, s));
+    TC_HIP_TRY(tck::launch_plain(b->rp + d.rp_off, b->ci + d.ci_off, 0, b->rows[i], nullptr, d_out + i, grid, s));
     TC_HIP_TRY(hipMemcpyAsync(b->pinned + i, d_out + i, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     TC_HIP_TRY(hipStreamSynchronize(s));
     counts[i] = b->pinned[i];

@@ -146,6 +146,7 @@ struct tc_graph {
   int64_t nnz = 0;
   int32_t row_begin = 0, row_end = 0;
   int64_t edge_begin = 0, edge_end = 0;  // DAG edge positions of rows [row_begin, row_end)
+  uint8_t* own = nullptr;                 // null, or per row 1 if it is a pivot row
   int32_t max_out = 0;
   tcr::Partners partners;
   tcr::LocalBufs local;

@@ -649,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void partner_count_kernel(LaunchArgs a, int
   const GraphDesc d = a.descs[g];
   const int64_t u = q - d.row_base;
   int32_t c1 = 0;
-  if (u >= d.own_begin && u < d.own_end) {
+  if (u >= d.own_begin && u < d.own_end && (!a.own || a.own[u])) {
     const int64_t* rp = a.rp + d.rp_off;
     const int32_t* ci = a.ci + d.ci_off;
     const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
   const int g = find_graph_rows(a.descs, a.ngraphs, q);
   const GraphDesc d = a.descs[g];
   const int64_t u = q - d.row_base;
-  if (u < d.own_begin || u >= d.own_end) return;
+  if (u < d.own_begin || u >= d.own_end || (a.own && !a.own[u])) return;
   const int64_t* rp = a.rp + d.rp_off;
   const int32_t* ci = a.ci + d.ci_off;
   ListRef* pci = (ListRef*)a.pci + d.pci_off;
@@ -1093,7 +1093,8 @@ __device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
 // v is in the same part or d+(u) >= d+(v) (pull), else v's part (push).
 __global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __restrict__ rp,
                                                              const int32_t* __restrict__ ci, int32_t row_begin,
-                                                             int32_t row_end, const uint32_t* __restrict__ ecount,
+                                                             int32_t row_end, const uint8_t* own,
+                                                             const uint32_t* __restrict__ ecount,
                                                              const int32_t* bounds, int32_t parts,
                                                              unsigned long long* pe) {
   __shared__ unsigned long long s_pe[kMaxParts];
@@ -1101,7 +1102,7 @@ __global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __re
   __syncthreads();
   const int64_t u = row_begin + ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
   const int gl = threadIdx.x % kBuildG;
-  if (u < row_end) {
+  if (u < row_end && (!own || own[u])) {
     const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
     const int pu = part_of(bounds, parts, u);
     for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
@@ -1131,7 +1132,7 @@ struct __align__(16) PlainLds {
 __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(const int64_t* __restrict__ rp,
                                                        const int32_t* __restrict__ ci,
                                                        int32_t row_begin, int32_t row_end,
-                                                       uint64_t* out) {
+                                                       const uint8_t* own, uint64_t* out) {
   __shared__ PlainLds L;
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t nw = (int64_t)gridDim.x * kWaves;
@@ -1139,7 +1140,7 @@ __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(co
   for (int64_t u = row_begin + (int64_t)blockIdx.x * kWaves + wid; u < row_end; u += nw) {
     const int64_t rb = rp[u];
     const int da = (int)(rp[u + 1] - rb);
-    if (da < 2) continue;
+    if (da < 2 || (own && !own[u])) continue;
     constexpr int kModes = kModeTree | kModeHbm;
     constexpr int kPlainInFlight = 6;
     if (da <= kCapWave - 1) {
@@ -1233,19 +1234,19 @@ hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s) {
 }
 
 hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                              const uint32_t* ecount, const int32_t* bounds, int32_t parts,
+                              const uint8_t* own, const uint32_t* ecount, const int32_t* bounds, int32_t parts,
                               unsigned long long* pe, hipStream_t s) {
   if (parts < 1 || parts > kMaxParts || !bounds || !pe) return hipErrorInvalidValue;
   const int64_t rows = (int64_t)row_end - row_begin;
   if (rows <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((rows * kBuildG + kBlock - 1) / kBlock);
-  local_parts_kernel<<<blocks, kBlock, 0, s>>>(rp, ci, row_begin, row_end, ecount, bounds, parts, pe);
+  local_parts_kernel<<<blocks, kBlock, 0, s>>>(rp, ci, row_begin, row_end, own, ecount, bounds, parts, pe);
   return hipGetLastError();
 }
 
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                        uint64_t* out, int grid, hipStream_t s) {
-  plain_kernel<<<grid, kBlock, 0, s>>>(rp, ci, row_begin, row_end, out);
+                        const uint8_t* own, uint64_t* out, int grid, hipStream_t s) {
+  plain_kernel<<<grid, kBlock, 0, s>>>(rp, ci, row_begin, row_end, own, out);
   return hipGetLastError();
 }
 

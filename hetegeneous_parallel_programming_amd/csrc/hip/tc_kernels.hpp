@@ -140,6 +140,7 @@ struct LaunchArgs {
   Tune tune;
   PlanBufs bufs;
   LocalOut local;
+  const uint8_t* own;   // one graph only: null, or per row 1 if it is a pivot row
 };
 
 // Planning blocks of a launch over P plan indices.
@@ -170,8 +171,10 @@ hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);
 hipError_t launch_plan(const LaunchArgs& a, hipStream_t s);
 // One full-residency persistent launch per bin (grid > 0 overrides the size).
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s);
+// own: null, or per row 1 if it is a pivot row (rows outside
+// [row_begin, row_end) are never pivots).
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                        uint64_t* out, int grid, hipStream_t s);
+                        const uint8_t* own, uint64_t* out, int grid, hipStream_t s);
 // Count of one graph that also fills a.local (ecount and vcount zeroed by
 // the caller): same plan, local variants of the bin kernels.
 hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s);
@@ -180,7 +183,7 @@ hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s);
 // (1 <= parts <= kMaxParts; pe zeroed by the caller).
 constexpr int kMaxParts = 1024;
 hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                              const uint32_t* ecount, const int32_t* bounds, int32_t parts,
+                              const uint8_t* own, const uint32_t* ecount, const int32_t* bounds, int32_t parts,
                               unsigned long long* pe, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
 // Resident workgroups per CU of the persistent and plain kernels.

@@ -8,6 +8,10 @@
 // loaded as a DAG, or "rmat:<scale>:<edge_factor>:<seed>" /
 // "er:<n>:<avg_degree>:<seed>" for a seeded synthetic graph.  Prints the
 // same lines as the reference, ending in "total_num_triangles = <count>".
+// "metis:<prefix>:<k>" reads the reference's per-PE METIS part files
+// <prefix>0.* .. <prefix>{k-1}.* (graph_partition.cc:501-566) and counts each
+// PE's own rows against the remote lists they reach, printing
+// "pe[<i>] n_real = <owned> count = <sum>" per PE before the total.
 // With --parts K it also prints, per PE of a K-way split, the sum the
 // reference's push/pull kernel reports for that PE
 // (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136), as
@@ -55,6 +59,47 @@ static int balance_of(const std::string& name) {
   throw Error(TC_EINVAL, "--balance takes vertices, edges, wedges or partner");
 }
 
+// Counts every PE of a METIS file set; returns the total.
+static uint64_t count_metis(const std::string& spec) {
+  const size_t c = spec.rfind(':');
+  if (c == std::string::npos || c <= 6) throw Error(TC_EINVAL, "expected metis:<prefix>:<k>");
+  const std::string prefix = spec.substr(6, c - 6);
+  const int k = std::atoi(spec.c_str() + c + 1);
+  if (k < 1) throw Error(TC_EINVAL, "expected metis:<prefix>:<k> with k >= 1");
+  std::vector<tch_metis*> pes((size_t)k, nullptr);
+  auto free_all = [&]() {
+    for (tch_metis* m : pes) tch_metis_free(m);
+  };
+  Device dev(0);
+  uint64_t total = 0;
+  try {
+    for (int i = 0; i < k; ++i) host_check(tch_metis_read(prefix.c_str(), i, &pes[(size_t)i]));
+    for (int i = 0; i < k; ++i) {
+      tch_metis_view mv;
+      host_check(tch_metis_get(pes[(size_t)i], &mv));
+      tch_csr* sub = nullptr;
+      std::vector<uint8_t> owned((size_t)mv.n);
+      host_check(tch_metis_assemble(pes.data(), k, i, &sub, owned.data()));
+      tc_csr_view v;
+      tch_csr_view(sub, &v);
+      tc_graph* dg = nullptr;
+      uint64_t cnt = 0;
+      tc_status st = tc_graph_upload_rows(dev.get(), &v, owned.data(), TC_CHECK_SHAPE, &dg);
+      if (st == TC_OK) st = tc_count(dev.get(), dg, &cnt, nullptr);
+      tc_graph_free(dev.get(), dg);
+      tch_csr_free(sub);
+      dev_check(st);
+      std::cout << "pe[" << i << "] n_real = " << mv.n_real << " count = " << cnt << "\n";
+      total += cnt;
+    }
+  } catch (...) {
+    free_all();
+    throw;
+  }
+  free_all();
+  return total;
+}
+
 int main(int argc, char* argv[]) {
   // positional arguments as main.cc:11-36; options anywhere after <graph>
   std::vector<std::string> pos;
@@ -78,6 +123,14 @@ int main(int argc, char* argv[]) {
   try {
     if (parts < 0 || parts > 1024) throw Error(TC_EINVAL, "--parts must be in [1, 1024]");
     const int bal = balance_of(balance);
+    if (pos[0].rfind("metis:", 0) == 0) {
+      auto t0 = std::chrono::steady_clock::now();
+      const uint64_t total = count_metis(pos[0]);
+      double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      std::cout << "runtime[gpu 0] = " << s << " sec\n";
+      std::cout << "total_num_triangles = " << total << "\n";
+      return 0;
+    }
     Graph g = load(pos[0]);
     int n_devices = 1, chunk_size = 1024, adj_sorted = 1;
     if (pos.size() > 1) n_devices = std::atoi(pos[1].c_str());

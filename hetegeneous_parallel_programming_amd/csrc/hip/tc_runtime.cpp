@@ -213,7 +213,7 @@ static tc_status check_dag(const tc_csr_view* dag, uint32_t check, int32_t* max_
 }
 
 static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_begin, int32_t row_end,
-                            int32_t max_out, tc_graph** out) {
+                            const uint8_t* owned, int32_t max_out, tc_graph** out) {
   std::unique_ptr<tc_graph> g(new tc_graph);
   g->ctx = ctx;
   g->n = dag->n_rows;
@@ -228,9 +228,12 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
     dev_free(g->rp);
     dev_free(g->ci);
     dev_free(g->d_desc);
+    dev_free(g->own);
     g->partners.release();
   };
   tc_status st = dev_alloc((void**)&g->rp, sizeof(int64_t) * ((size_t)g->n + 1), "rowptr");
+  if (st == TC_OK && owned)
+    st = dev_alloc((void**)&g->own, std::max<size_t>((size_t)g->n, 1), "owned-row mask");
   if (st == TC_OK) st = dev_alloc((void**)&g->ci, sizeof(int32_t) * (size_t)std::max<int64_t>(g->nnz, 1), "colidx");
   if (st == TC_OK) st = dev_alloc((void**)&g->d_desc, sizeof(tck::GraphDesc), "graph desc");
   if (st == TC_OK) st = g->partners.alloc(g->nnz, (int64_t)g->nv_max + 1, 1);
@@ -243,6 +246,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   if (e == hipSuccess && g->nnz > 0)
     e = hipMemcpy(g->ci, dag->colidx, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->d_desc, &d, sizeof(d), hipMemcpyHostToDevice);
+  if (e == hipSuccess && owned && g->n > 0) e = hipMemcpy(g->own, owned, (size_t)g->n, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     cleanup();
     return fail(TC_EDEVICE, std::string("graph upload: ") + hipGetErrorString(e));
@@ -254,6 +258,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   a.P = (int64_t)g->nv_max + 1;
   a.rp = g->rp;
   a.ci = g->ci;
+  a.own = g->own;
   g->partners.into(a);
   BuildScratch bs;
   hipStream_t s = ctx->stream;
@@ -276,7 +281,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   g->build_ms = ms;
   g->nv = *(const int32_t*)ctx->pinned;
   g->bytes = sizeof(int64_t) * ((int64_t)g->n + 1) + sizeof(int32_t) * g->nnz + sizeof(tck::GraphDesc) +
-             g->partners.bytes;
+             g->partners.bytes + (owned ? (int64_t)g->n : 0);
   *out = g.release();
   return TC_OK;
 }
@@ -376,7 +381,18 @@ tc_status tc_graph_upload(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, t
   if (st != TC_OK) return st;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
-  return upload_csr(ctx, dag, 0, dag->n_rows, max_out, out);
+  return upload_csr(ctx, dag, 0, dag->n_rows, nullptr, max_out, out);
+}
+
+tc_status tc_graph_upload_rows(tc_ctx* ctx, const tc_csr_view* dag, const uint8_t* owned, uint32_t check,
+                               tc_graph** out) {
+  if (!ctx || !out) return fail(TC_EINVAL, "null argument");
+  int32_t max_out = 0;
+  tc_status st = check_dag(dag, check, &max_out);
+  if (st != TC_OK) return st;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  return upload_csr(ctx, dag, 0, dag->n_rows, owned, max_out, out);
 }
 
 tc_status tc_partition_upload(tc_ctx* ctx, const tch_partition_view* part, uint32_t check,
@@ -392,7 +408,7 @@ tc_status tc_partition_upload(tc_ctx* ctx, const tch_partition_view* part, uint3
   if (st == TC_OK) {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     DeviceGuard guard(ctx->device);
-    st = upload_csr(ctx, &v, part->row_begin, part->row_end, max_out, out);
+    st = upload_csr(ctx, &v, part->row_begin, part->row_end, nullptr, max_out, out);
   }
   tch_csr_free(full);
   return st;
@@ -408,6 +424,7 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) {
   dev_free(g->rp);
   dev_free(g->ci);
   dev_free(g->d_desc);
+  dev_free(g->own);
   g->partners.release();
   g->local.release();
   delete g;
@@ -427,6 +444,22 @@ tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) {
   return TC_OK;
 }
 
+// Fills the LaunchArgs of a count of g (descs, pools, partners, tune, plan
+// scratch).
+static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
+  tck::LaunchArgs a{};
+  a.descs = g->d_desc;
+  a.ngraphs = 1;
+  a.P = (int64_t)g->nv_max + 1;
+  a.rp = g->rp;
+  a.ci = g->ci;
+  g->partners.into(a);
+  a.tune = to_kernel_tune(ctx->tune);
+  a.bufs = ctx->scratch.bufs();
+  a.own = g->own;
+  return a;
+}
+
 tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
   if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
@@ -436,15 +469,7 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   const int64_t P = (int64_t)g->nv_max + 1;
   tc_status st = ctx->scratch.ensure(P, 1);
   if (st != TC_OK) return st;
-  tck::LaunchArgs a{};
-  a.descs = g->d_desc;
-  a.ngraphs = 1;
-  a.P = P;
-  a.rp = g->rp;
-  a.ci = g->ci;
-  g->partners.into(a);
-  a.tune = to_kernel_tune(ctx->tune);
-  a.bufs = ctx->scratch.bufs();
+  const tck::LaunchArgs a = count_args(ctx, g);
   hipStream_t s = ctx->stream;
   TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
   TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
@@ -475,27 +500,12 @@ tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* st
   TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
   TC_HIP_TRY(tck::launch_zero(ctx->scratch.outs, 2, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, ctx->scratch.outs, grid, s));
+  TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, g->own, ctx->scratch.outs, grid, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->scratch.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
   return finish_stats(ctx, stats, t0, false, P);
-}
-
-// Fills the LaunchArgs of a count of g (descs, pools, partners, tune, plan
-// scratch).
-static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
-  tck::LaunchArgs a{};
-  a.descs = g->d_desc;
-  a.ngraphs = 1;
-  a.P = (int64_t)g->nv_max + 1;
-  a.rp = g->rp;
-  a.ci = g->ci;
-  g->partners.into(a);
-  a.tune = to_kernel_tune(ctx->tune);
-  a.bufs = ctx->scratch.bufs();
-  return a;
 }
 
 // First local count of g: its buffers, and a partner rebuild that also
@@ -564,7 +574,8 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
   TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, s));
-  if (k > 0) TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, g->row_begin, g->row_end, L.ecount, L.bounds, k, L.pe, s));
+  if (k > 0)
+    TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, g->row_begin, g->row_end, g->own, L.ecount, L.bounds, k, L.pe, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,

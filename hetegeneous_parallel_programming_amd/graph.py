@@ -21,7 +21,7 @@ import ctypes
 import numpy as np
 
 from . import _native as nat
-from ._native import CsrView, PartitionView, Work, host, host_check
+from ._native import CsrView, MetisView, PartitionView, Work, host, host_check
 
 
 def _as_view(rowptr: np.ndarray, colidx: np.ndarray) -> CsrView:
@@ -181,6 +181,16 @@ class Graph:
     def write(self, prefix: str) -> None:
         host_check(host().tch_write_graphminer(prefix.encode(), ctypes.byref(self._view)))
 
+    def write_metis(self, prefix: str, part) -> None:
+        """The reference's per-PE part files ``<prefix>{i}.*`` for every PE i
+        of the part array (graph_partition.cc:501-566; ``tch_metis_write``)."""
+        p = np.ascontiguousarray(part, dtype=np.int64)
+        if p.size != self.V():
+            raise ValueError("part must give every vertex a PE")
+        k = int(p.max()) + 1 if p.size else 1
+        host_check(host().tch_metis_write(prefix.encode(), ctypes.byref(self._view),
+                                          p.ctypes.data_as(nat.i64p), k))
+
 
 class HaloPartition:
     """Owned rows of one contiguous-range part of a DAG plus its halo cache."""
@@ -218,6 +228,53 @@ class HaloPartition:
             except Exception:
                 pass
             self._h = None
+
+
+class MetisPart:
+    """One PE's METIS part files (graph.cc:16-23,33-34,56-59; ``tch_metis_read``)."""
+
+    def __init__(self, prefix: str, i: int):
+        h = ctypes.c_void_p()
+        host_check(host().tch_metis_read(prefix.encode(), i, ctypes.byref(h)))
+        self._h = h
+        v = MetisView()
+        host_check(host().tch_metis_get(h, ctypes.byref(v)))
+        self.pe = i
+        self.n, self.n_real, self.nnz = v.n, v.n_real, v.nnz
+        self.max_subg_nv, self.max_subg_ne, self.max_degree = v.max_subg_nv, v.max_subg_ne, v.max_degree
+        self.rowptr = np.ctypeslib.as_array(v.rowptr, shape=(v.n + 1,))
+        self.colidx = (np.ctypeslib.as_array(v.colidx, shape=(v.nnz,)) if v.nnz
+                       else np.zeros(0, np.int32))
+        self.vertex_list = (np.ctypeslib.as_array(v.vertex_list, shape=(v.n_real,)) if v.n_real
+                            else np.zeros(0, np.int32))
+        self.part = np.ctypeslib.as_array(v.part, shape=(v.n,))
+
+    @staticmethod
+    def read_all(prefix: str, k: int) -> list:
+        return [MetisPart(prefix, i) for i in range(k)]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                host().tch_metis_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def assemble_metis(parts: list, i: int):
+    """(Graph, owned) for PE ``i`` of the METIS part files ``parts``: its own
+    rows plus the remote lists they reach, with the owned-row mask of its
+    pivots; ``i = -1`` gives the whole DAG (``tch_metis_assemble``)."""
+    k = len(parts)
+    arr = (ctypes.c_void_p * k)(*[p._h.value for p in parts])
+    out = ctypes.c_void_p()
+    owned = np.zeros(parts[0].n, np.uint8)
+    host_check(host().tch_metis_assemble(ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), k, i,
+                                         ctypes.byref(out),
+                                         owned.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    return Graph(out.value, f"metis-pe{i}"), owned
 
 
 class PartitionedGraph:

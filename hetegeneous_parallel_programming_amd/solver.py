@@ -110,6 +110,8 @@ def device_lib():
     S(L, "tc_set_tune", nat.c_i32, [P, ctypes.POINTER(Tune)])
     S(L, "tc_device_name", nat.c_i32, [P, ctypes.c_char_p, nat.c_i32])
     S(L, "tc_graph_upload", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_u32, PP])
+    S(L, "tc_graph_upload_rows", nat.c_i32, [P, ctypes.POINTER(CsrView), ctypes.POINTER(ctypes.c_uint8),
+                                             nat.c_u32, PP])
     S(L, "tc_partition_upload", nat.c_i32, [P, ctypes.POINTER(PartitionView), nat.c_u32, PP])
     S(L, "tc_graph_free", nat.c_i32, [P, P])
     S(L, "tc_graph_bytes", nat.c_i64, [P])
@@ -256,10 +258,20 @@ class Context:
         t = Tune.from_dict(cur)
         _check(device_lib().tc_set_tune(self._h, ctypes.byref(t)))
 
-    def upload(self, g: Graph, check: int = nat.TC_CHECK_SHAPE) -> DeviceGraph:
+    def upload(self, g: Graph, check: int = nat.TC_CHECK_SHAPE, owned=None) -> DeviceGraph:
+        """Make ``g`` resident; ``owned`` (one byte per row) restricts the
+        counted rows to those marked 1 (``tc_graph_upload_rows``)."""
         h = ctypes.c_void_p()
-        _check(device_lib().tc_graph_upload(self._h, ctypes.byref(g.view()), check,
-                                            ctypes.byref(h)))
+        if owned is None:
+            _check(device_lib().tc_graph_upload(self._h, ctypes.byref(g.view()), check,
+                                                ctypes.byref(h)))
+        else:
+            own = np.ascontiguousarray(owned, dtype=np.uint8)
+            if own.size != g.V():
+                raise ValueError("owned needs one entry per row")
+            _check(device_lib().tc_graph_upload_rows(
+                self._h, ctypes.byref(g.view()), own.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                check, ctypes.byref(h)))
         return DeviceGraph(self, h.value, g.V(), g.E())
 
     def upload_arrays(self, rowptr, colidx, check: int = nat.TC_CHECK_SHAPE) -> DeviceGraph:

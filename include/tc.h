@@ -77,6 +77,12 @@ tc_status tc_device_name(tc_ctx* ctx, char* buf, int32_t len);
  * rowptr/colidx + H2D).  `dag` must be the oriented DAG (tch_orient);
  * `check` (TC_CHECK_*) validates it on the host first. */
 tc_status tc_graph_upload(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, tc_graph** out);
+/* Same, counting only the rows u with owned[u] != 0 (n_rows bytes, copied):
+ * a PE's share when its rows are an arbitrary vertex set, as with METIS
+ * parts (bs_warp_vertex_nvshmem.cuh:25-28 loops over vertex_list); the other
+ * rows hold the remote lists they reach (tch_metis_assemble).  NULL = all. */
+tc_status tc_graph_upload_rows(tc_ctx* ctx, const tc_csr_view* dag, const uint8_t* owned, uint32_t check,
+                               tc_graph** out);
 
 /* One GPU's share of the push/pull scheme (GraphNVSHMEM,
  * successful_push_pull/graph_nvshmem.h:26-50, plus the pull branch of

@@ -153,6 +153,42 @@ tc_status tch_partition_induced(const tc_csr_view* g, int32_t k, int32_t part,
                                 int32_t* local_end, int32_t** idx_map_out);
 void tch_free_ids(int32_t* p);
 
+/* ---- the reference's per-PE METIS part files -----------------------------
+ * File set of PE i (graph_partition.cc:501-566, read by graph.cc:16-23,
+ * 33-34,56-59), <prefix> typically ".../metis/pgraph":
+ *   <prefix>i.meta.txt        n, n_real, E, max_subg_nv, max_subg_ne, max_degree
+ *   <prefix>i.vertex.bin      int64 rowptr[n+1] over all global ids, only
+ *                             PE i's rows non-empty
+ *   <prefix>i.edge.bin        int32 colidx[E], global ids
+ *   <prefix>i.vertex_list.bin int32 PE i's ids [n_real], ascending
+ *   <prefix>i.part.bin        int64 owner PE of every id [n]
+ * tch_metis_write lays out any DAG under any part array (values in [0, k))
+ * as k such file sets (METIS itself is not linked: its part array, or a range
+ * split, is an input).  tch_metis_read loads and checks one PE's set.
+ * tch_metis_assemble builds, from all k PEs' files, the CSR PE i counts: its
+ * own rows plus every remote list they reach (each from its owner's file,
+ * the lists the reference pulls, bs_warp_vertex_nvshmem.cuh:110-136), other
+ * rows empty; `owned` (n bytes, may be NULL) gets 1 for PE i's rows, the
+ * pivots of its count (vertex_list, bs_warp_vertex_nvshmem.cuh:25-28).
+ * i = -1 assembles the whole DAG (every row from its owner, owned all 1). */
+typedef struct tch_metis tch_metis;
+typedef struct tch_metis_view {
+  int32_t n;
+  int32_t n_real;
+  int64_t nnz;
+  int64_t max_subg_nv, max_subg_ne, max_degree;
+  const int64_t* rowptr;
+  const int32_t* colidx;
+  const int32_t* vertex_list;
+  const int64_t* part;
+} tch_metis_view;
+tc_status tch_metis_write(const char* prefix, const tc_csr_view* dag, const int64_t* part, int32_t k);
+tc_status tch_metis_read(const char* prefix, int32_t i, tch_metis** out);
+tc_status tch_metis_get(const tch_metis* m, tch_metis_view* out);
+void tch_metis_free(tch_metis* m);
+tc_status tch_metis_assemble(const tch_metis* const* pes, int32_t k, int32_t i, tch_csr** out,
+                             uint8_t* owned);
+
 /* ---- algorithmic work of a count ----------------------------------------- */
 
 typedef struct tch_work {

@@ -64,3 +64,24 @@ def test_per_pe_attribution_lines(balance):
         (i, int(bounds[i]), int(bounds[i + 1]), int(want[i])) for i in range(3)]
     m = re.search(r"total_num_triangles = (\d+)", p.stdout)
     assert int(m.group(1)) == int(want.sum())
+
+
+@pytest.mark.gpu
+def test_metis_part_files(tmp_path):
+    import numpy as np
+
+    d = Graph.rmat(1 << 13, 16, 2).oriented()
+    rp, ci = d.out_rowptr(), d.out_colidx()
+    k = 3
+    part = (np.arange(d.V(), dtype=np.int64) * 7919) % k
+    prefix = str(tmp_path / "pgraph")
+    d.write_metis(prefix, part)
+    p = run(f"metis:{prefix}:{k}")
+    assert p.returncode == 0, p.stderr
+    got = re.findall(r"pe\[(\d+)\] n_real = (\d+) count = (\d+)", p.stdout)
+    want = [(i, int((part == i).sum()), oracle.tc_count_rows(rp, ci, np.flatnonzero(part == i)))
+            for i in range(k)]
+    assert [(int(a), int(b), int(c)) for a, b, c in got] == want
+    m = re.search(r"total_num_triangles = (\d+)", p.stdout)
+    assert int(m.group(1)) == oracle.tc_count(rp, ci)
+    assert run(f"metis:{prefix}:{k + 1}").returncode == 2   # PE 3's files do not exist

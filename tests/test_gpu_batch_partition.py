@@ -78,3 +78,30 @@ def test_partition_halo_counts(ctx, k, balance):
         dg.free()
         got_sum += got
     assert got_sum == total
+
+
+def test_metis_parts_count_their_own_rows(ctx, tmp_path):
+    from hetegeneous_parallel_programming_amd import MetisPart, assemble_metis
+
+    g = Graph.rmat(1 << 15, 16, 8).oriented()
+    rp, ci = g.out_rowptr(), g.out_colidx()
+    k = 4
+    part = (np.arange(g.V(), dtype=np.int64) * 104729 + 5) % k   # scattered ownership
+    prefix = str(tmp_path / "pgraph")
+    g.write_metis(prefix, part)
+    pes = MetisPart.read_all(prefix, k)
+    want_v = oracle.tc_per_vertex(rp, ci)
+    acc, acc_v = 0, np.zeros(g.V(), np.uint64)
+    for i, pe in enumerate(pes):
+        sub, owned = assemble_metis(pes, i)
+        want = oracle.tc_count_rows(rp, ci, pe.vertex_list)
+        dg = ctx.upload(sub, owned=owned)
+        assert ctx.count(dg)[0] == want
+        assert ctx.count_plain(dg)[0] == want
+        total, out = ctx.count_local(dg, edges=False)
+        assert total == want
+        acc_v += out["vertex_counts"]
+        dg.free()
+        acc += want
+    assert acc == oracle.tc_count(rp, ci)
+    assert np.array_equal(acc_v, want_v)

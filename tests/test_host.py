@@ -273,6 +273,100 @@ def test_induced_partition_matches_reference_semantics():
     assert acc == oracle.tc_count(rp, ci)
 
 
+def _reference_metis_files(tmp_path, rp, ci, part, k):
+    """The file set graph_partition.cc:501-566 writes for each PE, built here
+    with numpy: meta lines n, n_real, E, max_subg_nv, max_subg_ne,
+    max_degree; int64 rowptr over all ids with only the PE's rows filled;
+    int32 colidx; int32 owned ids ascending; int64 part array."""
+    n = rp.size - 1
+    deg = np.diff(rp)
+    nreal = [int((part == i).sum()) for i in range(k)]
+    ne = [int(deg[part == i].sum()) for i in range(k)]
+    files = {}
+    for i in range(k):
+        mine = part == i
+        sub_rp = np.concatenate([[0], np.cumsum(np.where(mine, deg, 0))]).astype(np.int64)
+        sub_ci = np.concatenate([ci[rp[v]:rp[v + 1]] for v in np.flatnonzero(mine)] or
+                                [np.zeros(0, np.int32)]).astype(np.int32)
+        meta = f"{n}\n{nreal[i]}\n{ne[i]}\n{max(nreal)}\n{max(ne)}\n{int(deg.max())}\n"
+        files[i] = {".meta.txt": meta.encode(), ".vertex.bin": sub_rp.tobytes(),
+                    ".edge.bin": sub_ci.tobytes(),
+                    ".vertex_list.bin": np.flatnonzero(mine).astype(np.int32).tobytes(),
+                    ".part.bin": part.astype(np.int64).tobytes()}
+    return files
+
+
+def test_metis_files_match_the_reference_layout(tmp_path):
+    d = Graph.rmat(1 << 12, 8, 3).oriented()
+    rp, ci = d.out_rowptr(), d.out_colidx()
+    k = 3
+    part = (np.arange(d.V(), dtype=np.int64) * 7919) % k   # scattered, as a METIS part is
+    prefix = str(tmp_path / "pgraph")
+    d.write_metis(prefix, part)
+    want = _reference_metis_files(tmp_path, rp, ci, part, k)
+    for i in range(k):
+        for ext, data in want[i].items():
+            assert open(f"{prefix}{i}{ext}", "rb").read() == data, (i, ext)
+
+
+def test_metis_files_read_assemble_and_count(tmp_path):
+    from hetegeneous_parallel_programming_amd import MetisPart, assemble_metis
+
+    d = Graph.rmat(1 << 13, 16, 4).oriented()
+    rp, ci = d.out_rowptr(), d.out_colidx()
+    k = 4
+    part = (np.arange(d.V(), dtype=np.int64) * 104729 + 17) % k
+    prefix = str(tmp_path / "pgraph")
+    # files written by the reference's layout, read by the product
+    for i, files in _reference_metis_files(tmp_path, rp, ci, part, k).items():
+        for ext, data in files.items():
+            open(f"{prefix}{i}{ext}", "wb").write(data)
+    pes = MetisPart.read_all(prefix, k)
+    total = oracle.tc_count(rp, ci)
+    acc = 0
+    for i, pe in enumerate(pes):
+        assert (pe.n, pe.n_real) == (d.V(), int((part == i).sum()))
+        assert np.array_equal(pe.vertex_list, np.flatnonzero(part == i))
+        assert np.array_equal(pe.part, part)
+        g, owned = assemble_metis(pes, i)
+        assert np.array_equal(owned, (part == i).astype(np.uint8))
+        grp, gci = g.out_rowptr(), g.out_colidx()
+        # own rows as written, remote lists they reach as in the whole DAG, the rest empty
+        reach = np.zeros(d.V(), bool)
+        reach[part == i] = True
+        for u in np.flatnonzero(part == i):
+            reach[ci[rp[u]:rp[u + 1]]] = True
+        assert np.array_equal(np.diff(grp), np.where(reach, np.diff(rp), 0))
+        assert np.array_equal(gci, np.concatenate([ci[rp[v]:rp[v + 1]] for v in np.flatnonzero(reach)]))
+        c = oracle.tc_count_rows(grp, gci, pe.vertex_list)
+        assert c == oracle.tc_count_rows(rp, ci, pe.vertex_list)
+        acc += c
+    assert acc == total
+    whole, owned = assemble_metis(pes, -1)
+    assert owned.all()
+    assert np.array_equal(whole.out_rowptr(), rp) and np.array_equal(whole.out_colidx(), ci)
+
+
+def test_metis_read_rejects_bad_files(tmp_path):
+    from hetegeneous_parallel_programming_amd import MetisPart
+
+    d = Graph.er(500, 6, 1).oriented()
+    prefix = str(tmp_path / "pgraph")
+    part = np.arange(d.V(), dtype=np.int64) % 2
+    d.write_metis(prefix, part)
+    with pytest.raises(TcError) as e:
+        MetisPart(prefix, 2)          # no such PE
+    assert e.value.status == TC_EIO
+    vl = np.fromfile(f"{prefix}0.vertex_list.bin", np.int32)
+    vl[0] = 1                         # an id PE 1 owns
+    vl.tofile(f"{prefix}0.vertex_list.bin")
+    with pytest.raises(TcError) as e:
+        MetisPart(prefix, 0)
+    assert e.value.status == TC_EINVAL
+    with pytest.raises(TcError):
+        d.write_metis(prefix, np.full(d.V(), -1, np.int64))
+
+
 def test_graphminer_roundtrip(tmp_path):
     g = Graph.rmat(4096, 8, 3)
     prefix = str(tmp_path / "g")

@@ -1,13 +1,16 @@
 """GPU parity at the full workload sizes against the CPU oracle's committed
 counts (tests/golden/big.json, written by make_golden.py --big):
-RMAT scale 22 whole graph, the four contiguous parts of RMAT scale 24 with
-their halo caches, RMAT scale 20, and the 4096-graph batched workload."""
+RMAT scale 22 whole graph (total and per-edge / per-vertex / per-PE
+attribution sums), the four contiguous parts of RMAT scale 24 with their
+halo caches (and a partner-balanced split of it), RMAT scale 20, and the
+4096-graph batched workload."""
 import json
 import os
 
 import numpy as np
 import pytest
 
+import oracle
 from hetegeneous_parallel_programming_amd import Graph, PartitionedGraph
 
 pytestmark = pytest.mark.gpu
@@ -50,6 +53,40 @@ def test_rmat24_partitions_with_halo(ctx):
         assert got == p["triangles"], p
         total += got
     assert total == gold["triangles"]
+    # the partner-balanced split: other owned ranges, same total
+    bal = PartitionedGraph(g, 4, "partner")
+    b = bal.bounds()
+    assert b[0] == 0 and b[-1] == g.V() and (np.diff(b) > 0).all()
+    total = 0
+    for i in range(4):
+        hp = bal.halo_partition(i)
+        dg = ctx.upload_partition(hp)
+        total += ctx.count(dg)[0]
+        dg.free()
+        del hp
+    assert total == gold["triangles"]
+
+
+def test_rmat22_local_counts(ctx):
+    """Per-edge, per-vertex and per-PE attribution at full size: each sums to
+    the oracle's total (per-vertex to three times it)."""
+    g = Graph.rmat_scale(22, 16, 1)
+    g.orientation()
+    want = GOLD["rmat22_seed1"]["triangles"]
+    dg = ctx.upload(g)
+    bounds = PartitionedGraph(g, 4).bounds()
+    total, out = ctx.count_local(dg, bounds=bounds)
+    assert total == want
+    assert int(out["edge_counts"].sum(dtype=np.uint64)) == want
+    assert int(out["vertex_counts"].sum(dtype=np.uint64)) == 3 * want
+    assert int(out["part_counts"].sum()) == want
+    assert ctx.count(dg)[0] == want
+    dg.free()
+    # a row's per-edge counts sum to what the oracle counts at that pivot
+    rp, ci = g.out_rowptr(), g.out_colidx()
+    ec = out["edge_counts"].astype(np.int64)
+    for u in range(0, g.V(), 997)[:200]:
+        assert int(ec[rp[u]:rp[u + 1]].sum()) == oracle.tc_count(rp, ci, u, u + 1)
 
 
 def test_batch_4096(ctx):

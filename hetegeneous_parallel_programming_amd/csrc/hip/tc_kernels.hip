@@ -4,24 +4,31 @@
 // (bs_warp_vertex.cuh:13-22): sum over pivots u, over v in N+(u), of
 // |N+(u) ∩ N+(v)| on the degree-ordered DAG, i.e. every triangle once.
 //
-// Work unit.  A group of G lanes (G = 8, 16, 32 or 64) owns one pivot u.
-// It stages N+(u) in LDS, then walks the edges (u, v) in batches of G: lane
-// j of the batch reads rowptr[v], rowptr[v+1] for its edge, a group scan of
-// the d+(v) gives each edge's slice of a flat index space, and the group's
-// lanes then stride that space with coalesced int32 reads of the N+(v)
-// lists.  Each read neighbour w is binary-searched in the LDS copy of N+(u)
-// (the search of the reference's warp_intersect, successful_push_pull/
-// graph_gpu.h:228-252, with the pivot list as the searched side), and every
-// element of N+(v) is read exactly once per edge.  Several lookups per lane
-// (TC_IN_FLIGHT) are kept in flight so that the HBM/L2 reads and the LDS
-// search chains overlap.
+// Partner scheme.  The same sum is grouped by edge owner: edge (u, v) is
+// owned by the endpoint with the longer out-list (u when d+(v) <= d+(u)),
+// which keeps that list as the set; the other endpoint (its partner) has its
+// list streamed through it.  Each edge then streams min(d+(u), d+(v)) ids
+// instead of d+(v); the intersection is symmetric, so every edge contributes
+// the same term as in the reference.  A build pass (once per upload) writes
+// each owner's partners as list references and cuts owners into partner
+// rows of at most kPartnersPerRow partners -- the unit of work.
 //
-// Scheduling.  A planning pass bins pivots by d+(u) and by the wedge work
-// T(u) = sum of d+(v) over v in N+(u); light pivots go to 8/16/32-lane
-// groups, medium ones to a wave, heavy ones to a whole workgroup whose four
-// waves share one LDS copy of N+(u).  A persistent grid then pulls tiles
-// (bin, run of pivots) from a device queue, heaviest bins first.  Counts are
-// u64 sums, so the result does not depend on tile order.
+// Work unit.  A group of G lanes (G = 8, 16, 32 or 64, or a whole workgroup)
+// takes one partner row: it builds the owner's set in LDS (a two-choice
+// cuckoo hash, or an Eytzinger search tree when the hash does not fit or
+// does not converge; sets too large for LDS are searched in HBM), then walks
+// the partners in batches of G: a group scan of the batch's list lengths
+// gives each partner a slice of one flat index space, and the lanes stride
+// that space with coalesced int32 loads, each id looked up in the set.
+// Several lookups per lane (TC_IN_FLIGHT) stay in flight so that the HBM/L2
+// reads and the LDS probes overlap.
+//
+// Scheduling.  A planning pass bins partner rows by set size and work (the
+// total length of their partners' lists) into workgroup, wave and 32/16/8-
+// lane groups; one persistent kernel per bin pulls tiles (runs of rows of
+// that bin) from a device queue, heaviest bins first.  Counts are u64 sums,
+// so the result does not depend on tile order.  Local variants of the bin
+// kernels also attribute every hit to its edge and corners.
 #include <climits>
 
 #include "tc_kernels.hpp"

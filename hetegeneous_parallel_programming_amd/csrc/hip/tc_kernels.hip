@@ -1007,18 +1007,6 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
 #define TC_COUNT_WAVES_PER_EU 8
 #endif
 
-// Pivots per tile of each bin (rounds = pivots per group in light bins).
-__device__ __forceinline__ int tile_pivots(int bin, int rounds) {
-  switch (bin) {
-    case BIN_GLOBAL:
-    case BIN_BLOCK: return 1;
-    case BIN_WAVE: return kWaves;
-    case BIN_G32: return (kBlock / 32) * rounds;
-    case BIN_G16: return (kBlock / 16) * rounds;
-    default: return (kBlock / 8) * rounds;
-  }
-}
-
 // Local variants hold 1.5x the LDS of a plain count: 5 workgroups per CU.
 constexpr int kLocalWavesPerEU = 5;
 
@@ -1212,10 +1200,14 @@ static int bin_blocks_per_cu() {
   return n;
 }
 
+// tiles >= 0: the bin's tile count is known on the host; the grid is then
+// capped at it, and an empty bin is not launched.
 template <int BIN, bool kLocal = false>
-static void launch_bin(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
-  const int g = grid > 0 ? grid : bin_blocks_per_cu<BIN, kLocal>() * cus;
-  count_kernel<BIN, kLocal><<<g, kBlock, 0, s>>>(a);
+static void launch_bin(const LaunchArgs& a, int cus, int grid, hipStream_t s, int64_t tiles = -1) {
+  if (tiles == 0) return;
+  int64_t g = grid > 0 ? grid : (int64_t)bin_blocks_per_cu<BIN, kLocal>() * cus;
+  if (tiles > 0 && tiles < g) g = tiles;
+  count_kernel<BIN, kLocal><<<(unsigned)g, kBlock, 0, s>>>(a);
 }
 
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
@@ -1225,6 +1217,16 @@ hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
   launch_bin<BIN_G32>(a, cus, grid, s);
   launch_bin<BIN_G16>(a, cus, grid, s);
   launch_bin<BIN_G8>(a, cus, grid, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int64_t* tiles, hipStream_t s) {
+  launch_bin<BIN_GLOBAL>(a, cus, grid, s, tiles[BIN_GLOBAL]);
+  launch_bin<BIN_BLOCK>(a, cus, grid, s, tiles[BIN_BLOCK]);
+  launch_bin<BIN_WAVE>(a, cus, grid, s, tiles[BIN_WAVE]);
+  launch_bin<BIN_G32>(a, cus, grid, s, tiles[BIN_G32]);
+  launch_bin<BIN_G16>(a, cus, grid, s, tiles[BIN_G16]);
+  launch_bin<BIN_G8>(a, cus, grid, s, tiles[BIN_G8]);
   return hipGetLastError();
 }
 

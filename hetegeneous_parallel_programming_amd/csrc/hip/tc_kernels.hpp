@@ -32,6 +32,18 @@ constexpr int kCapG8 = 32, kCapG16 = 64, kCapG32 = 256, kCapWave = 1024, kCapBlo
 // every bin's areas fill the same 4096-word region of a workgroup.
 constexpr int kAreaG8 = 128, kAreaG16 = 256, kAreaG32 = 512, kAreaWave = 1024, kAreaBlock = 4096;
 
+// Partner rows per tile of each bin (rounds = rows per group in light bins).
+__host__ __device__ inline int tile_pivots(int bin, int rounds) {
+  switch (bin) {
+    case BIN_GLOBAL:
+    case BIN_BLOCK: return 1;
+    case BIN_WAVE: return kBlock / 64;
+    case BIN_G32: return (kBlock / 32) * rounds;
+    case BIN_G16: return (kBlock / 16) * rounds;
+    default: return (kBlock / 8) * rounds;
+  }
+}
+
 // Counter words of a plan (uint32).
 constexpr int kCntBinCount = 0;                   // [kNumBins]
 constexpr int kCntBinCursor = kNumBins;           // [kNumBins]
@@ -171,6 +183,9 @@ hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);
 hipError_t launch_plan(const LaunchArgs& a, hipStream_t s);
 // One full-residency persistent launch per bin (grid > 0 overrides the size).
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s);
+// Same, with each bin's tile count known on the host (tiles[kNumBins]):
+// empty bins are skipped and no grid exceeds its bin's tiles.
+hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int64_t* tiles, hipStream_t s);
 // own: null, or per row 1 if it is a pivot row (rows outside
 // [row_begin, row_end) are never pivots).
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,

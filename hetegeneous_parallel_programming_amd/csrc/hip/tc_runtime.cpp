@@ -286,13 +286,15 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   return TC_OK;
 }
 
-static tc_status finish_stats(tc_ctx* ctx, tc_stats* st, double t0, bool planned, int64_t pivots) {
+// plan_ms = ev[0] -> ev[plan_end]; count_ms = ev[1] -> ev[2].
+static tc_status finish_stats(tc_ctx* ctx, tc_stats* st, double t0, bool planned, int64_t pivots,
+                              int plan_end = 1) {
   if (!st) return TC_OK;
   std::memset(st, 0, sizeof(*st));
   st->total_ms = now_ms() - t0;
   float ms = 0;
   if (planned) {
-    TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[plan_end]));
     st->plan_ms = ms;
     const uint32_t* c = (const uint32_t*)(ctx->pinned + 8);
     for (int b = 0; b < TC_NUM_BINS; ++b) st->bin_pivots[b] = c[tck::kCntBinCount + b];
@@ -475,15 +477,26 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
   TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
   TC_HIP_TRY(tck::launch_plan(a, s));
-  TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  TC_HIP_TRY(tck::launch_count(a, ctx->cus, ctx->tune.grid, s));
-  TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
-  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[3], s));
+  // The plan's rows per bin size each bin's launch: empty bins are skipped
+  // and small ones get no more workgroups than tiles (a small graph would
+  // otherwise pay for six full-residency grids).
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,
                             hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
+  int64_t tiles[tck::kNumBins];
+  const uint32_t* c = (const uint32_t*)(ctx->pinned + 8);
+  for (int b = 0; b < tck::kNumBins; ++b) {
+    const int64_t piv = tck::tile_pivots(b, ctx->tune.rounds > 0 ? ctx->tune.rounds : 1);
+    tiles[b] = ((int64_t)c[tck::kCntBinCount + b] + piv - 1) / piv;
+  }
+  TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, tiles, s));
+  TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
-  return finish_stats(ctx, stats, t0, true, g->nv);
+  return finish_stats(ctx, stats, t0, true, g->nv, 3);
 }
 
 tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {

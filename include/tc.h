@@ -132,7 +132,8 @@ tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* st
  * optional):
  *   edge_counts[e]   for each DAG edge e = (u, v) of the counted rows, in
  *                    CSR order from the first counted row (nnz entries for a
- *                    whole graph, local_nnz for a partition):
+ *                    whole graph or a tc_graph_upload_rows graph, whose
+ *                    uncounted rows read 0; local_nnz for a partition):
  *                    |N+(u) ∩ N+(v)|, the per-edge term the reference sums
  *                    (bs_warp_vertex.cuh:20);
  *   vertex_counts[v] for each of the graph's n rows: the counted triangles

@@ -433,7 +433,7 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) {
   return TC_OK;
 }
 
-int64_t tc_graph_bytes(const tc_graph* g) { return g ? g->bytes : 0; }
+int64_t tc_graph_bytes(const tc_graph* g) { return g ? g->bytes + g->local.bytes : 0; }
 
 tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) {
   if (!g || !out) return fail(TC_EINVAL, "null argument");

@@ -624,10 +624,17 @@ struct StoreU32 {
   uint32_t* p;
   __device__ void operator()(int64_t i, int64_t v) const { p[i] = (uint32_t)v; }
 };
+// c1[i] plus row i's kXcds in-edge counters (stripe-major, stride n).
 struct LoadSum2 {
   const int32_t* a;
   const int32_t* b;
-  __device__ int64_t operator()(int64_t i) const { return (int64_t)a[i] + b[i]; }
+  int64_t n;
+  __device__ int64_t operator()(int64_t i) const {
+    int64_t t = a[i];
+#pragma unroll
+    for (int k = 0; k < kXcds; ++k) t += b[k * n + i];
+    return t;
+  }
 };
 struct LoadI32 {
   const int32_t* p;
@@ -660,10 +667,11 @@ __global__ __launch_bounds__(kBlock) void partner_count_kernel(LaunchArgs a, int
     const int64_t* rp = a.rp + d.rp_off;
     const int32_t* ci = a.ci + d.ci_off;
     const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
+    int32_t* c2 = b.c2 + (int64_t)(blockIdx.x % kXcds) * RP;
     for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
       const int v = ci[e];
       if (rp[v + 1] - rp[v] <= du) ++c1;
-      else atomicAdd(&b.c2[d.row_base + v], 1);
+      else atomicAdd(&c2[d.row_base + v], 1);
     }
   }
 #pragma unroll
@@ -672,8 +680,9 @@ __global__ __launch_bounds__(kBlock) void partner_count_kernel(LaunchArgs a, int
 }
 
 // Pass 2: partner ids.  Out-edge partners of u fill the front of u's range
-// in row order (ballot ranks); in-edge partners take the rest in arrival
-// order (c2 is counted down).  Order inside a range does not change any count.
+// in row order (ballot ranks); in-edge partners take the rest, stripe by
+// stripe, in arrival order within a stripe (a cursor per stripe).  Order
+// inside a range does not change any count.
 __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
   const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
   const int gl = threadIdx.x % kBuildG;
@@ -691,6 +700,7 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
   const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
   const int shift = (int)(threadIdx.x & 63) & ~(kBuildG - 1);
   const uint32_t below = (1u << gl) - 1u;
+  const int64_t sbase = (int64_t)(blockIdx.x % kXcds) * RP;
   for (int64_t eb = e0; eb < e1; eb += kBuildG) {
     const int64_t e = eb + gl;
     int v = 0;
@@ -709,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
         slot = w + __popc(gm & below);
         pci[slot] = list_ref(d.ci_off + vb, dv);
       } else {
-        slot = off[v] - base + b.c1[d.row_base + v] + (atomicSub(&b.c2[d.row_base + v], 1) - 1);
+        slot = (int64_t)atomicAdd(&b.cur[sbase + d.row_base + v], 1ull);
         pci[slot] = list_ref(d.ci_off + e0, du);
       }
       if (b.pedge) {  // both ways it is edge (u, v)
@@ -718,6 +728,21 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
       }
     }
     w += __popc(gm);
+  }
+}
+
+// Pass 1b (after the scan): each stripe's fill cursor starts where its
+// in-edge partners go: after the row's out-edge partners and the earlier
+// stripes' in-edge partners (offsets relative to the graph's first partner).
+__global__ __launch_bounds__(kBlock) void partner_cursor_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= RP) return;
+  const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
+  int64_t at = b.off[q] - b.off[d.row_base] + b.c1[q];
+#pragma unroll
+  for (int k = 0; k < kXcds; ++k) {
+    b.cur[k * RP + q] = (unsigned long long)at;
+    at += b.c2[k * RP + q];
   }
 }
 
@@ -786,12 +811,14 @@ hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, 
   if (RP <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
   const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
-  hipError_t e = launch_zero(b.c2, RP, s);
+  hipError_t e = launch_zero(b.c2, RP * kXcds, s);
   if (e == hipSuccess) e = launch_zero((void*)a.vrow, a.P * (int64_t)(sizeof(VRow) / 4), s);  // slots unused
   if (e != hipSuccess) return e;
   partner_count_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_scan(LoadSum2{b.c1, b.c2}, StoreI64{b.off}, RP, b.part, s)) != hipSuccess) return e;
+  if ((e = launch_scan(LoadSum2{b.c1, b.c2, RP}, StoreI64{b.off}, RP, b.part, s)) != hipSuccess) return e;
+  partner_cursor_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   partner_fill_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   partner_rows_count_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);

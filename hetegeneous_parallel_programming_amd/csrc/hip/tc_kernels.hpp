@@ -160,9 +160,18 @@ inline int64_t plan_blocks(int64_t P) { return (P + 255) / 256; }
 
 // Scratch of a partner build over RP row slots (the graphs' n_rows + 1
 // slots, concatenated in graph order from row_base 0).
+// Partners a row owns through in-edges are counted in kXcds stripes:
+// workgroup i updates stripe i mod kXcds, and the dispatcher places it on
+// XCD i mod kXcds, so each stripe's counters and cursors are touched by one
+// XCD only (a hub's in-edges then spread over kXcds cache lines).  Both
+// build passes use the same grid, so an edge falls in the same stripe in
+// each whatever the placement.
+constexpr int kXcds = 8;
+
 struct BuildBufs {
   int32_t* c1;          // [RP] partners owned through out-edges
-  int32_t* c2;          // [RP] partners owned through in-edges (consumed by the fill)
+  int32_t* c2;          // [kXcds * RP] partners owned through in-edges, per stripe
+  unsigned long long* cur;  // [kXcds * RP] fill cursor (partner slot) of each stripe
   int64_t* off;         // [RP] partner offsets (pool-global exclusive scan)
   int32_t* kc;          // [RP] partner rows per DAG row
   int64_t* voff;        // [RP] partner-row offsets (pool-global exclusive scan)

@@ -115,9 +115,11 @@ int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs
 }
 
 static int64_t build_words(int64_t rp_slots) {
-  // c1, c2, kc (int32) + off, voff (int64) + scan partials, 256-byte pieces
+  // c1, kc (int32), c2 (int32 per XCD stripe), cur (int64 per stripe) + off, voff (int64)
+  // + scan partials, 256-byte pieces
   auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
-  return up(4 * rp_slots) * 3 + up(8 * rp_slots) * 2 + up(8 * tck::scan_partials(rp_slots));
+  return up(4 * rp_slots) * 2 + up(4 * tck::kXcds * rp_slots) + up(8 * tck::kXcds * rp_slots) +
+         up(8 * rp_slots) * 2 + up(8 * tck::scan_partials(rp_slots));
 }
 
 int64_t BuildScratch::bytes_for(int64_t rp_slots) { return build_words(rp_slots); }
@@ -136,7 +138,8 @@ tc_status BuildScratch::ensure(int64_t rp_slots) {
   const int64_t n = std::max<int64_t>(rp_slots, 1);
   char* p = mem;
   b.c1 = (int32_t*)p;  p += up(4 * n);
-  b.c2 = (int32_t*)p;  p += up(4 * n);
+  b.c2 = (int32_t*)p;  p += up(4 * tck::kXcds * n);
+  b.cur = (unsigned long long*)p; p += up(8 * tck::kXcds * n);
   b.kc = (int32_t*)p;  p += up(4 * n);
   b.off = (int64_t*)p; p += up(8 * n);
   b.voff = (int64_t*)p; p += up(8 * n);

@@ -105,3 +105,22 @@ def test_metis_parts_count_their_own_rows(ctx, tmp_path):
         acc += want
     assert acc == oracle.tc_count(rp, ci)
     assert np.array_equal(acc_v, want_v)
+
+
+def test_owned_mask_edge_cases(ctx):
+    g = Graph.rmat(1 << 12, 8, 3).oriented()
+    rp, ci = g.out_rowptr(), g.out_colidx()
+    none = np.zeros(g.V(), np.uint8)
+    dg = ctx.upload(g, owned=none)
+    assert ctx.count(dg)[0] == 0 and ctx.count_plain(dg)[0] == 0
+    total, out = ctx.count_local(dg)
+    assert total == 0 and not out["vertex_counts"].any() and not out["edge_counts"].any()
+    dg.free()
+    every = np.ones(g.V(), np.uint8)
+    dg = ctx.upload(g, owned=every)
+    assert ctx.count(dg)[0] == oracle.tc_count(rp, ci)
+    dg.free()
+    odd = (np.arange(g.V()) % 2).astype(np.uint8)
+    dg = ctx.upload(g, owned=odd)
+    assert ctx.count(dg)[0] == oracle.tc_count_rows(rp, ci, np.flatnonzero(odd))
+    dg.free()

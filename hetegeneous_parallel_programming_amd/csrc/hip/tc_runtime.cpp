@@ -334,6 +334,8 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
   const std::string marketing = prop.name[0] ? std::string(prop.name) : std::string("AMD GPU");
   c->name = marketing + " (" + prop.gcnArchName + ", " + std::to_string(c->cus) + " CUs)";
   TC_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  TC_HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  TC_HIP_TRY(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
   for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
   TC_HIP_TRY(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));
   c->count_grid = tck::count_blocks_per_cu() * c->cus;
@@ -351,7 +353,10 @@ void tc_ctx_destroy(tc_ctx* ctx) {
     ctx->scratch.release();
     for (auto& ev : ctx->ev)
       if (ev) (void)hipEventDestroy(ev);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   }
   delete ctx;
@@ -494,7 +499,25 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
     tiles[b] = ((int64_t)c[tck::kCntBinCount + b] + piv - 1) / piv;
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, tiles, s));
+  const bool split = tiles[tck::BIN_GLOBAL] + tiles[tck::BIN_BLOCK] > 0 &&
+                     tiles[tck::BIN_WAVE] + tiles[tck::BIN_G32] + tiles[tck::BIN_G16] + tiles[tck::BIN_G8] > 0;
+  if (split) {
+    // The global-list and workgroup bins hold every resident workgroup until
+    // their queues drain; the light bins, on a second stream, take over the
+    // CUs their tails free.  The host has synchronised after planning, so the
+    // second stream needs no wait for the plan.
+    int64_t heavy[tck::kNumBins], light[tck::kNumBins];
+    for (int b = 0; b < tck::kNumBins; ++b) {
+      heavy[b] = b <= tck::BIN_BLOCK ? tiles[b] : 0;
+      light[b] = b <= tck::BIN_BLOCK ? 0 : tiles[b];
+    }
+    TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, heavy, s));
+    TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, light, ctx->side));
+    TC_HIP_TRY(hipEventRecord(ctx->join, ctx->side));
+    TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join, 0));
+  } else {
+    TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, tiles, s));
+  }
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));

@@ -128,6 +128,7 @@ struct tc_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;    // light bins of tc_count, beside the heavy ones
   hipEvent_t join = nullptr;     // side stream done (no timing)
+  hipEvent_t fork = nullptr;     // side stream may start (no timing)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int cus = 0;
   int count_grid = 0;   // full-residency grid of the persistent kernel

@@ -1247,6 +1247,16 @@ hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_count_bins(const LaunchArgs& a, int cus, int grid, int first, int last, hipStream_t s) {
+  if (first <= BIN_GLOBAL && BIN_GLOBAL <= last) launch_bin<BIN_GLOBAL>(a, cus, grid, s);
+  if (first <= BIN_BLOCK && BIN_BLOCK <= last) launch_bin<BIN_BLOCK>(a, cus, grid, s);
+  if (first <= BIN_WAVE && BIN_WAVE <= last) launch_bin<BIN_WAVE>(a, cus, grid, s);
+  if (first <= BIN_G32 && BIN_G32 <= last) launch_bin<BIN_G32>(a, cus, grid, s);
+  if (first <= BIN_G16 && BIN_G16 <= last) launch_bin<BIN_G16>(a, cus, grid, s);
+  if (first <= BIN_G8 && BIN_G8 <= last) launch_bin<BIN_G8>(a, cus, grid, s);
+  return hipGetLastError();
+}
+
 hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int64_t* tiles, hipStream_t s) {
   launch_bin<BIN_GLOBAL>(a, cus, grid, s, tiles[BIN_GLOBAL]);
   launch_bin<BIN_BLOCK>(a, cus, grid, s, tiles[BIN_BLOCK]);

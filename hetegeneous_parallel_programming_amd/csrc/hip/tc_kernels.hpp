@@ -192,6 +192,8 @@ hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);
 hipError_t launch_plan(const LaunchArgs& a, hipStream_t s);
 // One full-residency persistent launch per bin (grid > 0 overrides the size).
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s);
+// Bins first..last only, each at full residency.
+hipError_t launch_count_bins(const LaunchArgs& a, int cus, int grid, int first, int last, hipStream_t s);
 // Same, with each bin's tile count known on the host (tiles[kNumBins]):
 // empty bins are skipped and no grid exceeds its bin's tiles.
 hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int64_t* tiles, hipStream_t s);

@@ -336,6 +336,7 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
   TC_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   TC_HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   TC_HIP_TRY(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
+  TC_HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
   TC_HIP_TRY(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));
   c->count_grid = tck::count_blocks_per_cu() * c->cus;
@@ -356,6 +357,7 @@ void tc_ctx_destroy(tc_ctx* ctx) {
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->join) (void)hipEventDestroy(ctx->join);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   }

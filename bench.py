@@ -528,7 +528,7 @@ def leg_headline(ctx, args) -> dict:
         "roofline": {"bound": "hbm", "achieved": best["alg_GBps"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": best["alg_GBps"] / PEAK_HBM_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "count_kernel<bin> (6 per-bin persistent launches per step)",
+                     "kernel": "count_kernel<bin> (one persistent launch per non-empty bin per step; light bins on a second stream)",
                      "alg_bytes": work["partner_bytes"],
                      "alg_bytes_note": "bytes the partner-scheme count reads with no reuse: per partner row "
                                        "32 + 4 d+(x), per edge 8 + 4 min(d+(u), d+(v))",

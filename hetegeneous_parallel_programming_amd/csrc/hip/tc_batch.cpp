@@ -67,15 +67,21 @@ static tc_status capture(tc_ctx* ctx, tc_batch* b) {
   hipError_t e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s);
   if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)b->n, s);
   if (e == hipSuccess) e = tck::launch_plan(a, s);
-  // Heavy bins on the capture stream, light bins on a forked branch: the
-  // light bins' workgroups fill the CUs the heavy bins' tails free.
+  // Heavy bins on the capture stream, the wave bin and the sub-wave bins on
+  // two forked branches: each group's workgroups fill the CUs the others'
+  // tails free.
   if (e == hipSuccess) e = hipEventRecord(ctx->fork, s);
   if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->fork, 0);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side2, ctx->fork, 0);
   if (e == hipSuccess) e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_GLOBAL, tck::BIN_BLOCK, s);
   if (e == hipSuccess)
-    e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_WAVE, tck::BIN_G8, ctx->side);
+    e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_WAVE, tck::BIN_WAVE, ctx->side);
+  if (e == hipSuccess)
+    e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_G32, tck::BIN_G8, ctx->side2);
   if (e == hipSuccess) e = hipEventRecord(ctx->join, ctx->side);
+  if (e == hipSuccess) e = hipEventRecord(ctx->join2, ctx->side2);
   if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join, 0);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join2, 0);
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->pinned, a.bufs.outs, sizeof(uint64_t) * (size_t)b->n, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)

@@ -127,6 +127,8 @@ struct tc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;    // light bins of tc_count, beside the heavy ones
+  hipStream_t side2 = nullptr;   // sub-wave bins, when split from the wave bin
+  hipEvent_t join2 = nullptr;
   hipEvent_t join = nullptr;     // side stream done (no timing)
   hipEvent_t fork = nullptr;     // side stream may start (no timing)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

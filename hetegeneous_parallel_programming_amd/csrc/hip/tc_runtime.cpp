@@ -335,6 +335,8 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
   c->name = marketing + " (" + prop.gcnArchName + ", " + std::to_string(c->cus) + " CUs)";
   TC_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   TC_HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  TC_HIP_TRY(hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
+  TC_HIP_TRY(hipEventCreateWithFlags(&c->join2, hipEventDisableTiming));
   TC_HIP_TRY(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
   TC_HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
@@ -355,10 +357,13 @@ void tc_ctx_destroy(tc_ctx* ctx) {
     for (auto& ev : ctx->ev)
       if (ev) (void)hipEventDestroy(ev);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->side2) (void)hipStreamSynchronize(ctx->side2);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->join) (void)hipEventDestroy(ctx->join);
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->join2) (void)hipEventDestroy(ctx->join2);
+    if (ctx->side2) (void)hipStreamDestroy(ctx->side2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   }
   delete ctx;
@@ -501,24 +506,40 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
     tiles[b] = ((int64_t)c[tck::kCntBinCount + b] + piv - 1) / piv;
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  const bool split = tiles[tck::BIN_GLOBAL] + tiles[tck::BIN_BLOCK] > 0 &&
-                     tiles[tck::BIN_WAVE] + tiles[tck::BIN_G32] + tiles[tck::BIN_G16] + tiles[tck::BIN_G8] > 0;
-  if (split) {
-    // The global-list and workgroup bins hold every resident workgroup until
-    // their queues drain; the light bins, on a second stream, take over the
-    // CUs their tails free.  The host has synchronised after planning, so the
-    // second stream needs no wait for the plan.
-    int64_t heavy[tck::kNumBins], light[tck::kNumBins];
-    for (int b = 0; b < tck::kNumBins; ++b) {
-      heavy[b] = b <= tck::BIN_BLOCK ? tiles[b] : 0;
-      light[b] = b <= tck::BIN_BLOCK ? 0 : tiles[b];
-    }
-    TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, heavy, s));
-    TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, light, ctx->side));
-    TC_HIP_TRY(hipEventRecord(ctx->join, ctx->side));
-    TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join, 0));
-  } else {
+  // Three groups of bins on three streams (default priority; the host has
+  // synchronised after planning): the global-list and workgroup bins, the
+  // wave bin, and the sub-wave bins.
+  // Each group's persistent workgroups hold their CUs until its queues
+  // drain; the others take over the CUs that the tails free, and the
+  // dispatcher interleaves the groups' workgroups from the start.  One
+  // non-empty group runs on the library stream alone.
+  int64_t grp[3][tck::kNumBins] = {};
+  int used = 0;
+  for (int b = 0; b < tck::kNumBins; ++b) {
+    const int gi = b <= tck::BIN_BLOCK ? 0 : b == tck::BIN_WAVE ? 1 : 2;
+    grp[gi][b] = tiles[b];
+  }
+  bool any[3];
+  for (int gi = 0; gi < 3; ++gi) {
+    any[gi] = false;
+    for (int b = 0; b < tck::kNumBins; ++b) any[gi] = any[gi] || grp[gi][b] > 0;
+    used += any[gi] ? 1 : 0;
+  }
+  if (used <= 1) {
     TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, tiles, s));
+  } else {
+    hipStream_t gs[3] = {s, ctx->side, ctx->side2};
+    hipEvent_t gj[3] = {nullptr, ctx->join, ctx->join2};
+    for (int gi = 0; gi < 3; ++gi) {
+      if (!any[gi]) continue;
+      if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs[gi], ctx->ev[1], 0));  // timing starts before any group
+      TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, grp[gi], gs[gi]));
+    }
+    for (int gi = 1; gi < 3; ++gi)
+      if (any[gi]) {
+        TC_HIP_TRY(hipEventRecord(gj[gi], gs[gi]));
+        TC_HIP_TRY(hipStreamWaitEvent(s, gj[gi], 0));
+      }
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));

@@ -67,21 +67,20 @@ static tc_status capture(tc_ctx* ctx, tc_batch* b) {
   hipError_t e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s);
   if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)b->n, s);
   if (e == hipSuccess) e = tck::launch_plan(a, s);
-  // Heavy bins on the capture stream, the wave bin and the sub-wave bins on
-  // two forked branches: each group's workgroups fill the CUs the others'
-  // tails free.
+  // Heavy bins on the capture stream; the wave bin, the 32-lane bin and the
+  // 16/8-lane bins on three forked branches: each group's workgroups fill
+  // the CUs the others' tails free.
+  constexpr int kFirst[tc_ctx::kSides] = {tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G16};
+  constexpr int kLast[tc_ctx::kSides] = {tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G8};
   if (e == hipSuccess) e = hipEventRecord(ctx->fork, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->fork, 0);
-  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side2, ctx->fork, 0);
+  for (int i = 0; i < tc_ctx::kSides && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctx->side[i], ctx->fork, 0);
   if (e == hipSuccess) e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_GLOBAL, tck::BIN_BLOCK, s);
-  if (e == hipSuccess)
-    e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_WAVE, tck::BIN_WAVE, ctx->side);
-  if (e == hipSuccess)
-    e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_G32, tck::BIN_G8, ctx->side2);
-  if (e == hipSuccess) e = hipEventRecord(ctx->join, ctx->side);
-  if (e == hipSuccess) e = hipEventRecord(ctx->join2, ctx->side2);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join, 0);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join2, 0);
+  for (int i = 0; i < tc_ctx::kSides && e == hipSuccess; ++i)
+    e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, kFirst[i], kLast[i], ctx->side[i]);
+  for (int i = 0; i < tc_ctx::kSides && e == hipSuccess; ++i) {
+    e = hipEventRecord(ctx->join[i], ctx->side[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join[i], 0);
+  }
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->pinned, a.bufs.outs, sizeof(uint64_t) * (size_t)b->n, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)

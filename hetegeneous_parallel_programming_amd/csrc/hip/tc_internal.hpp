@@ -126,11 +126,13 @@ tc_tune default_tune();
 struct tc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;    // light bins of tc_count, beside the heavy ones
-  hipStream_t side2 = nullptr;   // sub-wave bins, when split from the wave bin
-  hipEvent_t join2 = nullptr;
-  hipEvent_t join = nullptr;     // side stream done (no timing)
-  hipEvent_t fork = nullptr;     // side stream may start (no timing)
+  // Bin groups 1.. of a count run on side streams beside group 0 on
+  // `stream` (default priority); join[i]: side stream i done, fork: side
+  // streams may start (events without timing).
+  static constexpr int kSides = 3;
+  hipStream_t side[kSides] = {};
+  hipEvent_t join[kSides] = {};
+  hipEvent_t fork = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int cus = 0;
   int count_grid = 0;   // full-residency grid of the persistent kernel

@@ -334,10 +334,10 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
   const std::string marketing = prop.name[0] ? std::string(prop.name) : std::string("AMD GPU");
   c->name = marketing + " (" + prop.gcnArchName + ", " + std::to_string(c->cus) + " CUs)";
   TC_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  TC_HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-  TC_HIP_TRY(hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
-  TC_HIP_TRY(hipEventCreateWithFlags(&c->join2, hipEventDisableTiming));
-  TC_HIP_TRY(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
+  for (int i = 0; i < tc_ctx::kSides; ++i) {
+    TC_HIP_TRY(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
+    TC_HIP_TRY(hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming));
+  }
   TC_HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
   TC_HIP_TRY(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));
@@ -356,14 +356,14 @@ void tc_ctx_destroy(tc_ctx* ctx) {
     ctx->scratch.release();
     for (auto& ev : ctx->ev)
       if (ev) (void)hipEventDestroy(ev);
-    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
-    if (ctx->side2) (void)hipStreamSynchronize(ctx->side2);
+    for (hipStream_t x : ctx->side)
+      if (x) (void)hipStreamSynchronize(x);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    if (ctx->join) (void)hipEventDestroy(ctx->join);
+    for (hipEvent_t x : ctx->join)
+      if (x) (void)hipEventDestroy(x);
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
-    if (ctx->side) (void)hipStreamDestroy(ctx->side);
-    if (ctx->join2) (void)hipEventDestroy(ctx->join2);
-    if (ctx->side2) (void)hipStreamDestroy(ctx->side2);
+    for (hipStream_t x : ctx->side)
+      if (x) (void)hipStreamDestroy(x);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   }
   delete ctx;
@@ -461,6 +461,16 @@ tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) {
   return TC_OK;
 }
 
+// Stream group of a bin in tc_count: 0 heavy (global-list, workgroup),
+// 1 wave, 2 32-lane, 3 16- and 8-lane (as many groups as the device's
+// default hardware queues per process, 4).
+static int bin_group(int bin) {
+  if (bin <= tck::BIN_BLOCK) return 0;
+  if (bin == tck::BIN_WAVE) return 1;
+  if (bin == tck::BIN_G32) return 2;
+  return 3;
+}
+
 // Fills the LaunchArgs of a count of g (descs, pools, partners, tune, plan
 // scratch).
 static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
@@ -506,21 +516,19 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
     tiles[b] = ((int64_t)c[tck::kCntBinCount + b] + piv - 1) / piv;
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  // Three groups of bins on three streams (default priority; the host has
-  // synchronised after planning): the global-list and workgroup bins, the
-  // wave bin, and the sub-wave bins.
-  // Each group's persistent workgroups hold their CUs until its queues
-  // drain; the others take over the CUs that the tails free, and the
-  // dispatcher interleaves the groups' workgroups from the start.  One
-  // non-empty group runs on the library stream alone.
-  int64_t grp[3][tck::kNumBins] = {};
+  // The bins run as groups on separate streams (default priority; the host
+  // has synchronised after planning): the global-list and workgroup bins,
+  // the wave bin, the 32-lane bin, and the 16- and 8-lane bins.  Each group's persistent
+  // workgroups hold their CUs until its queues drain; the dispatcher
+  // interleaves the groups' workgroups from the start and the others take
+  // over the CUs a group's tail frees.  One non-empty group runs on the
+  // library stream alone.
+  constexpr int kGroups = tc_ctx::kSides + 1;
+  int64_t grp[kGroups][tck::kNumBins] = {};
+  for (int b = 0; b < tck::kNumBins; ++b) grp[bin_group(b)][b] = tiles[b];
+  bool any[kGroups];
   int used = 0;
-  for (int b = 0; b < tck::kNumBins; ++b) {
-    const int gi = b <= tck::BIN_BLOCK ? 0 : b == tck::BIN_WAVE ? 1 : 2;
-    grp[gi][b] = tiles[b];
-  }
-  bool any[3];
-  for (int gi = 0; gi < 3; ++gi) {
+  for (int gi = 0; gi < kGroups; ++gi) {
     any[gi] = false;
     for (int b = 0; b < tck::kNumBins; ++b) any[gi] = any[gi] || grp[gi][b] > 0;
     used += any[gi] ? 1 : 0;
@@ -528,17 +536,16 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   if (used <= 1) {
     TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, tiles, s));
   } else {
-    hipStream_t gs[3] = {s, ctx->side, ctx->side2};
-    hipEvent_t gj[3] = {nullptr, ctx->join, ctx->join2};
-    for (int gi = 0; gi < 3; ++gi) {
+    for (int gi = 0; gi < kGroups; ++gi) {
       if (!any[gi]) continue;
-      if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs[gi], ctx->ev[1], 0));  // timing starts before any group
-      TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, grp[gi], gs[gi]));
+      hipStream_t gs = gi == 0 ? s : ctx->side[gi - 1];
+      if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs, ctx->ev[1], 0));  // timing starts before any group
+      TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, grp[gi], gs));
     }
-    for (int gi = 1; gi < 3; ++gi)
+    for (int gi = 1; gi < kGroups; ++gi)
       if (any[gi]) {
-        TC_HIP_TRY(hipEventRecord(gj[gi], gs[gi]));
-        TC_HIP_TRY(hipStreamWaitEvent(s, gj[gi], 0));
+        TC_HIP_TRY(hipEventRecord(ctx->join[gi - 1], ctx->side[gi - 1]));
+        TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[gi - 1], 0));
       }
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));

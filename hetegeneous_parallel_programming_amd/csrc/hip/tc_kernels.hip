@@ -1267,15 +1267,15 @@ hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int6
   return hipGetLastError();
 }
 
-hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s) {
+hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, hipStream_t s) {
   if (a.ngraphs != 1 || !a.local.pedge || !a.local.pvid || !a.local.pown || !a.local.ecount || !a.local.vcount)
     return hipErrorInvalidValue;
-  launch_bin<BIN_GLOBAL, true>(a, cus, 0, s);
-  launch_bin<BIN_BLOCK, true>(a, cus, 0, s);
-  launch_bin<BIN_WAVE, true>(a, cus, 0, s);
-  launch_bin<BIN_G32, true>(a, cus, 0, s);
-  launch_bin<BIN_G16, true>(a, cus, 0, s);
-  launch_bin<BIN_G8, true>(a, cus, 0, s);
+  if (first <= BIN_GLOBAL && BIN_GLOBAL <= last) launch_bin<BIN_GLOBAL, true>(a, cus, 0, s);
+  if (first <= BIN_BLOCK && BIN_BLOCK <= last) launch_bin<BIN_BLOCK, true>(a, cus, 0, s);
+  if (first <= BIN_WAVE && BIN_WAVE <= last) launch_bin<BIN_WAVE, true>(a, cus, 0, s);
+  if (first <= BIN_G32 && BIN_G32 <= last) launch_bin<BIN_G32, true>(a, cus, 0, s);
+  if (first <= BIN_G16 && BIN_G16 <= last) launch_bin<BIN_G16, true>(a, cus, 0, s);
+  if (first <= BIN_G8 && BIN_G8 <= last) launch_bin<BIN_G8, true>(a, cus, 0, s);
   return hipGetLastError();
 }
 

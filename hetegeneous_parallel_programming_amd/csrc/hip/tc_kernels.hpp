@@ -202,8 +202,8 @@ hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int6
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
                         const uint8_t* own, uint64_t* out, int grid, hipStream_t s);
 // Count of one graph that also fills a.local (ecount and vcount zeroed by
-// the caller): same plan, local variants of the bin kernels.
-hipError_t launch_count_local(const LaunchArgs& a, int cus, hipStream_t s);
+// the caller): same plan, local variants of bins first..last.
+hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, hipStream_t s);
 // After launch_count_local: sums each counted edge's ecount into pe[owner]
 // by the reference push/pull rule over the split bounds[0..parts]
 // (1 <= parts <= kMaxParts; pe zeroed by the caller).

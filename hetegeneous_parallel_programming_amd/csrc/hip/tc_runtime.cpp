@@ -642,7 +642,17 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   TC_HIP_TRY(tck::launch_zero(L.pe, 2 * (int64_t)tck::kMaxParts, s));
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, s));
+  {  // bin groups on the library stream and the side streams, as in tc_count
+    constexpr int kFirst[tc_ctx::kSides + 1] = {tck::BIN_GLOBAL, tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G16};
+    constexpr int kLast[tc_ctx::kSides + 1] = {tck::BIN_BLOCK, tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G8};
+    TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, kFirst[0], kLast[0], s));
+    for (int i = 0; i < tc_ctx::kSides; ++i) {
+      TC_HIP_TRY(hipStreamWaitEvent(ctx->side[i], ctx->ev[1], 0));
+      TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, kFirst[i + 1], kLast[i + 1], ctx->side[i]));
+      TC_HIP_TRY(hipEventRecord(ctx->join[i], ctx->side[i]));
+    }
+    for (int i = 0; i < tc_ctx::kSides; ++i) TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[i], 0));
+  }
   if (k > 0)
     TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, g->row_begin, g->row_end, g->own, L.ecount, L.bounds, k, L.pe, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));

@@ -101,7 +101,7 @@ static tc_status capture(tc_ctx* ctx, tc_batch* b) {
 extern "C" {
 
 tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs, uint32_t check,
-                          tc_batch** out) {
+                          tc_batch** out) try {
   if (!ctx || !out || n_graphs < 0 || (n_graphs > 0 && !dags)) return fail(TC_EINVAL, "bad arguments");
   for (int32_t i = 0; i < n_graphs; ++i) {
     const tc_csr_view& d = dags[i];
@@ -242,9 +242,11 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
              b->partners.bytes + PlanScratch::bytes_for(b->P) + 8 * (int64_t)n_graphs;
   *out = b.release();
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_batch_count(tc_ctx* ctx, tc_batch* b, uint64_t* counts, tc_stats* stats) {
+tc_status tc_batch_count(tc_ctx* ctx, tc_batch* b, uint64_t* counts, tc_stats* stats) try {
   if (!ctx || !b || (!counts && b->n > 0)) return fail(TC_EINVAL, "null argument");
   if (b->ctx != ctx) return fail(TC_EINVAL, "batch belongs to another context");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -270,10 +272,12 @@ tc_status tc_batch_count(tc_ctx* ctx, tc_batch* b, uint64_t* counts, tc_stats* s
     for (int k = 0; k < TC_NUM_BINS; ++k) stats->bin_pivots[k] = c[tck::kCntBinCount + k];
   }
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double* ms_each,
-                              tc_stats* stats) {
+                              tc_stats* stats) try {
   if (!ctx || !b || (!counts && b->n > 0)) return fail(TC_EINVAL, "null argument");
   if (b->ctx != ctx) return fail(TC_EINVAL, "batch belongs to another context");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -305,9 +309,11 @@ tc_status tc_batch_count_each(tc_ctx* ctx, tc_batch* b, uint64_t* counts, double
     stats->pivots = (int64_t)b->RP - b->n;
   }
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b) {
+tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b) try {
   if (!b) return TC_OK;
   if (ctx && b->ctx != ctx) return fail(TC_EINVAL, "batch belongs to another context");
   std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
@@ -316,12 +322,14 @@ tc_status tc_batch_free(tc_ctx* ctx, tc_batch* b) {
   b->release();
   delete b;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 int64_t tc_batch_bytes(const tc_batch* b) { return b ? b->bytes : 0; }
 int32_t tc_batch_graphs(const tc_batch* b) { return b ? b->n : 0; }
 
-tc_status tc_batch_get_info(const tc_batch* b, tc_graph_info* out) {
+tc_status tc_batch_get_info(const tc_batch* b, tc_graph_info* out) try {
   if (!b || !out) return fail(TC_EINVAL, "null argument");
   out->graphs = b->n;
   out->rows = b->RP - b->n;
@@ -333,6 +341,8 @@ tc_status tc_batch_get_info(const tc_batch* b, tc_graph_info* out) {
   out->build_ms = b->build_ms;
   out->bytes = b->bytes;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 }  // extern "C"

@@ -17,6 +17,10 @@ namespace tcr {
 
 // Records the calling thread's error text and returns `st`.
 tc_status fail(tc_status st, const std::string& msg);
+// Called from a catch block: turns the exception in flight into a status, so
+// that no C++ exception crosses the C ABI (a failed host allocation is
+// TC_ENOMEM).
+tc_status caught() noexcept;
 
 #define TC_HIP_TRY(expr)                                                              \
   do {                                                                                \

@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 
 #include "tc_internal.hpp"
 
@@ -17,6 +19,23 @@ static thread_local std::string g_err;
 tc_status fail(tc_status st, const std::string& msg) {
   g_err = msg;
   return st;
+}
+
+tc_status caught() noexcept {
+  try {
+    try {
+      throw;
+    } catch (const std::bad_alloc&) {
+      return fail(TC_ENOMEM, "host allocation failed");
+    } catch (const std::length_error&) {
+      return fail(TC_ENOMEM, "host allocation too large");
+    } catch (const std::exception& e) {
+      return fail(TC_EINVAL, std::string("unexpected error: ") + e.what());
+    }
+  } catch (...) {
+    // the error text itself could not be stored
+  }
+  return TC_ENOMEM;
 }
 
 tc_status dev_alloc(void** p, size_t bytes, const char* what) {
@@ -316,7 +335,7 @@ extern "C" {
 
 const char* tc_last_error(const tc_ctx*) { return tcr::g_err.c_str(); }
 
-tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
+tc_status tc_ctx_create(int32_t device, tc_ctx** out) try {
   if (!out) return fail(TC_EINVAL, "null out");
   *out = nullptr;
   int ndev = 0;
@@ -346,6 +365,8 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) {
   c->tune = default_tune();
   *out = c.release();
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 void tc_ctx_destroy(tc_ctx* ctx) {
@@ -369,29 +390,35 @@ void tc_ctx_destroy(tc_ctx* ctx) {
   delete ctx;
 }
 
-tc_status tc_get_tune(tc_ctx* ctx, tc_tune* out) {
+tc_status tc_get_tune(tc_ctx* ctx, tc_tune* out) try {
   if (!ctx || !out) return fail(TC_EINVAL, "null argument");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   *out = ctx->tune;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_set_tune(tc_ctx* ctx, const tc_tune* t) {
+tc_status tc_set_tune(tc_ctx* ctx, const tc_tune* t) try {
   if (!ctx || !t) return fail(TC_EINVAL, "null argument");
   tc_status st = check_tune(*t);
   if (st != TC_OK) return st;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   ctx->tune = *t;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_device_name(tc_ctx* ctx, char* buf, int32_t len) {
+tc_status tc_device_name(tc_ctx* ctx, char* buf, int32_t len) try {
   if (!ctx || !buf || len <= 0) return fail(TC_EINVAL, "null argument");
   std::snprintf(buf, (size_t)len, "%s", ctx->name.c_str());
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_graph_upload(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, tc_graph** out) {
+tc_status tc_graph_upload(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, tc_graph** out) try {
   if (!ctx || !out) return fail(TC_EINVAL, "null argument");
   int32_t max_out = 0;
   tc_status st = check_dag(dag, check, &max_out);
@@ -399,10 +426,12 @@ tc_status tc_graph_upload(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, t
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
   return upload_csr(ctx, dag, 0, dag->n_rows, nullptr, max_out, out);
+} catch (...) {
+  return tcr::caught();
 }
 
 tc_status tc_graph_upload_rows(tc_ctx* ctx, const tc_csr_view* dag, const uint8_t* owned, uint32_t check,
-                               tc_graph** out) {
+                               tc_graph** out) try {
   if (!ctx || !out) return fail(TC_EINVAL, "null argument");
   int32_t max_out = 0;
   tc_status st = check_dag(dag, check, &max_out);
@@ -410,10 +439,12 @@ tc_status tc_graph_upload_rows(tc_ctx* ctx, const tc_csr_view* dag, const uint8_
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
   return upload_csr(ctx, dag, 0, dag->n_rows, owned, max_out, out);
+} catch (...) {
+  return tcr::caught();
 }
 
 tc_status tc_partition_upload(tc_ctx* ctx, const tch_partition_view* part, uint32_t check,
-                              tc_graph** out) {
+                              tc_graph** out) try {
   if (!ctx || !part || !out) return fail(TC_EINVAL, "null argument");
   tch_csr* full = nullptr;
   tc_status st = tch_partition_assemble(part, &full);
@@ -429,9 +460,11 @@ tc_status tc_partition_upload(tc_ctx* ctx, const tch_partition_view* part, uint3
   }
   tch_csr_free(full);
   return st;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) {
+tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) try {
   if (!g) return TC_OK;
   if (ctx && g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
   tc_ctx* c = g->ctx;
@@ -446,11 +479,13 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) {
   g->local.release();
   delete g;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 int64_t tc_graph_bytes(const tc_graph* g) { return g ? g->bytes + g->local.bytes : 0; }
 
-tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) {
+tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) try {
   if (!g || !out) return fail(TC_EINVAL, "null argument");
   out->graphs = 1;
   out->rows = g->n;
@@ -459,6 +494,8 @@ tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) {
   out->build_ms = g->build_ms;
   out->bytes = g->bytes + g->local.bytes;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 // Stream group of a bin in tc_count: 0 heavy (global-list, workgroup),
@@ -487,7 +524,7 @@ static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
   return a;
 }
 
-tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
   if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
   if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -553,9 +590,11 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
   return finish_stats(ctx, stats, t0, true, g->nv, 3);
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
   if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
   if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -575,6 +614,8 @@ tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* st
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
   return finish_stats(ctx, stats, t0, false, P);
+} catch (...) {
+  return tcr::caught();
 }
 
 // First local count of g: its buffers, and a partner rebuild that also
@@ -609,7 +650,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
   return TC_OK;
 }
 
-tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* out, tc_stats* stats) {
+tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* out, tc_stats* stats) try {
   if (!ctx || !g || !total || !out) return fail(TC_EINVAL, "null argument");
   if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
   const int32_t k = out->parts;
@@ -670,9 +711,11 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   if (k > 0)
     TC_HIP_TRY(hipMemcpy(out->part_counts, L.pe, sizeof(uint64_t) * (size_t)k, hipMemcpyDeviceToHost));
   return finish_stats(ctx, stats, t0, true, g->nv);
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter) {
+tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter) try {
   if (!ctx || !ms_per_iter || bytes <= 0 || bytes % 16 != 0 || iters < 1)
     return fail(TC_EINVAL, "need bytes > 0, a multiple of 16, and iters >= 1");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -699,6 +742,8 @@ tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_pe
   if (e != hipSuccess) return fail(TC_EDEVICE, std::string("copy probe: ") + hipGetErrorString(e));
   *ms_per_iter = ms / iters;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 }  // extern "C"

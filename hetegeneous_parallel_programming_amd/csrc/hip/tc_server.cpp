@@ -354,7 +354,7 @@ uint64_t tc_request_footprint(int32_t n_rows, int64_t nnz) {
   return Layout(n_rows, nnz).total;
 }
 
-tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out) {
+tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out) try {
   if (!ctx || !cfg || !out) return fail(TC_EINVAL, "null argument");
   if (cfg->slots < 1 || cfg->slots > 8) return fail(TC_EINVAL, "slots must be in [1, 8]");
   if (cfg->max_batch_graphs < 1 || cfg->max_batch_rows < 1 || cfg->max_queue < 1 || cfg->lookahead < 0)
@@ -419,9 +419,11 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
   raw->th = std::thread([raw]() { raw->run(); });
   *out = raw;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check) {
+tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check) try {
   if (!s || !dag) return fail(TC_EINVAL, "null argument");
   tc_status st = tch_validate(dag, check | TC_CHECK_SHAPE);
   if (st != TC_OK) {
@@ -476,10 +478,12 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
   ++s->submitted;
   s->cv_work.notify_one();
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 tc_status tc_server_submit_at(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check,
-                              double t_ms) {
+                              double t_ms) try {
   for (;;) {
     const double left = t_ms - now_ms();
     if (left <= 0) break;
@@ -487,9 +491,11 @@ tc_status tc_server_submit_at(tc_server* s, const tc_csr_view* dag, uint64_t tag
     else std::this_thread::yield();
   }
   return tc_server_submit(s, dag, tag, check);
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_server_poll(tc_server* s, tc_request_result* out, int32_t max, int32_t* n_out, int32_t timeout_ms) {
+tc_status tc_server_poll(tc_server* s, tc_request_result* out, int32_t max, int32_t* n_out, int32_t timeout_ms) try {
   if (!s || !n_out || (max > 0 && !out)) return fail(TC_EINVAL, "null argument");
   std::unique_lock<std::mutex> lk(s->mu);
   if (s->results.empty() && timeout_ms > 0)
@@ -501,9 +507,11 @@ tc_status tc_server_poll(tc_server* s, tc_request_result* out, int32_t max, int3
   }
   *n_out = k;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_server_drain(tc_server* s, int32_t timeout_ms) {
+tc_status tc_server_drain(tc_server* s, int32_t timeout_ms) try {
   if (!s) return fail(TC_EINVAL, "null argument");
   std::unique_lock<std::mutex> lk(s->mu);
   s->cv_work.notify_one();
@@ -511,9 +519,11 @@ tc_status tc_server_drain(tc_server* s, int32_t timeout_ms) {
                            [&] { return s->outstanding == 0; }))
     return fail(TC_ETIMEDOUT, "requests still unfinished after the drain timeout");
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_server_get_stats(tc_server* s, tc_server_stats* o) {
+tc_status tc_server_get_stats(tc_server* s, tc_server_stats* o) try {
   if (!s || !o) return fail(TC_EINVAL, "null argument");
   tch_arena_stats as;
   tch_arena_stats_get(s->arena, &as);
@@ -529,9 +539,11 @@ tc_status tc_server_get_stats(tc_server* s, tc_server_stats* o) {
   o->pool_used = as.used;
   o->pool_peak = as.peak_used;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_server_destroy(tc_server* s) {
+tc_status tc_server_destroy(tc_server* s) try {
   if (!s) return TC_OK;
   {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -543,6 +555,8 @@ tc_status tc_server_destroy(tc_server* s) {
   s->release();
   delete s;
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
 }  // extern "C"

@@ -76,7 +76,7 @@ std::string trim(const std::string& s) {
 extern "C" {
 
 tc_status tc_autotune(tc_ctx* ctx, tc_graph* g, tc_batch* b, int32_t reps, double budget_ms,
-                      tc_tune* best, tc_autotune_report* report) {
+                      tc_tune* best, tc_autotune_report* report) try {
   if (!ctx || (!g) == (!b)) return fail(TC_EINVAL, "autotune: give exactly one of a graph or a batch");
   if (reps < 1 || reps > 100) return fail(TC_EINVAL, "autotune: reps must be in [1, 100]");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -142,9 +142,11 @@ tc_status tc_autotune(tc_ctx* ctx, tc_graph* g, tc_batch* b, int32_t reps, doubl
     report->elapsed_ms = tcr::now_ms() - t_start;
   }
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_tune_save(tc_ctx* ctx, const char* path, const char* key) {
+tc_status tc_tune_save(tc_ctx* ctx, const char* path, const char* key) try {
   if (!ctx || !path || !key) return fail(TC_EINVAL, "null argument");
   if (std::strpbrk(key, " \t\r\n|")) return fail(TC_EINVAL, "tune key must not hold blanks or '|'");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -174,9 +176,11 @@ tc_status tc_tune_save(tc_ctx* ctx, const char* path, const char* key) {
   }
   if (std::rename(tmp.c_str(), path) != 0) return fail(TC_EIO, std::string("cannot replace ") + path);
   return TC_OK;
+} catch (...) {
+  return tcr::caught();
 }
 
-tc_status tc_tune_load(tc_ctx* ctx, const char* path, const char* key) {
+tc_status tc_tune_load(tc_ctx* ctx, const char* path, const char* key) try {
   if (!ctx || !path || !key) return fail(TC_EINVAL, "null argument");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   const std::string id = device_id(ctx) + "|" + key;
@@ -196,6 +200,8 @@ tc_status tc_tune_load(tc_ctx* ctx, const char* path, const char* key) {
     return TC_OK;
   }
   return fail(TC_EIO, std::string("no tune record for ") + id + " in " + path);
+} catch (...) {
+  return tcr::caught();
 }
 
 }  // extern "C"

@@ -18,7 +18,7 @@ using namespace tch;
 
 extern "C" {
 
-tc_status tch_arena_create(uint64_t capacity, uint64_t align, tch_arena** out) {
+tc_status tch_arena_create(uint64_t capacity, uint64_t align, tch_arena** out) try {
   if (!out) return fail(TC_EINVAL, "null out");
   if (align == 0 || (align & (align - 1))) return fail(TC_EINVAL, "align must be a power of two");
   std::unique_ptr<tch_arena> a(new tch_arena);
@@ -27,11 +27,13 @@ tc_status tch_arena_create(uint64_t capacity, uint64_t align, tch_arena** out) {
   if (a->capacity) a->free_[0] = a->capacity;
   *out = a.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 void tch_arena_destroy(tch_arena* a) { delete a; }
 
-tc_status tch_arena_alloc(tch_arena* a, uint64_t bytes, uint64_t* offset) {
+tc_status tch_arena_alloc(tch_arena* a, uint64_t bytes, uint64_t* offset) try {
   if (!a || !offset) return fail(TC_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(a->mu);
   const uint64_t need = bytes == 0 ? a->align : (bytes + a->align - 1) / a->align * a->align;
@@ -49,9 +51,11 @@ tc_status tch_arena_alloc(tch_arena* a, uint64_t bytes, uint64_t* offset) {
   }
   ++a->failed;
   return fail(TC_ENOMEM, "arena has no free extent of " + std::to_string(need) + " bytes");
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_arena_free(tch_arena* a, uint64_t offset) {
+tc_status tch_arena_free(tch_arena* a, uint64_t offset) try {
   if (!a) return fail(TC_EINVAL, "null arena");
   std::lock_guard<std::mutex> lk(a->mu);
   auto it = a->live_.find(offset);
@@ -73,9 +77,11 @@ tc_status tch_arena_free(tch_arena* a, uint64_t offset) {
   }
   a->free_[off] = size;
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_arena_stats_get(const tch_arena* a, tch_arena_stats* out) {
+tc_status tch_arena_stats_get(const tch_arena* a, tch_arena_stats* out) try {
   if (!a || !out) return fail(TC_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(a->mu);
   out->capacity = a->capacity;
@@ -89,6 +95,8 @@ tc_status tch_arena_stats_get(const tch_arena* a, tch_arena_stats* out) {
   out->allocs = a->allocs;
   out->failed_allocs = a->failed;
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 }  // extern "C"

@@ -9,7 +9,11 @@
 #include <atomic>
 #include <cstdio>
 #include <fstream>
+#include <new>
 #include <sstream>
+#include <stdexcept>
+
+#include <sys/stat.h>
 
 #include "host_common.hpp"
 
@@ -22,6 +26,45 @@ tc_status fail(tc_status st, const std::string& msg) {
   return st;
 }
 void clear_error() { g_last_error.clear(); }
+
+tc_status caught() noexcept {
+  try {
+    try {
+      throw;
+    } catch (const std::bad_alloc&) {
+      return fail(TC_ENOMEM, "host allocation failed");
+    } catch (const std::length_error&) {
+      return fail(TC_ENOMEM, "host allocation too large");
+    } catch (const std::exception& e) {
+      return fail(TC_EINVAL, std::string("unexpected error: ") + e.what());
+    }
+  } catch (...) {
+    // the error text itself could not be stored
+  }
+  return TC_ENOMEM;
+}
+
+int64_t file_bytes(const std::string& path) {
+  struct stat sb;
+  if (::stat(path.c_str(), &sb) != 0 || !S_ISREG(sb.st_mode)) return -1;
+  return (int64_t)sb.st_size;
+}
+
+bool read_exact(const std::string& path, void* dst, size_t bytes) {
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  size_t got = bytes ? std::fread(dst, 1, bytes, fp) : 0;
+  int extra = std::fgetc(fp);
+  std::fclose(fp);
+  return got == bytes && extra == EOF;
+}
+
+bool write_exact(const std::string& path, const void* src, size_t bytes) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) return false;
+  size_t put = bytes ? std::fwrite(src, 1, bytes, fp) : 0;
+  return std::fclose(fp) == 0 && put == bytes;
+}
 
 void Csr::compute_max_degree() {
   int32_t md = 0;
@@ -198,20 +241,24 @@ const char* tch_last_error(void) { return tch::g_last_error.c_str(); }
 
 void tch_csr_free(tch_csr* g) { delete g; }
 
-tc_status tch_csr_view(const tch_csr* g, tc_csr_view* out) {
+tc_status tch_csr_view(const tch_csr* g, tc_csr_view* out) try {
   if (!g || !out) return fail(TC_EINVAL, "null argument");
   *out = g->g.view();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 int32_t tch_csr_max_degree(const tch_csr* g) { return g ? g->g.max_degree : -1; }
 
-tc_status tch_validate(const tc_csr_view* g, uint32_t check) {
+tc_status tch_validate(const tc_csr_view* g, uint32_t check) try {
   if (!g) return fail(TC_EINVAL, "null view");
   return tch::validate(*g, check);
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_csr_copy(const tc_csr_view* src, uint32_t check, tch_csr** out) {
+tc_status tch_csr_copy(const tc_csr_view* src, uint32_t check, tch_csr** out) try {
   if (!src || !out) return fail(TC_EINVAL, "null argument");
   tc_status st = tch::validate(*src, check | TC_CHECK_SHAPE);
   if (st != TC_OK) return st;
@@ -222,19 +269,23 @@ tc_status tch_csr_copy(const tc_csr_view* src, uint32_t check, tch_csr** out) {
   h->g.compute_max_degree();
   *out = h.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 tc_status tch_csr_from_pairs(int32_t n, const int32_t* src, const int32_t* dst, int64_t m,
-                             tch_csr** out) {
+                             tch_csr** out) try {
   if (!out) return fail(TC_EINVAL, "null out");
   std::unique_ptr<tch_csr> h(new tch_csr);
   tc_status st = build_symmetric(n, src, dst, m, omp_get_max_threads(), h->g);
   if (st != TC_OK) return st;
   *out = h.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_orient(const tc_csr_view* sym, tch_csr** dag_out) {
+tc_status tch_orient(const tc_csr_view* sym, tch_csr** dag_out) try {
   if (!sym || !dag_out) return fail(TC_EINVAL, "null argument");
   tc_status st = tch::validate(*sym, TC_CHECK_SHAPE);
   if (st != TC_OK) return st;
@@ -243,18 +294,22 @@ tc_status tch_orient(const tc_csr_view* sym, tch_csr** dag_out) {
   if (st != TC_OK) return st;
   *dag_out = h.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_sort_neighbors(tch_csr* g) {
+tc_status tch_sort_neighbors(tch_csr* g) try {
   if (!g) return fail(TC_EINVAL, "null graph");
   Csr& c = g->g;
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t v = 0; v < c.n; ++v)
     std::sort(c.colidx.get() + c.rowptr[v], c.colidx.get() + c.rowptr[v + 1]);
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_read_graphminer(const char* prefix, int use_dag, tch_csr** out) {
+tc_status tch_read_graphminer(const char* prefix, int use_dag, tch_csr** out) try {
   if (!prefix || !out) return fail(TC_EINVAL, "null argument");
   std::string p(prefix);
   std::ifstream meta(p + ".meta.txt");
@@ -270,19 +325,17 @@ tc_status tch_read_graphminer(const char* prefix, int use_dag, tch_csr** out) {
   if (n <= 0 || n > INT32_MAX || m < 0) return fail(TC_ERANGE, "vertex or edge count out of range");
   if (!(max_degree > 0 && max_degree < n))
     return fail(TC_EINVAL, "meta max_degree must satisfy 0 < max_degree < n");
+  // the files must have the sizes the meta data gives before anything is
+  // allocated from those counts
+  if (file_bytes(p + ".vertex.bin") != 8 * (n + 1))
+    return fail(TC_EIO, "cannot read " + p + ".vertex.bin (size must be 8*(n+1))");
+  if (m > INT64_MAX / 4 || file_bytes(p + ".edge.bin") != 4 * m)
+    return fail(TC_EIO, "cannot read " + p + ".edge.bin (size must be 4*m)");
   std::unique_ptr<tch_csr> h(new tch_csr);
   h->g.allocate((int32_t)n, m);
-  auto read_all = [](const std::string& f, void* dst, size_t bytes) -> bool {
-    FILE* fp = std::fopen(f.c_str(), "rb");
-    if (!fp) return false;
-    size_t got = bytes ? std::fread(dst, 1, bytes, fp) : 0;
-    int extra = std::fgetc(fp);
-    std::fclose(fp);
-    return got == bytes && extra == EOF;
-  };
-  if (!read_all(p + ".vertex.bin", h->g.rowptr.get(), sizeof(int64_t) * ((size_t)n + 1)))
+  if (!read_exact(p + ".vertex.bin", h->g.rowptr.get(), sizeof(int64_t) * ((size_t)n + 1)))
     return fail(TC_EIO, "cannot read " + p + ".vertex.bin (size must be 8*(n+1))");
-  if (!read_all(p + ".edge.bin", h->g.colidx.get(), sizeof(int32_t) * (size_t)m))
+  if (!read_exact(p + ".edge.bin", h->g.colidx.get(), sizeof(int32_t) * (size_t)m))
     return fail(TC_EIO, "cannot read " + p + ".edge.bin (size must be 4*m)");
   tc_status st = tch::validate(h->g.view(), TC_CHECK_SHAPE);
   if (st != TC_OK) return st;
@@ -295,9 +348,11 @@ tc_status tch_read_graphminer(const char* prefix, int use_dag, tch_csr** out) {
   }
   *out = h.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_write_graphminer(const char* prefix, const tc_csr_view* g) {
+tc_status tch_write_graphminer(const char* prefix, const tc_csr_view* g) try {
   if (!prefix || !g) return fail(TC_EINVAL, "null argument");
   tc_status st = tch::validate(*g, TC_CHECK_SHAPE);
   if (st != TC_OK) return st;
@@ -309,17 +364,14 @@ tc_status tch_write_graphminer(const char* prefix, const tc_csr_view* g) {
   // n m vid_size eid_size vlabel_size elabel_size max_degree feat_len nvc nec
   meta << g->n_rows << "\n" << g->nnz << "\n4 8 1 2\n" << md << "\n0\n0\n0\n";
   meta.close();
-  auto write_all = [](const std::string& f, const void* src, size_t bytes) -> bool {
-    FILE* fp = std::fopen(f.c_str(), "wb");
-    if (!fp) return false;
-    size_t put = bytes ? std::fwrite(src, 1, bytes, fp) : 0;
-    return std::fclose(fp) == 0 && put == bytes;
-  };
-  if (!write_all(p + ".vertex.bin", g->rowptr, sizeof(int64_t) * ((size_t)g->n_rows + 1)))
+  if (!meta) return fail(TC_EIO, "cannot write " + p + ".meta.txt");
+  if (!write_exact(p + ".vertex.bin", g->rowptr, sizeof(int64_t) * ((size_t)g->n_rows + 1)))
     return fail(TC_EIO, "cannot write " + p + ".vertex.bin");
-  if (!write_all(p + ".edge.bin", g->colidx, sizeof(int32_t) * (size_t)g->nnz))
+  if (!write_exact(p + ".edge.bin", g->colidx, sizeof(int32_t) * (size_t)g->nnz))
     return fail(TC_EIO, "cannot write " + p + ".edge.bin");
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 }  // extern "C"

@@ -118,40 +118,48 @@ using namespace tch;
 
 extern "C" {
 
-tc_status tch_gen_er(int32_t n, double avg_degree, uint64_t seed, tch_csr** out) {
+tc_status tch_gen_er(int32_t n, double avg_degree, uint64_t seed, tch_csr** out) try {
   if (!out) return fail(TC_EINVAL, "null out");
   std::unique_ptr<tch_csr> h(new tch_csr);
   tc_status st = gen_er(n, avg_degree, seed, omp_get_max_threads(), h->g);
   if (st != TC_OK) return st;
   *out = h.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 tc_status tch_gen_rmat(int32_t n, double edge_factor, double a, double b, double c, uint64_t seed,
-                       tch_csr** out) {
+                       tch_csr** out) try {
   if (!out) return fail(TC_EINVAL, "null out");
   std::unique_ptr<tch_csr> h(new tch_csr);
   tc_status st = gen_rmat(n, edge_factor, a, b, c, seed, omp_get_max_threads(), h->g);
   if (st != TC_OK) return st;
   *out = h.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_batch_spec_of(uint64_t batch_seed, int32_t index, tch_batch_spec* out) {
+tc_status tch_batch_spec_of(uint64_t batch_seed, int32_t index, tch_batch_spec* out) try {
   if (!out) return fail(TC_EINVAL, "null out");
   return batch_spec(batch_seed, index, *out);
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_batch_graph(uint64_t batch_seed, int32_t index, tch_csr** dag_out) {
+tc_status tch_batch_graph(uint64_t batch_seed, int32_t index, tch_csr** dag_out) try {
   if (!dag_out) return fail(TC_EINVAL, "null out");
   std::unique_ptr<tch_csr> h(new tch_csr);
   tc_status st = batch_graph(batch_seed, index, omp_get_max_threads(), h->g);
   if (st != TC_OK) return st;
   *dag_out = h.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_batch_graphs(uint64_t batch_seed, int32_t first, int32_t count, tch_csr** dags_out) {
+tc_status tch_batch_graphs(uint64_t batch_seed, int32_t first, int32_t count, tch_csr** dags_out) try {
   if (!dags_out || first < 0 || count < 0) return fail(TC_EINVAL, "bad batch range");
   std::vector<tc_status> st(count, TC_OK);
   std::vector<std::string> err(count);
@@ -179,6 +187,8 @@ tc_status tch_batch_graphs(uint64_t batch_seed, int32_t first, int32_t count, tc
     }
   }
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 }  // extern "C"

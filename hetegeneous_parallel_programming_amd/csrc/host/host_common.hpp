@@ -37,6 +37,17 @@ struct Csr {
 tc_status fail(tc_status st, const std::string& msg);
 void clear_error();
 
+// Called from a catch block: turns the exception in flight into a status, so
+// that no C++ exception crosses the C ABI (a failed host allocation, e.g. for
+// sizes read from a file, is TC_ENOMEM).
+tc_status caught() noexcept;
+
+// Size of a file in bytes, or -1 when it cannot be examined.
+int64_t file_bytes(const std::string& path);
+// Reads exactly `bytes` bytes (the whole file) into dst / writes them out.
+bool read_exact(const std::string& path, void* dst, size_t bytes);
+bool write_exact(const std::string& path, const void* src, size_t bytes);
+
 // splitmix64 finaliser: a bijective 64-bit mixer.
 static inline uint64_t mix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;

@@ -18,7 +18,6 @@
 // any part array (a range split, a balanced split, an external METIS run)
 // produces the same file set.
 #include <algorithm>
-#include <cstdio>
 #include <fstream>
 
 #include "host_common.hpp"
@@ -35,22 +34,6 @@ namespace {
 
 using tch::fail;
 
-bool write_all(const std::string& f, const void* src, size_t bytes) {
-  FILE* fp = std::fopen(f.c_str(), "wb");
-  if (!fp) return false;
-  size_t put = bytes ? std::fwrite(src, 1, bytes, fp) : 0;
-  return std::fclose(fp) == 0 && put == bytes;
-}
-
-bool read_all(const std::string& f, void* dst, size_t bytes) {
-  FILE* fp = std::fopen(f.c_str(), "rb");
-  if (!fp) return false;
-  size_t got = bytes ? std::fread(dst, 1, bytes, fp) : 0;
-  int extra = std::fgetc(fp);
-  std::fclose(fp);
-  return got == bytes && extra == EOF;
-}
-
 std::string file_of(const char* prefix, int32_t i, const char* ext) {
   return std::string(prefix) + std::to_string(i) + ext;
 }
@@ -59,7 +42,7 @@ std::string file_of(const char* prefix, int32_t i, const char* ext) {
 
 extern "C" {
 
-tc_status tch_metis_write(const char* prefix, const tc_csr_view* g, const int64_t* part, int32_t k) {
+tc_status tch_metis_write(const char* prefix, const tc_csr_view* g, const int64_t* part, int32_t k) try {
   if (!prefix || !g || !part) return fail(TC_EINVAL, "null argument");
   if (k < 1) return fail(TC_EINVAL, "need k >= 1");
   tc_status st = tch_validate(g, TC_CHECK_SHAPE);
@@ -96,16 +79,18 @@ tc_status tch_metis_write(const char* prefix, const tc_csr_view* g, const int64_
          << md << "\n";
     meta.close();
     if (!meta) return fail(TC_EIO, "cannot write " + file_of(prefix, i, ".meta.txt"));
-    if (!write_all(file_of(prefix, i, ".vertex.bin"), rp.get(), sizeof(int64_t) * ((size_t)n + 1)) ||
-        !write_all(file_of(prefix, i, ".edge.bin"), ci.data(), sizeof(int32_t) * ci.size()) ||
-        !write_all(file_of(prefix, i, ".vertex_list.bin"), owned.data(), sizeof(int32_t) * owned.size()) ||
-        !write_all(file_of(prefix, i, ".part.bin"), part, sizeof(int64_t) * (size_t)n))
+    if (!tch::write_exact(file_of(prefix, i, ".vertex.bin"), rp.get(), sizeof(int64_t) * ((size_t)n + 1)) ||
+        !tch::write_exact(file_of(prefix, i, ".edge.bin"), ci.data(), sizeof(int32_t) * ci.size()) ||
+        !tch::write_exact(file_of(prefix, i, ".vertex_list.bin"), owned.data(), sizeof(int32_t) * owned.size()) ||
+        !tch::write_exact(file_of(prefix, i, ".part.bin"), part, sizeof(int64_t) * (size_t)n))
       return fail(TC_EIO, "cannot write the binary files of " + file_of(prefix, i, ""));
   }
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_metis_read(const char* prefix, int32_t i, tch_metis** out) {
+tc_status tch_metis_read(const char* prefix, int32_t i, tch_metis** out) try {
   if (!prefix || !out) return fail(TC_EINVAL, "null argument");
   if (i < 0) return fail(TC_EINVAL, "PE index must be >= 0");
   std::unique_ptr<tch_metis> m(new tch_metis);
@@ -116,20 +101,33 @@ tc_status tch_metis_read(const char* prefix, int32_t i, tch_metis** out) {
     if (!meta) return fail(TC_EINVAL, "malformed meta file " + file_of(prefix, i, ".meta.txt"));
   }
   const int64_t n = m->n;
-  if (n <= 0 || n > INT32_MAX || m->n_real < 0 || m->n_real > n || m->nnz < 0)
+  if (n <= 0 || n > INT32_MAX || m->n_real < 0 || m->n_real > n || m->nnz < 0 || m->nnz > INT64_MAX / 4)
     return fail(TC_ERANGE, "meta counts out of range in " + file_of(prefix, i, ".meta.txt"));
+  // the files must have the sizes the meta data gives before anything is
+  // allocated from those counts
+  const struct {
+    const char* ext;
+    int64_t bytes;
+    const char* rule;
+  } expect[] = {{".vertex.bin", 8 * (n + 1), "8*(n+1)"},
+                {".edge.bin", 4 * m->nnz, "4*E"},
+                {".vertex_list.bin", 4 * m->n_real, "4*n_real"},
+                {".part.bin", 8 * n, "8*n"}};
+  for (const auto& f : expect)
+    if (tch::file_bytes(file_of(prefix, i, f.ext)) != f.bytes)
+      return fail(TC_EIO, "cannot read " + file_of(prefix, i, f.ext) + " (size must be " + f.rule + ")");
   m->rowptr.reset(new int64_t[(size_t)n + 1]);
   m->colidx.reset(new int32_t[m->nnz > 0 ? (size_t)m->nnz : 1]);
   m->vertex_list.reset(new int32_t[m->n_real > 0 ? (size_t)m->n_real : 1]);
   m->part.reset(new int64_t[(size_t)n]);
-  if (!read_all(file_of(prefix, i, ".vertex.bin"), m->rowptr.get(), sizeof(int64_t) * ((size_t)n + 1)))
+  if (!tch::read_exact(file_of(prefix, i, ".vertex.bin"), m->rowptr.get(), sizeof(int64_t) * ((size_t)n + 1)))
     return fail(TC_EIO, "cannot read " + file_of(prefix, i, ".vertex.bin") + " (size must be 8*(n+1))");
-  if (!read_all(file_of(prefix, i, ".edge.bin"), m->colidx.get(), sizeof(int32_t) * (size_t)m->nnz))
+  if (!tch::read_exact(file_of(prefix, i, ".edge.bin"), m->colidx.get(), sizeof(int32_t) * (size_t)m->nnz))
     return fail(TC_EIO, "cannot read " + file_of(prefix, i, ".edge.bin") + " (size must be 4*E)");
-  if (!read_all(file_of(prefix, i, ".vertex_list.bin"), m->vertex_list.get(),
+  if (!tch::read_exact(file_of(prefix, i, ".vertex_list.bin"), m->vertex_list.get(),
                 sizeof(int32_t) * (size_t)m->n_real))
     return fail(TC_EIO, "cannot read " + file_of(prefix, i, ".vertex_list.bin") + " (size must be 4*n_real)");
-  if (!read_all(file_of(prefix, i, ".part.bin"), m->part.get(), sizeof(int64_t) * (size_t)n))
+  if (!tch::read_exact(file_of(prefix, i, ".part.bin"), m->part.get(), sizeof(int64_t) * (size_t)n))
     return fail(TC_EIO, "cannot read " + file_of(prefix, i, ".part.bin") + " (size must be 8*n)");
   const tc_csr_view v{m->rowptr.get(), m->colidx.get(), (int32_t)n, m->nnz};
   tc_status st = tch_validate(&v, TC_CHECK_SHAPE);
@@ -145,9 +143,11 @@ tc_status tch_metis_read(const char* prefix, int32_t i, tch_metis** out) {
   if (owned_edges != m->nnz) return fail(TC_EINVAL, "rows outside vertex_list must be empty");
   *out = m.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_metis_get(const tch_metis* m, tch_metis_view* o) {
+tc_status tch_metis_get(const tch_metis* m, tch_metis_view* o) try {
   if (!m || !o) return fail(TC_EINVAL, "null argument");
   o->n = (int32_t)m->n;
   o->n_real = (int32_t)m->n_real;
@@ -160,12 +160,14 @@ tc_status tch_metis_get(const tch_metis* m, tch_metis_view* o) {
   o->vertex_list = m->vertex_list.get();
   o->part = m->part.get();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 void tch_metis_free(tch_metis* m) { delete m; }
 
 tc_status tch_metis_assemble(const tch_metis* const* pes, int32_t k, int32_t i, tch_csr** out,
-                             uint8_t* owned) {
+                             uint8_t* owned) try {
   if (!pes || !out) return fail(TC_EINVAL, "null argument");
   if (k < 1 || i < -1 || i >= k) return fail(TC_EINVAL, "need k >= 1 and -1 <= i < k");
   for (int32_t p = 0; p < k; ++p)
@@ -210,6 +212,8 @@ tc_status tch_metis_assemble(const tch_metis* const* pes, int32_t k, int32_t i, 
     for (int64_t v = 0; v < n; ++v) owned[v] = (i < 0 || part[v] == i) ? 1 : 0;
   *out = c.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 }  // extern "C"

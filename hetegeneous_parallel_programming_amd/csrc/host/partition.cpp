@@ -51,15 +51,17 @@ using namespace tch;
 
 extern "C" {
 
-tc_status tch_partition_range(const tc_csr_view* dag, int32_t k, int32_t part, tch_partition** out) {
+tc_status tch_partition_range(const tc_csr_view* dag, int32_t k, int32_t part, tch_partition** out) try {
   if (!dag || !out) return fail(TC_EINVAL, "null argument");
   if (k < 1 || part < 0 || part >= k) return fail(TC_EINVAL, "need k >= 1 and 0 <= part < k");
   int32_t b, e;
   range_of(dag->n_rows, k, part, b, e);
   return tch_partition_span(dag, b, e, out);
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_partition_bounds(const tc_csr_view* dag, int32_t k, int32_t balance, int32_t* bounds) {
+tc_status tch_partition_bounds(const tc_csr_view* dag, int32_t k, int32_t balance, int32_t* bounds) try {
   if (!dag || !bounds) return fail(TC_EINVAL, "null argument");
   if (k < 1) return fail(TC_EINVAL, "need k >= 1");
   if (balance < TCH_BALANCE_VERTICES || balance > TCH_BALANCE_PARTNER)
@@ -99,10 +101,12 @@ tc_status tch_partition_bounds(const tc_csr_view* dag, int32_t k, int32_t balanc
     bounds[i] = std::max(bounds[i - 1], (int32_t)r);
   }
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 tc_status tch_partition_span(const tc_csr_view* dag, int32_t row_begin, int32_t row_end,
-                             tch_partition** out) {
+                             tch_partition** out) try {
   if (!dag || !out) return fail(TC_EINVAL, "null argument");
   const int32_t n = dag->n_rows;
   if (row_begin < 0 || row_end > n || row_begin > row_end)
@@ -149,9 +153,11 @@ tc_status tch_partition_span(const tc_csr_view* dag, int32_t row_begin, int32_t 
   p->halo.compute_max_degree();
   *out = p.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_partition_get(const tch_partition* p, tch_partition_view* o) {
+tc_status tch_partition_get(const tch_partition* p, tch_partition_view* o) try {
   if (!p || !o) return fail(TC_EINVAL, "null argument");
   o->n_global = p->n_global;
   o->row_begin = p->row_begin;
@@ -165,11 +171,13 @@ tc_status tch_partition_get(const tch_partition* p, tch_partition_view* o) {
   o->halo_colidx = p->halo.colidx.get();
   o->halo_nnz = p->halo.nnz;
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 void tch_partition_free(tch_partition* p) { delete p; }
 
-tc_status tch_partition_assemble(const tch_partition_view* p, tch_csr** out) {
+tc_status tch_partition_assemble(const tch_partition_view* p, tch_csr** out) try {
   if (!p || !out) return fail(TC_EINVAL, "null argument");
   const int32_t n = p->n_global, b = p->row_begin, e = p->row_end;
   if (n < 0 || b < 0 || e > n || b > e || p->n_halo < 0)
@@ -204,12 +212,14 @@ tc_status tch_partition_assemble(const tch_partition_view* p, tch_csr** out) {
   g->g.compute_max_degree();
   *out = g.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 void tch_free_ids(int32_t* p) { delete[] p; }
 
 tc_status tch_partition_induced(const tc_csr_view* g, int32_t k, int32_t part, tch_csr** sub_out,
-                                int32_t* local_begin, int32_t* local_end, int32_t** idx_map_out) {
+                                int32_t* local_begin, int32_t* local_end, int32_t** idx_map_out) try {
   if (!g || !sub_out || !local_begin || !local_end || !idx_map_out)
     return fail(TC_EINVAL, "null argument");
   if (k < 1 || part < 0 || part >= k) return fail(TC_EINVAL, "need k >= 1 and 0 <= part < k");
@@ -260,6 +270,8 @@ tc_status tch_partition_induced(const tc_csr_view* g, int32_t k, int32_t part, t
   *idx_map_out = map;
   *sub_out = sub.release();
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 }  // extern "C"
@@ -305,7 +317,7 @@ static void partner_units(int64_t nl, const int64_t* rowptr, const int32_t* coli
 
 extern "C" {
 
-tc_status tch_work_of(const tc_csr_view* dag, int32_t row_begin, int32_t row_end, tch_work* out) {
+tc_status tch_work_of(const tc_csr_view* dag, int32_t row_begin, int32_t row_end, tch_work* out) try {
   if (!dag || !out) return fail(TC_EINVAL, "null argument");
   if (row_begin < 0 || row_end > dag->n_rows || row_begin > row_end)
     return fail(TC_EINVAL, "row range outside [0, n]");
@@ -344,9 +356,11 @@ tc_status tch_work_of(const tc_csr_view* dag, int32_t row_begin, int32_t row_end
   out->partner_rows = units;
   out->partner_bytes = ub + setb + 8 * m + 4 * mins;
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
-tc_status tch_work_of_partition(const tch_partition_view* p, tch_work* out) {
+tc_status tch_work_of_partition(const tch_partition_view* p, tch_work* out) try {
   if (!p || !out) return fail(TC_EINVAL, "null argument");
   const int32_t b = p->row_begin, nl = p->row_end - p->row_begin;
   int64_t m = 0, wedge = 0, mins = 0;
@@ -400,6 +414,8 @@ tc_status tch_work_of_partition(const tch_partition_view* p, tch_work* out) {
   out->partner_rows = units;
   out->partner_bytes = ub + setb + 8 * m + 4 * mins;
   return TC_OK;
+} catch (...) {
+  return tch::caught();
 }
 
 }  // extern "C"

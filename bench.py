@@ -11,7 +11,8 @@ the DAG already resident in HBM.  Beside it:
   the shorter list 4 min(d+(u), d+(v)) -- divided by their average duration
   from HIP events on the library's stream, against 8 TB/s; the reference
   algorithm's edge-iterator bytes 8(R+1) + 20 m_R + 4 sum d+(v) over the same
-  time are reported beside it;
+  time are reported beside it, and so is the device's practical read ceiling
+  (a streaming read of 4 GiB, measured in the same run);
 * cpu_baseline: the CPU oracle (OpenMP merge count, the reference's CPU
   algorithm) on a bounded random sample of pivot rows of the same graph;
 * extra legs: the plain one-wave-per-pivot kernel, the 4096-graph batched
@@ -63,6 +64,17 @@ def cores_available() -> int:
     return max(1, n)
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def time_steps(fn, steps: int, warmup: int):
     for _ in range(warmup):
         fn()
@@ -94,8 +106,9 @@ def cpu_baseline(g, target_s: float, seed: int = 7) -> dict:
     oracle.tc_count_rows(rp, ci, rows, threads=threads)
     t = time.perf_counter() - t0
     return {"value": edges / t, "unit": "edges/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{rows.size} random pivot rows ({frac:.4%} of {n}, {edges} DAG edges) of the "
-                      f"same RMAT-22 DAG, OpenMP merge count, {t:.2f} s"}
+                      f"same RMAT-22 DAG, OpenMP merge count (dynamic schedule), {t:.2f} s"}
 
 
 def leg_er(ctx, seed, steps, warmup):
@@ -457,7 +470,7 @@ def main() -> int:
     ap.add_argument("--tune-seconds", type=float, default=12.0, help="autotuner wall-time budget")
     ap.add_argument("--tune-file", default="", help="load the batch setting from / save it to this file")
     ap.add_argument("--copy-probe", type=int, default=0,
-                    help="only run a plain device copy of this many bytes (counter calibration)")
+                    help="only run a plain device copy (counter calibration) and a plain read of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this triangle-counting build has no multi-GPU "
@@ -470,9 +483,12 @@ def main() -> int:
     if args.copy_probe:
         ctx = Context(0)
         ms = ctx.probe_copy(args.copy_probe, max(1, args.steps))
+        ms_read = ctx.probe_read(args.copy_probe, max(1, args.steps))
         ctx.close()
         print(json.dumps({"copy_bytes": args.copy_probe, "iters": max(1, args.steps), "ms": ms,
-                          "GBps_read_plus_write": 2 * args.copy_probe / (ms / 1e3) / 1e9}), flush=True)
+                          "GBps_read_plus_write": 2 * args.copy_probe / (ms / 1e3) / 1e9,
+                          "read_ms": ms_read,
+                          "GBps_read_only": args.copy_probe / (ms_read / 1e3) / 1e9}), flush=True)
         return 0
     ctx = Context(0)
     log(f"device: {ctx.name}")
@@ -515,6 +531,12 @@ def leg_headline(ctx, args) -> dict:
                               plain=not args.no_plain, local=not args.no_local)
     best = single["binned"]
     traffic, traffic_src = traffic_of(f"rmat{args.scale}")
+    # the practical read ceiling of this device, measured in the same run: a
+    # streaming read of 4 GiB (well beyond the 256 MB Infinity Cache)
+    read_bytes = 1 << 32
+    read_GBps = read_bytes / (ctx.probe_read(read_bytes, 10) / 1e3) / 1e9
+    log(f"read probe: 4 GiB streaming read {read_GBps:.0f} GB/s; count at "
+        f"{best['alg_GBps'] / read_GBps:.2f} of it")
     line = {
         "metric": "TC edges/s, %HBM peak; batched graphs/s + p99 vs one-launch-per-graph",
         "value": best["edges_per_s"], "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -528,13 +550,15 @@ def leg_headline(ctx, args) -> dict:
         "roofline": {"bound": "hbm", "achieved": best["alg_GBps"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": best["alg_GBps"] / PEAK_HBM_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "count_kernel<bin> (one persistent launch per non-empty bin per step; light bins on a second stream)",
+                     "kernel": "count_kernel<bin> (one persistent launch per non-empty bin per step; four bin groups on four streams)",
                      "alg_bytes": work["partner_bytes"],
                      "alg_bytes_note": "bytes the partner-scheme count reads with no reuse: per partner row "
                                        "32 + 4 d+(x), per edge 8 + 4 min(d+(u), d+(v))",
                      "alg_bytes_edge_iterator": work["alg_bytes"],
                      "edge_iterator_GBps": best["edge_iterator_GBps"],
-                     "kernel_ms_avg": best["kernel_ms_avg"]},
+                     "kernel_ms_avg": best["kernel_ms_avg"],
+                     "read_probe_GBps": read_GBps,
+                     "frac_of_read_probe": best["alg_GBps"] / read_GBps},
         "cpu_baseline": None,
         "extra": {"plain_kernel": single.get("plain"), "binned": best, "upload": single["upload"],
                   "local_counts": single.get("local")},

@@ -1191,6 +1191,33 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const uint4* __restrict__ 
     dst[i] = src[i];
 }
 
+// plain streaming read: the practical HBM read rate, the ceiling of a kernel
+// that only reads.  Eight 16-byte nontemporal loads in flight per lane (a
+// sweep of loads in flight, grid size and the hint on MI355X found this the
+// fastest form).  The xor of the words is stored only if it equals a value
+// the probe's byte fill cannot produce, which keeps every load and writes
+// nothing.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kReadUnroll = 8;
+__global__ __launch_bounds__(kBlock) void read_kernel(const u32x4* __restrict__ src, int64_t n,
+                                                      uint32_t* __restrict__ sink) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint32_t acc = 0;
+  for (; i + (kReadUnroll - 1) * stride < n; i += kReadUnroll * stride) {
+    u32x4 x[kReadUnroll];
+#pragma unroll
+    for (int k = 0; k < kReadUnroll; ++k) x[k] = __builtin_nontemporal_load(src + i + k * stride);
+#pragma unroll
+    for (int k = 0; k < kReadUnroll; ++k) acc ^= x[k].x ^ x[k].y ^ x[k].z ^ x[k].w;
+  }
+  for (; i < n; i += stride) {
+    const u32x4 x = __builtin_nontemporal_load(src + i);
+    acc ^= x.x ^ x.y ^ x.z ^ x.w;
+  }
+  if (acc == 0x9E3779B9u) sink[(int64_t)blockIdx.x * kBlock + threadIdx.x] = acc;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1298,6 +1325,11 @@ hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin,
 
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s) {
   copy_kernel<<<grid, kBlock, 0, s>>>((const uint4*)src, (uint4*)dst, bytes / 16);
+  return hipGetLastError();
+}
+
+hipError_t launch_read(const void* src, int64_t bytes, uint32_t* sink, int grid, hipStream_t s) {
+  read_kernel<<<grid, kBlock, 0, s>>>((const u32x4*)src, bytes / 16, sink);
   return hipGetLastError();
 }
 

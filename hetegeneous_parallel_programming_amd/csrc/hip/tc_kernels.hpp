@@ -212,6 +212,8 @@ hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_
                               const uint8_t* own, const uint32_t* ecount, const int32_t* bounds, int32_t parts,
                               unsigned long long* pe, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
+// sink: grid * kBlock words (never written for a byte-filled source).
+hipError_t launch_read(const void* src, int64_t bytes, uint32_t* sink, int grid, hipStream_t s);
 // Resident workgroups per CU of the persistent and plain kernels.
 int count_blocks_per_cu();
 int plain_blocks_per_cu();

@@ -746,4 +746,36 @@ tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_pe
   return tcr::caught();
 }
 
+tc_status tc_probe_read(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter) try {
+  if (!ctx || !ms_per_iter || bytes <= 0 || bytes % 16 != 0 || iters < 1)
+    return fail(TC_EINVAL, "need bytes > 0, a multiple of 16, and iters >= 1");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  const int grid = ctx->cus * 32;
+  void *src = nullptr, *sink = nullptr;
+  tc_status st = dev_alloc(&src, (size_t)bytes, "read src");
+  if (st == TC_OK) st = dev_alloc(&sink, sizeof(uint32_t) * (size_t)grid * tck::kBlock, "read sink");
+  if (st != TC_OK) {
+    dev_free(src);
+    return st;
+  }
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemsetAsync(src, 1, (size_t)bytes, s);
+  uint32_t* k = (uint32_t*)sink;
+  if (e == hipSuccess) e = tck::launch_read(src, bytes, k, grid, s);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], s);
+  for (int i = 0; i < iters && e == hipSuccess; ++i) e = tck::launch_read(src, bytes, k, grid, s);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[2], s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]);
+  dev_free(src);
+  dev_free(sink);
+  if (e != hipSuccess) return fail(TC_EDEVICE, std::string("read probe: ") + hipGetErrorString(e));
+  *ms_per_iter = ms / iters;
+  return TC_OK;
+} catch (...) {
+  return tcr::caught();
+}
+
 }  // extern "C"

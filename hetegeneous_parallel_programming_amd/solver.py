@@ -132,6 +132,7 @@ def device_lib():
     S(L, "tc_tune_save", nat.c_i32, [P, ctypes.c_char_p, ctypes.c_char_p])
     S(L, "tc_tune_load", nat.c_i32, [P, ctypes.c_char_p, ctypes.c_char_p])
     S(L, "tc_probe_copy", nat.c_i32, [P, nat.c_i64, nat.c_i32, ctypes.POINTER(nat.c_dbl)])
+    S(L, "tc_probe_read", nat.c_i32, [P, nat.c_i64, nat.c_i32, ctypes.POINTER(nat.c_dbl)])
     S(L, "tc_clock_ms", nat.c_dbl, [])
     S(L, "tc_request_footprint", nat.c_u64, [nat.c_i32, nat.c_i64])
     S(L, "tc_server_create", nat.c_i32, [P, ctypes.POINTER(ServerConfig), PP])
@@ -368,6 +369,12 @@ class Context:
         """Mean ms of a plain device copy of nbytes (counter calibration)."""
         ms = nat.c_dbl(0)
         _check(device_lib().tc_probe_copy(self._h, nbytes, iters, ctypes.byref(ms)))
+        return ms.value
+
+    def probe_read(self, nbytes: int, iters: int = 10) -> float:
+        """Mean ms of a plain streaming read of nbytes (practical read rate)."""
+        ms = nat.c_dbl(0)
+        _check(device_lib().tc_probe_read(self._h, nbytes, iters, ctypes.byref(ms)))
         return ms.value
 
     def close(self) -> None:

@@ -265,6 +265,11 @@ tc_status tc_server_destroy(tc_server* s);
  * counterpart in the reference: it gives a kernel of known HBM traffic
  * (bytes read + bytes written) to calibrate profiler traffic counters. */
 tc_status tc_probe_copy(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter);
+/* Reads `bytes` (a multiple of 16) of a device buffer with a plain streaming
+ * kernel `iters` times and reports the mean kernel time: the practical HBM
+ * read rate, against which a kernel that only reads is measured.  No
+ * counterpart in the reference. */
+tc_status tc_probe_read(tc_ctx* ctx, int64_t bytes, int32_t iters, double* ms_per_iter);
 
 #ifdef __cplusplus
 }

@@ -40,7 +40,7 @@ void dev_free(void* p);
 
 // Plan scratch (binning output and counters) sized for P pivots / k outputs.
 struct PlanScratch {
-  uint8_t* bin_of = nullptr;
+  uint16_t* bin_of = nullptr;
   uint64_t* items = nullptr;
   uint32_t* blk = nullptr;
   int64_t* part = nullptr;

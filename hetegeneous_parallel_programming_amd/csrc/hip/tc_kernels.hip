@@ -863,8 +863,25 @@ __device__ __forceinline__ PRow partner_row(const LaunchArgs& a, int64_t k) {
   return pr;
 }
 
-// Pass 1: bin of every plan index and, per planning block, the count per
-// bin (bin-major: blk[bin * nblk + block]).
+// Work class of a binned partner row: its estimated cost in quarter octaves
+// (4 floor(log2 c) + the two bits below the leading one), capped.  Within a
+// planning block a bin's rows are placed in class order, so the groups that
+// share a wave (8, 4 or 2 rows in lockstep) get rows of similar cost and
+// fewer of their lanes idle while the heaviest row finishes.
+// The cost estimate weighs the streamed ids (work) against the per-edge
+// batch bookkeeping (np partners) and the set build (s ids).
+constexpr int kWorkClasses = 64;
+__device__ __forceinline__ int work_class(uint32_t work, int64_t np, int s) {
+  const uint64_t c = (uint64_t)work + 8 * (uint64_t)np + 4 * (uint64_t)s;
+  const uint64_t x = c > 0 ? c : 1;
+  const int e = 63 - __clzll((long long)x);
+  const int q = 4 * e + (int)(((x << 2) >> e) & 3);
+  return min(kWorkClasses - 1, q);
+}
+
+// Pass 1: bin and work class of every plan index (bin_of = class << 3 | bin)
+// and, per planning block, the count per bin (bin-major:
+// blk[bin * nblk + block]).
 __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
   __shared__ uint32_t s_cnt[kNumBins];
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -873,7 +890,8 @@ __global__ __launch_bounds__(kBlock) void plan_bin_kernel(LaunchArgs a) {
   if (p < a.P) {
     const PRow pr = partner_row(a, p);
     const int bin = pr.set ? choose_bin(pr.pe - pr.pb, list_len(pr.set), pr.work, a.tune) : (int)kNoBin;
-    a.bufs.bin_of[p] = (uint8_t)bin;
+    a.bufs.bin_of[p] = bin == kNoBin ? (uint16_t)kNoBin
+                                     : (uint16_t)(work_class(pr.work, pr.pe - pr.pb, list_len(pr.set)) << 3 | bin);
     if (bin != kNoBin) atomicAdd(&s_cnt[bin], 1u);
   }
   __syncthreads();
@@ -888,26 +906,33 @@ __global__ void plan_totals_kernel(LaunchArgs a, int64_t nblk) {
   if (b < kNumBins) a.bufs.counters[kCntBinCount + b] = a.bufs.blk[(b + 1) * nblk] - a.bufs.blk[b * nblk];
 }
 
-// Pass 3: stable scatter of (graph, row) items into their bin segment.
+// Pass 3: scatter of (graph, row) items into their bin segment.  A planning
+// block keeps its place in the segment (so items stay grouped by graph); its
+// items are ordered by work class (any order within a class).
 __global__ __launch_bounds__(kBlock) void plan_scatter_kernel(LaunchArgs a) {
-  __shared__ uint32_t s_wc[kWaves][kNumBins];
+  constexpr int kKeys = kNumBins * kWorkClasses;
+  __shared__ uint32_t s_key[kKeys];  // per (bin, class): count, then next rank
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int bin = (p < a.P) ? (int)a.bufs.bin_of[p] : (int)kNoBin;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t rank = 0;
-#pragma unroll
-  for (int b = 0; b < kNumBins; ++b) {
-    const uint64_t m = __ballot(bin == b);
-    if (bin == b) rank = (uint32_t)__popcll(m & lt);
-    if (lane == 0) s_wc[wid][b] = (uint32_t)__popcll(m);
+  const int code = (p < a.P) ? (int)a.bufs.bin_of[p] : (int)kNoBin;
+  const bool has = code != kNoBin;
+  const int bin = code & 7, key = bin * kWorkClasses + (code >> 3);
+  for (int i = threadIdx.x; i < kKeys; i += kBlock) s_key[i] = 0;
+  __syncthreads();
+  if (has) atomicAdd(&s_key[key], 1u);
+  __syncthreads();
+  if (threadIdx.x < kNumBins) {  // per bin: exclusive scan over its classes
+    uint32_t run = 0;
+    for (int c = 0; c < kWorkClasses; ++c) {
+      const uint32_t n = s_key[threadIdx.x * kWorkClasses + c];
+      s_key[threadIdx.x * kWorkClasses + c] = run;
+      run += n;
+    }
   }
   __syncthreads();
-  if (bin != kNoBin) {
+  if (has) {
     const int64_t nblk = gridDim.x;
-    uint32_t off = a.bufs.blk[bin * nblk + blockIdx.x] + rank;
-    for (int w = 0; w < wid; ++w) off += s_wc[w][bin];
-    a.bufs.items[off] = (uint64_t)p;
+    const uint32_t rank = atomicAdd(&s_key[key], 1u);
+    a.bufs.items[a.bufs.blk[bin * nblk + blockIdx.x] + rank] = (uint64_t)p;
   }
 }
 

@@ -98,7 +98,7 @@ struct Tune {
 };
 
 struct PlanBufs {
-  uint8_t* bin_of;      // [P] bin of each pivot
+  uint16_t* bin_of;     // [P] work class << 3 | bin of each pivot (kNoBin: none)
   uint64_t* items;      // [P] (graph << 32 | partner row), ordered by (bin, plan index)
   uint32_t* blk;        // [kNumBins * nblk + 1] per-(bin, planning block) counts -> offsets
   int64_t* part;        // [scan_partials(kNumBins * nblk + 1)] scan partials

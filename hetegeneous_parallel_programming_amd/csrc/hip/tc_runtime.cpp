@@ -73,7 +73,7 @@ tc_status PlanScratch::ensure(int64_t pivots, int64_t nouts) {
     cap = 0;
     int64_t c = std::max<int64_t>(pivots, 1024);
     const int64_t nb = tck::kNumBins * tck::plan_blocks(c) + 1;
-    tc_status st = dev_alloc((void**)&bin_of, (size_t)c, "pivot bins");
+    tc_status st = dev_alloc((void**)&bin_of, sizeof(uint16_t) * (size_t)c, "pivot bins");
     if (st == TC_OK) st = dev_alloc((void**)&items, sizeof(uint64_t) * (size_t)c, "pivot items");
     if (st == TC_OK) st = dev_alloc((void**)&blk, sizeof(uint32_t) * (size_t)nb, "plan blocks");
     if (st == TC_OK) st = dev_alloc((void**)&part, sizeof(int64_t) * (size_t)tck::scan_partials(nb), "plan scan");
@@ -104,7 +104,7 @@ void PlanScratch::release() {
 
 int64_t PlanScratch::bytes_for(int64_t pivots) {
   const int64_t nb = tck::kNumBins * tck::plan_blocks(pivots) + 1;
-  return 9 * pivots + 4 * nb + 8 * tck::scan_partials(nb);
+  return 10 * pivots + 4 * nb + 8 * tck::scan_partials(nb);
 }
 
 tc_status Partners::alloc(int64_t ci_slots, int64_t row_slots, int32_t ngraphs) {

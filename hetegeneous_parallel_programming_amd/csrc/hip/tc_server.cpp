@@ -386,7 +386,7 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
   }
   const int64_t R = cfg->max_batch_rows, G = cfg->max_batch_graphs;
   const int64_t nb = tck::kNumBins * tck::plan_blocks(R) + 1;
-  const uint64_t o_items = 0, o_bin = o_items + align_up(8 * R), o_blk = o_bin + align_up(R),
+  const uint64_t o_items = 0, o_bin = o_items + align_up(8 * R), o_blk = o_bin + align_up(2 * R),
                  o_part = o_blk + align_up(4 * nb), o_cnt = o_part + align_up(8 * tck::scan_partials(nb)),
                  o_outs = o_cnt + align_up(4 * 64), o_desc = o_outs + align_up(8 * G),
                  o_nv = o_desc + align_up(sizeof(tck::GraphDesc) * G), o_vrow = o_nv + align_up(4 * G),
@@ -408,7 +408,7 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
       return st;
     }
     char* p = (char*)sl.plan;
-    sl.bufs = tck::PlanBufs{(uint8_t*)(p + o_bin), (uint64_t*)(p + o_items), (uint32_t*)(p + o_blk),
+    sl.bufs = tck::PlanBufs{(uint16_t*)(p + o_bin), (uint64_t*)(p + o_items), (uint32_t*)(p + o_blk),
                             (int64_t*)(p + o_part), (uint32_t*)(p + o_cnt), (uint64_t*)(p + o_outs)};
     sl.d_descs = (tck::GraphDesc*)(p + o_desc);
     sl.d_nv = (int32_t*)(p + o_nv);

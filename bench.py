@@ -223,6 +223,8 @@ def leg_batch(ctx, graphs, seed, reps, tuner=None):
     autotuned one (tuned on a different batch of the same mix)."""
     n_graphs = len(graphs)
     edges = sum(g.E() for g in graphs)
+    # partner-scheme bytes of the whole batch, as for the headline roofline
+    alg_bytes = sum(g.work()["partner_bytes"] for g in graphs)
     b = ctx.batch(graphs)
     each, ms_each, st_each = b.count_each()
     each2, ms_each2, st_each2 = b.count_each()
@@ -249,7 +251,8 @@ def leg_batch(ctx, graphs, seed, reps, tuner=None):
         res["tune"], res["autotune"] = tune, report
         batched_s, kms = timed()
     res.update({"batched_graphs_per_s": n_graphs / batched_s, "batched_ms": batched_s * 1e3,
-                "batched_p99_ms": float(np.percentile(kms, 99)), "speedup": base_s / batched_s})
+                "batched_p99_ms": float(np.percentile(kms, 99)), "speedup": base_s / batched_s,
+                "alg_bytes": alg_bytes, "alg_GBps": alg_bytes / batched_s / 1e9})
     gold = golden_big().get(f"batch_seed{seed}_{n_graphs}")
     res["exact_vs_cpu_oracle"] = (bool(np.array_equal(each, np.array(gold["triangles"], np.uint64)))
                                   if gold else None)

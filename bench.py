@@ -348,8 +348,15 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
             cnt = np.array([got[i]["count"] for i in range(n_requests)], np.uint64)
             ok = bool(np.array_equal(cnt, want)) and all(got[i]["status"] == 0 for i in range(n_requests))
             span = (done.max() - (sched[0] if sched is not None else start)) / 1e3
+            light = lat[picks >= 0]
+            heavy_lat = lat[picks < 0]
             return {"graphs_per_s": n_requests / span, "p50_ms": float(np.percentile(lat, 50)),
-                    "p99_ms": float(np.percentile(lat, 99)), "exact": ok, "clients": clients,
+                    "p99_ms": float(np.percentile(lat, 99)),
+                    "light_p50_ms": float(np.percentile(light, 50)),
+                    "light_p99_ms": float(np.percentile(light, 99)),
+                    "heavy_p50_ms": float(np.percentile(heavy_lat, 50)) if heavy_lat.size else None,
+                    "heavy_p99_ms": float(np.percentile(heavy_lat, 99)) if heavy_lat.size else None,
+                    "exact": ok, "clients": clients,
                     "mean_batch": float(np.mean([got[i]["batch_graphs"] for i in range(n_requests)]))}
 
         sat = run(None, clients)
@@ -363,7 +370,8 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
             r["offered_per_s"] = lam
             out[f"load_{int(frac * 100)}"] = r
             log(f"serving {int(frac * 100)}% load ({lam:.0f}/s): {r['graphs_per_s']:.0f} graphs/s, "
-                f"p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, exact={r['exact']}, "
+                f"p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms (mix graphs p50 {r['light_p50_ms']:.2f} / "
+                f"p99 {r['light_p99_ms']:.2f} ms, RMAT-20 p50 {r['heavy_p50_ms'] or float('nan'):.2f} ms), exact={r['exact']}, "
                 f"mean batch {r['mean_batch']:.1f}")
         out["server_stats"] = srv.stats()
     out["exact_all"] = all(out[k]["exact"] for k in ("saturated", "load_50", "load_90"))

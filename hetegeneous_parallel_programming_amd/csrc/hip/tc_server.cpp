@@ -36,6 +36,31 @@ struct Layout {
   }
 };
 
+// Copies a request into the pinned staging area on the submitting thread;
+// large requests are split over a few helper threads, since a single
+// memcpy moves ~9 GB/s and an RMAT-20 request holds ~71 MB of CSR.
+constexpr size_t kCopySplit = (size_t)8 << 20;  // bytes per helper thread at least
+constexpr int kCopyThreads = 4;
+void copy_in(char* dst, const char* src, size_t n) {
+  const int nt = (int)std::min<size_t>(kCopyThreads, n / kCopySplit);
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t per = (n + nt - 1) / nt;
+  std::vector<std::thread> helpers;
+  for (int t = 1; t < nt; ++t) {
+    const size_t a = per * t, b = std::min(n, a + per);
+    try {
+      helpers.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+    } catch (...) {  // no thread to be had: copy this part here
+      std::memcpy(dst + a, src + a, b - a);
+    }
+  }
+  std::memcpy(dst, src, std::min(n, per));
+  for (auto& h : helpers) h.join();
+}
+
 // One admission's host-to-device copies: launches of its requests wait on
 // the event; the last request to finish returns the event to the pool.
 struct Upload {
@@ -466,8 +491,9 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
     r->stage_off = so;
     r->staged = s->staging + so;
     const uint64_t rp_bytes = sizeof(int64_t) * ((uint64_t)dag->n_rows + 1);
-    std::memcpy(r->staged, dag->rowptr, rp_bytes);
-    if (dag->nnz) std::memcpy(r->staged + align_up(rp_bytes), dag->colidx, sizeof(int32_t) * (size_t)dag->nnz);
+    copy_in(r->staged, (const char*)dag->rowptr, rp_bytes);
+    if (dag->nnz)
+      copy_in(r->staged + align_up(rp_bytes), (const char*)dag->colidx, sizeof(int32_t) * (size_t)dag->nnz);
   } else {
     r->rp.assign(dag->rowptr, dag->rowptr + dag->n_rows + 1);
     r->ci.assign(dag->colidx, dag->colidx + dag->nnz);

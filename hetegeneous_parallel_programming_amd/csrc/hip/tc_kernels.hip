@@ -1145,21 +1145,33 @@ __global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __re
                                                              const int32_t* bounds, int32_t parts,
                                                              unsigned long long* pe) {
   __shared__ unsigned long long s_pe[kMaxParts];
+  __shared__ int32_t s_b[kMaxParts + 1];
   for (int i = threadIdx.x; i < parts; i += kBlock) s_pe[i] = 0;
+  for (int i = threadIdx.x; i <= parts; i += kBlock) s_b[i] = bounds[i];
   __syncthreads();
   const int64_t u = row_begin + ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
   const int gl = threadIdx.x % kBuildG;
   if (u < row_end && (!own || own[u])) {
     const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
-    const int pu = part_of(bounds, parts, u);
+    const int pu = part_of(s_b, parts, u);
+    // A lane sums a run of edges with the same owner part (most of a row's
+    // edges stay with u's part) and adds it to the workgroup's sums once.
+    int run_owner = pu;
+    unsigned long long run = 0;
     for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
       const uint32_t c = ecount[e];
       if (c == 0) continue;
       const int v = ci[e];
-      const int pv = part_of(bounds, parts, v);
+      const int pv = part_of(s_b, parts, v);
       const int owner = (pv != pu && du < rp[v + 1] - rp[v]) ? pv : pu;
-      atomicAdd(&s_pe[owner], (unsigned long long)c);
+      if (owner != run_owner) {
+        if (run) atomicAdd(&s_pe[run_owner], run);
+        run_owner = owner;
+        run = 0;
+      }
+      run += c;
     }
+    if (run) atomicAdd(&s_pe[run_owner], run);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < parts; i += kBlock)

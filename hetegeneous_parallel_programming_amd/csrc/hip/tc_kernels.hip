@@ -1134,10 +1134,14 @@ __device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
   return lo;
 }
 
-// A thread per counted DAG edge slot of a row group: edge (u, v) with count
-// c goes to the part that computes it in the reference's push/pull kernel
-// (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136): u's part when
-// v is in the same part or d+(u) >= d+(v) (pull), else v's part (push).
+// Every counted DAG edge (u, v) with count c goes to the part that computes
+// it in the reference's push/pull kernel
+// (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136): u's part when v
+// is in the same part or d+(u) >= d+(v) (pull), else v's part (push).  A
+// wave walks the edges of 64 consecutive rows as one flattened range (its
+// lanes stay busy whatever the row lengths; a lane finds its row by a binary
+// search over the rows' degree prefix in LDS) and sums runs of edges with
+// the same owner part before one LDS atomic.
 __global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __restrict__ rp,
                                                              const int32_t* __restrict__ ci, int32_t row_begin,
                                                              int32_t row_end, const uint8_t* own,
@@ -1146,33 +1150,45 @@ __global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __re
                                                              unsigned long long* pe) {
   __shared__ unsigned long long s_pe[kMaxParts];
   __shared__ int32_t s_b[kMaxParts + 1];
+  __shared__ int64_t s_e0[kBlock];
+  __shared__ int32_t s_end[kBlock], s_du[kBlock], s_pu[kBlock];
   for (int i = threadIdx.x; i < parts; i += kBlock) s_pe[i] = 0;
   for (int i = threadIdx.x; i <= parts; i += kBlock) s_b[i] = bounds[i];
-  __syncthreads();
-  const int64_t u = row_begin + ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
-  const int gl = threadIdx.x % kBuildG;
+  const int lane = threadIdx.x & 63, w0 = threadIdx.x & ~63;
+  const int64_t u = row_begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int32_t deg = 0;
   if (u < row_end && (!own || own[u])) {
-    const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
-    const int pu = part_of(s_b, parts, u);
-    // A lane sums a run of edges with the same owner part (most of a row's
-    // edges stay with u's part) and adds it to the workgroup's sums once.
-    int run_owner = pu;
-    unsigned long long run = 0;
-    for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
-      const uint32_t c = ecount[e];
-      if (c == 0) continue;
-      const int v = ci[e];
-      const int pv = part_of(s_b, parts, v);
-      const int owner = (pv != pu && du < rp[v + 1] - rp[v]) ? pv : pu;
-      if (owner != run_owner) {
-        if (run) atomicAdd(&s_pe[run_owner], run);
-        run_owner = owner;
-        run = 0;
-      }
-      run += c;
-    }
-    if (run) atomicAdd(&s_pe[run_owner], run);
+    const int64_t e0 = rp[u];
+    deg = (int32_t)(rp[u + 1] - e0);  // a row holds fewer than 2^31 / 64 ids
+    s_e0[threadIdx.x] = e0;
   }
+  int total;
+  const int before = group_excl_scan<64>(deg, lane, total);
+  s_end[threadIdx.x] = before + deg;
+  s_du[threadIdx.x] = deg;
+  __syncthreads();  // split points and this wave's rows
+  s_pu[threadIdx.x] = deg ? part_of(s_b, parts, u) : 0;
+  wave_sync();
+  int run_owner = 0;
+  unsigned long long run = 0;
+  for (int t = lane; t < total; t += 64) {
+    const int r = w0 + find_edge<64>(s_end + w0, t);
+    const int32_t du = s_du[r];
+    const int64_t e = s_e0[r] + (t - (s_end[r] - du));
+    const uint32_t c = ecount[e];
+    if (c == 0) continue;
+    const int v = ci[e];
+    const int pu = s_pu[r];
+    const int pv = part_of(s_b, parts, v);
+    const int owner = (pv != pu && du < rp[v + 1] - rp[v]) ? pv : pu;
+    if (owner != run_owner) {
+      if (run) atomicAdd(&s_pe[run_owner], run);
+      run_owner = owner;
+      run = 0;
+    }
+    run += c;
+  }
+  if (run) atomicAdd(&s_pe[run_owner], run);
   __syncthreads();
   for (int i = threadIdx.x; i < parts; i += kBlock)
     if (s_pe[i]) atomicAdd(&pe[i], s_pe[i]);
@@ -1349,7 +1365,7 @@ hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_
   if (parts < 1 || parts > kMaxParts || !bounds || !pe) return hipErrorInvalidValue;
   const int64_t rows = (int64_t)row_end - row_begin;
   if (rows <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((rows * kBuildG + kBlock - 1) / kBlock);
+  const unsigned blocks = (unsigned)((rows + kBlock - 1) / kBlock);
   local_parts_kernel<<<blocks, kBlock, 0, s>>>(rp, ci, row_begin, row_end, own, ecount, bounds, parts, pe);
   return hipGetLastError();
 }

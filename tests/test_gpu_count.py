@@ -196,3 +196,19 @@ def test_upload_rejects_malformed(ctx):
         ctx.upload_arrays(rp, np.array([1, 1, 0], np.int32), check=7)  # duplicate in row
     with pytest.raises(TcError):
         ctx.upload_arrays(np.array([0, 3, 2], np.int64), np.array([1, 1, 0], np.int32))
+
+
+def test_stream_probes(ctx):
+    """The copy and read probes (traffic-counter calibration, read ceiling)
+    time a known number of bytes at a plausible rate and reject malformed
+    sizes."""
+    nbytes = 64 << 20
+    for probe, moved in ((ctx.probe_copy, 2 * nbytes), (ctx.probe_read, nbytes)):
+        ms = probe(nbytes, 3)
+        gbps = moved / (ms / 1e3) / 1e9
+        assert 10 < gbps < 20000, (probe.__name__, gbps)
+        for bad in (0, 17, -16):
+            with pytest.raises(TcError):
+                probe(bad, 1)
+        with pytest.raises(TcError):
+            probe(nbytes, 0)

@@ -14,7 +14,9 @@ the DAG already resident in HBM.  Beside it:
   time are reported beside it, and so is the device's practical read ceiling
   (a streaming read of 4 GiB, measured in the same run);
 * cpu_baseline: the CPU oracle (OpenMP merge count, the reference's CPU
-  algorithm) on a bounded random sample of pivot rows of the same graph;
+  algorithm) on the same graph -- the whole graph when a run fits the CPU
+  budget, else a random sample of pivot rows -- in a child process with
+  OMP_PROC_BIND=close, median of 3 runs;
 * extra legs: the plain one-wave-per-pivot kernel, the 4096-graph batched
   workload (one captured persistent launch, default and autotuned, vs one
   plain launch per graph), the serving loop under Poisson arrivals, and
@@ -85,30 +87,83 @@ def time_steps(fn, steps: int, warmup: int):
     return (time.perf_counter() - t0) * 1e3, outs
 
 
-def cpu_baseline(g, target_s: float, seed: int = 7) -> dict:
-    """Oracle count of a random sample of pivot rows (bounded CPU time)."""
+def physical_cores() -> int:
+    """Distinct (socket, core) pairs of the host in /proc/cpuinfo."""
+    pairs, phys = set(), None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("physical id"):
+                    phys = line.split(":", 1)[1].strip()
+                elif line.startswith("core id"):
+                    pairs.add((phys, line.split(":", 1)[1].strip()))
+    except OSError:
+        pass
+    return len(pairs) or (os.cpu_count() or 1)
+
+
+def cpu_baseline_child(scale: int, seed: int, target_s: float, runs: int = 3, sample_seed: int = 7) -> dict:
+    """Runs in a child process started with OMP_PROC_BIND=close (so the
+    binding never reaches the GPU process's server and client threads): the
+    oracle's OpenMP merge count -- the reference's CPU algorithm,
+    graph.cc:376-391 inside the tc_omp_base loop, dynamic schedule of 1024
+    pivots -- on the same RMAT DAG, median of `runs` timed runs.  The whole
+    graph when one run fits target_s, else a random sample of pivot rows
+    sized to it (dynamic schedule of 64 rows)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, used here only as the timed CPU baseline
+    from hetegeneous_parallel_programming_amd import Graph
 
+    g = Graph.rmat_scale(scale, 16, seed)
+    g.orientation()
     threads = cores_available()
     rp, ci = g.out_rowptr(), g.out_colidx()
-    rng = np.random.default_rng(seed)
     n = g.V()
-    frac = 0.002
+    rng = np.random.default_rng(sample_seed)
+    frac = 0.01
     rows = np.sort(rng.choice(n, max(1, int(n * frac)), replace=False)).astype(np.int32)
     t0 = time.perf_counter()
     oracle.tc_count_rows(rp, ci, rows, threads=threads)
-    t_probe = time.perf_counter() - t0
-    frac = min(1.0, frac * target_s / max(t_probe, 1e-3))
-    rows = np.sort(rng.choice(n, max(1, int(n * frac)), replace=False)).astype(np.int32)
-    edges = int((rp[rows + 1] - rp[rows]).sum())
-    t0 = time.perf_counter()
-    oracle.tc_count_rows(rp, ci, rows, threads=threads)
-    t = time.perf_counter() - t0
+    est_whole = (time.perf_counter() - t0) / frac
+    times = []
+    if est_whole <= target_s:
+        edges, what = g.E(), f"the whole RMAT-{scale} DAG ({n} rows, {g.E()} DAG edges)"
+        count = None
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            count = oracle.tc_count(rp, ci, chunk=1024, threads=threads)
+            times.append(time.perf_counter() - t0)
+    else:
+        frac = target_s / est_whole
+        rows = np.sort(rng.choice(n, max(1, int(n * frac)), replace=False)).astype(np.int32)
+        edges = int((rp[rows + 1] - rp[rows]).sum())
+        what = f"{rows.size} random pivot rows ({frac:.4%} of {n}, {edges} DAG edges) of the RMAT-{scale} DAG"
+        count = None
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            count = oracle.tc_count_rows(rp, ci, rows, chunk=64, threads=threads)
+            times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
     return {"value": edges / t, "unit": "edges/s", "cores": threads, "kind": "port",
-            "cpu_model": cpu_model(),
-            "sample": f"{rows.size} random pivot rows ({frac:.4%} of {n}, {edges} DAG edges) of the "
-                      f"same RMAT-22 DAG, OpenMP merge count (dynamic schedule), {t:.2f} s"}
+            "cpu_model": cpu_model(), "physical_cores_host": physical_cores(),
+            "logical_cpus_host": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "runs_s": times, "count": int(count),
+            "sample": f"{what}: oracle OpenMP merge count on {threads} threads (the CPUs this process may "
+                      f"use; OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), median of {runs} runs = {t:.2f} s"}
+
+
+def cpu_baseline(args) -> dict:
+    """The CPU baseline in a child process (see cpu_baseline_child); the
+    child builds the same graph from the same seed and never touches the GPU."""
+    import subprocess
+
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cores_available()))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--scale", str(args.scale),
+           "--seed", str(args.seed), "--cpu-seconds", str(args.cpu_seconds)]
+    p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed ({p.returncode}): {p.stderr[-2000:]}")
+    return json.loads(p.stdout.strip().splitlines()[-1])
 
 
 def leg_er(ctx, seed, steps, warmup):
@@ -467,7 +522,9 @@ def main() -> int:
     ap.add_argument("--no-split", action="store_true",
                     help="skip timing the other parts of the RMAT-24 4-way splits")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="CPU time per baseline run (3 runs); the whole graph when it fits")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-rmat", action="store_true", help="skip the headline RMAT leg")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
@@ -483,6 +540,9 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy (counter calibration) and a plain read of this many bytes")
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        print(json.dumps(cpu_baseline_child(args.scale, args.seed, args.cpu_seconds)), flush=True)
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this triangle-counting build has no multi-GPU "
               "path (run with --gpus 1).", file=sys.stderr)
@@ -575,7 +635,9 @@ def leg_headline(ctx, args) -> dict:
                   "local_counts": single.get("local")},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args)
+        if line["cpu_baseline"]["count"] != best["count"]:
+            raise RuntimeError("CPU baseline count differs from the GPU count")
         log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} edges/s on "
             f"{line['cpu_baseline']['cores']} cores")
     gold = golden_big().get(f"rmat{args.scale}_seed{args.seed}")

@@ -105,6 +105,8 @@ def host():
     _sig(L, "tch_csr_copy", c_i32, [ctypes.POINTER(CsrView), c_u32, ctypes.POINTER(P)])
     _sig(L, "tch_csr_from_pairs", c_i32, [c_i32, i32p, i32p, c_i64, ctypes.POINTER(P)])
     _sig(L, "tch_validate", c_i32, [ctypes.POINTER(CsrView), c_u32])
+    _sig(L, "tch_intersect_num", c_i64, [ctypes.POINTER(CsrView), c_i32, c_i32,
+                                        ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint8])
     _sig(L, "tch_gen_er", c_i32, [c_i32, c_dbl, c_u64, ctypes.POINTER(P)])
     _sig(L, "tch_gen_rmat", c_i32, [c_i32, c_dbl, c_dbl, c_dbl, c_dbl, c_u64, ctypes.POINTER(P)])
     _sig(L, "tch_batch_spec_of", c_i32, [c_u64, c_i32, ctypes.POINTER(BatchSpec)])

@@ -61,6 +61,7 @@ static tc_status capture(tc_ctx* ctx, tc_batch* b) {
   b->partners.into(a);
   a.tune = to_kernel_tune(ctx->tune);
   a.bufs = b->scratch.bufs();
+  for (int32_t r : b->rows) a.wide_ids |= r > tck::kNarrowIds ? 1 : 0;
   hipStream_t s = ctx->stream;
   TC_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   // Zero fills are kernels, not memset nodes.
@@ -270,6 +271,7 @@ tc_status tc_batch_count(tc_ctx* ctx, tc_batch* b, uint64_t* counts, tc_stats* s
     stats->pivots = b->nv;
     const uint32_t* c = (const uint32_t*)(b->pinned + b->n);
     for (int k = 0; k < TC_NUM_BINS; ++k) stats->bin_pivots[k] = c[tck::kCntBinCount + k];
+    stats->hash_fallbacks = c[tck::kCntFallback];
   }
   return TC_OK;
 } catch (...) {

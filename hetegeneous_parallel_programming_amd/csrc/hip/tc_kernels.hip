@@ -116,7 +116,11 @@ constexpr int kInFlightGroup = TC_IN_FLIGHT_GROUP;
 //   group's LDS area or its construction did not converge.
 // kSetHbm: the sorted list in HBM, binary-searched (pivots whose list does
 //   not fit a workgroup's LDS).
-enum SetMode : int { kSetHash = 0, kSetTree = 1, kSetHbm = 2 };
+// kSetHashWide: the same hash over keys w ^ (w >> 24), for graphs whose ids
+//   reach 2^24 (the 24-bit multiply would otherwise put ids equal modulo 2^24
+//   in the same buckets of both tables, and sets holding more than four of
+//   them could not be built).
+enum SetMode : int { kSetHash = 0, kSetTree = 1, kSetHbm = 2, kSetHashWide = 3 };
 constexpr int kModeHash = 1 << kSetHash, kModeTree = 1 << kSetTree, kModeHbm = 1 << kSetHbm;
 
 struct PivotSet {
@@ -138,6 +142,12 @@ constexpr uint32_t kHashA1 = 0x7F4A7Du;
 __device__ __forceinline__ uint32_t hash_bucket(int w, uint32_t mul, int sh) {
   return (uint32_t)__umul24((uint32_t)w, mul) >> sh;
 }
+// The key hashed for id w: w itself, or with its bits 24..30 folded into the
+// low bits (a bijection on ids, so distinct ids stay distinct keys).
+template <bool kWide>
+__device__ __forceinline__ int hash_key(int w) {
+  return kWide ? (w ^ (w >> 24)) : w;
+}
 
 // log2 of the buckets per table for a list of da ids in an LDS area of
 // `area` words (4B words for both tables): load <= 1/2 when the area allows,
@@ -155,16 +165,18 @@ __device__ __forceinline__ int hash_log2_buckets(int da, int area) {
 // safe: every id is always either in exactly one slot or carried by exactly
 // one lane.  Returns false when an evicted id could not be placed within
 // `kicks` moves (the set is then incomplete and must not be used).
+template <bool kWide>
 __device__ __forceinline__ bool hash_insert(int32_t* T, int lb, int key, int kicks) {
   const int sh = 32 - lb;
   const int B = 1 << lb;
   int t = 0;
   for (int it = 0; it < kicks; ++it) {
-    int32_t* b = T + (t ? 2 * B : 0) + 2 * (int)hash_bucket(key, t ? kHashA1 : kHashA0, sh);
+    const int hk = hash_key<kWide>(key);
+    int32_t* b = T + (t ? 2 * B : 0) + 2 * (int)hash_bucket(hk, t ? kHashA1 : kHashA0, sh);
     if (atomicCAS(&b[0], kEmpty, key) == kEmpty) return true;
     if (atomicCAS(&b[1], kEmpty, key) == kEmpty) return true;
     if (it == 0) {  // the other table's bucket before evicting anyone
-      int32_t* c = T + 2 * B + 2 * (int)hash_bucket(key, kHashA1, sh);
+      int32_t* c = T + 2 * B + 2 * (int)hash_bucket(hk, kHashA1, sh);
       if (atomicCAS(&c[0], kEmpty, key) == kEmpty) return true;
       if (atomicCAS(&c[1], kEmpty, key) == kEmpty) return true;
     }
@@ -215,9 +227,12 @@ __device__ __forceinline__ void group_sync() {
 // then fit the tree cap).
 // kLocal: C (area / 2 words: a 16-bit hit counter per slot word, two per
 // word) gets zeroed over the slots the set uses.
+// wide: ids reach 2^24 (kSetHashWide).  fallbacks: counts the groups whose
+// hash build did not converge (one increment per fallen-back set).
 template <int G, bool kLocal = false>
 __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
-                                                    int da, int gl, const Tune& tune, uint32_t* C = nullptr) {
+                                                    int da, int gl, const Tune& tune, bool wide,
+                                                    uint32_t* fallbacks, uint32_t* C = nullptr) {
   PivotSet S{slot, kSetTree, 0, 0};
   // tree_only: 1 = every set a tree; 2 = trees for sub-wave groups only
   const bool tree = tune.tree_only == 1 || (tune.tree_only == 2 && G < 64);
@@ -231,13 +246,17 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
     }
     group_sync<G>();
     bool ok = true;
-    for (int j = gl; j < da; j += G) ok = hash_insert(slot, lb, src[j], tune.hash_kicks) && ok;
+    if (wide)
+      for (int j = gl; j < da; j += G) ok = hash_insert<true>(slot, lb, src[j], tune.hash_kicks) && ok;
+    else
+      for (int j = gl; j < da; j += G) ok = hash_insert<false>(slot, lb, src[j], tune.hash_kicks) && ok;
     if (group_all<G>(ok)) {
-      S.mode = kSetHash;
+      S.mode = wide ? kSetHashWide : kSetHash;
       S.p0 = 32 - lb;
       S.p1 = 8 * B;
       return S;
     }
+    if (gl == 0 && fallbacks) atomicAdd(fallbacks, 1u);
   }
   stage_tree(slot, src, da, gl, G);
   S.p0 = tree_levels(da);
@@ -272,7 +291,7 @@ __device__ __forceinline__ void count_slot_hit(uint32_t* C, int idx) {
 template <int G>
 __device__ __forceinline__ void flush_set_hits(const int32_t* slot, const uint32_t* C, const PivotSet& S, int gl,
                                                unsigned long long* vcount) {
-  const int used = S.mode == kSetHash ? (4 << (32 - S.p0)) : (1 << S.p0);
+  const int used = (S.mode == kSetHash || S.mode == kSetHashWide) ? (4 << (32 - S.p0)) : (1 << S.p0);
   for (int x = gl; x < used; x += G) {
     const uint32_t c = (C[x >> 1] >> ((x & 1) << 4)) & 0xFFFFu;
     const int key = slot[x];
@@ -286,6 +305,35 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t x) {
 #pragma unroll
   for (int o = G / 2; o >= 1; o >>= 1) x += __shfl_xor(x, o, G);
   return x;
+}
+
+// NF lookups of ids w[] in a hash set: two 8-byte LDS reads (one bucket of
+// each table) and four compares per id.  kLocal: a hit also bumps the hit
+// counter of its slot and of its batch edge kq[].
+template <int NF, bool kLocal, bool kWide>
+__device__ __forceinline__ void hash_probe(const PivotSet& S, const int (&w)[NF], const int (&kq)[NF],
+                                           uint32_t& hits, const LocalSink& ls) {
+  const char* Tb = (const char*)S.A;
+  const int sh = S.p0;
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const int hk = hash_key<kWide>(w[q]);
+    const uint32_t b0 = hash_bucket(hk, kHashA0, sh), b1 = hash_bucket(hk, kHashA1, sh);
+    const int2 x = *(const int2*)(Tb + (b0 << 3));
+    const int2 y = *(const int2*)(Tb + S.p1 + (b1 << 3));
+    const bool f = (x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q]);
+    hits += f ? 1u : 0u;
+    if constexpr (kLocal) {
+      if (f) {
+        const int t1 = S.p1 >> 2;  // word offset of table 1
+        const int idx = x.x == w[q] ? 2 * (int)b0
+                      : x.y == w[q] ? 2 * (int)b0 + 1
+                      : y.x == w[q] ? t1 + 2 * (int)b1 : t1 + 2 * (int)b1 + 1;
+        count_slot_hit(ls.C, idx);
+        atomicAdd(&ls.EC[kq[q]], 1u);
+      }
+    }
+  }
 }
 
 // Counts, for the edges j = jfirst, jfirst + jstride, ... of pivot u (list
@@ -380,26 +428,9 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
         }
       }
       if ((kModes & kModeHash) && S.mode == kSetHash) {
-        const char* Tb = (const char*)A;
-        const int sh = S.p0;
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-          const uint32_t b0 = hash_bucket(w[q], kHashA0, sh), b1 = hash_bucket(w[q], kHashA1, sh);
-          const int2 x = *(const int2*)(Tb + (b0 << 3));
-          const int2 y = *(const int2*)(Tb + S.p1 + (b1 << 3));
-          const bool f = (x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q]);
-          hits += f ? 1u : 0u;
-          if constexpr (kLocal) {
-            if (f) {
-              const int t1 = S.p1 >> 2;  // word offset of table 1
-              const int idx = x.x == w[q] ? 2 * (int)b0
-                            : x.y == w[q] ? 2 * (int)b0 + 1
-                            : y.x == w[q] ? t1 + 2 * (int)b1 : t1 + 2 * (int)b1 + 1;
-              count_slot_hit(ls.C, idx);
-              atomicAdd(&ls.EC[kq[q]], 1u);
-            }
-          }
-        }
+        hash_probe<NF, kLocal, false>(S, w, kq, hits, ls);
+      } else if ((kModes & kModeHash) && S.mode == kSetHashWide) {
+        hash_probe<NF, kLocal, true>(S, w, kq, hits, ls);
       } else if ((kModes & kModeTree) && ((kModes == kModeTree) || S.mode == kSetTree)) {
         // h descent steps; then the lower bound is k with its trailing one
         // bits and the following zero shifted out (0 = past the end -> -1).
@@ -999,7 +1030,7 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
     const int64_t item = (int64_t)items[i];
     const PRow pr = partner_row(a, item);
     const PivotSet S = build_pivot_set<G, kLocal>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set), gl,
-                                                  a.tune, C);
+                                                  a.tune, a.wide_ids != 0, a.bufs.counters + kCntFallback, C);
     const LocalSink ls{C, kLocal ? LL.ec() + grp * G : nullptr, kLocal ? a.local.pedge + pr.pb : nullptr,
                        kLocal ? a.local.pvid + pr.pb : nullptr, a.local.ecount, a.local.vcount};
     const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kLocal>(
@@ -1031,7 +1062,8 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int32_t* pvid = kLocal ? a.local.pvid + pr.pb : nullptr;
   uint32_t h;
   if (kLdsPivot) {
-    const PivotSet S = build_pivot_set<kBlock, kLocal>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune, LL.c());
+    const PivotSet S = build_pivot_set<kBlock, kLocal>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune,
+                                                       a.wide_ids != 0, a.bufs.counters + kCntFallback, LL.c());
     const LocalSink ls{LL.c(), EC, pedge, pvid, a.local.ecount, a.local.vcount};
     h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock,
                                                                    L.End + wid * 64, L.Base + wid * 64, lane, ls);

@@ -48,7 +48,8 @@ __host__ __device__ inline int tile_pivots(int bin, int rounds) {
 constexpr int kCntBinCount = 0;                   // [kNumBins]
 constexpr int kCntBinCursor = kNumBins;           // [kNumBins]
 constexpr int kCntTileHead = 2 * kNumBins;        // [kNumBins] tile queue heads
-constexpr int kNumCounters = 3 * kNumBins + 2;
+constexpr int kCntFallback = 3 * kNumBins + 2;    // hash builds that fell back to the tree
+constexpr int kNumCounters = 3 * kNumBins + 3;
 
 // Partner rows.  Every DAG edge (u, v) is counted from the endpoint whose
 // out-list is longer: its "owner" x keeps N+(x) as the LDS set and its
@@ -153,7 +154,11 @@ struct LaunchArgs {
   PlanBufs bufs;
   LocalOut local;
   const uint8_t* own;   // one graph only: null, or per row 1 if it is a pivot row
+  int32_t wide_ids;     // 1 when ids reach 2^24: hash sets fold the high id bits into the bucket
 };
+
+// Ids at or above this need the folded hash (the bucket multiply reads 24 bits).
+constexpr int64_t kNarrowIds = (int64_t)1 << 24;
 
 // Planning blocks of a launch over P plan indices.
 inline int64_t plan_blocks(int64_t P) { return (P + 255) / 256; }

@@ -15,6 +15,9 @@
 namespace tcr {
 
 static thread_local std::string g_err;
+// ctx->pinned layout (uint64 words): [0] count, [8..) plan counters,
+// [kPinnedFallback] the count's hash-fallback counter.
+constexpr int kPinnedFallback = 24;
 
 tc_status fail(tc_status st, const std::string& msg) {
   g_err = msg;
@@ -309,10 +312,12 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
 }
 
 // plan_ms = ev[0] -> ev[plan_end]; count_ms = ev[1] -> ev[2].
+// fallbacks: the hash builds that fell back to the tree (-1: not read).
 static tc_status finish_stats(tc_ctx* ctx, tc_stats* st, double t0, bool planned, int64_t pivots,
-                              int plan_end = 1) {
+                              int plan_end = 1, int64_t fallbacks = 0) {
   if (!st) return TC_OK;
   std::memset(st, 0, sizeof(*st));
+  st->hash_fallbacks = fallbacks;
   st->total_ms = now_ms() - t0;
   float ms = 0;
   if (planned) {
@@ -521,6 +526,7 @@ static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
   a.tune = to_kernel_tune(ctx->tune);
   a.bufs = ctx->scratch.bufs();
   a.own = g->own;
+  a.wide_ids = g->n > tck::kNarrowIds ? 1 : 0;
   return a;
 }
 
@@ -587,9 +593,11 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) t
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + kPinnedFallback, a.bufs.counters + tck::kCntFallback, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
-  return finish_stats(ctx, stats, t0, true, g->nv, 3);
+  return finish_stats(ctx, stats, t0, true, g->nv, 3, *(const uint32_t*)(ctx->pinned + kPinnedFallback));
 } catch (...) {
   return tcr::caught();
 }
@@ -702,6 +710,7 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
                             hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
+  const int64_t fallbacks = ((const uint32_t*)(ctx->pinned + 8))[tck::kCntFallback];
   // pageable destinations: synchronous copies after the stream is done
   if (out->edge_counts && g->edge_end > g->edge_begin)
     TC_HIP_TRY(hipMemcpy(out->edge_counts, L.ecount + g->edge_begin,
@@ -710,7 +719,7 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
     TC_HIP_TRY(hipMemcpy(out->vertex_counts, L.vcount, sizeof(uint64_t) * (size_t)g->n, hipMemcpyDeviceToHost));
   if (k > 0)
     TC_HIP_TRY(hipMemcpy(out->part_counts, L.pe, sizeof(uint64_t) * (size_t)k, hipMemcpyDeviceToHost));
-  return finish_stats(ctx, stats, t0, true, g->nv);
+  return finish_stats(ctx, stats, t0, true, g->nv, 1, fallbacks);
 } catch (...) {
   return tcr::caught();
 }

@@ -278,6 +278,7 @@ bool tc_server::launch() {
       a.tune = to_kernel_tune(ctx->tune);
     }
     a.bufs = s.bufs;
+    for (const Req* r : s.reqs) a.wide_ids |= r->n > tck::kNarrowIds ? 1 : 0;
     const double t = now_ms();
     for (Req* r : s.reqs) r->t_launch = t;
     hipError_t e = hipSuccess;

@@ -258,6 +258,29 @@ tc_status tch_validate(const tc_csr_view* g, uint32_t check) try {
   return tch::caught();
 }
 
+int64_t tch_intersect_num(const tc_csr_view* g, int32_t v, int32_t u, const uint8_t* vlabels, uint8_t label) {
+  if (!g || !g->rowptr || v < 0 || u < 0 || v >= g->n_rows || u >= g->n_rows) {
+    tch::fail(TC_EINVAL, "intersect_num: vertex out of range");
+    return -1;
+  }
+  const int32_t* a = g->colidx + g->rowptr[v];
+  const int32_t* b = g->colidx + g->rowptr[u];
+  const int64_t na = g->rowptr[v + 1] - g->rowptr[v], nb = g->rowptr[u + 1] - g->rowptr[u];
+  int64_t i = 0, j = 0, num = 0;
+  while (i < na && j < nb) {
+    if (a[i] < b[j]) {
+      ++i;
+    } else if (b[j] < a[i]) {
+      ++j;
+    } else {
+      if (!vlabels || vlabels[a[i]] == label) ++num;
+      ++i;
+      ++j;
+    }
+  }
+  return num;
+}
+
 tc_status tch_csr_copy(const tc_csr_view* src, uint32_t check, tch_csr** out) try {
   if (!src || !out) return fail(TC_EINVAL, "null argument");
   tc_status st = tch::validate(*src, check | TC_CHECK_SHAPE);

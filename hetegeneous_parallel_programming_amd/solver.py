@@ -24,12 +24,14 @@ BIN_NAMES = ("global", "workgroup", "wave", "g32", "g16", "g8")
 
 class Stats(ctypes.Structure):
     _fields_ = [("total_ms", nat.c_dbl), ("plan_ms", nat.c_dbl), ("count_ms", nat.c_dbl),
-                ("pivots", nat.c_i64), ("bin_pivots", nat.c_i64 * NUM_BINS)]
+                ("pivots", nat.c_i64), ("bin_pivots", nat.c_i64 * NUM_BINS),
+                ("hash_fallbacks", nat.c_i64)]
 
     def as_dict(self) -> dict:
         return {"total_ms": self.total_ms, "plan_ms": self.plan_ms, "count_ms": self.count_ms,
                 "pivots": self.pivots,
-                "bin_pivots": dict(zip(BIN_NAMES, list(self.bin_pivots)))}
+                "bin_pivots": dict(zip(BIN_NAMES, list(self.bin_pivots))),
+                "hash_fallbacks": self.hash_fallbacks}
 
 
 class Tune(ctypes.Structure):

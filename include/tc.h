@@ -58,6 +58,8 @@ typedef struct tc_stats {
   int64_t pivots;             /* pivot rows considered                             */
   int64_t bin_pivots[TC_NUM_BINS]; /* pivots per bin: global-list, workgroup, wave,
                                       32-lane, 16-lane, 8-lane                      */
+  int64_t hash_fallbacks;     /* pivot sets whose LDS hash build did not converge and
+                                 fell back to the search tree (binned counts)       */
 } tc_stats;
 
 /* ---- context ------------------------------------------------------------

@@ -47,6 +47,13 @@ tc_status tch_csr_from_pairs(int32_t n, const int32_t* src, const int32_t* dst,
 /* Structural checks on a CSR (TC_CHECK_* flags). */
 tc_status tch_validate(const tc_csr_view* g, uint32_t check);
 
+/* |N(v) ∩ N(u)| of two sorted rows by a two-pointer merge, counting only
+ * common ids w with vlabels[w] == label when vlabels is not NULL (one byte
+ * per row).  Graph::intersect_num (graph.cc:376-391) on the host; it is the
+ * reference's Graph API, not part of the GPU count.  Returns -1 (and sets
+ * the error) when v or u is out of range. */
+int64_t tch_intersect_num(const tc_csr_view* g, int32_t v, int32_t u, const uint8_t* vlabels, uint8_t label);
+
 /* Seeded generators (symmetric CSR out).
  *  ER: G(n, M) with M = llround(n*avg_degree/2) uniformly drawn pairs (with
  *      replacement), self-loops dropped, symmetrised and deduplicated.

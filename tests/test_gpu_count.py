@@ -212,3 +212,35 @@ def test_stream_probes(ctx):
                 probe(bad, 1)
         with pytest.raises(TcError):
             probe(nbytes, 0)
+
+
+def test_ids_beyond_2_24_in_hash_sets(ctx):
+    """Ids at and above 2^24 in LDS hash sets.  200 "mid" vertices
+    c + j * 2^24 (40 low values c, j = 0..4) form a clique and 600 leaves join
+    every mid: each leaf's DAG list is the 200 mids, five ids per residue
+    class modulo 2^24.  A bucket hash of the low 24 bits alone could not place
+    five equal-residue ids in two 2-slot buckets; with the high bits folded
+    in every set builds as a hash (no fallback to the search tree), and the
+    count is the closed form C(200, 3) + 600 * C(200, 2), also the oracle's."""
+    rng = np.random.default_rng(24)
+    n = 4 * (1 << 24) + (1 << 20)
+    low = rng.choice(1 << 20, 40, replace=False)
+    mids = np.sort((low[None, :] + (np.arange(5) << 24)[:, None]).ravel()).astype(np.int32)
+    leaves = rng.choice(np.arange(1 << 20, 1 << 24), 600, replace=False).astype(np.int32)
+    i, j = np.triu_indices(mids.size, 1)
+    src = np.concatenate([mids[i], np.repeat(leaves, mids.size)])
+    dst = np.concatenate([mids[j], np.tile(mids, leaves.size)])
+    g = Graph.from_pairs(n, src, dst).oriented()
+    want = math.comb(200, 3) + 600 * math.comb(200, 2)
+    assert oracle.tc_count(g.out_rowptr(), g.out_colidx()) == want
+    dg = ctx.upload(g)
+    old = ctx.tune
+    try:
+        ctx.tune = {"pivot_set": 0}  # hash sets wherever they fit
+        got, st = ctx.count(dg)
+        got_plain, _ = ctx.count_plain(dg)
+    finally:
+        ctx.tune = old
+        dg.free()
+    assert got == want and got_plain == want, (got, got_plain, want, st)
+    assert st["hash_fallbacks"] == 0, st

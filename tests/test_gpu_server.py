@@ -133,3 +133,44 @@ def test_shutdown_finishes_submitted_work(ctx, mix):
     res = {r["tag"]: r for r in srv.poll(max_results=4096)}
     srv.close()
     assert len(res) == 60 and all(res[i]["count"] == want[i] for i in range(60))
+
+
+def test_rmat20_request_alone_and_in_the_mix(ctx):
+    """The serving workload's heavy request -- RMAT scale 20, edge factor 16,
+    seed 1 (~71 MB of CSR) -- through the server, alone and mixed into
+    requests of the batched workload (seed-1 batch, graphs 0..299), checked
+    against the oracle's full-size counts.  The staging area is large enough
+    that submit takes the multi-threaded copy into pinned memory."""
+    import json
+    import os
+
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "big.json")))
+    heavy = Graph.rmat_scale(20, 16, 1)
+    heavy.orientation()
+    want_heavy = gold["rmat20_seed1"]["triangles"]
+    light = Graph.batch(1, 0, 300)
+    want_light = gold["batch_seed1_4096"]["triangles"][:300]
+    with Server(ctx, hbm_budget=2 << 30, slots=3, max_batch_graphs=512, max_batch_rows=4 << 20,
+                host_staging=1 << 30) as srv:
+        srv.submit(heavy, 1_000_000)
+        srv.drain()
+        (alone,) = srv.poll()
+        assert alone["status"] == 0 and alone["batch_graphs"] == 1
+        assert alone["count"] == want_heavy
+        tags = []
+        for rep in range(2):  # heavy requests inside a stream of light ones
+            for i in range(300):
+                srv.submit(light[i], rep * 1000 + i)
+                tags.append(rep * 1000 + i)
+                if i == 150:
+                    srv.submit(heavy, 2_000_000 + rep)
+                    tags.append(2_000_000 + rep)
+        res = collect(srv, len(tags), timeout_s=300)
+    assert len(res) == len(tags)
+    for rep in range(2):
+        r = res[2_000_000 + rep]
+        assert r["status"] == 0 and r["count"] == want_heavy, r
+        for i in range(300):
+            r = res[rep * 1000 + i]
+            assert r["status"] == 0 and r["count"] == want_light[i], (rep, i, r)
+    assert max(r["batch_graphs"] for r in res.values()) > 1

@@ -167,9 +167,12 @@ def cpu_baseline(args) -> dict:
 
 
 def leg_er(ctx, seed, steps, warmup):
-    """ER G(n=16384, avg degree 16): binned and plain counts of the resident
-    DAG beside the CPU oracle's count of the same DAG on all host cores
-    (median of 3).  A ~131k-edge graph is launch-bound on the GPU."""
+    """ER G(n=16384, avg degree 16): tc_count (which takes the single plain
+    launch at this size), the binned count (its plan kept after the first
+    call) and the plain kernel on the resident DAG, beside the CPU oracle's
+    count of the same DAG on all host cores (median of 3); and TCSolver's
+    one-shot call (upload + partner build + count + free).  A ~131k-edge
+    graph is launch-bound on the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, and here the timed CPU figure beside the GPU one
     from hetegeneous_parallel_programming_amd import Graph
@@ -185,19 +188,27 @@ def leg_er(ctx, seed, steps, warmup):
     cpu_s = sorted(cpu)[1]
     res = {"vertices": g.V(), "dag_edges": g.E(), "count": want,
            "cpu": {"ms": cpu_s * 1e3, "edges_per_s": g.E() / cpu_s, "cores": threads}}
+    from hetegeneous_parallel_programming_amd.solver import TCSolver
+
     dg = ctx.upload(g)
     exact = True
-    for kind, fn in (("binned", ctx.count), ("plain", ctx.count_plain)):
+    for kind, fn in (("tc_count", ctx.count), ("binned", ctx.count_binned), ("plain", ctx.count_plain)):
         ms_total, outs = time_steps(lambda: fn(dg), steps, warmup)
         exact = exact and all(o[0] == want for o in outs)
         k_avg = float(np.mean([o[1]["count_ms"] for o in outs]))
         res[kind] = {"ms_per_step": ms_total / steps, "kernel_ms_avg": k_avg,
-                     "edges_per_s": g.E() / (ms_total / steps / 1e3)}
+                     "edges_per_s": g.E() / (ms_total / steps / 1e3),
+                     "single_launch": bool(outs[-1][1]["single_launch"])}
     dg.free()
+    ms_total, outs = time_steps(lambda: TCSolver(g, ctx=ctx), steps, warmup)
+    exact = exact and all(o == want for o in outs)
+    res["tcsolver_one_shot_ms"] = ms_total / steps
     res["exact_vs_cpu_oracle"] = exact
-    log(f"er16384: count={want} binned step={res['binned']['ms_per_step']:.3f} ms "
+    log(f"er16384: count={want} tc_count step={res['tc_count']['ms_per_step']:.3f} ms "
+        f"(single launch: {res['tc_count']['single_launch']}), binned {res['binned']['ms_per_step']:.3f} ms "
         f"(kernels {res['binned']['kernel_ms_avg']:.3f}), plain {res['plain']['ms_per_step']:.3f} ms, "
-        f"cpu {cpu_s * 1e3:.2f} ms on {threads} cores, exact={exact}")
+        f"TCSolver one-shot {res['tcsolver_one_shot_ms']:.3f} ms, cpu {cpu_s * 1e3:.2f} ms on {threads} cores, "
+        f"exact={exact}")
     return res
 
 
@@ -272,18 +283,52 @@ def golden_big() -> dict:
     return json.load(open(p)) if os.path.exists(p) else {}
 
 
+def per_graph_tc_count(ctx, graphs):
+    """Second baseline: every graph uploaded on its own (tc_graph_upload) and
+    counted by tc_count, one call per graph (the single plain launch for a
+    small graph, the binned count with its kept plan otherwise); per-graph
+    host time of the call (each ends in a stream sync), after one untimed
+    pass.  Returns (counts, ms per graph)."""
+    dgs = [ctx.upload(g) for g in graphs]
+    try:
+        for dg in dgs:
+            ctx.count(dg)
+        counts = np.zeros(len(graphs), np.uint64)
+        ms = np.zeros(len(graphs))
+        for i, dg in enumerate(dgs):
+            t0 = time.perf_counter()
+            counts[i] = ctx.count(dg)[0]
+            ms[i] = (time.perf_counter() - t0) * 1e3
+    finally:
+        for dg in dgs:
+            dg.free()
+    return counts, ms
+
+
 def leg_batch(ctx, graphs, seed, reps, tuner=None):
-    """4096-graph mix: one plain launch per graph (baseline) vs one captured
-    persistent launch, first with the default launch setting, then with the
-    autotuned one (tuned on a different batch of the same mix)."""
+    """4096-graph mix: one captured persistent launch, first with the default
+    launch setting, then with the autotuned one (tuned on a different batch of
+    the same mix), against two per-graph baselines -- one plain launch per
+    graph (memset + plain kernel + copy back + sync, tc_batch_count_each), and
+    tc_count per separately uploaded graph -- with the speed-up also quoted
+    against the faster of the two for every graph.  The batch's own creation
+    (host-to-device copies and the partner build of all graphs) is reported
+    beside it, with a build-inclusive rate."""
     n_graphs = len(graphs)
     edges = sum(g.E() for g in graphs)
     # partner-scheme bytes of the whole batch, as for the headline roofline
     alg_bytes = sum(g.work()["partner_bytes"] for g in graphs)
+    t0 = time.perf_counter()
     b = ctx.batch(graphs)
+    create_ms = (time.perf_counter() - t0) * 1e3
+    build_ms = b.info()["build_ms"]
     each, ms_each, st_each = b.count_each()
     each2, ms_each2, st_each2 = b.count_each()
     base_s = st_each2["total_ms"] / 1e3
+    tcc, ms_tcc = per_graph_tc_count(ctx, graphs)
+    if not np.array_equal(tcc, each):
+        raise RuntimeError("tc_count per graph differs from the plain launch per graph")
+    best_s = float(np.minimum(ms_each2, ms_tcc).sum()) / 1e3
 
     def timed():
         b.count()  # capture + first replay
@@ -298,6 +343,10 @@ def leg_batch(ctx, graphs, seed, reps, tuner=None):
            "baseline_graphs_per_s": n_graphs / base_s,
            "baseline_p50_ms": float(np.percentile(ms_each2, 50)),
            "baseline_p99_ms": float(np.percentile(ms_each2, 99)),
+           "baseline_tc_count_graphs_per_s": n_graphs / (ms_tcc.sum() / 1e3),
+           "baseline_tc_count_p99_ms": float(np.percentile(ms_tcc, 99)),
+           "baseline_best_per_graph_graphs_per_s": n_graphs / best_s,
+           "create_ms": create_ms, "partner_build_ms": build_ms,
            "default_tune": ctx.tune, "default_graphs_per_s": n_graphs / default_s,
            "default_speedup": base_s / default_s}
     batched_s, kms = default_s, kms_default
@@ -307,13 +356,21 @@ def leg_batch(ctx, graphs, seed, reps, tuner=None):
         batched_s, kms = timed()
     res.update({"batched_graphs_per_s": n_graphs / batched_s, "batched_ms": batched_s * 1e3,
                 "batched_p99_ms": float(np.percentile(kms, 99)), "speedup": base_s / batched_s,
+                "speedup_vs_best_per_graph": best_s / batched_s,
+                "build_inclusive_graphs_per_s": n_graphs / (batched_s + build_ms / 1e3),
+                "build_inclusive_speedup_vs_best_per_graph": best_s / (batched_s + build_ms / 1e3),
                 "alg_bytes": alg_bytes, "alg_GBps": alg_bytes / batched_s / 1e9})
     gold = golden_big().get(f"batch_seed{seed}_{n_graphs}")
     res["exact_vs_cpu_oracle"] = (bool(np.array_equal(each, np.array(gold["triangles"], np.uint64)))
                                   if gold else None)
-    log(f"batch: baseline {res['baseline_graphs_per_s']:.0f} graphs/s, batched default "
+    log(f"batch: baselines {res['baseline_graphs_per_s']:.0f} graphs/s (plain launch per graph), "
+        f"{res['baseline_tc_count_graphs_per_s']:.0f} (tc_count per graph), "
+        f"{res['baseline_best_per_graph_graphs_per_s']:.0f} (faster per graph); batched default "
         f"{res['default_graphs_per_s']:.0f} graphs/s, batched tuned {res['batched_graphs_per_s']:.0f} "
-        f"graphs/s, speedup {res['speedup']:.2f}x, exact vs oracle: {res['exact_vs_cpu_oracle']}")
+        f"graphs/s ({res['batched_ms']:.2f} ms), speedup {res['speedup']:.2f}x / "
+        f"{res['speedup_vs_best_per_graph']:.2f}x vs the faster baseline; creation {create_ms:.0f} ms "
+        f"(partner build {build_ms:.1f} ms), build-inclusive {res['build_inclusive_graphs_per_s']:.0f} graphs/s "
+        f"({res['build_inclusive_speedup_vs_best_per_graph']:.2f}x); exact vs oracle: {res['exact_vs_cpu_oracle']}")
     b.free()
     return res, each
 
@@ -341,9 +398,31 @@ def batch_tuner(args):
     return tune
 
 
+def serving_mix_capacity(ctx, req, want, k=2000, reps=3):
+    """Batched capacity of the serving mix: the first k requests (the same
+    draw: batched-workload graphs plus ~1% RMAT-20) counted by one captured
+    batched launch.  Replay-only graphs/s, and with the batch's partner build
+    included (a served graph arrives without its partner structure)."""
+    k = min(k, len(req))
+    b = ctx.batch(req[:k])
+    try:
+        build_ms = b.info()["build_ms"]
+        b.count()
+        ms_total, outs = time_steps(lambda: b.count(), reps, 1)
+        exact = all(np.array_equal(o[0], want[:k]) for o in outs)
+    finally:
+        b.free()
+    replay_s = ms_total / reps / 1e3
+    return {"requests": k, "replay_ms": replay_s * 1e3, "partner_build_ms": build_ms,
+            "replay_graphs_per_s": k / replay_s, "graphs_per_s": k / (replay_s + build_ms / 1e3),
+            "exact": exact}
+
+
 def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clients=8):
     """Poisson TC requests from the batched mix plus 1% RMAT-20 through the
-    continuous-batching server: saturated capacity, then 50% and 90% load."""
+    continuous-batching server: saturated throughput; then Poisson arrivals at
+    50% and 90% of the batched capacity of the same mix (build included), and
+    at 50% and 90% of the server's own saturated throughput."""
     from hetegeneous_parallel_programming_amd.solver import Server, clock_ms
 
     rng = np.random.default_rng(seed)
@@ -351,6 +430,10 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
     want = np.array([heavy_count if k < 0 else counts[k] for k in picks], np.uint64)
     req = [heavy if k < 0 else graphs[k] for k in picks]
     out = {"requests": n_requests, "heavy_fraction": float((picks < 0).mean())}
+    cap = serving_mix_capacity(ctx, req, want)
+    out["batched_capacity"] = cap
+    log(f"serving mix, batched: {cap['graphs_per_s']:.0f} graphs/s with the partner build "
+        f"({cap['replay_graphs_per_s']:.0f} replay only; {cap['requests']} requests), exact={cap['exact']}")
     with Server(ctx, hbm_budget=16 << 30, slots=3, max_batch_graphs=2048, max_batch_rows=16 << 20,
                 max_queue=1 << 20, host_staging=4 << 30) as srv:
         for i in range(64):  # warm-up
@@ -416,20 +499,25 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
 
         sat = run(None, clients)
         out["saturated"] = sat
-        cap = sat["graphs_per_s"]
-        log(f"serving saturated: {cap:.0f} graphs/s, p99 {sat['p99_ms']:.1f} ms, exact={sat['exact']}")
-        for frac in (0.5, 0.9):
-            lam = frac * cap
+        log(f"serving saturated: {sat['graphs_per_s']:.0f} graphs/s, p99 {sat['p99_ms']:.1f} ms, "
+            f"exact={sat['exact']}")
+        # load_50 / load_90: fractions of the batched capacity of this mix;
+        # load_50_of_saturated / ...: of the server's own saturated rate
+        for key, frac, base in (("load_50", 0.5, cap["graphs_per_s"]), ("load_90", 0.9, cap["graphs_per_s"]),
+                                ("load_50_of_saturated", 0.5, sat["graphs_per_s"]),
+                                ("load_90_of_saturated", 0.9, sat["graphs_per_s"])):
+            lam = frac * base
             arr = np.cumsum(rng.exponential(1.0 / lam, n_requests)) * 1e3
             r = run(arr, clients)
             r["offered_per_s"] = lam
-            out[f"load_{int(frac * 100)}"] = r
-            log(f"serving {int(frac * 100)}% load ({lam:.0f}/s): {r['graphs_per_s']:.0f} graphs/s, "
+            out[key] = r
+            log(f"serving {key} ({lam:.0f}/s offered): {r['graphs_per_s']:.0f} graphs/s, "
                 f"p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms (mix graphs p50 {r['light_p50_ms']:.2f} / "
-                f"p99 {r['light_p99_ms']:.2f} ms, RMAT-20 p50 {r['heavy_p50_ms'] or float('nan'):.2f} ms), exact={r['exact']}, "
-                f"mean batch {r['mean_batch']:.1f}")
+                f"p99 {r['light_p99_ms']:.2f} ms, RMAT-20 p50 {r['heavy_p50_ms'] or float('nan'):.2f} ms), "
+                f"exact={r['exact']}, mean batch {r['mean_batch']:.1f}")
         out["server_stats"] = srv.stats()
-    out["exact_all"] = all(out[k]["exact"] for k in ("saturated", "load_50", "load_90"))
+    out["exact_all"] = cap["exact"] and all(
+        out[k]["exact"] for k in ("saturated", "load_50", "load_90", "load_50_of_saturated", "load_90_of_saturated"))
     return out
 
 

@@ -125,7 +125,7 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
   int64_t rpo = 0, cio = 0, pro = 0;
   for (int32_t i = 0; i < n_graphs; ++i) {
     const int32_t nvm = tck::partner_rows_max(dags[i].n_rows, dags[i].nnz);
-    b->descs[i] = tck::GraphDesc{rpo, cio, cio, rpo, pro, dags[i].n_rows, nvm, 0, dags[i].n_rows};
+    b->descs[i] = tck::GraphDesc{rpo, cio, cio, rpo, pro, dags[i].n_rows, nvm, 0, dags[i].n_rows, cio};
     b->rows[i] = dags[i].n_rows;
     rpo += dags[i].n_rows + 1;
     cio += dags[i].nnz;
@@ -221,7 +221,7 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
     BuildScratch bs;
     st = TC_OK;
     e = hipEventRecord(ctx->ev[0], ctx->stream);
-    if (e == hipSuccess) st = launch_build(a, b->RP, bs, ctx->stream);
+    if (e == hipSuccess) st = launch_build(a, b->RP, b->cio, bs, ctx->stream);
     if (st == TC_OK && e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
     std::vector<int32_t> nv((size_t)n_graphs);
     if (st == TC_OK && e == hipSuccess)

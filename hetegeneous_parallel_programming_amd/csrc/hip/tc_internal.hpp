@@ -91,21 +91,23 @@ struct LocalBufs {
   void release();
 };
 
-// Temporary arrays of a partner build over RP rowptr slots.
+// Temporary arrays of a partner build over RP rowptr slots and M DAG ids
+// (the ids' part -- 24 bytes per id -- serves the bucketed transpose; with
+// ids = 0 only the per-edge atomic path is available).
 struct BuildScratch {
   char* mem = nullptr;
   int64_t cap = 0;
   tck::BuildBufs b{};
-  tc_status ensure(int64_t rp_slots);
+  tc_status ensure(int64_t rp_slots, int64_t ids);
   void release();
-  static int64_t bytes_for(int64_t rp_slots);
+  static int64_t bytes_for(int64_t rp_slots, int64_t ids);
 };
 
 // Builds the partners of the graphs of `a` (descs on the device, pools and
-// P set; RP row slots) on stream s.  Stream-ordered, no sync.
+// P set; RP row slots, M ids in all) on stream s.  Stream-ordered, no sync.
 // loc: null, or where to write each partner entry's DAG edge position and
 // partner vertex and each partner row's owner (pedge, pvid, pown).
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s,
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t M, BuildScratch& bs, hipStream_t s,
                        const LocalBufs* loc = nullptr);
 
 // Makes ctx's device current for the scope of one API call.
@@ -166,4 +168,11 @@ struct tc_graph {
   double build_ms = 0;  // partner build time on the device
   tck::GraphDesc* d_desc = nullptr;
   int64_t bytes = 0;
+  // The binned count's plan, kept across calls: valid for plan_tune; its
+  // rows per bin and tiles per bin (which size the launches) on the host.
+  tcr::PlanScratch plan;
+  bool plan_valid = false;
+  tc_tune plan_tune{};
+  int64_t plan_bins[tck::kNumBins] = {};
+  int64_t plan_tiles[tck::kNumBins] = {};
 };

@@ -671,6 +671,24 @@ struct LoadI32 {
   const int32_t* p;
   __device__ int64_t operator()(int64_t i) const { return p[i]; }
 };
+struct LoadI64 {
+  const int64_t* p;
+  __device__ int64_t operator()(int64_t i) const { return p[i]; }
+};
+struct LoadU64 {
+  const unsigned long long* p;
+  __device__ int64_t operator()(int64_t i) const { return (int64_t)p[i]; }
+};
+struct StoreU64 {
+  unsigned long long* p;
+  __device__ void operator()(int64_t i, int64_t v) const { p[i] = (unsigned long long)v; }
+};
+// c1[i] + c2[i]: a row's partners through out-edges and in-edges.
+struct LoadAdd {
+  const int32_t* a;
+  const int32_t* b;
+  __device__ int64_t operator()(int64_t i) const { return (int64_t)a[i] + b[i]; }
+};
 struct StoreI64 {
   int64_t* p;
   __device__ void operator()(int64_t i, int64_t v) const { p[i] = v; }
@@ -838,8 +856,268 @@ __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
   if (gl == 0) vr->work = (uint32_t)min(w, (int64_t)0xFFFFFFFF);
 }
 
-hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s) {
+// ---------------------------------------------------------------------------
+// partner build by a bucketed transpose (no per-edge global atomics)
+// ---------------------------------------------------------------------------
+// Most edges of a degree-ordered DAG are owned by their head (77% on RMAT-22),
+// so the build is mostly a transpose: in-owned edges grouped by owner.
+// Per-edge atomics on owner counters make that transpose wait on the memory
+// side's atomic rate (and on hub vertices).  Here instead:
+//   split   -- per owned row (16 lanes), in row order: out-owned partners
+//              compacted at the front of the row's slots of `out`, counted
+//              in c1; an in-owned edge leaves its owner's row slot in `key`
+//              and is counted in its owner's bucket (block LDS histogram,
+//              one atomic per (block, bucket));
+//   bucket  -- the same row walk places every (owner, partner list) pair in
+//              its bucket's range of pkey/pref (one reservation per
+//              (block, bucket), LDS ranks inside it);
+//   incount -- per chunk of a bucket, an LDS histogram over the bucket's
+//              owners gives c2 (a store, or one atomic per owner and chunk
+//              when the bucket has several chunks);
+//   (scan of c1 + c2: every row's partner offset, as for the atomic path)
+//   outcopy -- each row's out-owned partners moved to its partner range, and
+//              the first in-partner slot of every row;
+//   infill  -- per chunk, the LDS histogram again reserves each owner's slots
+//              (per owner and chunk), LDS ranks place the pairs.
+// The partner pool comes out laid out as by the atomic path (a row's
+// out-edge partners in row order, then its in-edge partners in any order),
+// so rows, work, planning and counting are unchanged.
+constexpr int kRowsPerBlock = 2048;  // rows of one block of the split and bucket walks
+
+__global__ __launch_bounds__(kBlock) void partner_split_kernel(LaunchArgs a, int64_t RP, BuildBufs b, int bshift,
+                                                               int nb) {
+  extern __shared__ uint32_t s_hist[];  // [nb] in-owned edges per owner bucket
+  for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0;
+  __syncthreads();
+  const int gl = threadIdx.x % kBuildG;
+  const int shift = (int)(threadIdx.x & 63) & ~(kBuildG - 1);
+  const uint32_t below = (1u << gl) - 1u;
+  const int64_t q1 = min((int64_t)(blockIdx.x + 1) * kRowsPerBlock, RP);
+  for (int64_t q = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kBuildG; q < q1; q += kBlock / kBuildG) {
+    const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
+    const int64_t u = q - d.row_base;
+    int32_t c1 = 0;
+    if (u < d.n_rows && u >= d.own_begin && u < d.own_end && (!a.own || a.own[u])) {
+      const int64_t* rp = a.rp + d.rp_off;
+      const int32_t* ci = a.ci + d.ci_off;
+      const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
+      ListRef* out = b.out + d.eb + e0;
+      int32_t* key = b.key + d.eb;
+      for (int64_t eb = e0; eb < e1; eb += kBuildG) {
+        const int64_t e = eb + gl;
+        int v = 0;
+        int64_t vb = 0, dv = 0;
+        bool own = false;
+        if (e < e1) {
+          v = ci[e];
+          vb = rp[v];
+          dv = rp[v + 1] - vb;
+          own = dv <= du;
+        }
+        const uint32_t gm = (uint32_t)(__ballot(own) >> shift) & ((1u << kBuildG) - 1u);
+        if (e < e1) {
+          if (own) {
+            out[c1 + __popc(gm & below)] = list_ref(d.ci_off + vb, dv);
+            key[e] = -1;
+          } else {
+            const int32_t k = (int32_t)(d.row_base + v);
+            key[e] = k;
+            atomicAdd(&s_hist[k >> bshift], 1u);
+          }
+        }
+        c1 += __popc(gm);
+      }
+    }
+    if (gl == 0) b.c1[q] = c1;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += kBlock)
+    if (s_hist[i]) atomicAdd(&b.btot[i], (unsigned long long)s_hist[i]);
+}
+
+// After the scan of btot (now bucket starts, btot[nb] = all pairs): bucket
+// fill cursors, and the chunks of each bucket (cstart[nb] = 0, scanned next).
+__global__ __launch_bounds__(kBlock) void bucket_chunks_kernel(BuildBufs b, int nb) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nb) {
+    const unsigned long long p0 = b.btot[i], p1 = b.btot[i + 1];
+    b.bcur[i] = p0;
+    b.cstart[i] = (int64_t)((p1 - p0 + kChunkPairs - 1) / kChunkPairs);
+  } else if (i == nb) {
+    b.cstart[nb] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void partner_bucket_kernel(LaunchArgs a, int64_t RP, BuildBufs b, int bshift,
+                                                                int nb) {
+  extern __shared__ uint32_t s_w[];  // [nb] pairs per bucket, then the next slot of each
+  for (int i = threadIdx.x; i < nb; i += kBlock) s_w[i] = 0;
+  __syncthreads();
+  const int gl = threadIdx.x % kBuildG;
+  const int64_t qf = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kBuildG;
+  const int64_t q1 = min((int64_t)(blockIdx.x + 1) * kRowsPerBlock, RP);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int64_t q = qf; q < q1; q += kBlock / kBuildG) {
+      const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
+      const int64_t u = q - d.row_base;
+      if (u >= d.n_rows || u < d.own_begin || u >= d.own_end || (a.own && !a.own[u])) continue;
+      const int64_t* rp = a.rp + d.rp_off;
+      const int64_t e0 = rp[u], e1 = rp[u + 1];
+      const int32_t* key = b.key + d.eb;
+      const ListRef ref = list_ref(d.ci_off + e0, e1 - e0);
+      for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
+        const int32_t k = key[e];
+        if (k < 0) continue;
+        const uint32_t slot = atomicAdd(&s_w[k >> bshift], 1u);
+        if (pass == 1) {
+          b.pkey[slot] = k;
+          b.pref[slot] = ref;
+        }
+      }
+    }
+    __syncthreads();
+    if (pass == 0) {  // reserve this block's range of each bucket (pair positions are below 2^32)
+      for (int i = threadIdx.x; i < nb; i += kBlock)
+        if (s_w[i]) s_w[i] = (uint32_t)atomicAdd(&b.bcur[i], (unsigned long long)s_w[i]);
+      __syncthreads();
+    }
+  }
+}
+
+// Chunk blockIdx.x of the bucketed pairs: its bucket, pair range and whether
+// it is its bucket's only chunk.  False for blocks past the last chunk.
+struct PairChunk {
+  int bk;
+  int64_t p0, p1;
+  bool single;
+};
+__device__ __forceinline__ bool pair_chunk(const BuildBufs& b, int nb, PairChunk& c) {
+  const int64_t i = blockIdx.x;
+  if (i >= b.cstart[nb]) return false;
+  int lo = 0, hi = nb - 1;  // last bucket with cstart <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.cstart[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int64_t k = i - b.cstart[lo];
+  const int64_t bp0 = (int64_t)b.btot[lo], bp1 = (int64_t)b.btot[lo + 1];
+  c.bk = lo;
+  c.p0 = bp0 + k * kChunkPairs;
+  c.p1 = min(bp1, c.p0 + kChunkPairs);
+  c.single = b.cstart[lo + 1] - b.cstart[lo] == 1;
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void partner_incount_kernel(BuildBufs b, int bshift, int nb) {
+  extern __shared__ uint32_t s_h[];  // [1 << bshift] in-owned edges per owner of the bucket
+  PairChunk c;
+  if (!pair_chunk(b, nb, c)) return;
+  const int nk = 1 << bshift;
+  const int32_t kb = c.bk << bshift;
+  for (int j = threadIdx.x; j < nk; j += kBlock) s_h[j] = 0;
+  __syncthreads();
+  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_h[b.pkey[p] - kb], 1u);
+  __syncthreads();
+  for (int j = threadIdx.x; j < nk; j += kBlock) {
+    const uint32_t h = s_h[j];
+    if (!h) continue;
+    if (c.single) b.c2[kb + j] = (int32_t)h;
+    else atomicAdd(&b.c2[kb + j], (int32_t)h);
+  }
+}
+
+// Per row (kBuildG lanes): its out-owned partners to the front of its
+// partner range; cur = absolute slot of its first in-edge partner.
+__global__ __launch_bounds__(kBlock) void partner_outcopy_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
+  const int gl = threadIdx.x % kBuildG;
+  if (q >= RP) return;
+  const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
+  const int64_t u = q - d.row_base;
+  const int64_t base = d.pci_off + b.off[q] - b.off[d.row_base];
+  const int32_t c = b.c1[q];
+  if (c > 0) {
+    const ListRef* src = b.out + d.eb + a.rp[d.rp_off + u];
+    ListRef* dst = (ListRef*)a.pci + base;
+    for (int j = gl; j < c; j += kBuildG) dst[j] = src[j];
+  }
+  if (gl == 0) b.cur[q] = (unsigned long long)(base + c);
+}
+
+__global__ __launch_bounds__(kBlock) void partner_infill_kernel(LaunchArgs a, BuildBufs b, int bshift, int nb) {
+  extern __shared__ uint32_t s_w[];  // [1 << bshift] pairs per owner, then the next rank of each
+  PairChunk c;
+  if (!pair_chunk(b, nb, c)) return;
+  const int nk = 1 << bshift;
+  const int32_t kb = c.bk << bshift;
+  for (int j = threadIdx.x; j < nk; j += kBlock) s_w[j] = 0;
+  __syncthreads();
+  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_w[b.pkey[p] - kb], 1u);
+  __syncthreads();
+  // this chunk's first rank of each owner: 0 when the bucket is one chunk,
+  // else reserved on the owner's counter (kc, zeroed)
+  for (int j = threadIdx.x; j < nk; j += kBlock) {
+    const uint32_t h = s_w[j];
+    if (h) s_w[j] = c.single ? 0u : atomicAdd((uint32_t*)&b.kc[kb + j], h);
+  }
+  __syncthreads();
+  ListRef* pci = (ListRef*)a.pci;
+  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) {
+    const int32_t k = b.pkey[p];
+    const uint32_t r = atomicAdd(&s_w[k - kb], 1u);
+    pci[b.cur[k] + r] = b.pref[p];
+  }
+}
+
+// Rows, row offsets and work after the partner pool is filled (both paths).
+static hipError_t launch_partner_rows(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s) {
+  const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
+  hipError_t e;
+  partner_rows_count_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(LoadI32{b.kc}, StoreI64{b.voff}, RP, b.part, s)) != hipSuccess) return e;
+  partner_rows_fill_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (a.P > 0) partner_work_kernel<<<(unsigned)((a.P * kBuildG + kBlock - 1) / kBlock), kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+static hipError_t launch_partners_bucketed(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b,
+                                           int bshift, hipStream_t s) {
+  const int nb = (int)((RP + ((int64_t)1 << bshift) - 1) >> bshift);
+  const unsigned wblocks = (unsigned)((RP + kRowsPerBlock - 1) / kRowsPerBlock);
+  const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
+  const unsigned cblocks = (unsigned)(nb + (M + kChunkPairs - 1) / kChunkPairs);  // >= chunks
+  const size_t wlds = sizeof(uint32_t) * (size_t)nb, clds = sizeof(uint32_t) << bshift;
+  hipError_t e = launch_zero(b.c2, RP, s);
+  if (e == hipSuccess) e = launch_zero(b.btot, 2 * ((int64_t)nb + 1), s);
+  if (e == hipSuccess) e = launch_zero((void*)a.vrow, a.P * (int64_t)(sizeof(VRow) / 4), s);  // slots unused
+  if (e != hipSuccess) return e;
+  partner_split_kernel<<<wblocks, kBlock, wlds, s>>>(a, RP, b, bshift, nb);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(LoadU64{b.btot}, StoreU64{b.btot}, (int64_t)nb + 1, b.part, s)) != hipSuccess) return e;
+  bucket_chunks_kernel<<<(unsigned)((nb + kBlock) / kBlock), kBlock, 0, s>>>(b, nb);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(LoadI64{b.cstart}, StoreI64{b.cstart}, (int64_t)nb + 1, b.part, s)) != hipSuccess) return e;
+  partner_bucket_kernel<<<wblocks, kBlock, wlds, s>>>(a, RP, b, bshift, nb);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  partner_incount_kernel<<<cblocks, kBlock, clds, s>>>(b, bshift, nb);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_scan(LoadAdd{b.c1, b.c2}, StoreI64{b.off}, RP, b.part, s)) != hipSuccess) return e;
+  partner_outcopy_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_zero(b.kc, RP, s)) != hipSuccess) return e;
+  partner_infill_kernel<<<cblocks, kBlock, clds, s>>>(a, b, bshift, nb);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return launch_partner_rows(a, RP, b, s);
+}
+
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b, hipStream_t s) {
   if (RP <= 0) return hipSuccess;
+  const int bshift = bucket_shift(RP);
+  if (b.key && !b.pedge && bshift >= 0 && M <= b.ids && M < ((int64_t)1 << 32))
+    return launch_partners_bucketed(a, RP, M, b, bshift, s);
   const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
   const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
   hipError_t e = launch_zero(b.c2, RP * kXcds, s);
@@ -852,14 +1130,9 @@ hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, 
   if ((e = hipGetLastError()) != hipSuccess) return e;
   partner_fill_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  partner_rows_count_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_scan(LoadI32{b.kc}, StoreI64{b.voff}, RP, b.part, s)) != hipSuccess) return e;
-  partner_rows_fill_kernel<<<blocks, kBlock, 0, s>>>(a, RP, b);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  if (a.P > 0) partner_work_kernel<<<(unsigned)((a.P * kBuildG + kBlock - 1) / kBlock), kBlock, 0, s>>>(a);
-  return hipGetLastError();
+  return launch_partner_rows(a, RP, b, s);
 }
+
 
 // ---------------------------------------------------------------------------
 // planning: bin every partner row, then scatter (graph, row) items by bin
@@ -1311,6 +1584,18 @@ hipError_t launch_zero(void* p, int64_t words32, hipStream_t s) {
   int64_t blocks = (words32 + kBlock - 1) / kBlock;
   if (blocks > 4096) blocks = 4096;
   zero_kernel<<<(unsigned)blocks, kBlock, 0, s>>>((uint32_t*)p, words32);
+  return hipGetLastError();
+}
+
+__global__ void rearm_kernel(uint32_t* counters, uint64_t* outs, int32_t nouts) {
+  const int i = threadIdx.x;
+  if (i < kNumBins) counters[kCntTileHead + i] = 0u;
+  if (i == 0) counters[kCntFallback] = 0u;
+  for (int j = i; j < nouts; j += blockDim.x) outs[j] = 0ull;
+}
+
+hipError_t launch_rearm(uint32_t* counters, uint64_t* outs, int32_t nouts, hipStream_t s) {
+  rearm_kernel<<<1, 64, 0, s>>>(counters, outs, nouts);
   return hipGetLastError();
 }
 

@@ -71,7 +71,8 @@ constexpr int kPartnersPerRow = TCH_PARTNERS_PER_ROW;
 //        take slots [pr_base, pr_base + nv_max + 1) of the launch's VRow and
 //        row-offset arrays; those slots are also the plan indices.
 //   Build: the graph's n_rows + 1 slots of a partner build's per-row
-//        scratch start at row_base (graphs in order, contiguous).
+//        scratch start at row_base, and its ids' slots of the build's
+//        per-id scratch at eb (graphs in order, contiguous).
 // List references (set and partners) and row offsets are absolute in their
 // pools, so planning and counting never read this descriptor.
 struct GraphDesc {
@@ -84,6 +85,7 @@ struct GraphDesc {
   int32_t nv_max;
   int32_t own_begin;
   int32_t own_end;
+  int64_t eb;
 };
 
 // Partner rows a graph of n rows and m DAG ids may need (it takes this
@@ -184,16 +186,48 @@ struct BuildBufs {
   int64_t* pedge;       // null, or [partner pool]: DAG edge position of each partner entry
   int32_t* pvid;        // with pedge: [partner pool] partner vertex of each entry
   int32_t* pown;        // with pedge: [P] owner vertex of each partner row
+  // Bucketed transpose (null when the scratch has no per-id part): per id
+  // of the build (M = sum of the graphs' ids, graph g's at eb):
+  int32_t* key;         // [M] owner row slot of an edge its head owns, else -1
+  ListRef* out;         // [M] out-edge partners of each row, compacted at its front
+  int32_t* pkey;        // [M] owner row slots of the in-owned edges, grouped by bucket
+  ListRef* pref;        // [M] their partners' lists, in the same order
+  unsigned long long* btot;  // [kMaxBuckets + 1] pairs per bucket, then bucket starts
+  unsigned long long* bcur;  // [kMaxBuckets] bucket fill cursors
+  int64_t* cstart;      // [kMaxBuckets + 1] first chunk of each bucket (scan of chunk counts)
+  int64_t ids;          // M capacity
 };
 int64_t scan_partials(int64_t n);
 
+// Bucketed transpose: owner row slots are bucketed by their high bits
+// (1 << bshift slots per bucket, at most kMaxBuckets buckets, at most
+// kMaxBucketSlots slots per bucket, so a bucket's counters fit LDS), and a
+// bucket's pairs are processed in chunks of at most kChunkPairs.
+constexpr int kMaxBuckets = 8192;
+constexpr int kMaxBucketShift = 14;
+constexpr int kMinBucketShift = 12;
+constexpr int64_t kChunkPairs = 65536;
+// Bucket shift of a build over RP row slots, or -1 when RP needs more than
+// kMaxBuckets buckets of 2^kMaxBucketShift slots (the build then takes the
+// per-edge atomic path).
+inline int bucket_shift(int64_t RP) {
+  int sh = kMinBucketShift;
+  while (sh <= kMaxBucketShift && (RP + ((int64_t)1 << sh) - 1) >> sh > kMaxBuckets) ++sh;
+  return sh <= kMaxBucketShift ? sh : -1;
+}
+
 // Builds the partner lists, rows and counts of every graph of `a` (descs,
 // rp, ci, P as for a count; pci/vrp/vrow/nv are written); RP = sum of
-// n_rows + 1 over the graphs.
-hipError_t launch_partners(const LaunchArgs& a, int64_t RP, const BuildBufs& b, hipStream_t s);
+// n_rows + 1 over the graphs, M = sum of their ids.  With the per-id
+// scratch (b.key) and no local outputs (b.pedge null) the in-edge partners
+// are placed by a bucketed transpose; otherwise by per-edge atomics.
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b, hipStream_t s);
 
 // Launchers (stream-ordered; return the launch status).
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);
+// Re-arms a kept plan for another count: tile queue heads and the fallback
+// counter to 0 (bin counts kept), outs[0..nouts) to 0.
+hipError_t launch_rearm(uint32_t* counters, uint64_t* outs, int32_t nouts, hipStream_t s);
 hipError_t launch_plan(const LaunchArgs& a, hipStream_t s);
 // One full-residency persistent launch per bin (grid > 0 overrides the size).
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s);

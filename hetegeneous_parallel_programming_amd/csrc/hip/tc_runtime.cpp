@@ -136,18 +136,25 @@ int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs
          4 * (int64_t)ngraphs;
 }
 
-static int64_t build_words(int64_t rp_slots) {
-  // c1, kc (int32), c2 (int32 per XCD stripe), cur (int64 per stripe) + off, voff (int64)
-  // + scan partials, 256-byte pieces
-  auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
-  return up(4 * rp_slots) * 2 + up(4 * tck::kXcds * rp_slots) + up(8 * tck::kXcds * rp_slots) +
-         up(8 * rp_slots) * 2 + up(8 * tck::scan_partials(rp_slots));
+// Piece sizes of a build scratch, in allocation order (256-byte pieces).
+static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+static int64_t build_bytes(int64_t n, int64_t ids) {
+  // per row: c1, kc (int32), c2 (int32 per XCD stripe), cur (int64 per
+  // stripe), off, voff (int64), scan partials; per id: key, pkey (int32),
+  // out, pref (list references); per bucket: totals, cursors, chunk starts
+  int64_t b = up256(4 * n) * 2 + up256(4 * tck::kXcds * n) + up256(8 * tck::kXcds * n) + up256(8 * n) * 2 +
+              up256(8 * tck::scan_partials(n));
+  if (ids > 0) b += up256(4 * ids) * 2 + up256(8 * ids) * 2 + up256(8 * (tck::kMaxBuckets + 1)) * 3;
+  return b;
 }
 
-int64_t BuildScratch::bytes_for(int64_t rp_slots) { return build_words(rp_slots); }
+int64_t BuildScratch::bytes_for(int64_t rp_slots, int64_t ids) {
+  return build_bytes(std::max<int64_t>(rp_slots, 1), ids);
+}
 
-tc_status BuildScratch::ensure(int64_t rp_slots) {
-  const int64_t need = build_words(std::max<int64_t>(rp_slots, 1));
+tc_status BuildScratch::ensure(int64_t rp_slots, int64_t ids) {
+  const int64_t n = std::max<int64_t>(rp_slots, 1);
+  const int64_t need = build_bytes(n, ids);
   if (need > cap) {
     dev_free(mem);
     mem = nullptr;
@@ -156,16 +163,25 @@ tc_status BuildScratch::ensure(int64_t rp_slots) {
     if (st != TC_OK) return st;
     cap = need;
   }
-  auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
-  const int64_t n = std::max<int64_t>(rp_slots, 1);
   char* p = mem;
-  b.c1 = (int32_t*)p;  p += up(4 * n);
-  b.c2 = (int32_t*)p;  p += up(4 * tck::kXcds * n);
-  b.cur = (unsigned long long*)p; p += up(8 * tck::kXcds * n);
-  b.kc = (int32_t*)p;  p += up(4 * n);
-  b.off = (int64_t*)p; p += up(8 * n);
-  b.voff = (int64_t*)p; p += up(8 * n);
-  b.part = (int64_t*)p;
+  b = tck::BuildBufs{};
+  b.c1 = (int32_t*)p;  p += up256(4 * n);
+  b.c2 = (int32_t*)p;  p += up256(4 * tck::kXcds * n);
+  b.cur = (unsigned long long*)p; p += up256(8 * tck::kXcds * n);
+  b.kc = (int32_t*)p;  p += up256(4 * n);
+  b.off = (int64_t*)p; p += up256(8 * n);
+  b.voff = (int64_t*)p; p += up256(8 * n);
+  b.part = (int64_t*)p; p += up256(8 * tck::scan_partials(n));
+  if (ids > 0) {
+    b.key = (int32_t*)p; p += up256(4 * ids);
+    b.pkey = (int32_t*)p; p += up256(4 * ids);
+    b.out = (tck::ListRef*)p; p += up256(8 * ids);
+    b.pref = (tck::ListRef*)p; p += up256(8 * ids);
+    b.btot = (unsigned long long*)p; p += up256(8 * (tck::kMaxBuckets + 1));
+    b.bcur = (unsigned long long*)p; p += up256(8 * (tck::kMaxBuckets + 1));
+    b.cstart = (int64_t*)p;
+    b.ids = ids;
+  }
   return TC_OK;
 }
 
@@ -174,14 +190,15 @@ void BuildScratch::release() {
   *this = BuildScratch();
 }
 
-tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, BuildScratch& bs, hipStream_t s,
+tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t M, BuildScratch& bs, hipStream_t s,
                        const LocalBufs* loc) {
-  tc_status st = bs.ensure(RP);
+  // the local rebuild writes edge positions beside the partners: atomic path
+  tc_status st = bs.ensure(RP, loc ? 0 : M);
   if (st != TC_OK) return st;
   bs.b.pedge = loc ? loc->pedge : nullptr;
   bs.b.pvid = loc ? loc->pvid : nullptr;
   bs.b.pown = loc ? loc->pown : nullptr;
-  TC_HIP_TRY(tck::launch_partners(a, RP, bs.b, s));
+  TC_HIP_TRY(tck::launch_partners(a, RP, M, bs.b, s));
   return TC_OK;
 }
 
@@ -266,7 +283,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
     cleanup();
     return st;
   }
-  const tck::GraphDesc d{0, 0, 0, 0, 0, g->n, g->nv_max, row_begin, row_end};
+  const tck::GraphDesc d{0, 0, 0, 0, 0, g->n, g->nv_max, row_begin, row_end, 0};
   hipError_t e = hipMemcpy(g->rp, dag->rowptr, sizeof(int64_t) * ((size_t)g->n + 1), hipMemcpyHostToDevice);
   if (e == hipSuccess && g->nnz > 0)
     e = hipMemcpy(g->ci, dag->colidx, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice);
@@ -289,7 +306,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   hipStream_t s = ctx->stream;
   e = hipEventRecord(ctx->ev[0], s);
   if (e == hipSuccess) {
-    st = launch_build(a, (int64_t)g->n + 1, bs, s);
+    st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, s);
     if (st == TC_OK) e = hipEventRecord(ctx->ev[1], s);
     if (st == TC_OK && e == hipSuccess)
       e = hipMemcpyAsync(ctx->pinned, g->partners.nv, sizeof(int32_t), hipMemcpyDeviceToHost, s);
@@ -482,6 +499,7 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) try {
   dev_free(g->own);
   g->partners.release();
   g->local.release();
+  g->plan.release();
   delete g;
   return TC_OK;
 } catch (...) {
@@ -530,42 +548,61 @@ static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
   return a;
 }
 
-tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
-  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
-  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  DeviceGuard guard(ctx->device);
+}  // extern "C"
+
+namespace tcr {
+
+// The binned count.  Its plan (rows per bin, bin-major item lists) depends
+// only on the graph's partner rows and the launch setting, so it is built
+// on the first call and kept in the graph: later calls re-arm the tile
+// queues and launch the sized bins directly -- no planning kernels and no
+// host round trip before the count.
+static tc_status count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   const double t0 = now_ms();
   const int64_t P = (int64_t)g->nv_max + 1;
-  tc_status st = ctx->scratch.ensure(P, 1);
-  if (st != TC_OK) return st;
-  const tck::LaunchArgs a = count_args(ctx, g);
+  const bool kept = g->plan_valid && std::memcmp(&g->plan_tune, &ctx->tune, sizeof(tc_tune)) == 0;
+  if (!kept) {
+    const int64_t had = g->plan.cap;
+    tc_status st = g->plan.ensure(P, 1);
+    if (st != TC_OK) return st;
+    if (g->plan.cap != had) g->bytes += PlanScratch::bytes_for(g->plan.cap) - (had ? PlanScratch::bytes_for(had) : 0);
+  }
+  tck::LaunchArgs a = count_args(ctx, g);
+  a.bufs = g->plan.bufs();
   hipStream_t s = ctx->stream;
   TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
-  TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
-  TC_HIP_TRY(tck::launch_plan(a, s));
-  TC_HIP_TRY(hipEventRecord(ctx->ev[3], s));
-  // The plan's rows per bin size each bin's launch: empty bins are skipped
-  // and small ones get no more workgroups than tiles (a small graph would
-  // otherwise pay for six full-residency grids).
-  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,
-                            hipMemcpyDeviceToHost, s));
-  TC_HIP_TRY(hipStreamSynchronize(s));
-  int64_t tiles[tck::kNumBins];
-  const uint32_t* c = (const uint32_t*)(ctx->pinned + 8);
-  for (int b = 0; b < tck::kNumBins; ++b) {
-    const int64_t piv = tck::tile_pivots(b, ctx->tune.rounds > 0 ? ctx->tune.rounds : 1);
-    tiles[b] = ((int64_t)c[tck::kCntBinCount + b] + piv - 1) / piv;
+  if (kept) {
+    TC_HIP_TRY(tck::launch_rearm(a.bufs.counters, a.bufs.outs, 2, s));
+    TC_HIP_TRY(hipEventRecord(ctx->ev[3], s));
+  } else {
+    g->plan_valid = false;
+    TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
+    TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
+    TC_HIP_TRY(tck::launch_plan(a, s));
+    TC_HIP_TRY(hipEventRecord(ctx->ev[3], s));
+    // The plan's rows per bin size each bin's launch: empty bins are skipped
+    // and small ones get no more workgroups than tiles (a small graph would
+    // otherwise pay for six full-residency grids).
+    TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,
+                              hipMemcpyDeviceToHost, s));
+    TC_HIP_TRY(hipStreamSynchronize(s));
+    const uint32_t* c = (const uint32_t*)(ctx->pinned + 8);
+    for (int b = 0; b < tck::kNumBins; ++b) {
+      const int64_t piv = tck::tile_pivots(b, ctx->tune.rounds > 0 ? ctx->tune.rounds : 1);
+      g->plan_bins[b] = c[tck::kCntBinCount + b];
+      g->plan_tiles[b] = (g->plan_bins[b] + piv - 1) / piv;
+    }
+    g->plan_tune = ctx->tune;
+    g->plan_valid = true;
   }
+  const int64_t* tiles = g->plan_tiles;
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  // The bins run as groups on separate streams (default priority; the host
-  // has synchronised after planning): the global-list and workgroup bins,
-  // the wave bin, the 32-lane bin, and the 16- and 8-lane bins.  Each group's persistent
-  // workgroups hold their CUs until its queues drain; the dispatcher
-  // interleaves the groups' workgroups from the start and the others take
-  // over the CUs a group's tail frees.  One non-empty group runs on the
-  // library stream alone.
+  // The bins run as groups on separate streams (default priority): the
+  // global-list and workgroup bins, the wave bin, the 32-lane bin, and the
+  // 16- and 8-lane bins.  Each group's persistent workgroups hold their CUs
+  // until its queues drain; the dispatcher interleaves the groups'
+  // workgroups from the start and the others take over the CUs a group's
+  // tail frees.  One non-empty group runs on the library stream alone.
   constexpr int kGroups = tc_ctx::kSides + 1;
   int64_t grp[kGroups][tck::kNumBins] = {};
   for (int b = 0; b < tck::kNumBins; ++b) grp[bin_group(b)][b] = tiles[b];
@@ -597,16 +634,23 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) t
                             hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
-  return finish_stats(ctx, stats, t0, true, g->nv, 3, *(const uint32_t*)(ctx->pinned + kPinnedFallback));
-} catch (...) {
-  return tcr::caught();
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->total_ms = now_ms() - t0;
+    float ms = 0;
+    TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+    stats->plan_ms = ms;
+    TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+    stats->count_ms = ms;
+    stats->pivots = g->nv;
+    for (int b = 0; b < TC_NUM_BINS; ++b) stats->bin_pivots[b] = g->plan_bins[b];
+    stats->hash_fallbacks = *(const uint32_t*)(ctx->pinned + kPinnedFallback);
+    stats->plan_kept = kept ? 1 : 0;
+  }
+  return TC_OK;
 }
 
-tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
-  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
-  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  DeviceGuard guard(ctx->device);
+static tc_status count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
   const double t0 = now_ms();
   const int64_t P = (int64_t)g->row_end - g->row_begin;
   tc_status st = ctx->scratch.ensure(0, 1);
@@ -621,7 +665,51 @@ tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* st
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->scratch.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
-  return finish_stats(ctx, stats, t0, false, P);
+  st = finish_stats(ctx, stats, t0, false, P);
+  if (st == TC_OK && stats) stats->single_launch = 1;
+  return st;
+}
+
+// Graphs whose counted rows hold at most this many DAG ids, and whose lists
+// all fit a wave's LDS tree, are counted by the single plain launch in
+// tc_count: below this size its one launch beats planning and the binned
+// kernels' launches (measured on MI355X).
+constexpr int64_t kPlainMaxIds = (int64_t)1 << 18;
+
+static bool small_graph(const tc_graph* g) {
+  return g->edge_end - g->edge_begin <= kPlainMaxIds && g->max_out < tck::kCapWave;
+}
+
+}  // namespace tcr
+
+extern "C" {
+
+tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
+  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  return small_graph(g) ? count_plain(ctx, g, total, stats) : count_binned(ctx, g, total, stats);
+} catch (...) {
+  return tcr::caught();
+}
+
+tc_status tc_count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
+  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  return count_binned(ctx, g, total, stats);
+} catch (...) {
+  return tcr::caught();
+}
+
+tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
+  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  return count_plain(ctx, g, total, stats);
 } catch (...) {
   return tcr::caught();
 }
@@ -644,7 +732,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
   if (st == TC_OK) {
     tck::LaunchArgs a = count_args(ctx, g);
     BuildScratch bs;
-    st = launch_build(a, (int64_t)g->n + 1, bs, ctx->stream, &L);
+    st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L);
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     bs.release();
     if (st == TC_OK && e != hipSuccess)
@@ -654,6 +742,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
     L.release();
     return st;
   }
+  g->plan_valid = false;  // the rebuild rewrote the partner rows the kept plan lists
   L.bytes = 16 * m + 4 * P + 8 * ((int64_t)g->n + 1) + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
   return TC_OK;
 }

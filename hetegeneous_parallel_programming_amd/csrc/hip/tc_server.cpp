@@ -22,6 +22,7 @@ using namespace tcr;
 namespace {
 
 constexpr uint64_t kAlign = 256;
+constexpr int64_t kDefaultBatchIds = (int64_t)32 << 20;  // max_batch_ids = 0
 inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 // A request's extent in the HBM pool: rowptr | colidx | partner list
@@ -115,6 +116,7 @@ struct tc_server {
   std::deque<Req*> waiting, ready;
   std::deque<tc_request_result> results;
   int64_t outstanding = 0, submitted = 0, completed = 0, rejected = 0, launches = 0, launched_graphs = 0;
+  int64_t max_ids = 0;  // DAG ids per launch (cfg.max_batch_ids or its default)
   bool stop = false;
   std::thread th;
 
@@ -244,24 +246,26 @@ bool tc_server::launch() {
   bool any = false;
   for (auto& s : slots) {
     if (s.busy || ready.empty()) continue;
-    int64_t rows = 0;
+    int64_t rows = 0, ids = 0;
     while (!ready.empty() && (int)s.reqs.size() < cfg.max_batch_graphs &&
-           rows + ready.front()->nv_max + 1 <= cfg.max_batch_rows) {
+           rows + ready.front()->nv_max + 1 <= cfg.max_batch_rows && ids + ready.front()->nnz <= max_ids) {
       rows += ready.front()->nv_max + 1;
+      ids += ready.front()->nnz;
       s.reqs.push_back(ready.front());
       ready.pop_front();
     }
     if (s.reqs.empty()) continue;
     const int k = (int)s.reqs.size();
     tck::GraphDesc* hd = (tck::GraphDesc*)(s.pinned + cfg.max_batch_graphs);
-    int64_t pb = 0, rb = 0;
+    int64_t pb = 0, rb = 0, eb = 0;
     for (int i = 0; i < k; ++i) {
       Req* r = s.reqs[i];
       const Layout L(r->n, r->nnz);
       hd[i] = tck::GraphDesc{(int64_t)(r->off / sizeof(int64_t)), (int64_t)((r->off + L.ci) / sizeof(int32_t)),
-                             (int64_t)((r->off + L.pci) / sizeof(tck::ListRef)), rb, pb, r->n, r->nv_max, 0, r->n};
+                             (int64_t)((r->off + L.pci) / sizeof(tck::ListRef)), rb, pb, r->n, r->nv_max, 0, r->n, eb};
       pb += (int64_t)r->nv_max + 1;
       rb += (int64_t)r->n + 1;
+      eb += r->nnz;
     }
     tck::LaunchArgs a{};
     a.descs = s.d_descs;
@@ -291,8 +295,9 @@ bool tc_server::launch() {
     if (e == hipSuccess)
       e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
     // Partner structures of this launch's graphs, then plan and count.
-    if (e == hipSuccess) e = s.build.ensure(rb) == TC_OK ? tck::launch_partners(a, rb, s.build.b, s.stream)
-                                                         : hipErrorOutOfMemory;
+    if (e == hipSuccess)
+      e = s.build.ensure(rb, max_ids) == TC_OK ? tck::launch_partners(a, rb, eb, s.build.b, s.stream)
+                                              : hipErrorOutOfMemory;
     // A batch of one graph would take the single-graph reduction path.
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s.stream);
     if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)k, s.stream);
@@ -387,9 +392,11 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
     return fail(TC_EINVAL, "batch limits and queue size must be positive");
   if (cfg->max_batch_rows >= ((int64_t)1 << 31)) return fail(TC_ERANGE, "max_batch_rows must be < 2^31");
   if (cfg->hbm_budget < kAlign) return fail(TC_EINVAL, "hbm_budget too small");
+  if (cfg->max_batch_ids < 0) return fail(TC_EINVAL, "max_batch_ids must be >= 0");
   std::unique_ptr<tc_server> s(new tc_server);
   s->ctx = ctx;
   s->cfg = *cfg;
+  s->max_ids = cfg->max_batch_ids > 0 ? cfg->max_batch_ids : kDefaultBatchIds;
   DeviceGuard guard(ctx->device);
   tc_status st = dev_alloc((void**)&s->pool, cfg->hbm_budget, "server pool");
   if (st != TC_OK) return st;
@@ -428,7 +435,7 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
       return fail(TC_EDEVICE, std::string("server slot: ") + hipGetErrorString(e));
     }
     st = dev_alloc(&sl.plan, total, "slot plan scratch");
-    if (st == TC_OK) st = sl.build.ensure(R);
+    if (st == TC_OK) st = sl.build.ensure(R, s->max_ids);
     if (st != TC_OK) {
       s->release();
       return st;
@@ -469,6 +476,7 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
   const int32_t nvm = tck::partner_rows_max(dag->n_rows, dag->nnz);
   if ((int64_t)nvm + 1 > s->cfg.max_batch_rows)
     return reject(TC_ERANGE, "request needs more plan rows than max_batch_rows (rows + nnz/1024 + 2)");
+  if (dag->nnz > s->max_ids) return reject(TC_ERANGE, "request has more DAG ids than max_batch_ids");
   for (int64_t v = 0; v < dag->n_rows; ++v)
     if ((dag->rowptr[v + 1] - dag->rowptr[v]) * 64 >= ((int64_t)1 << 31))
       return reject(TC_ERANGE, "DAG out-degree too large");

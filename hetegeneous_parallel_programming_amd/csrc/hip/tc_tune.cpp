@@ -39,7 +39,7 @@ tc_status evaluate(Workload& w, int32_t reps, double* best_ms) {
   for (int32_t r = -1; r < reps; ++r) {
     tc_stats st;
     const tc_status s = w.b ? tc_batch_count(w.ctx, w.b, w.got.data(), &st)
-                            : tc_count(w.ctx, w.g, w.got.data(), &st);
+                            : tc_count_binned(w.ctx, w.g, w.got.data(), &st);
     if (s != TC_OK) return s;
     if (!w.have_want) {
       w.want = w.got;

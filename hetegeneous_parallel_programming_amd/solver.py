@@ -25,13 +25,14 @@ BIN_NAMES = ("global", "workgroup", "wave", "g32", "g16", "g8")
 class Stats(ctypes.Structure):
     _fields_ = [("total_ms", nat.c_dbl), ("plan_ms", nat.c_dbl), ("count_ms", nat.c_dbl),
                 ("pivots", nat.c_i64), ("bin_pivots", nat.c_i64 * NUM_BINS),
-                ("hash_fallbacks", nat.c_i64)]
+                ("hash_fallbacks", nat.c_i64), ("single_launch", nat.c_i64), ("plan_kept", nat.c_i64)]
 
     def as_dict(self) -> dict:
         return {"total_ms": self.total_ms, "plan_ms": self.plan_ms, "count_ms": self.count_ms,
                 "pivots": self.pivots,
                 "bin_pivots": dict(zip(BIN_NAMES, list(self.bin_pivots))),
-                "hash_fallbacks": self.hash_fallbacks}
+                "hash_fallbacks": self.hash_fallbacks, "single_launch": bool(self.single_launch),
+                "plan_kept": bool(self.plan_kept)}
 
 
 class Tune(ctypes.Structure):
@@ -74,7 +75,7 @@ class AutotuneReport(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", nat.c_u64), ("slots", nat.c_i32), ("max_batch_graphs", nat.c_i32),
                 ("max_batch_rows", nat.c_i64), ("max_queue", nat.c_i32), ("lookahead", nat.c_i32),
-                ("host_staging", nat.c_u64)]
+                ("host_staging", nat.c_u64), ("max_batch_ids", nat.c_i64)]
 
 
 class RequestResult(ctypes.Structure):
@@ -119,6 +120,7 @@ def device_lib():
     S(L, "tc_graph_bytes", nat.c_i64, [P])
     S(L, "tc_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_plain", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
+    S(L, "tc_count_binned", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_local", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Local), ctypes.POINTER(Stats)])
     S(L, "tc_batch_create", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_i32, nat.c_u32, PP])
     S(L, "tc_batch_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
@@ -292,9 +294,17 @@ class Context:
         return DeviceGraph(self, h.value, p.row_end - p.row_begin, int(p.local_colidx.size))
 
     def count(self, dg: DeviceGraph):
+        """tc_count: the binned count, or the single plain launch for small graphs."""
         total = nat.c_u64(0)
         st = Stats()
         _check(device_lib().tc_count(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
+        return int(total.value), st.as_dict()
+
+    def count_binned(self, dg: DeviceGraph):
+        """The degree-binned count whatever the graph's size (tc_count_binned)."""
+        total = nat.c_u64(0)
+        st = Stats()
+        _check(device_lib().tc_count_binned(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
         return int(total.value), st.as_dict()
 
     def count_plain(self, dg: DeviceGraph):
@@ -411,9 +421,10 @@ class Server:
 
     def __init__(self, ctx: Context, hbm_budget: int = 8 << 30, slots: int = 3,
                  max_batch_graphs: int = 1024, max_batch_rows: int = 8 << 20,
-                 max_queue: int = 1 << 16, lookahead: int = 64, host_staging: int = 1 << 30):
+                 max_queue: int = 1 << 16, lookahead: int = 64, host_staging: int = 1 << 30,
+                 max_batch_ids: int = 0):
         cfg = ServerConfig(hbm_budget, slots, max_batch_graphs, max_batch_rows, max_queue, lookahead,
-                           host_staging)
+                           host_staging, max_batch_ids)
         h = ctypes.c_void_p()
         _check(device_lib().tc_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
         self.ctx = ctx

@@ -60,6 +60,8 @@ typedef struct tc_stats {
                                       32-lane, 16-lane, 8-lane                      */
   int64_t hash_fallbacks;     /* pivot sets whose LDS hash build did not converge and
                                  fell back to the search tree (binned counts)       */
+  int64_t single_launch;      /* 1: counted by the single plain launch                */
+  int64_t plan_kept;          /* 1: the binned count reused the graph's kept plan      */
 } tc_stats;
 
 /* ---- context ------------------------------------------------------------
@@ -123,7 +125,16 @@ tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out);
  * (successful_push_pull/multigpu_nvshmem_pushpull.cu:69-83) around
  * warp_vertex (bs_warp_vertex.cuh:2-35). */
 tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
-/* tc_count_plain: one launch of the plain kernel, one wave64 per pivot,
+/* tc_count_binned: always the degree-binned count (tc_count takes it for all
+ * but small graphs; see below).  The binned count's plan is built on a
+ * graph's first binned count and kept in the graph (device bytes: 10 per
+ * partner-row slot) for later counts under the same tc_tune. */
+tc_status tc_count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
+/* tc_count on a graph whose counted rows hold at most 2^18 DAG ids, with
+ * every list under 1024 ids, runs tc_count_plain's single launch instead
+ * (stats->single_launch = 1): at that size one launch is faster than
+ * planning plus the binned kernels' launches.  Both are exact.
+ * tc_count_plain: one launch of the plain kernel, one wave64 per pivot,
  * grid-stride over pivots at full residency -- the direct restatement of
  * warp_vertex (bs_warp_vertex.cuh:2-35, grid a17 of
  * multigpu_nvshmem_pushpull.cu:41-51). */
@@ -222,6 +233,8 @@ typedef struct tc_server_config {
   int32_t lookahead;         /* requests that may be admitted past a blocked head */
   uint64_t host_staging;     /* bytes of pinned host memory submit copies requests
                                 into (async DMA to HBM); 0 = pageable copies     */
+  int64_t max_batch_ids;     /* DAG ids per launch (sizes per-slot build scratch,
+                                24 bytes per id); 0 = 32 Mi ids                  */
 } tc_server_config;
 
 typedef struct tc_request_result {
@@ -245,7 +258,7 @@ uint64_t tc_request_footprint(int32_t n_rows, int64_t nnz);
 tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out);
 /* Copies the DAG (caller keeps ownership).  TC_EBUSY: queue full;
  * TC_ENOMEM: footprint exceeds the whole pool; TC_ERANGE: more rows than
- * max_batch_rows. */
+ * max_batch_rows or more ids than max_batch_ids. */
 tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check);
 /* tc_server_submit at a scheduled time: waits (sleeping, then spinning for
  * the last ~2 ms) until tc_clock_ms() >= t_ms, then submits.  For clients

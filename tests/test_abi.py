@@ -27,7 +27,7 @@ def test_headers_declare_the_expected_surface():
     host = declared_functions("tc_host.h")
     dev = declared_functions("tc.h")
     assert {"tch_gen_rmat", "tch_orient", "tch_partition_range", "tch_read_graphminer"} <= host
-    assert {"tc_ctx_create", "tc_graph_upload", "tc_count", "tc_count_plain",
+    assert {"tc_ctx_create", "tc_graph_upload", "tc_count", "tc_count_plain", "tc_count_binned",
             "tc_partition_upload", "tc_batch_create", "tc_batch_count"} <= dev
 
 

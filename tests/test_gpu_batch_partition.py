@@ -52,7 +52,7 @@ def test_batch_interleaved_with_single_counts(ctx):
     single = ctx.upload(graphs[3])
     for _ in range(3):
         assert np.array_equal(b.count()[0], want)
-        assert ctx.count(single)[0] == want[3]
+        assert ctx.count_binned(single)[0] == want[3]
     single.free()
     b.free()
 
@@ -72,7 +72,7 @@ def test_partition_halo_counts(ctx, k, balance):
                                     part.halo_colidx)
         dg = ctx.upload_partition(part)
         assert dg.info()["partner_rows"] == part.work()["partner_rows"]
-        got, st = ctx.count(dg)
+        got, st = ctx.count_binned(dg)
         assert got == want, (p, got, want)
         assert ctx.count_plain(dg)[0] == want
         dg.free()
@@ -96,7 +96,7 @@ def test_metis_parts_count_their_own_rows(ctx, tmp_path):
         sub, owned = assemble_metis(pes, i)
         want = oracle.tc_count_rows(rp, ci, pe.vertex_list)
         dg = ctx.upload(sub, owned=owned)
-        assert ctx.count(dg)[0] == want
+        assert ctx.count_binned(dg)[0] == want
         assert ctx.count_plain(dg)[0] == want
         total, out = ctx.count_local(dg, edges=False)
         assert total == want
@@ -112,15 +112,15 @@ def test_owned_mask_edge_cases(ctx):
     rp, ci = g.out_rowptr(), g.out_colidx()
     none = np.zeros(g.V(), np.uint8)
     dg = ctx.upload(g, owned=none)
-    assert ctx.count(dg)[0] == 0 and ctx.count_plain(dg)[0] == 0
+    assert ctx.count_binned(dg)[0] == 0 and ctx.count_plain(dg)[0] == 0
     total, out = ctx.count_local(dg)
     assert total == 0 and not out["vertex_counts"].any() and not out["edge_counts"].any()
     dg.free()
     every = np.ones(g.V(), np.uint8)
     dg = ctx.upload(g, owned=every)
-    assert ctx.count(dg)[0] == oracle.tc_count(rp, ci)
+    assert ctx.count_binned(dg)[0] == oracle.tc_count(rp, ci)
     dg.free()
     odd = (np.arange(g.V()) % 2).astype(np.uint8)
     dg = ctx.upload(g, owned=odd)
-    assert ctx.count(dg)[0] == oracle.tc_count_rows(rp, ci, np.flatnonzero(odd))
+    assert ctx.count_binned(dg)[0] == oracle.tc_count_rows(rp, ci, np.flatnonzero(odd))
     dg.free()

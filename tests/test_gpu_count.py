@@ -19,10 +19,16 @@ pytestmark = pytest.mark.gpu
 
 
 def _both(ctx, g):
+    """Binned and plain counts of g (and tc_count, which takes one of them
+    by size, must agree with both)."""
     dg = ctx.upload(g)
-    a, st = ctx.count(dg)
+    a, st = ctx.count_binned(dg)
     b, _ = ctx.count_plain(dg)
+    c, st_c = ctx.count(dg)
+    a2, st2 = ctx.count_binned(dg)  # the kept plan
     dg.free()
+    assert c == a and a2 == a, (a, c, a2)
+    assert st2["plan_kept"] and st2["bin_pivots"] == st["bin_pivots"]
     return a, b, st
 
 
@@ -40,7 +46,7 @@ def test_er_16384_d16(ctx, seed):
 
 
 @pytest.mark.parametrize("n,ef,seed", [(1 << 12, 8, 1), (1 << 14, 16, 2), (1 << 16, 16, 3),
-                                       (50000, 24, 4), (1 << 17, 8, 5)])
+                                       (50000, 24, 4), (1 << 17, 8, 5), (1 << 18, 16, 6)])
 def test_rmat(ctx, n, ef, seed):
     g = Graph.rmat(n, ef, seed).oriented()
     want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
@@ -61,7 +67,7 @@ def test_clique_beyond_lds_slot(ctx):
     n = 4200
     s, d = _clique_pairs(n)
     g = Graph.from_pairs(n, s, d).oriented()
-    a, st = ctx.count(ctx.upload(g))
+    a, st = ctx.count_binned(ctx.upload(g))
     assert a == math.comb(n, 3)
     assert st["bin_pivots"]["global"] > 0 and st["bin_pivots"]["workgroup"] > 0
 
@@ -105,7 +111,8 @@ def test_disjoint_cliques_all_bins(ctx):
 def test_repeat_is_stable(ctx):
     g = Graph.rmat(1 << 15, 16, 9).oriented()
     dg = ctx.upload(g)
-    vals = {ctx.count(dg)[0] for _ in range(5)} | {ctx.count_plain(dg)[0] for _ in range(3)}
+    vals = ({ctx.count(dg)[0] for _ in range(3)} | {ctx.count_binned(dg)[0] for _ in range(3)}
+            | {ctx.count_plain(dg)[0] for _ in range(3)})
     dg.free()
     assert vals == {oracle.tc_count(g.out_rowptr(), g.out_colidx())}
 
@@ -120,7 +127,7 @@ def test_tune_changes_schedule_not_result(ctx):
                   {"t_g8": 1 << 20, "t_g16": 1 << 20, "t_g32": 1 << 20, "t_wave": 1 << 20},
                   {"t_g8": 64, "t_g16": 64, "t_g32": 512, "t_wave": 4096, "rounds": 1, "grid": 7}):
             ctx.tune = t
-            assert ctx.count(dg)[0] == want, t
+            assert ctx.count_binned(dg)[0] == want, t
     finally:
         ctx.tune = saved
     dg.free()
@@ -156,8 +163,8 @@ def test_pivot_set_layouts_same_count(ctx, layout):
     try:
         ctx.tune = layout
         dcl, drm = ctx.upload(cl), ctx.upload(rm)
-        assert ctx.count(dcl)[0] == sum(math.comb(k, 3) for k in sizes)
-        assert ctx.count(drm)[0] == oracle.tc_count(rm.out_rowptr(), rm.out_colidx())
+        assert ctx.count_binned(dcl)[0] == sum(math.comb(k, 3) for k in sizes)
+        assert ctx.count_binned(drm)[0] == oracle.tc_count(rm.out_rowptr(), rm.out_colidx())
         dcl.free()
         drm.free()
         graphs = Graph.batch(5, 0, 48)
@@ -237,10 +244,28 @@ def test_ids_beyond_2_24_in_hash_sets(ctx):
     old = ctx.tune
     try:
         ctx.tune = {"pivot_set": 0}  # hash sets wherever they fit
-        got, st = ctx.count(dg)
+        got, st = ctx.count_binned(dg)
         got_plain, _ = ctx.count_plain(dg)
     finally:
         ctx.tune = old
         dg.free()
     assert got == want and got_plain == want, (got, got_plain, want, st)
     assert st["hash_fallbacks"] == 0, st
+
+
+def test_tc_count_picks_the_single_launch_for_small_graphs(ctx):
+    """tc_count runs the plain kernel's single launch for a graph of at most
+    2^18 DAG ids (ER n=16384 d=16: ~131k) and the binned count above that,
+    whose plan it keeps after the first call; every count is the oracle's."""
+    small = Graph.er(16384, 16, 1).oriented()
+    big = Graph.rmat(1 << 16, 16, 3).oriented()
+    assert small.E() <= 1 << 18 < big.E()
+    for g, single in ((small, True), (big, False)):
+        want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
+        dg = ctx.upload(g)
+        (c1, st1), (c2, st2) = ctx.count(dg), ctx.count(dg)
+        dg.free()
+        assert c1 == c2 == want
+        assert st1["single_launch"] == single and st2["single_launch"] == single
+        if not single:
+            assert not st1["plan_kept"] and st2["plan_kept"]

@@ -37,7 +37,7 @@ def _check_whole(ctx, g, bounds=None):
     if bounds is not None:
         assert np.array_equal(out["part_counts"], oracle.tc_pushpull_bounds(rp, ci, bounds))
     # the plain count still agrees after the local call rebuilt the partner entries
-    assert ctx.count(dg)[0] == total
+    assert ctx.count_binned(dg)[0] == total
     dg.free()
     return out
 

@@ -44,7 +44,7 @@ def test_autotune_graph_keeps_count(ctx, restore_tune):
     dg = ctx.upload(g)
     t, rep = ctx.autotune(graph=dg, reps=1, budget_ms=3000)
     assert ordered(t)
-    assert ctx.count(dg)[0] == want
+    assert ctx.count_binned(dg)[0] == want
     dg.free()
 
 

@@ -268,7 +268,9 @@ def leg_single(ctx, g, steps, warmup, name, plain=True, local=False):
                      "edges_per_s": work["edges"] / (ms_total / steps / 1e3),
                      "alg_bytes": alg[kind], "alg_GBps": alg[kind] / (k_avg / 1e3) / 1e9,
                      "edge_iterator_GBps": work["alg_bytes"] / (k_avg / 1e3) / 1e9,
-                     "bins": outs[-1][1]["bin_pivots"]}
+                     "bins": outs[-1][1]["bin_pivots"],
+                     "hash_fallbacks": outs[-1][1]["hash_fallbacks"],
+                     "single_launch": outs[-1][1]["single_launch"]}
         log(f"{name} {kind}: count={res[kind]['count']} step={res[kind]['ms_per_step']:.3f} ms "
             f"kernel={k_avg:.3f} ms alg={res[kind]['alg_GBps']:.0f} GB/s")
     if local:

@@ -109,9 +109,11 @@ constexpr int kInFlightGroup = TC_IN_FLIGHT_GROUP;
 //   slots (8 bytes) per bucket; id w may sit only in bucket h0(w) of table 0
 //   or bucket h1(w) of table 1, so a lookup is two 8-byte LDS reads and four
 //   compares -- a fixed cost, independent of d+(u), with no data-dependent
-//   branch.  h0/h1 are the top log2(B) bits of w * A0 / w * A1 modulo 2^32
-//   with 24-bit odd multipliers (one full-rate v_mul_u32_u24 each).  Empty
-//   slots hold -1, which is never an id.
+//   branch.  h0/h1 are the top log2(B) bits of w * A0 / w * A1 modulo 2^24
+//   (one full-rate v_mul_u32_u24 and one v_bfe_u32 each): multiplicative
+//   (Fibonacci) hashing with A / 2^24 near 1/phi and 1/sqrt(2), so ids that
+//   differ by a few land far apart -- neighbour lists often hold runs of
+//   nearby ids.  Empty slots hold -1, which is never an id.
 // kSetTree: the Eytzinger tree above, used when the hash does not fit the
 //   group's LDS area or its construction did not converge.
 // kSetHbm: the sorted list in HBM, binary-searched (pivots whose list does
@@ -126,7 +128,7 @@ constexpr int kModeHash = 1 << kSetHash, kModeTree = 1 << kSetTree, kModeHbm = 1
 struct PivotSet {
   const int32_t* A;  // LDS slot area (hash tables or tree) or the HBM list
   int mode;
-  int p0;            // hash: 32 - log2(B); tree: levels; hbm: list length
+  int p0;            // hash: log2(B); tree: levels; hbm: list length
   int p1;            // hash: byte offset of table 1 (8B)
 };
 
@@ -136,11 +138,12 @@ struct PivotSet {
 typedef __attribute__((address_space(1))) const int32_t gint32;
 
 constexpr int32_t kEmpty = -1;
-constexpr uint32_t kHashA0 = 0x9E3779u;  // odd 24-bit multipliers
-constexpr uint32_t kHashA1 = 0x7F4A7Du;
+constexpr uint32_t kHashA0 = 0x9E3779u;  // odd 24-bit multipliers: 2^24 / phi,
+constexpr uint32_t kHashA1 = 0xB504F3u;  // 2^24 / sqrt(2)
 
-__device__ __forceinline__ uint32_t hash_bucket(int w, uint32_t mul, int sh) {
-  return (uint32_t)__umul24((uint32_t)w, mul) >> sh;
+// Bucket of key w in a table of 2^lb buckets: bits [24 - lb, 24) of w * mul.
+__device__ __forceinline__ uint32_t hash_bucket(int w, uint32_t mul, int lb) {
+  return __builtin_amdgcn_ubfe((uint32_t)__umul24((uint32_t)w, mul), (uint32_t)(24 - lb), (uint32_t)lb);
 }
 // The key hashed for id w: w itself, or with its bits 24..30 folded into the
 // low bits (a bijection on ids, so distinct ids stay distinct keys).
@@ -167,16 +170,15 @@ __device__ __forceinline__ int hash_log2_buckets(int da, int area) {
 // `kicks` moves (the set is then incomplete and must not be used).
 template <bool kWide>
 __device__ __forceinline__ bool hash_insert(int32_t* T, int lb, int key, int kicks) {
-  const int sh = 32 - lb;
   const int B = 1 << lb;
   int t = 0;
   for (int it = 0; it < kicks; ++it) {
     const int hk = hash_key<kWide>(key);
-    int32_t* b = T + (t ? 2 * B : 0) + 2 * (int)hash_bucket(hk, t ? kHashA1 : kHashA0, sh);
+    int32_t* b = T + (t ? 2 * B : 0) + 2 * (int)hash_bucket(hk, t ? kHashA1 : kHashA0, lb);
     if (atomicCAS(&b[0], kEmpty, key) == kEmpty) return true;
     if (atomicCAS(&b[1], kEmpty, key) == kEmpty) return true;
     if (it == 0) {  // the other table's bucket before evicting anyone
-      int32_t* c = T + 2 * B + 2 * (int)hash_bucket(hk, kHashA1, sh);
+      int32_t* c = T + 2 * B + 2 * (int)hash_bucket(hk, kHashA1, lb);
       if (atomicCAS(&c[0], kEmpty, key) == kEmpty) return true;
       if (atomicCAS(&c[1], kEmpty, key) == kEmpty) return true;
     }
@@ -227,12 +229,12 @@ __device__ __forceinline__ void group_sync() {
 // then fit the tree cap).
 // kLocal: C (area / 2 words: a 16-bit hit counter per slot word, two per
 // word) gets zeroed over the slots the set uses.
-// wide: ids reach 2^24 (kSetHashWide).  fallbacks: counts the groups whose
+// kWide: ids reach 2^24 (kSetHashWide).  fallbacks: counts the groups whose
 // hash build did not converge (one increment per fallen-back set).
-template <int G, bool kLocal = false>
+template <int G, bool kLocal, bool kWide>
 __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
-                                                    int da, int gl, const Tune& tune, bool wide,
-                                                    uint32_t* fallbacks, uint32_t* C = nullptr) {
+                                                    int da, int gl, const Tune& tune, uint32_t* fallbacks,
+                                                    uint32_t* C = nullptr) {
   PivotSet S{slot, kSetTree, 0, 0};
   // tree_only: 1 = every set a tree; 2 = trees for sub-wave groups only
   const bool tree = tune.tree_only == 1 || (tune.tree_only == 2 && G < 64);
@@ -246,13 +248,10 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
     }
     group_sync<G>();
     bool ok = true;
-    if (wide)
-      for (int j = gl; j < da; j += G) ok = hash_insert<true>(slot, lb, src[j], tune.hash_kicks) && ok;
-    else
-      for (int j = gl; j < da; j += G) ok = hash_insert<false>(slot, lb, src[j], tune.hash_kicks) && ok;
+    for (int j = gl; j < da; j += G) ok = hash_insert<kWide>(slot, lb, src[j], tune.hash_kicks) && ok;
     if (group_all<G>(ok)) {
-      S.mode = wide ? kSetHashWide : kSetHash;
-      S.p0 = 32 - lb;
+      S.mode = kWide ? kSetHashWide : kSetHash;
+      S.p0 = lb;
       S.p1 = 8 * B;
       return S;
     }
@@ -291,7 +290,7 @@ __device__ __forceinline__ void count_slot_hit(uint32_t* C, int idx) {
 template <int G>
 __device__ __forceinline__ void flush_set_hits(const int32_t* slot, const uint32_t* C, const PivotSet& S, int gl,
                                                unsigned long long* vcount) {
-  const int used = (S.mode == kSetHash || S.mode == kSetHashWide) ? (4 << (32 - S.p0)) : (1 << S.p0);
+  const int used = (S.mode == kSetHash || S.mode == kSetHashWide) ? (4 << S.p0) : (1 << S.p0);
   for (int x = gl; x < used; x += G) {
     const uint32_t c = (C[x >> 1] >> ((x & 1) << 4)) & 0xFFFFu;
     const int key = slot[x];
@@ -314,11 +313,11 @@ template <int NF, bool kLocal, bool kWide>
 __device__ __forceinline__ void hash_probe(const PivotSet& S, const int (&w)[NF], const int (&kq)[NF],
                                            uint32_t& hits, const LocalSink& ls) {
   const char* Tb = (const char*)S.A;
-  const int sh = S.p0;
+  const int lb = S.p0;
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const int hk = hash_key<kWide>(w[q]);
-    const uint32_t b0 = hash_bucket(hk, kHashA0, sh), b1 = hash_bucket(hk, kHashA1, sh);
+    const uint32_t b0 = hash_bucket(hk, kHashA0, lb), b1 = hash_bucket(hk, kHashA1, lb);
     const int2 x = *(const int2*)(Tb + (b0 << 3));
     const int2 y = *(const int2*)(Tb + S.p1 + (b1 << 3));
     const bool f = (x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q]);
@@ -365,7 +364,9 @@ struct RefEdges {
   }
 };
 
-template <int G, int kModes, int NF = kInFlight, bool kLocal = false, class Src>
+// kWide: hash sets hold keys folded for ids past 2^24 (kSetHashWide); one
+// instantiation per hash kind keeps a single lookup path in the loop.
+template <int G, int kModes, int NF = kInFlight, bool kLocal = false, bool kWide = false, class Src>
 __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, const Src src, int da,
                                                 const PivotSet S, int jfirst, int jstride,
                                                 int32_t* sEnd, gint32** sBase, int gl,
@@ -427,10 +428,8 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
           }
         }
       }
-      if ((kModes & kModeHash) && S.mode == kSetHash) {
-        hash_probe<NF, kLocal, false>(S, w, kq, hits, ls);
-      } else if ((kModes & kModeHash) && S.mode == kSetHashWide) {
-        hash_probe<NF, kLocal, true>(S, w, kq, hits, ls);
+      if ((kModes & kModeHash) && S.mode == (kWide ? kSetHashWide : kSetHash)) {
+        hash_probe<NF, kLocal, kWide>(S, w, kq, hits, ls);
       } else if ((kModes & kModeTree) && ((kModes == kModeTree) || S.mode == kSetTree)) {
         // h descent steps; then the lower bound is k with its trailing one
         // bits and the following zero shifted out (0 = past the end -> -1).
@@ -1287,7 +1286,7 @@ struct __align__(16) LocalLds<true> {
   __device__ uint32_t* ec() { return EC; }
 };
 
-template <int G, int AREA, bool kLocal>
+template <int G, int AREA, bool kLocal, bool kWide>
 __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* items, int64_t first,
                                            int64_t last, CountLds& L, LocalLds<kLocal>& LL, bool batch,
                                            Acc& acc) {
@@ -1302,11 +1301,11 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   for (int64_t i = first + grp; i < last; i += NGRP) {
     const int64_t item = (int64_t)items[i];
     const PRow pr = partner_row(a, item);
-    const PivotSet S = build_pivot_set<G, kLocal>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set), gl,
-                                                  a.tune, a.wide_ids != 0, a.bufs.counters + kCntFallback, C);
+    const PivotSet S = build_pivot_set<G, kLocal, kWide>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set),
+                                                         gl, a.tune, a.bufs.counters + kCntFallback, C);
     const LocalSink ls{C, kLocal ? LL.ec() + grp * G : nullptr, kLocal ? a.local.pedge + pr.pb : nullptr,
                        kLocal ? a.local.pvid + pr.pb : nullptr, a.local.ecount, a.local.vcount};
-    const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kLocal>(
+    const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kLocal, kWide>(
         a.ci, RefEdges{a.pci + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl, ls);
     if constexpr (kLocal) {
       group_sync<G>();
@@ -1320,7 +1319,7 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   }
 }
 
-template <bool kLdsPivot, bool kLocal>
+template <bool kLdsPivot, bool kLocal, bool kWide>
 __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, CountLds& L,
                                             LocalLds<kLocal>& LL, bool batch, Acc& acc) {
   const PRow pr = partner_row(a, (int64_t)item);
@@ -1335,11 +1334,12 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int32_t* pvid = kLocal ? a.local.pvid + pr.pb : nullptr;
   uint32_t h;
   if (kLdsPivot) {
-    const PivotSet S = build_pivot_set<kBlock, kLocal>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune,
-                                                       a.wide_ids != 0, a.bufs.counters + kCntFallback, LL.c());
+    const PivotSet S = build_pivot_set<kBlock, kLocal, kWide>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune,
+                                                              a.bufs.counters + kCntFallback, LL.c());
     const LocalSink ls{LL.c(), EC, pedge, pvid, a.local.ecount, a.local.vcount};
-    h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock,
-                                                                   L.End + wid * 64, L.Base + wid * 64, lane, ls);
+    h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kLocal, kWide>(ci, part, np, S, wid * 64, kBlock,
+                                                                          L.End + wid * 64, L.Base + wid * 64, lane,
+                                                                          ls);
     if constexpr (kLocal) {
       __syncthreads();
       flush_set_hits<kBlock>(L.A, LL.c(), S, threadIdx.x, a.local.vcount);
@@ -1371,7 +1371,8 @@ constexpr int kLocalWavesPerEU = 5;
 // working set in registers.  Blocks pull tiles from the bin's queue head.
 // kLocal: also the per-edge and per-vertex attribution of a.local (one
 // graph only).
-template <int BIN, bool kLocal>
+// kWide: the instantiation for graphs whose ids reach 2^24 (folded hash keys).
+template <int BIN, bool kLocal, bool kWide>
 __global__ __launch_bounds__(kBlock, kLocal ? kLocalWavesPerEU : TC_COUNT_WAVES_PER_EU) void count_kernel(
     LaunchArgs a) {
   __shared__ CountLds L;
@@ -1398,12 +1399,12 @@ __global__ __launch_bounds__(kBlock, kLocal ? kLocalWavesPerEU : TC_COUNT_WAVES_
     if (t >= ntiles) break;
     const int64_t first = t * piv;
     const int64_t last = min(first + (int64_t)piv, cnt);
-    if (BIN == BIN_GLOBAL) block_pivot<false, kLocal>(a, items[first], L, LL, batch, acc);
-    else if (BIN == BIN_BLOCK) block_pivot<true, kLocal>(a, items[first], L, LL, batch, acc);
-    else if (BIN == BIN_WAVE) group_tile<64, kAreaWave, kLocal>(a, items, first, last, L, LL, batch, acc);
-    else if (BIN == BIN_G32) group_tile<32, kAreaG32, kLocal>(a, items, first, last, L, LL, batch, acc);
-    else if (BIN == BIN_G16) group_tile<16, kAreaG16, kLocal>(a, items, first, last, L, LL, batch, acc);
-    else group_tile<8, kAreaG8, kLocal>(a, items, first, last, L, LL, batch, acc);
+    if (BIN == BIN_GLOBAL) block_pivot<false, kLocal, kWide>(a, items[first], L, LL, batch, acc);
+    else if (BIN == BIN_BLOCK) block_pivot<true, kLocal, kWide>(a, items[first], L, LL, batch, acc);
+    else if (BIN == BIN_WAVE) group_tile<64, kAreaWave, kLocal, kWide>(a, items, first, last, L, LL, batch, acc);
+    else if (BIN == BIN_G32) group_tile<32, kAreaG32, kLocal, kWide>(a, items, first, last, L, LL, batch, acc);
+    else if (BIN == BIN_G16) group_tile<16, kAreaG16, kLocal, kWide>(a, items, first, last, L, LL, batch, acc);
+    else group_tile<8, kAreaG8, kLocal, kWide>(a, items, first, last, L, LL, batch, acc);
     if (batch) {
       // Items are ordered by graph, so a tile almost always belongs to one
       // graph: then one block-reduced atomic; otherwise one per lane.
@@ -1613,11 +1614,12 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int BIN, bool kLocal>
+template <int BIN, bool kLocal, bool kWide>
 static int bin_blocks_per_cu() {
   static const int n = [] {
     int k = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, count_kernel<BIN, kLocal>, kBlock, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, count_kernel<BIN, kLocal, kWide>, kBlock, 0) !=
+        hipSuccess)
       k = 1;
     return k > 0 ? k : 1;
   }();
@@ -1629,9 +1631,12 @@ static int bin_blocks_per_cu() {
 template <int BIN, bool kLocal = false>
 static void launch_bin(const LaunchArgs& a, int cus, int grid, hipStream_t s, int64_t tiles = -1) {
   if (tiles == 0) return;
-  int64_t g = grid > 0 ? grid : (int64_t)bin_blocks_per_cu<BIN, kLocal>() * cus;
+  const bool wide = a.wide_ids != 0;
+  int64_t g = grid > 0 ? grid
+                       : (int64_t)(wide ? bin_blocks_per_cu<BIN, kLocal, true>() : bin_blocks_per_cu<BIN, kLocal, false>()) * cus;
   if (tiles > 0 && tiles < g) g = tiles;
-  count_kernel<BIN, kLocal><<<(unsigned)g, kBlock, 0, s>>>(a);
+  if (wide) count_kernel<BIN, kLocal, true><<<(unsigned)g, kBlock, 0, s>>>(a);
+  else count_kernel<BIN, kLocal, false><<<(unsigned)g, kBlock, 0, s>>>(a);
 }
 
 hipError_t launch_count(const LaunchArgs& a, int cus, int grid, hipStream_t s) {
@@ -1705,7 +1710,8 @@ hipError_t launch_read(const void* src, int64_t bytes, uint32_t* sink, int grid,
 
 int count_blocks_per_cu() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel<BIN_WAVE, false>, kBlock, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel<BIN_WAVE, false, false>, kBlock, 0) !=
+      hipSuccess)
     return 1;
   return n > 0 ? n : 1;
 }

@@ -377,7 +377,8 @@ def test_graphminer_roundtrip(tmp_path):
     assert np.array_equal(h.out_rowptr(), g.out_rowptr())
     assert np.array_equal(h.out_colidx(), g.out_colidx())
     dag = Graph.from_file(prefix, use_dag=True)  # graph.cc:135-138
-    assert np.array_equal(dag.out_colidx(), g.oriented().out_colidx())
+    ref = g.oriented()  # keep the graph alive: out_colidx() views its memory
+    assert np.array_equal(dag.out_colidx(), ref.out_colidx())
     with pytest.raises(TcError) as e:
         Graph.from_file(str(tmp_path / "missing"))
     assert e.value.status == TC_EIO

@@ -110,13 +110,16 @@ def cpu_baseline_child(scale: int, seed: int, target_s: float, runs: int = 3, sa
     pivots -- on the same RMAT DAG, median of `runs` timed runs.  The whole
     graph when one run fits target_s, else a random sample of pivot rows
     sized to it (dynamic schedule of 64 rows)."""
+    # the thread count first: with OMP_PROC_BIND set, the first OpenMP region
+    # (graph generation below) binds this thread to one CPU, and the
+    # affinity mask no longer shows the CPUs the process may use
+    threads = cores_available()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, used here only as the timed CPU baseline
     from hetegeneous_parallel_programming_amd import Graph
 
     g = Graph.rmat_scale(scale, 16, seed)
     g.orientation()
-    threads = cores_available()
     rp, ci = g.out_rowptr(), g.out_colidx()
     n = g.V()
     rng = np.random.default_rng(sample_seed)
@@ -147,7 +150,7 @@ def cpu_baseline_child(scale: int, seed: int, target_s: float, runs: int = 3, sa
     return {"value": edges / t, "unit": "edges/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "physical_cores_host": physical_cores(),
             "logical_cpus_host": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-            "runs_s": times, "count": int(count),
+            "runs_s": times, "count": int(count), "whole_graph": est_whole <= target_s,
             "sample": f"{what}: oracle OpenMP merge count on {threads} threads (the CPUs this process may "
                       f"use; OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), median of {runs} runs = {t:.2f} s"}
 
@@ -612,7 +615,7 @@ def main() -> int:
     ap.add_argument("--no-split", action="store_true",
                     help="skip timing the other parts of the RMAT-24 4-way splits")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU time per baseline run (3 runs); the whole graph when it fits")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-serving", action="store_true")
@@ -726,7 +729,7 @@ def leg_headline(ctx, args) -> dict:
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
-        if line["cpu_baseline"]["count"] != best["count"]:
+        if line["cpu_baseline"]["whole_graph"] and line["cpu_baseline"]["count"] != best["count"]:
             raise RuntimeError("CPU baseline count differs from the GPU count")
         log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} edges/s on "
             f"{line['cpu_baseline']['cores']} cores")

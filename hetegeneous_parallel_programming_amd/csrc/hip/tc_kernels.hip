@@ -862,14 +862,16 @@ __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
 // so the build is mostly a transpose: in-owned edges grouped by owner.
 // Per-edge atomics on owner counters make that transpose wait on the memory
 // side's atomic rate (and on hub vertices).  Here instead:
-//   split   -- per owned row (16 lanes), in row order: out-owned partners
-//              compacted at the front of the row's slots of `out`, counted
-//              in c1; an in-owned edge leaves its owner's row slot in `key`
-//              and is counted in its owner's bucket (block LDS histogram,
-//              one atomic per (block, bucket));
-//   bucket  -- the same row walk places every (owner, partner list) pair in
-//              its bucket's range of pkey/pref (one reservation per
-//              (block, bucket), LDS ranks inside it);
+//   split   -- per owned row (16 lanes; blocks of 16 rows, so the dispatcher
+//              balances rows of any length): out-owned partners compacted at
+//              the front of the row's slots of `out` (row order, ballot
+//              ranks), counted in c1; in-owned edges compacted at the back of
+//              the same slots -- the partner's list in `out`, the owner's row
+//              slot + 1 in `key` (key is 0 elsewhere);
+//   bhist   -- flat over `key`, even blocks of kKeysPerBlock: pairs per owner
+//              bucket (block LDS histogram, one atomic per (block, bucket));
+//   bucket  -- the same blocks place every pair in its bucket's range of
+//              pkey/pref (one reservation per (block, bucket), LDS ranks);
 //   incount -- per chunk of a bucket, an LDS histogram over the bucket's
 //              owners gives c2 (a store, or one atomic per owner and chunk
 //              when the bucket has several chunks);
@@ -881,53 +883,64 @@ __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
 // The partner pool comes out laid out as by the atomic path (a row's
 // out-edge partners in row order, then its in-edge partners in any order),
 // so rows, work, planning and counting are unchanged.
-constexpr int kRowsPerBlock = 2048;  // rows of one block of the split and bucket walks
+constexpr int64_t kKeysPerBlock = 65536;  // keys per block of the bhist and bucket passes
 
-__global__ __launch_bounds__(kBlock) void partner_split_kernel(LaunchArgs a, int64_t RP, BuildBufs b, int bshift,
-                                                               int nb) {
-  extern __shared__ uint32_t s_hist[];  // [nb] in-owned edges per owner bucket
+__global__ __launch_bounds__(kBlock) void partner_split_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
+  const int gl = threadIdx.x % kBuildG;
+  if (q >= RP) return;  // the whole group
+  const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
+  const int64_t u = q - d.row_base;
+  int32_t c1 = 0;
+  if (u < d.n_rows && u >= d.own_begin && u < d.own_end && (!a.own || a.own[u])) {
+    const int64_t* rp = a.rp + d.rp_off;
+    const int32_t* ci = a.ci + d.ci_off;
+    const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
+    ListRef* out = b.out + d.eb;
+    int32_t* key = b.key + d.eb;
+    const ListRef uref = list_ref(d.ci_off + e0, du);
+    const int shift = (int)(threadIdx.x & 63) & ~(kBuildG - 1);
+    const uint32_t below = (1u << gl) - 1u, mask = (1u << kBuildG) - 1u;
+    int32_t cin = 0;
+    for (int64_t eb = e0; eb < e1; eb += kBuildG) {
+      const int64_t e = eb + gl;
+      int v = 0;
+      int64_t vb = 0, dv = 0;
+      bool own = false, in = false;
+      if (e < e1) {
+        v = ci[e];
+        vb = rp[v];
+        dv = rp[v + 1] - vb;
+        own = dv <= du;
+        in = !own;
+      }
+      const uint32_t go = (uint32_t)(__ballot(own) >> shift) & mask;
+      const uint32_t gi = (uint32_t)(__ballot(in) >> shift) & mask;
+      if (own) {
+        const int64_t at = e0 + c1 + __popc(go & below);
+        out[at] = list_ref(d.ci_off + vb, dv);
+        key[at] = 0;
+      } else if (in) {
+        const int64_t at = e1 - 1 - (cin + __popc(gi & below));
+        out[at] = uref;
+        key[at] = (int32_t)(d.row_base + v) + 1;
+      }
+      c1 += __popc(go);
+      cin += __popc(gi);
+    }
+  }
+  if (gl == 0) b.c1[q] = c1;
+}
+
+// Pairs per owner bucket in keys [blockIdx.x * kKeysPerBlock, ...).
+__global__ __launch_bounds__(kBlock) void partner_bhist_kernel(BuildBufs b, int64_t M, int bshift, int nb) {
+  extern __shared__ uint32_t s_hist[];  // [nb]
   for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0;
   __syncthreads();
-  const int gl = threadIdx.x % kBuildG;
-  const int shift = (int)(threadIdx.x & 63) & ~(kBuildG - 1);
-  const uint32_t below = (1u << gl) - 1u;
-  const int64_t q1 = min((int64_t)(blockIdx.x + 1) * kRowsPerBlock, RP);
-  for (int64_t q = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kBuildG; q < q1; q += kBlock / kBuildG) {
-    const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
-    const int64_t u = q - d.row_base;
-    int32_t c1 = 0;
-    if (u < d.n_rows && u >= d.own_begin && u < d.own_end && (!a.own || a.own[u])) {
-      const int64_t* rp = a.rp + d.rp_off;
-      const int32_t* ci = a.ci + d.ci_off;
-      const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
-      ListRef* out = b.out + d.eb + e0;
-      int32_t* key = b.key + d.eb;
-      for (int64_t eb = e0; eb < e1; eb += kBuildG) {
-        const int64_t e = eb + gl;
-        int v = 0;
-        int64_t vb = 0, dv = 0;
-        bool own = false;
-        if (e < e1) {
-          v = ci[e];
-          vb = rp[v];
-          dv = rp[v + 1] - vb;
-          own = dv <= du;
-        }
-        const uint32_t gm = (uint32_t)(__ballot(own) >> shift) & ((1u << kBuildG) - 1u);
-        if (e < e1) {
-          if (own) {
-            out[c1 + __popc(gm & below)] = list_ref(d.ci_off + vb, dv);
-            key[e] = -1;
-          } else {
-            const int32_t k = (int32_t)(d.row_base + v);
-            key[e] = k;
-            atomicAdd(&s_hist[k >> bshift], 1u);
-          }
-        }
-        c1 += __popc(gm);
-      }
-    }
-    if (gl == 0) b.c1[q] = c1;
+  const int64_t i0 = (int64_t)blockIdx.x * kKeysPerBlock, i1 = min(i0 + kKeysPerBlock, M);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
+    const int32_t k = b.key[i];
+    if (k > 0) atomicAdd(&s_hist[(k - 1) >> bshift], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kBlock)
@@ -947,39 +960,27 @@ __global__ __launch_bounds__(kBlock) void bucket_chunks_kernel(BuildBufs b, int 
   }
 }
 
-__global__ __launch_bounds__(kBlock) void partner_bucket_kernel(LaunchArgs a, int64_t RP, BuildBufs b, int bshift,
-                                                                int nb) {
+// Same blocks as bhist: this block's pairs to their buckets.
+__global__ __launch_bounds__(kBlock) void partner_bucket_kernel(BuildBufs b, int64_t M, int bshift, int nb) {
   extern __shared__ uint32_t s_w[];  // [nb] pairs per bucket, then the next slot of each
   for (int i = threadIdx.x; i < nb; i += kBlock) s_w[i] = 0;
   __syncthreads();
-  const int gl = threadIdx.x % kBuildG;
-  const int64_t qf = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kBuildG;
-  const int64_t q1 = min((int64_t)(blockIdx.x + 1) * kRowsPerBlock, RP);
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int64_t q = qf; q < q1; q += kBlock / kBuildG) {
-      const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
-      const int64_t u = q - d.row_base;
-      if (u >= d.n_rows || u < d.own_begin || u >= d.own_end || (a.own && !a.own[u])) continue;
-      const int64_t* rp = a.rp + d.rp_off;
-      const int64_t e0 = rp[u], e1 = rp[u + 1];
-      const int32_t* key = b.key + d.eb;
-      const ListRef ref = list_ref(d.ci_off + e0, e1 - e0);
-      for (int64_t e = e0 + gl; e < e1; e += kBuildG) {
-        const int32_t k = key[e];
-        if (k < 0) continue;
-        const uint32_t slot = atomicAdd(&s_w[k >> bshift], 1u);
-        if (pass == 1) {
-          b.pkey[slot] = k;
-          b.pref[slot] = ref;
-        }
-      }
-    }
-    __syncthreads();
-    if (pass == 0) {  // reserve this block's range of each bucket (pair positions are below 2^32)
-      for (int i = threadIdx.x; i < nb; i += kBlock)
-        if (s_w[i]) s_w[i] = (uint32_t)atomicAdd(&b.bcur[i], (unsigned long long)s_w[i]);
-      __syncthreads();
-    }
+  const int64_t i0 = (int64_t)blockIdx.x * kKeysPerBlock, i1 = min(i0 + kKeysPerBlock, M);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
+    const int32_t k = b.key[i];
+    if (k > 0) atomicAdd(&s_w[(k - 1) >> bshift], 1u);
+  }
+  __syncthreads();
+  // reserve this block's range of each bucket (pair positions are below 2^32)
+  for (int i = threadIdx.x; i < nb; i += kBlock)
+    if (s_w[i]) s_w[i] = (uint32_t)atomicAdd(&b.bcur[i], (unsigned long long)s_w[i]);
+  __syncthreads();
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
+    const int32_t k = b.key[i];
+    if (k <= 0) continue;
+    const uint32_t slot = atomicAdd(&s_w[(k - 1) >> bshift], 1u);
+    b.pkey[slot] = k - 1;
+    b.pref[slot] = b.out[i];
   }
 }
 
@@ -1026,11 +1027,13 @@ __global__ __launch_bounds__(kBlock) void partner_incount_kernel(BuildBufs b, in
   }
 }
 
-// Per row (kBuildG lanes): its out-owned partners to the front of its
-// partner range; cur = absolute slot of its first in-edge partner.
+// Per row (kCopyG lanes: rows average a few out-owned partners): its
+// out-owned partners to the front of its partner range; cur = absolute slot
+// of its first in-edge partner.
+constexpr int kCopyG = 4;
 __global__ __launch_bounds__(kBlock) void partner_outcopy_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
-  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
-  const int gl = threadIdx.x % kBuildG;
+  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kCopyG;
+  const int gl = threadIdx.x % kCopyG;
   if (q >= RP) return;
   const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
   const int64_t u = q - d.row_base;
@@ -1039,7 +1042,7 @@ __global__ __launch_bounds__(kBlock) void partner_outcopy_kernel(LaunchArgs a, i
   if (c > 0) {
     const ListRef* src = b.out + d.eb + a.rp[d.rp_off + u];
     ListRef* dst = (ListRef*)a.pci + base;
-    for (int j = gl; j < c; j += kBuildG) dst[j] = src[j];
+    for (int j = gl; j < c; j += kCopyG) dst[j] = src[j];
   }
   if (gl == 0) b.cur[q] = (unsigned long long)(base + c);
 }
@@ -1085,26 +1088,34 @@ static hipError_t launch_partner_rows(const LaunchArgs& a, int64_t RP, const Bui
 static hipError_t launch_partners_bucketed(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b,
                                            int bshift, hipStream_t s) {
   const int nb = (int)((RP + ((int64_t)1 << bshift) - 1) >> bshift);
-  const unsigned wblocks = (unsigned)((RP + kRowsPerBlock - 1) / kRowsPerBlock);
   const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
+  const unsigned cpblocks = (unsigned)((RP * kCopyG + kBlock - 1) / kBlock);
+  const unsigned kblocks = (unsigned)((M + kKeysPerBlock - 1) / kKeysPerBlock);
   const unsigned cblocks = (unsigned)(nb + (M + kChunkPairs - 1) / kChunkPairs);  // >= chunks
-  const size_t wlds = sizeof(uint32_t) * (size_t)nb, clds = sizeof(uint32_t) << bshift;
+  const size_t blds = sizeof(uint32_t) * (size_t)nb, clds = sizeof(uint32_t) << bshift;
   hipError_t e = launch_zero(b.c2, RP, s);
+  if (e == hipSuccess) e = launch_zero(b.key, M, s);  // keys of rows that are not split stay "none"
   if (e == hipSuccess) e = launch_zero(b.btot, 2 * ((int64_t)nb + 1), s);
   if (e == hipSuccess) e = launch_zero((void*)a.vrow, a.P * (int64_t)(sizeof(VRow) / 4), s);  // slots unused
   if (e != hipSuccess) return e;
-  partner_split_kernel<<<wblocks, kBlock, wlds, s>>>(a, RP, b, bshift, nb);
+  partner_split_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (kblocks > 0) {
+    partner_bhist_kernel<<<kblocks, kBlock, blds, s>>>(b, M, bshift, nb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   if ((e = launch_scan(LoadU64{b.btot}, StoreU64{b.btot}, (int64_t)nb + 1, b.part, s)) != hipSuccess) return e;
   bucket_chunks_kernel<<<(unsigned)((nb + kBlock) / kBlock), kBlock, 0, s>>>(b, nb);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(LoadI64{b.cstart}, StoreI64{b.cstart}, (int64_t)nb + 1, b.part, s)) != hipSuccess) return e;
-  partner_bucket_kernel<<<wblocks, kBlock, wlds, s>>>(a, RP, b, bshift, nb);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (kblocks > 0) {
+    partner_bucket_kernel<<<kblocks, kBlock, blds, s>>>(b, M, bshift, nb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   partner_incount_kernel<<<cblocks, kBlock, clds, s>>>(b, bshift, nb);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(LoadAdd{b.c1, b.c2}, StoreI64{b.off}, RP, b.part, s)) != hipSuccess) return e;
-  partner_outcopy_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
+  partner_outcopy_kernel<<<cpblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_zero(b.kc, RP, s)) != hipSuccess) return e;
   partner_infill_kernel<<<cblocks, kBlock, clds, s>>>(a, b, bshift, nb);

@@ -148,6 +148,7 @@ struct tc_ctx {
   std::string name;
   tcr::PlanScratch scratch;
   uint64_t* pinned = nullptr;  // 64 B count + 64 B bin counters
+  void* noalloc = nullptr;     // device address of the static scratch of tc_count_noalloc
   int grid() const { return tune.grid > 0 ? tune.grid : count_grid; }
 };
 

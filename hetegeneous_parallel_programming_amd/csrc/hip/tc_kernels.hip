@@ -1719,6 +1719,39 @@ hipError_t launch_read(const void* src, int64_t bytes, uint32_t* sink, int grid,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// static scratch of the allocation-free count
+// ---------------------------------------------------------------------------
+// Pieces (256-byte aligned): bin_of [slots] u16, items [slots] u64, blk
+// [bins * plan_blocks(slots) + 1] u32, scan partials of blk, counters [64]
+// u32, outs [16] u64.
+constexpr int64_t nalloc_up(int64_t b) { return (b + 255) & ~(int64_t)255; }
+constexpr int64_t kNaBlk = kNumBins * ((kNoAllocSlots + 255) / 256) + 1;
+constexpr int64_t kNaBinOf = 0;
+constexpr int64_t kNaItems = kNaBinOf + nalloc_up(2 * kNoAllocSlots);
+constexpr int64_t kNaBlkOff = kNaItems + nalloc_up(8 * kNoAllocSlots);
+constexpr int64_t kNaPart = kNaBlkOff + nalloc_up(4 * kNaBlk);
+constexpr int64_t kNaCounters = kNaPart + nalloc_up(8 * ((kNaBlk + kScanChunk - 1) / kScanChunk + 1));
+constexpr int64_t kNaOuts = kNaCounters + nalloc_up(4 * 64);
+constexpr int64_t kNaBytes = kNaOuts + nalloc_up(8 * 16);
+
+// The scratch itself.  Host code never passes this name to a kernel or a
+// copy: it is a host-side stand-in there.  tc_ctx_create takes its device
+// address once per context with the context's device current
+// (noalloc_scratch_address), and every use goes through that address.  All
+// calls on one device share it (overlapping calls race).
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+int64_t noalloc_scratch_bytes() { return kNaBytes; }
+
+PlanBufs noalloc_plan_bufs(void* base) {
+  char* p = (char*)base;
+  return PlanBufs{(uint16_t*)(p + kNaBinOf), (uint64_t*)(p + kNaItems), (uint32_t*)(p + kNaBlkOff),
+                  (int64_t*)(p + kNaPart), (uint32_t*)(p + kNaCounters), (uint64_t*)(p + kNaOuts)};
+}
+
+hipError_t noalloc_scratch_address(void** ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)); }
+
 int count_blocks_per_cu() {
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, count_kernel<BIN_WAVE, false, false>, kBlock, 0) !=

@@ -257,4 +257,15 @@ hipError_t launch_read(const void* src, int64_t bytes, uint32_t* sink, int grid,
 int count_blocks_per_cu();
 int plain_blocks_per_cu();
 
+// Static plan scratch of the allocation-free count (tc_count_noalloc): a
+// count over at most kNoAllocSlots plan indices keeps its bins, items, scan
+// and counters there instead of in allocated memory.
+constexpr int64_t kNoAllocSlots = (int64_t)1 << 21;
+// Byte size of the scratch and the plan buffers laid out in it (base = its
+// device address, from noalloc_scratch_address).
+int64_t noalloc_scratch_bytes();
+PlanBufs noalloc_plan_bufs(void* base);
+// Device address of the scratch on the current device.
+hipError_t noalloc_scratch_address(void** ptr);
+
 }  // namespace tck

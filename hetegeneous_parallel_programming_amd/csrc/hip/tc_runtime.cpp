@@ -382,6 +382,8 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) try {
   TC_HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
   TC_HIP_TRY(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));
+  // device address of the allocation-free count's static scratch (this device's copy)
+  TC_HIP_TRY(tck::noalloc_scratch_address(&c->noalloc));
   c->count_grid = tck::count_blocks_per_cu() * c->cus;
   c->plain_grid = tck::plain_blocks_per_cu() * c->cus;
   c->tune = default_tune();
@@ -552,30 +554,22 @@ static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
 
 namespace tcr {
 
-// The binned count.  Its plan (rows per bin, bin-major item lists) depends
-// only on the graph's partner rows and the launch setting, so it is built
-// on the first call and kept in the graph: later calls re-arm the tile
-// queues and launch the sized bins directly -- no planning kernels and no
-// host round trip before the count.
-static tc_status count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+// The binned count over plan buffers `bufs`.  kept: the plan in bufs is
+// already built for this graph and tune (its rows per bin in bins[], tiles
+// per bin in tiles[]): re-arm the tile queues and count.  Otherwise plan
+// first (one host round trip for the rows per bin, which size the launches)
+// and fill bins[] and tiles[].
+static tc_status binned_count(tc_ctx* ctx, tc_graph* g, const tck::PlanBufs& bufs, bool kept, int64_t* bins,
+                              int64_t* tiles, uint64_t* total, tc_stats* stats) {
   const double t0 = now_ms();
-  const int64_t P = (int64_t)g->nv_max + 1;
-  const bool kept = g->plan_valid && std::memcmp(&g->plan_tune, &ctx->tune, sizeof(tc_tune)) == 0;
-  if (!kept) {
-    const int64_t had = g->plan.cap;
-    tc_status st = g->plan.ensure(P, 1);
-    if (st != TC_OK) return st;
-    if (g->plan.cap != had) g->bytes += PlanScratch::bytes_for(g->plan.cap) - (had ? PlanScratch::bytes_for(had) : 0);
-  }
   tck::LaunchArgs a = count_args(ctx, g);
-  a.bufs = g->plan.bufs();
+  a.bufs = bufs;
   hipStream_t s = ctx->stream;
   TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
   if (kept) {
     TC_HIP_TRY(tck::launch_rearm(a.bufs.counters, a.bufs.outs, 2, s));
     TC_HIP_TRY(hipEventRecord(ctx->ev[3], s));
   } else {
-    g->plan_valid = false;
     TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
     TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
     TC_HIP_TRY(tck::launch_plan(a, s));
@@ -589,13 +583,10 @@ static tc_status count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stat
     const uint32_t* c = (const uint32_t*)(ctx->pinned + 8);
     for (int b = 0; b < tck::kNumBins; ++b) {
       const int64_t piv = tck::tile_pivots(b, ctx->tune.rounds > 0 ? ctx->tune.rounds : 1);
-      g->plan_bins[b] = c[tck::kCntBinCount + b];
-      g->plan_tiles[b] = (g->plan_bins[b] + piv - 1) / piv;
+      bins[b] = c[tck::kCntBinCount + b];
+      tiles[b] = (bins[b] + piv - 1) / piv;
     }
-    g->plan_tune = ctx->tune;
-    g->plan_valid = true;
   }
-  const int64_t* tiles = g->plan_tiles;
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
   // The bins run as groups on separate streams (default priority): the
   // global-list and workgroup bins, the wave bin, the 32-lane bin, and the
@@ -643,31 +634,59 @@ static tc_status count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stat
     TC_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
     stats->count_ms = ms;
     stats->pivots = g->nv;
-    for (int b = 0; b < TC_NUM_BINS; ++b) stats->bin_pivots[b] = g->plan_bins[b];
+    for (int b = 0; b < TC_NUM_BINS; ++b) stats->bin_pivots[b] = bins[b];
     stats->hash_fallbacks = *(const uint32_t*)(ctx->pinned + kPinnedFallback);
     stats->plan_kept = kept ? 1 : 0;
   }
   return TC_OK;
 }
 
-static tc_status count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+// The binned count with the plan kept in the graph.  The plan (rows per bin,
+// bin-major item lists) depends only on the graph's partner rows and the
+// launch setting, so it is built on the first call: later calls re-arm the
+// tile queues and launch the sized bins directly -- no planning kernels and
+// no host round trip before the count.
+static tc_status count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+  const int64_t P = (int64_t)g->nv_max + 1;
+  const bool kept = g->plan_valid && std::memcmp(&g->plan_tune, &ctx->tune, sizeof(tc_tune)) == 0;
+  if (!kept) {
+    const int64_t had = g->plan.cap;
+    tc_status st = g->plan.ensure(P, 1);
+    if (st != TC_OK) return st;
+    if (g->plan.cap != had) g->bytes += PlanScratch::bytes_for(g->plan.cap) - (had ? PlanScratch::bytes_for(had) : 0);
+    g->plan_valid = false;
+  }
+  tc_status st = binned_count(ctx, g, g->plan.bufs(), kept, g->plan_bins, g->plan_tiles, total, stats);
+  if (st == TC_OK && !kept) {
+    g->plan_tune = ctx->tune;
+    g->plan_valid = true;
+  }
+  return st;
+}
+
+// The plain kernel's single launch, its u64 sum at `out` on the device.
+static tc_status plain_count(tc_ctx* ctx, tc_graph* g, uint64_t* out, uint64_t* total, tc_stats* stats) {
   const double t0 = now_ms();
   const int64_t P = (int64_t)g->row_end - g->row_begin;
-  tc_status st = ctx->scratch.ensure(0, 1);
-  if (st != TC_OK) return st;
   hipStream_t s = ctx->stream;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->plain_grid, (P + tck::kWaves - 1) / tck::kWaves));
   TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  TC_HIP_TRY(tck::launch_zero(ctx->scratch.outs, 2, s));
+  TC_HIP_TRY(tck::launch_zero(out, 2, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, g->own, ctx->scratch.outs, grid, s));
+  TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, g->own, out, grid, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
-  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->scratch.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, out, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
-  st = finish_stats(ctx, stats, t0, false, P);
+  tc_status st = finish_stats(ctx, stats, t0, false, P);
   if (st == TC_OK && stats) stats->single_launch = 1;
   return st;
+}
+
+static tc_status count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
+  tc_status st = ctx->scratch.ensure(0, 1);
+  if (st != TC_OK) return st;
+  return plain_count(ctx, g, ctx->scratch.outs, total, stats);
 }
 
 // Graphs whose counted rows hold at most this many DAG ids, and whose lists
@@ -710,6 +729,24 @@ tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* st
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
   return count_plain(ctx, g, total, stats);
+} catch (...) {
+  return tcr::caught();
+}
+
+tc_status tc_count_noalloc(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) try {
+  if (!ctx || !g || !total) return fail(TC_EINVAL, "null argument");
+  if (g->ctx != ctx) return fail(TC_EINVAL, "graph belongs to another context");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  DeviceGuard guard(ctx->device);
+  const tck::PlanBufs bufs = tck::noalloc_plan_bufs(ctx->noalloc);
+  if (small_graph(g)) return plain_count(ctx, g, bufs.outs, total, stats);
+  const int64_t P = (int64_t)g->nv_max + 1;
+  if (P > tck::kNoAllocSlots)
+    return fail(TC_ERANGE, "tc_count_noalloc: the graph needs " + std::to_string(P) +
+                               " plan slots (rows + ids/1024 + 2); the static scratch holds " +
+                               std::to_string(tck::kNoAllocSlots));
+  int64_t bins[tck::kNumBins], tiles[tck::kNumBins];
+  return binned_count(ctx, g, bufs, false, bins, tiles, total, stats);
 } catch (...) {
   return tcr::caught();
 }

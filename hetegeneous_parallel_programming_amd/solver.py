@@ -121,6 +121,7 @@ def device_lib():
     S(L, "tc_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_plain", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_binned", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
+    S(L, "tc_count_noalloc", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_local", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Local), ctypes.POINTER(Stats)])
     S(L, "tc_batch_create", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_i32, nat.c_u32, PP])
     S(L, "tc_batch_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
@@ -305,6 +306,15 @@ class Context:
         total = nat.c_u64(0)
         st = Stats()
         _check(device_lib().tc_count_binned(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
+        return int(total.value), st.as_dict()
+
+    def count_noalloc(self, dg: DeviceGraph):
+        """tc_count's count without any device allocation (tc_count_noalloc):
+        its plan lives in the library's static scratch, at most 2^21 plan
+        slots; larger graphs raise TC_ERANGE."""
+        total = nat.c_u64(0)
+        st = Stats()
+        _check(device_lib().tc_count_noalloc(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
         return int(total.value), st.as_dict()
 
     def count_plain(self, dg: DeviceGraph):

@@ -140,6 +140,19 @@ tc_status tc_count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* s
  * multigpu_nvshmem_pushpull.cu:41-51). */
 tc_status tc_count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
 
+/* tc_count_noalloc: tc_count's count (the single plain launch for a small
+ * graph, else the binned count, planned on every call) that allocates
+ * nothing: no hipMalloc, hipFree or allocating library call.  No counterpart
+ * in the reference; it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  Its plan
+ * lives in a static device array of the library, about 20 MB, held on every
+ * GPU the library runs on until the process ends: at most 2^21 plan slots
+ * (rows + ids/1024 + 2), larger graphs return TC_ERANGE.  The graph must be
+ * resident already (tc_graph_upload allocates).  Calls on one device share
+ * that array: calls that overlap on one GPU race (one context serialises
+ * its own calls; two contexts on one GPU must not count at the same time). */
+tc_status tc_count_noalloc(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
+
 /* tc_count_local: the same count, also attributing every triangle to its
  * edges, its corners and the parts of a split.  Outputs (host memory, each
  * optional):

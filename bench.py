@@ -465,6 +465,7 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
                 except Exception as e:  # surfaced after the run
                     errors.append(e)
 
+            st0 = srv.stats()
             start = clock_ms()
             ths = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
             for t in ths:
@@ -493,19 +494,24 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
             span = (done.max() - (sched[0] if sched is not None else start)) / 1e3
             light = lat[picks >= 0]
             heavy_lat = lat[picks < 0]
+            st1 = srv.stats()
+            # where the time went during this run: ms per second of the run's span
+            where = {k: (st1[k] - st0[k]) / span for k in
+                     ("submit_ms", "retire_ms", "admit_ms", "launch_ms", "wait_busy_ms", "wait_idle_ms", "device_ms")}
+            where["launches"] = st1["launches"] - st0["launches"]
             return {"graphs_per_s": n_requests / span, "p50_ms": float(np.percentile(lat, 50)),
                     "p99_ms": float(np.percentile(lat, 99)),
                     "light_p50_ms": float(np.percentile(light, 50)),
                     "light_p99_ms": float(np.percentile(light, 99)),
                     "heavy_p50_ms": float(np.percentile(heavy_lat, 50)) if heavy_lat.size else None,
                     "heavy_p99_ms": float(np.percentile(heavy_lat, 99)) if heavy_lat.size else None,
-                    "exact": ok, "clients": clients,
+                    "exact": ok, "clients": clients, "time_per_s": where,
                     "mean_batch": float(np.mean([got[i]["batch_graphs"] for i in range(n_requests)]))}
 
         sat = run(None, clients)
         out["saturated"] = sat
         log(f"serving saturated: {sat['graphs_per_s']:.0f} graphs/s, p99 {sat['p99_ms']:.1f} ms, "
-            f"exact={sat['exact']}")
+            f"exact={sat['exact']}; ms per s: " + ", ".join(f"{k} {v:.0f}" for k, v in sat["time_per_s"].items()))
         # load_50 / load_90: fractions of the batched capacity of this mix;
         # load_50_of_saturated / ...: of the server's own saturated rate
         for key, frac, base in (("load_50", 0.5, cap["graphs_per_s"]), ("load_90", 0.9, cap["graphs_per_s"]),

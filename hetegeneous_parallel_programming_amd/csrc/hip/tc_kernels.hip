@@ -884,6 +884,20 @@ __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
 // out-edge partners in row order, then its in-edge partners in any order),
 // so rows, work, planning and counting are unchanged.
 constexpr int64_t kKeysPerBlock = 65536;  // keys per block of the bhist and bucket passes
+constexpr int kKeyUnroll = 4;             // keys (or pairs) in flight per thread in the flat passes
+
+// Out-degree of every row slot, saturated at 65535 (8.4 MB for 4M rows, where
+// the rowptr is 33.6 MB): the split's per-edge degree gather reads this.
+constexpr int32_t kDegSat = 65535;
+__global__ __launch_bounds__(kBlock) void partner_deg_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= RP) return;
+  const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
+  const int64_t u = q - d.row_base;
+  int64_t du = 0;
+  if (u < d.n_rows) du = a.rp[d.rp_off + u + 1] - a.rp[d.rp_off + u];
+  b.deg16[q] = (uint16_t)min(du, (int64_t)kDegSat);
+}
 
 __global__ __launch_bounds__(kBlock) void partner_split_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
   const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
@@ -898,6 +912,7 @@ __global__ __launch_bounds__(kBlock) void partner_split_kernel(LaunchArgs a, int
     const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
     ListRef* out = b.out + d.eb;
     int32_t* key = b.key + d.eb;
+    const uint16_t* deg = b.deg16 + d.row_base;
     const ListRef uref = list_ref(d.ci_off + e0, du);
     const int shift = (int)(threadIdx.x & 63) & ~(kBuildG - 1);
     const uint32_t below = (1u << gl) - 1u, mask = (1u << kBuildG) - 1u;
@@ -909,10 +924,11 @@ __global__ __launch_bounds__(kBlock) void partner_split_kernel(LaunchArgs a, int
       bool own = false, in = false;
       if (e < e1) {
         v = ci[e];
-        vb = rp[v];
-        dv = rp[v + 1] - vb;
+        dv = deg[v];
+        if (dv == kDegSat && du >= kDegSat) dv = rp[v + 1] - rp[v];  // both saturated: exact
         own = dv <= du;
         in = !own;
+        if (own) vb = rp[v];  // the partner's list: only out-owned edges read the rowptr
       }
       const uint32_t go = (uint32_t)(__ballot(own) >> shift) & mask;
       const uint32_t gi = (uint32_t)(__ballot(in) >> shift) & mask;
@@ -938,9 +954,13 @@ __global__ __launch_bounds__(kBlock) void partner_bhist_kernel(BuildBufs b, int6
   for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0;
   __syncthreads();
   const int64_t i0 = (int64_t)blockIdx.x * kKeysPerBlock, i1 = min(i0 + kKeysPerBlock, M);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
-    const int32_t k = b.key[i];
-    if (k > 0) atomicAdd(&s_hist[(k - 1) >> bshift], 1u);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kKeyUnroll * kBlock) {
+    int32_t k[kKeyUnroll];
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j) k[j] = i + j * kBlock < i1 ? b.key[i + j * kBlock] : 0;
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j)
+      if (k[j] > 0) atomicAdd(&s_hist[(k[j] - 1) >> bshift], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += kBlock)
@@ -966,21 +986,33 @@ __global__ __launch_bounds__(kBlock) void partner_bucket_kernel(BuildBufs b, int
   for (int i = threadIdx.x; i < nb; i += kBlock) s_w[i] = 0;
   __syncthreads();
   const int64_t i0 = (int64_t)blockIdx.x * kKeysPerBlock, i1 = min(i0 + kKeysPerBlock, M);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
-    const int32_t k = b.key[i];
-    if (k > 0) atomicAdd(&s_w[(k - 1) >> bshift], 1u);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kKeyUnroll * kBlock) {
+    int32_t k[kKeyUnroll];
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j) k[j] = i + j * kBlock < i1 ? b.key[i + j * kBlock] : 0;
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j)
+      if (k[j] > 0) atomicAdd(&s_w[(k[j] - 1) >> bshift], 1u);
   }
   __syncthreads();
   // reserve this block's range of each bucket (pair positions are below 2^32)
   for (int i = threadIdx.x; i < nb; i += kBlock)
     if (s_w[i]) s_w[i] = (uint32_t)atomicAdd(&b.bcur[i], (unsigned long long)s_w[i]);
   __syncthreads();
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
-    const int32_t k = b.key[i];
-    if (k <= 0) continue;
-    const uint32_t slot = atomicAdd(&s_w[(k - 1) >> bshift], 1u);
-    b.pkey[slot] = k - 1;
-    b.pref[slot] = b.out[i];
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kKeyUnroll * kBlock) {
+    int32_t k[kKeyUnroll];
+    ListRef r[kKeyUnroll];
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j) k[j] = i + j * kBlock < i1 ? b.key[i + j * kBlock] : 0;
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j) r[j] = k[j] > 0 ? b.out[i + j * kBlock] : 0;
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j) {
+      if (k[j] <= 0) continue;
+      const uint32_t slot = atomicAdd(&s_w[(k[j] - 1) >> bshift], 1u);
+      b.pkey[slot] = k[j] - 1;
+      b.pref[slot] = r[j];
+    }
   }
 }
 
@@ -1065,10 +1097,21 @@ __global__ __launch_bounds__(kBlock) void partner_infill_kernel(LaunchArgs a, Bu
   }
   __syncthreads();
   ListRef* pci = (ListRef*)a.pci;
-  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) {
-    const int32_t k = b.pkey[p];
-    const uint32_t r = atomicAdd(&s_w[k - kb], 1u);
-    pci[b.cur[k] + r] = b.pref[p];
+  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kKeyUnroll * kBlock) {
+    int32_t k[kKeyUnroll];
+    ListRef ref[kKeyUnroll];
+    unsigned long long at[kKeyUnroll];
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j) {
+      const bool in = p + j * kBlock < c.p1;
+      k[j] = in ? b.pkey[p + j * kBlock] : -1;
+      ref[j] = in ? b.pref[p + j * kBlock] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j) at[j] = k[j] >= 0 ? b.cur[k[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < kKeyUnroll; ++j)
+      if (k[j] >= 0) pci[at[j] + atomicAdd(&s_w[k[j] - kb], 1u)] = ref[j];
   }
 }
 
@@ -1098,6 +1141,8 @@ static hipError_t launch_partners_bucketed(const LaunchArgs& a, int64_t RP, int6
   if (e == hipSuccess) e = launch_zero(b.btot, 2 * ((int64_t)nb + 1), s);
   if (e == hipSuccess) e = launch_zero((void*)a.vrow, a.P * (int64_t)(sizeof(VRow) / 4), s);  // slots unused
   if (e != hipSuccess) return e;
+  partner_deg_kernel<<<(unsigned)((RP + kBlock - 1) / kBlock), kBlock, 0, s>>>(a, RP, b);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   partner_split_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (kblocks > 0) {

@@ -188,7 +188,8 @@ struct BuildBufs {
   int32_t* pown;        // with pedge: [P] owner vertex of each partner row
   // Bucketed transpose (null when the scratch has no per-id part): per id
   // of the build (M = sum of the graphs' ids, graph g's at eb):
-  int32_t* key;         // [M] owner row slot of an edge its head owns, else -1
+  uint16_t* deg16;      // [RP] out-degree of each row slot, saturated at 65535
+  int32_t* key;         // [M] owner row slot + 1 of an edge its head owns, else 0
   ListRef* out;         // [M] out-edge partners of each row, compacted at its front
   int32_t* pkey;        // [M] owner row slots of the in-owned edges, grouped by bucket
   ListRef* pref;        // [M] their partners' lists, in the same order

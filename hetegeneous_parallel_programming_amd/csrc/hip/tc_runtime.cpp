@@ -140,11 +140,13 @@ int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs
 static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 static int64_t build_bytes(int64_t n, int64_t ids) {
   // per row: c1, kc (int32), c2 (int32 per XCD stripe), cur (int64 per
-  // stripe), off, voff (int64), scan partials; per id: key, pkey (int32),
-  // out, pref (list references); per bucket: totals, cursors, chunk starts
+  // stripe), off, voff (int64), scan partials; with the per-id part also
+  // deg16 per row and per id: key, pkey (int32), out, pref (list
+  // references); per bucket: totals, cursors, chunk starts
   int64_t b = up256(4 * n) * 2 + up256(4 * tck::kXcds * n) + up256(8 * tck::kXcds * n) + up256(8 * n) * 2 +
               up256(8 * tck::scan_partials(n));
-  if (ids > 0) b += up256(4 * ids) * 2 + up256(8 * ids) * 2 + up256(8 * (tck::kMaxBuckets + 1)) * 3;
+  if (ids > 0)
+    b += up256(2 * n) + up256(4 * ids) * 2 + up256(8 * ids) * 2 + up256(8 * (tck::kMaxBuckets + 1)) * 3;
   return b;
 }
 
@@ -173,6 +175,7 @@ tc_status BuildScratch::ensure(int64_t rp_slots, int64_t ids) {
   b.voff = (int64_t*)p; p += up256(8 * n);
   b.part = (int64_t*)p; p += up256(8 * tck::scan_partials(n));
   if (ids > 0) {
+    b.deg16 = (uint16_t*)p; p += up256(2 * n);
     b.key = (int32_t*)p; p += up256(4 * ids);
     b.pkey = (int32_t*)p; p += up256(4 * ids);
     b.out = (tck::ListRef*)p; p += up256(8 * ids);

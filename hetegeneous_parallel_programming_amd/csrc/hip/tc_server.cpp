@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
+#include <atomic>
 #include <deque>
 #include <thread>
 
@@ -85,6 +86,7 @@ struct Req {
 
 struct Slot {
   hipStream_t stream = nullptr;
+  hipEvent_t start = nullptr;        // timing events around a launch's device work
   hipEvent_t done = nullptr;
   void* plan = nullptr;              // device: items | bin_of | blk | part | counters | outs | descs | nv |
                                      //         partner rows | row offsets
@@ -117,6 +119,10 @@ struct tc_server {
   std::deque<tc_request_result> results;
   int64_t outstanding = 0, submitted = 0, completed = 0, rejected = 0, launches = 0, launched_graphs = 0;
   int64_t max_ids = 0;  // DAG ids per launch (cfg.max_batch_ids or its default)
+  // time accounting (ns): submit (callers, atomic); the server thread's
+  // phases, read by get_stats (atomic)
+  std::atomic<int64_t> t_submit{0}, t_retire{0}, t_admit{0}, t_launch{0}, t_wait_busy{0}, t_wait_idle{0},
+      t_device{0};
   bool stop = false;
   std::thread th;
 
@@ -162,6 +168,9 @@ bool tc_server::retire() {
     if (e == hipErrorNotReady) continue;
     const double t = now_ms();
     const int k = (int)s.reqs.size();
+    float dev_ms = 0;
+    if (e == hipSuccess && hipEventElapsedTime(&dev_ms, s.start, s.done) == hipSuccess)
+      t_device += (int64_t)((double)dev_ms * 1e6);
     for (int i = 0; i < k; ++i)
       finish(s.reqs[i], e == hipSuccess ? s.pinned[i] : 0, e == hipSuccess ? TC_OK : TC_EDEVICE, k, t);
     if (e != hipSuccess) (void)hipGetLastError();
@@ -292,6 +301,7 @@ bool tc_server::launch() {
       for (int j = 0; j < i && !seen; ++j) seen = s.reqs[j]->upload == up;
       if (up && !seen) e = hipStreamWaitEvent(s.stream, up->ev, 0);
     }
+    if (e == hipSuccess) e = hipEventRecord(s.start, s.stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
     // Partner structures of this launch's graphs, then plan and count.
@@ -324,22 +334,36 @@ bool tc_server::launch() {
   return any;
 }
 
+static int64_t ns_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void tc_server::run() {
   (void)hipSetDevice(ctx->device);
+  using clk = std::chrono::steady_clock;
   for (;;) {
+    auto t = clk::now();
     bool progressed = retire();
+    t_retire += ns_since(t);
+    t = clk::now();
     progressed |= admit();
+    t_admit += ns_since(t);
+    t = clk::now();
     progressed |= launch();
+    t_launch += ns_since(t);
     if (progressed) continue;
+    t = clk::now();
     std::unique_lock<std::mutex> lk(mu);
     const bool idle = waiting.empty() && ready.empty() &&
                       std::none_of(slots.begin(), slots.end(), [](const Slot& s) { return s.busy; });
     if (stop && idle) break;
     if (idle) {
       cv_work.wait_for(lk, std::chrono::milliseconds(50));
+      t_wait_idle += ns_since(t);
     } else {
       lk.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(20));  // launches in flight
+      t_wait_busy += ns_since(t);
     }
   }
 }
@@ -348,6 +372,7 @@ void tc_server::release() {
   for (auto& s : slots) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.done) (void)hipEventDestroy(s.done);
+    if (s.start) (void)hipEventDestroy(s.start);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     dev_free(s.plan);
     s.build.release();
@@ -427,7 +452,8 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
   s->slots.resize(cfg->slots);
   for (auto& sl : s->slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&sl.start);
+    if (e == hipSuccess) e = hipEventCreate(&sl.done);
     if (e == hipSuccess)
       e = hipHostMalloc((void**)&sl.pinned, 8 * G + sizeof(tck::GraphDesc) * G, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -456,8 +482,19 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
   return tcr::caught();
 }
 
+static tc_status submit_request(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check);
+
 tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check) try {
   if (!s || !dag) return fail(TC_EINVAL, "null argument");
+  const auto t0 = std::chrono::steady_clock::now();
+  const tc_status st = submit_request(s, dag, tag, check);
+  s->t_submit += ns_since(t0);
+  return st;
+} catch (...) {
+  return tcr::caught();
+}
+
+static tc_status submit_request(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check) {
   tc_status st = tch_validate(dag, check | TC_CHECK_SHAPE);
   if (st != TC_OK) {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -513,8 +550,6 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
   ++s->submitted;
   s->cv_work.notify_one();
   return TC_OK;
-} catch (...) {
-  return tcr::caught();
 }
 
 tc_status tc_server_submit_at(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check,
@@ -573,6 +608,13 @@ tc_status tc_server_get_stats(tc_server* s, tc_server_stats* o) try {
   o->pool_capacity = as.capacity;
   o->pool_used = as.used;
   o->pool_peak = as.peak_used;
+  o->submit_ms = s->t_submit.load() / 1e6;
+  o->retire_ms = s->t_retire.load() / 1e6;
+  o->admit_ms = s->t_admit.load() / 1e6;
+  o->launch_ms = s->t_launch.load() / 1e6;
+  o->wait_busy_ms = s->t_wait_busy.load() / 1e6;
+  o->wait_idle_ms = s->t_wait_idle.load() / 1e6;
+  o->device_ms = s->t_device.load() / 1e6;
   return TC_OK;
 } catch (...) {
   return tcr::caught();

@@ -88,7 +88,10 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", nat.c_i64), ("completed", nat.c_i64), ("rejected", nat.c_i64),
                 ("launches", nat.c_i64), ("queued", nat.c_i64), ("in_flight", nat.c_i64),
                 ("mean_batch_graphs", nat.c_dbl), ("pool_capacity", nat.c_u64),
-                ("pool_used", nat.c_u64), ("pool_peak", nat.c_u64)]
+                ("pool_used", nat.c_u64), ("pool_peak", nat.c_u64),
+                ("submit_ms", nat.c_dbl), ("retire_ms", nat.c_dbl), ("admit_ms", nat.c_dbl),
+                ("launch_ms", nat.c_dbl), ("wait_busy_ms", nat.c_dbl), ("wait_idle_ms", nat.c_dbl),
+                ("device_ms", nat.c_dbl)]
 
 
 _dev = None

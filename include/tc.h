@@ -262,6 +262,13 @@ typedef struct tc_server_stats {
   int64_t submitted, completed, rejected, launches, queued, in_flight;
   double mean_batch_graphs;
   uint64_t pool_capacity, pool_used, pool_peak;
+  /* where the time went since the server started (ms): callers' time inside
+   * tc_server_submit (checks + copy into staging, summed over threads); the
+   * server thread's time retiring finished launches, admitting requests,
+   * issuing launches, and sleeping with work in flight or waiting for work;
+   * device time of the launches (first command to completion, summed over
+   * launches, which overlap on the slots' streams) */
+  double submit_ms, retire_ms, admit_ms, launch_ms, wait_busy_ms, wait_idle_ms, device_ms;
 } tc_server_stats;
 
 /* Monotonic clock (ms) used for every timestamp of the serving loop. */

@@ -1633,9 +1633,48 @@ __global__ __launch_bounds__(kBlock) void read_kernel(const u32x4* __restrict__ 
   if (acc == 0x9E3779B9u) sink[(int64_t)blockIdx.x * kBlock + threadIdx.x] = acc;
 }
 
+// Staging copy of a server launch: block b finds its copy (the last with
+// first <= b) and moves one kStageChunk of it, 16 bytes per lane and load
+// (the reads cross the host link, so many are kept in flight), then the
+// copy's tail of 4-byte words.
+__global__ __launch_bounds__(kBlock) void stage_copy_kernel(const StageCopy* copies, int32_t n) {
+  const int64_t b = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (copies[mid].first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const StageCopy c = copies[lo];
+  const int64_t c0 = (b - c.first) * kStageChunk;
+  if (c0 >= c.bytes) return;
+  const int64_t c1 = min(c0 + kStageChunk, c.bytes);
+  const int64_t v1 = c0 + ((c1 - c0) & ~(int64_t)15);  // 16-byte part of this chunk
+  constexpr int kU = (int)(kStageChunk / (16 * kBlock));  // 16-byte words per lane
+  u32x4 x[kU];
+#pragma unroll
+  for (int k = 0; k < kU; ++k) {
+    const int64_t i = c0 + ((int64_t)k * kBlock + threadIdx.x) * 16;
+    if (i < v1) x[k] = *(const u32x4*)(c.src + i);
+  }
+#pragma unroll
+  for (int k = 0; k < kU; ++k) {
+    const int64_t i = c0 + ((int64_t)k * kBlock + threadIdx.x) * 16;
+    if (i < v1) *(u32x4*)(c.dst + i) = x[k];
+  }
+  for (int64_t i = v1 + 4 * (int64_t)threadIdx.x; i < c1; i += 4 * kBlock)
+    *(uint32_t*)(c.dst + i) = *(const uint32_t*)(c.src + i);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_stage_copy(const StageCopy* copies, int32_t n, int64_t blocks, hipStream_t s) {
+  if (n <= 0 || blocks <= 0) return hipSuccess;
+  stage_copy_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(copies, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s) {
   if (words32 <= 0) return hipSuccess;
   int64_t blocks = (words32 + kBlock - 1) / kBlock;

@@ -226,6 +226,22 @@ hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t M, const Bui
 
 // Launchers (stream-ordered; return the launch status).
 hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);
+
+// One request's CSR on its way from pinned host memory (src, mapped for the
+// device) to its extent of the device pool (dst): `bytes`, a multiple of 4,
+// both ends 16-byte aligned; blocks [first, first + ceil(bytes / kStageChunk))
+// of the staging copy kernel move it.
+struct StageCopy {
+  const char* src;
+  char* dst;
+  int64_t bytes;
+  int64_t first;
+};
+constexpr int64_t kStageChunk = (int64_t)64 << 10;  // bytes per block of the staging copy
+inline int64_t stage_blocks(int64_t bytes) { return (bytes + kStageChunk - 1) / kStageChunk; }
+// All copies of a launch in one kernel (`blocks` = the sum of their blocks;
+// copies on the device, ordered by first).
+hipError_t launch_stage_copy(const StageCopy* copies, int32_t n, int64_t blocks, hipStream_t s);
 // Re-arms a kept plan for another count: tile queue heads and the fallback
 // counter to 0 (bin counts kept), outs[0..nouts) to 0.
 hipError_t launch_rearm(uint32_t* counters, uint64_t* outs, int32_t nouts, hipStream_t s);

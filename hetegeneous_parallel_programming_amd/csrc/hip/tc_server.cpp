@@ -2,11 +2,14 @@
 // device-resident CSR pool.  One server thread owns all device work:
 //   retire  -- finished launches: deliver counts, free their pool extents;
 //   admit   -- waiting requests whose CSR fits the pool (arrival order, with
-//              bounded look-ahead past a blocked head) are uploaded on a
-//              copy stream while earlier launches keep running;
-//   launch  -- any free slot takes every uploaded request (up to the batch
-//              limits) and runs partner build -> zero -> plan -> per-bin
-//              count kernels and the copy-back on its own stream.
+//              bounded look-ahead past a blocked head) get their extent;
+//   launch  -- any free slot takes every admitted request (up to the batch
+//              limits) and runs, on its own stream: one staging-copy kernel
+//              that moves all of their CSRs from pinned host memory into
+//              the pool, partner build -> zero -> plan -> per-bin count
+//              kernels, and the copy-back.  (A request that did not fit the
+//              staging area was copied from pageable memory on a copy
+//              stream at admission; its launch waits for that.)
 // Launch slots run concurrently on separate streams, so requests arriving
 // while others are on the GPU overlap with them on the device.
 #include <algorithm>
@@ -38,29 +41,74 @@ struct Layout {
   }
 };
 
-// Copies a request into the pinned staging area on the submitting thread;
-// large requests are split over a few helper threads, since a single
-// memcpy moves ~9 GB/s and an RMAT-20 request holds ~71 MB of CSR.
+// A request's CSR is copied on the submitting thread (into the pinned
+// staging area, or a pageable buffer when that is full) by the same pass
+// that checks its shape: rowptr non-decreasing with out-degrees below
+// kMaxOutDegree, ids in [0, n).  One read of the caller's arrays, in plain
+// loops -- an OpenMP region per call, from several client threads at once,
+// costs far more than the loops themselves for graphs of a few thousand ids.
+// A large request is split over a few helper threads, since one thread
+// moves ~9 GB/s and an RMAT-20 request holds ~71 MB of CSR.
 constexpr size_t kCopySplit = (size_t)8 << 20;  // bytes per helper thread at least
 constexpr int kCopyThreads = 4;
-void copy_in(char* dst, const char* src, size_t n) {
-  const int nt = (int)std::min<size_t>(kCopyThreads, n / kCopySplit);
-  if (nt <= 1) {
-    std::memcpy(dst, src, n);
-    return;
+constexpr int64_t kMaxOutDegree = (int64_t)1 << 25;  // d * 64 < 2^31 in the plan's wedge-work field
+
+struct ShapeFlags {
+  uint32_t down = 0, wide = 0, id = 0;  // rowptr decreases / degree too large / id outside [0, n)
+};
+
+void check_copy(const tc_csr_view& g, int64_t* rp_dst, int32_t* ci_dst, int64_t v0, int64_t v1, int64_t e0,
+                int64_t e1, ShapeFlags& f) {
+  const int64_t* rp = g.rowptr;
+  uint32_t down = 0, wide = 0;
+  for (int64_t v = v0; v < v1; ++v) {
+    const int64_t d = rp[v + 1] - rp[v];
+    down |= d < 0;
+    wide |= d >= kMaxOutDegree;
+    rp_dst[v + 1] = rp[v + 1];
   }
-  const size_t per = (n + nt - 1) / nt;
+  const int32_t* ci = g.colidx;
+  const uint32_t n = (uint32_t)g.n_rows;
+  uint32_t id = 0;
+  for (int64_t e = e0; e < e1; ++e) {
+    const int32_t c = ci[e];
+    id |= (uint32_t)c >= n;
+    ci_dst[e] = c;
+  }
+  f = ShapeFlags{down, wide, id};
+}
+
+// Copies g (rowptr[0] == 0 and rowptr[n] == nnz already checked) to rp_dst /
+// ci_dst; returns null, or what is wrong with its shape.
+const char* stage_checked(const tc_csr_view& g, int64_t* rp_dst, int32_t* ci_dst) {
+  const int64_t n = g.n_rows, m = g.nnz;
+  rp_dst[0] = 0;
+  const size_t bytes = sizeof(int64_t) * (size_t)n + sizeof(int32_t) * (size_t)m;
+  const int nt = (int)std::max<size_t>(1, std::min<size_t>(kCopyThreads, bytes / kCopySplit));
+  ShapeFlags f[kCopyThreads];
+  auto part = [&](int t) {
+    check_copy(g, rp_dst, ci_dst, n * t / nt, n * (t + 1) / nt, m * t / nt, m * (t + 1) / nt, f[t]);
+  };
   std::vector<std::thread> helpers;
   for (int t = 1; t < nt; ++t) {
-    const size_t a = per * t, b = std::min(n, a + per);
     try {
-      helpers.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
-    } catch (...) {  // no thread to be had: copy this part here
-      std::memcpy(dst + a, src + a, b - a);
+      helpers.emplace_back(part, t);
+    } catch (...) {  // no thread to be had: this part runs here
+      part(t);
     }
   }
-  std::memcpy(dst, src, std::min(n, per));
+  part(0);
   for (auto& h : helpers) h.join();
+  ShapeFlags all;
+  for (int t = 0; t < nt; ++t) {
+    all.down |= f[t].down;
+    all.wide |= f[t].wide;
+    all.id |= f[t].id;
+  }
+  if (all.down) return "rowptr is not monotone";
+  if (all.wide) return "DAG out-degree too large";
+  if (all.id) return "neighbour id out of range [0, n)";
+  return nullptr;
 }
 
 // One admission's host-to-device copies: launches of its requests wait on
@@ -79,6 +127,7 @@ struct Req {
   std::vector<int32_t> ci;
   char* staged = nullptr;   // or: copy in the pinned staging area
   uint64_t stage_off = 0;
+  uint64_t stage_bytes = 0;  // rowptr | pad | colidx, laid out as in the pool
   uint64_t off = 0, bytes = 0;
   Upload* upload = nullptr;
   double t_submit = 0, t_admit = 0, t_launch = 0;
@@ -90,7 +139,7 @@ struct Slot {
   hipEvent_t done = nullptr;
   void* plan = nullptr;              // device: items | bin_of | blk | part | counters | outs | descs | nv |
                                      //         partner rows | row offsets
-  uint64_t* pinned = nullptr;        // host: outs[G] then descs staging
+  uint64_t* pinned = nullptr;        // host: outs[G], then descs[k] and stage copies[k] of a launch
   tck::PlanBufs bufs{};
   tck::GraphDesc* d_descs = nullptr;
   int32_t* d_nv = nullptr;
@@ -109,6 +158,7 @@ struct tc_server {
   char* pool = nullptr;
   tch_arena* arena = nullptr;
   char* staging = nullptr;         // pinned host memory
+  const char* staging_dev = nullptr;  // the same memory, mapped for the device
   tch_arena* stage_arena = nullptr;
   hipStream_t copy_stream = nullptr;
   std::vector<hipEvent_t> free_events;
@@ -200,25 +250,25 @@ bool tc_server::admit() {
     }
   }
   if (take.empty()) return false;
-  // Copies go out on the copy stream without waiting: one event marks their
+  // Staged requests are copied by their launch's staging kernel.  Pageable
+  // ones go out on the copy stream without waiting: one event marks their
   // end, and the launches of these requests wait on it on the device.
   hipError_t e = hipSuccess;
+  bool pageable = false;
   for (Req* r : take) {
+    if (r->staged) continue;
+    pageable = true;
     const Layout L(r->n, r->nnz);
     char* base = pool + r->off;
-    if (r->staged) {  // rowptr | pad | colidx, laid out as in the pool
-      e = hipMemcpyAsync(base, r->staged, L.pci, hipMemcpyHostToDevice, copy_stream);
-    } else {
-      e = hipMemcpyAsync(base, r->rp.data(), sizeof(int64_t) * ((size_t)r->n + 1), hipMemcpyHostToDevice,
+    e = hipMemcpyAsync(base, r->rp.data(), sizeof(int64_t) * ((size_t)r->n + 1), hipMemcpyHostToDevice,
+                       copy_stream);
+    if (e == hipSuccess && r->nnz > 0)
+      e = hipMemcpyAsync(base + L.ci, r->ci.data(), sizeof(int32_t) * (size_t)r->nnz, hipMemcpyHostToDevice,
                          copy_stream);
-      if (e == hipSuccess && r->nnz > 0)
-        e = hipMemcpyAsync(base + L.ci, r->ci.data(), sizeof(int32_t) * (size_t)r->nnz, hipMemcpyHostToDevice,
-                           copy_stream);
-    }
     if (e != hipSuccess) break;
   }
   Upload* up = nullptr;
-  if (e == hipSuccess) {
+  if (e == hipSuccess && pageable) {
     up = new Upload;
     if (!free_events.empty()) {
       up->ev = free_events.back();
@@ -243,8 +293,10 @@ bool tc_server::admit() {
       finish(r, 0, TC_EDEVICE, 0, t);
       continue;
     }
-    r->upload = up;
-    ++up->refs;
+    if (!r->staged) {
+      r->upload = up;
+      ++up->refs;
+    }
     r->t_admit = t;
     ready.push_back(r);
   }
@@ -265,8 +317,12 @@ bool tc_server::launch() {
     }
     if (s.reqs.empty()) continue;
     const int k = (int)s.reqs.size();
+    // descs[k] then the staging copies, in pinned memory and (one copy) on the device
     tck::GraphDesc* hd = (tck::GraphDesc*)(s.pinned + cfg.max_batch_graphs);
-    int64_t pb = 0, rb = 0, eb = 0;
+    const uint64_t o_copies = align_up(sizeof(tck::GraphDesc) * k);
+    tck::StageCopy* hc = (tck::StageCopy*)((char*)hd + o_copies);
+    int32_t ncopies = 0;
+    int64_t pb = 0, rb = 0, eb = 0, cblocks = 0;
     for (int i = 0; i < k; ++i) {
       Req* r = s.reqs[i];
       const Layout L(r->n, r->nnz);
@@ -275,6 +331,10 @@ bool tc_server::launch() {
       pb += (int64_t)r->nv_max + 1;
       rb += (int64_t)r->n + 1;
       eb += r->nnz;
+      if (r->staged) {
+        hc[ncopies++] = tck::StageCopy{staging_dev + r->stage_off, pool + r->off, (int64_t)r->stage_bytes, cblocks};
+        cblocks += tck::stage_blocks((int64_t)r->stage_bytes);
+      }
     }
     tck::LaunchArgs a{};
     a.descs = s.d_descs;
@@ -303,7 +363,11 @@ bool tc_server::launch() {
     }
     if (e == hipSuccess) e = hipEventRecord(s.start, s.stream);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(s.d_descs, hd, sizeof(tck::GraphDesc) * k, hipMemcpyHostToDevice, s.stream);
+      e = hipMemcpyAsync(s.d_descs, hd, o_copies + sizeof(tck::StageCopy) * ncopies, hipMemcpyHostToDevice,
+                         s.stream);
+    if (e == hipSuccess)
+      e = tck::launch_stage_copy((const tck::StageCopy*)((char*)s.d_descs + o_copies), ncopies, cblocks,
+                                 s.stream);
     // Partner structures of this launch's graphs, then plan and count.
     if (e == hipSuccess)
       e = s.build.ensure(rb, max_ids) == TC_OK ? tck::launch_partners(a, rb, eb, s.build.b, s.stream)
@@ -319,7 +383,7 @@ bool tc_server::launch() {
     if (e != hipSuccess) {
       (void)hipGetLastError();
       (void)hipStreamSynchronize(s.stream);
-      (void)hipStreamSynchronize(copy_stream);  // staging is released below
+      (void)hipStreamSynchronize(copy_stream);  // pageable copies may still read their buffers
       const double tf = now_ms();
       for (Req* r : s.reqs) finish(r, 0, TC_EDEVICE, k, tf);
       s.reqs.clear();
@@ -435,19 +499,23 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
     return fail(TC_EDEVICE, "copy stream creation failed");
   }
   if (cfg->host_staging >= kAlign) {
+    void* dev = nullptr;
     if (hipHostMalloc((void**)&s->staging, cfg->host_staging, hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer(&dev, s->staging, 0) != hipSuccess ||
         tch_arena_create(cfg->host_staging, kAlign, &s->stage_arena) != TC_OK) {
       (void)hipGetLastError();
       s->release();
       return fail(TC_ENOMEM, "pinned staging allocation failed");
     }
+    s->staging_dev = (const char*)dev;
   }
   const int64_t R = cfg->max_batch_rows, G = cfg->max_batch_graphs;
   const int64_t nb = tck::kNumBins * tck::plan_blocks(R) + 1;
   const uint64_t o_items = 0, o_bin = o_items + align_up(8 * R), o_blk = o_bin + align_up(2 * R),
                  o_part = o_blk + align_up(4 * nb), o_cnt = o_part + align_up(8 * tck::scan_partials(nb)),
                  o_outs = o_cnt + align_up(4 * 64), o_desc = o_outs + align_up(8 * G),
-                 o_nv = o_desc + align_up(sizeof(tck::GraphDesc) * G), o_vrow = o_nv + align_up(4 * G),
+                 o_nv = o_desc + align_up(sizeof(tck::GraphDesc) * G) + align_up(sizeof(tck::StageCopy) * G),
+                 o_vrow = o_nv + align_up(4 * G),
                  o_vrp = o_vrow + align_up(sizeof(tck::VRow) * R), total = o_vrp + align_up(8 * (R + 1));
   s->slots.resize(cfg->slots);
   for (auto& sl : s->slots) {
@@ -455,7 +523,9 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
     if (e == hipSuccess) e = hipEventCreate(&sl.start);
     if (e == hipSuccess) e = hipEventCreate(&sl.done);
     if (e == hipSuccess)
-      e = hipHostMalloc((void**)&sl.pinned, 8 * G + sizeof(tck::GraphDesc) * G, hipHostMallocDefault);
+      e = hipHostMalloc((void**)&sl.pinned,
+                        8 * G + align_up(sizeof(tck::GraphDesc) * G) + sizeof(tck::StageCopy) * G,
+                        hipHostMallocDefault);
     if (e != hipSuccess) {
       s->release();
       return fail(TC_EDEVICE, std::string("server slot: ") + hipGetErrorString(e));
@@ -495,28 +565,30 @@ tc_status tc_server_submit(tc_server* s, const tc_csr_view* dag, uint64_t tag, u
 }
 
 static tc_status submit_request(tc_server* s, const tc_csr_view* dag, uint64_t tag, uint32_t check) {
-  tc_status st = tch_validate(dag, check | TC_CHECK_SHAPE);
-  if (st != TC_OK) {
-    std::lock_guard<std::mutex> lk(s->mu);
-    ++s->rejected;
-    return fail(st, tch_last_error());
-  }
-  const uint64_t fp = tc_request_footprint(dag->n_rows, dag->nnz);
   auto reject = [&](tc_status code, const std::string& msg) {
     std::lock_guard<std::mutex> lk(s->mu);
     ++s->rejected;
     return fail(code, msg);
   };
+  const tc_csr_view& g = *dag;
+  // constant-time shape checks here; the O(n + nnz) ones run with the copy
+  if (g.n_rows < 0 || g.nnz < 0) return reject(TC_EINVAL, "negative n_rows or nnz");
+  if (!g.rowptr) return reject(TC_EINVAL, "null rowptr");
+  if (g.nnz > 0 && !g.colidx) return reject(TC_EINVAL, "null colidx");
+  if (g.rowptr[0] != 0) return reject(TC_EINVAL, "rowptr[0] != 0");
+  if (g.rowptr[g.n_rows] != g.nnz) return reject(TC_EINVAL, "rowptr[n] != nnz");
+  if (check & ~(uint32_t)TC_CHECK_SHAPE) {  // row order / self-loop checks: the host library's pass
+    const tc_status st = tch_validate(dag, check | TC_CHECK_SHAPE);
+    if (st != TC_OK) return reject(st, tch_last_error());
+  }
+  const uint64_t fp = tc_request_footprint(g.n_rows, g.nnz);
   tch_arena_stats as;
   tch_arena_stats_get(s->arena, &as);
   if (fp > as.capacity) return reject(TC_ENOMEM, "request footprint exceeds the server's HBM pool");
-  const int32_t nvm = tck::partner_rows_max(dag->n_rows, dag->nnz);
+  const int32_t nvm = tck::partner_rows_max(g.n_rows, g.nnz);
   if ((int64_t)nvm + 1 > s->cfg.max_batch_rows)
     return reject(TC_ERANGE, "request needs more plan rows than max_batch_rows (rows + nnz/1024 + 2)");
-  if (dag->nnz > s->max_ids) return reject(TC_ERANGE, "request has more DAG ids than max_batch_ids");
-  for (int64_t v = 0; v < dag->n_rows; ++v)
-    if ((dag->rowptr[v + 1] - dag->rowptr[v]) * 64 >= ((int64_t)1 << 31))
-      return reject(TC_ERANGE, "DAG out-degree too large");
+  if (g.nnz > s->max_ids) return reject(TC_ERANGE, "request has more DAG ids than max_batch_ids");
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return fail(TC_ESTATE, "server is shutting down");
@@ -528,21 +600,32 @@ static tc_status submit_request(tc_server* s, const tc_csr_view* dag, uint64_t t
   }
   std::unique_ptr<Req> r(new Req);
   r->tag = tag;
-  r->n = dag->n_rows;
+  r->n = g.n_rows;
   r->nv_max = nvm;
-  r->nnz = dag->nnz;
+  r->nnz = g.nnz;
   r->bytes = fp;
   uint64_t so = 0;
-  if (s->stage_arena && tch_arena_alloc(s->stage_arena, Layout(dag->n_rows, dag->nnz).pci, &so) == TC_OK) {
+  int64_t* rp_dst;
+  int32_t* ci_dst;
+  if (s->stage_arena && tch_arena_alloc(s->stage_arena, Layout(g.n_rows, g.nnz).pci, &so) == TC_OK) {
     r->stage_off = so;
     r->staged = s->staging + so;
-    const uint64_t rp_bytes = sizeof(int64_t) * ((uint64_t)dag->n_rows + 1);
-    copy_in(r->staged, (const char*)dag->rowptr, rp_bytes);
-    if (dag->nnz)
-      copy_in(r->staged + align_up(rp_bytes), (const char*)dag->colidx, sizeof(int32_t) * (size_t)dag->nnz);
+    r->stage_bytes = Layout(g.n_rows, g.nnz).pci;
+    rp_dst = (int64_t*)r->staged;
+    ci_dst = (int32_t*)(r->staged + align_up(sizeof(int64_t) * ((uint64_t)g.n_rows + 1)));
   } else {
-    r->rp.assign(dag->rowptr, dag->rowptr + dag->n_rows + 1);
-    r->ci.assign(dag->colidx, dag->colidx + dag->nnz);
+    r->rp.resize((size_t)g.n_rows + 1);
+    r->ci.resize((size_t)g.nnz);
+    rp_dst = r->rp.data();
+    ci_dst = r->ci.data();
+  }
+  if (const char* bad = stage_checked(g, rp_dst, ci_dst)) {
+    if (r->staged) tch_arena_free(s->stage_arena, r->stage_off);
+    std::lock_guard<std::mutex> lk(s->mu);
+    --s->outstanding;
+    ++s->rejected;
+    s->cv_done.notify_all();
+    return fail(TC_EINVAL, bad);
   }
   r->t_submit = now_ms();
   std::lock_guard<std::mutex> lk(s->mu);
@@ -595,8 +678,9 @@ tc_status tc_server_drain(tc_server* s, int32_t timeout_ms) try {
 
 tc_status tc_server_get_stats(tc_server* s, tc_server_stats* o) try {
   if (!s || !o) return fail(TC_EINVAL, "null argument");
-  tch_arena_stats as;
+  tch_arena_stats as, ss{};
   tch_arena_stats_get(s->arena, &as);
+  if (s->stage_arena) tch_arena_stats_get(s->stage_arena, &ss);
   std::lock_guard<std::mutex> lk(s->mu);
   o->submitted = s->submitted;
   o->completed = s->completed;
@@ -615,6 +699,7 @@ tc_status tc_server_get_stats(tc_server* s, tc_server_stats* o) try {
   o->wait_busy_ms = s->t_wait_busy.load() / 1e6;
   o->wait_idle_ms = s->t_wait_idle.load() / 1e6;
   o->device_ms = s->t_device.load() / 1e6;
+  o->staging_used = ss.used;
   return TC_OK;
 } catch (...) {
   return tcr::caught();

@@ -91,7 +91,7 @@ class ServerStats(ctypes.Structure):
                 ("pool_used", nat.c_u64), ("pool_peak", nat.c_u64),
                 ("submit_ms", nat.c_dbl), ("retire_ms", nat.c_dbl), ("admit_ms", nat.c_dbl),
                 ("launch_ms", nat.c_dbl), ("wait_busy_ms", nat.c_dbl), ("wait_idle_ms", nat.c_dbl),
-                ("device_ms", nat.c_dbl)]
+                ("device_ms", nat.c_dbl), ("staging_used", nat.c_u64)]
 
 
 _dev = None
@@ -446,6 +446,13 @@ class Server:
 
     def submit(self, g: Graph, tag: int, check: int = nat.TC_CHECK_SHAPE) -> None:
         _check(device_lib().tc_server_submit(self._h, ctypes.byref(g.view()), tag, check))
+
+    def submit_arrays(self, rowptr, colidx, tag: int, check: int = nat.TC_CHECK_SHAPE) -> None:
+        """Submit a DAG given as CSR arrays (copied before this returns)."""
+        rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+        ci = np.ascontiguousarray(colidx, dtype=np.int32)
+        v = CsrView(rp.ctypes.data_as(nat.i64p), ci.ctypes.data_as(nat.i32p), rp.size - 1, ci.size)
+        _check(device_lib().tc_server_submit(self._h, ctypes.byref(v), tag, check))
 
     def try_submit(self, g: Graph, tag: int, check: int = nat.TC_CHECK_SHAPE) -> int:
         """Status code instead of an exception (TC_EBUSY when the queue is full)."""

@@ -269,6 +269,7 @@ typedef struct tc_server_stats {
    * device time of the launches (first command to completion, summed over
    * launches, which overlap on the slots' streams) */
   double submit_ms, retire_ms, admit_ms, launch_ms, wait_busy_ms, wait_idle_ms, device_ms;
+  uint64_t staging_used;     /* pinned staging bytes held by requests not yet uploaded */
 } tc_server_stats;
 
 /* Monotonic clock (ms) used for every timestamp of the serving loop. */

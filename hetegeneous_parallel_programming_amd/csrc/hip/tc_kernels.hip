@@ -540,6 +540,10 @@ __global__ void zero_kernel(uint32_t* p, int64_t n) {
 // planning: bin every pivot, then scatter (graph, row) items grouped by bin
 // ---------------------------------------------------------------------------
 // Largest g with row_base <= q (build row slots are in graph order).
+// Graph of a row slot from the radix build's gid (~graph marks a row that is
+// not split).
+__device__ __forceinline__ int gid_graph(int32_t x) { return x >= 0 ? x : ~x; }
+
 __device__ __forceinline__ int find_graph_rows(const GraphDesc* descs, int ngraphs, int64_t q) {
   int lo = 0, hi = ngraphs - 1;
   while (lo < hi) {
@@ -798,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void partner_cursor_kernel(LaunchArgs a, in
 __global__ __launch_bounds__(kBlock) void partner_rows_count_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= RP) return;
-  const int g = find_graph_rows(a.descs, a.ngraphs, q);
+  const int g = b.gid ? gid_graph(b.gid[q]) : find_graph_rows(a.descs, a.ngraphs, q);
   const GraphDesc d = a.descs[g];
   const int64_t u = q - d.row_base;
   int32_t k = 0;
@@ -815,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void partner_rows_count_kernel(LaunchArgs a
 __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= RP) return;
-  const int g = find_graph_rows(a.descs, a.ngraphs, q);
+  const int g = b.gid ? gid_graph(b.gid[q]) : find_graph_rows(a.descs, a.ngraphs, q);
   const GraphDesc d = a.descs[g];
   const int64_t u = q - d.row_base;
   const int64_t pbase = b.off[d.row_base], vbase = b.voff[d.row_base];
@@ -856,262 +860,498 @@ __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// partner build by a bucketed transpose (no per-edge global atomics)
+// partner build by a radix transpose (no per-edge global atomics)
 // ---------------------------------------------------------------------------
-// Most edges of a degree-ordered DAG are owned by their head (77% on RMAT-22),
-// so the build is mostly a transpose: in-owned edges grouped by owner.
-// Per-edge atomics on owner counters make that transpose wait on the memory
-// side's atomic rate (and on hub vertices).  Here instead:
-//   split   -- per owned row (16 lanes; blocks of 16 rows, so the dispatcher
-//              balances rows of any length): out-owned partners compacted at
-//              the front of the row's slots of `out` (row order, ballot
-//              ranks), counted in c1; in-owned edges compacted at the back of
-//              the same slots -- the partner's list in `out`, the owner's row
-//              slot + 1 in `key` (key is 0 elsewhere);
-//   bhist   -- flat over `key`, even blocks of kKeysPerBlock: pairs per owner
-//              bucket (block LDS histogram, one atomic per (block, bucket));
-//   bucket  -- the same blocks place every pair in its bucket's range of
-//              pkey/pref (one reservation per (block, bucket), LDS ranks);
-//   incount -- per chunk of a bucket, an LDS histogram over the bucket's
-//              owners gives c2 (a store, or one atomic per owner and chunk
-//              when the bucket has several chunks);
-//   (scan of c1 + c2: every row's partner offset, as for the atomic path)
-//   outcopy -- each row's out-owned partners moved to its partner range, and
-//              the first in-partner slot of every row;
-//   infill  -- per chunk, the LDS histogram again reserves each owner's slots
-//              (per owner and chunk), LDS ranks place the pairs.
-// The partner pool comes out laid out as by the atomic path (a row's
-// out-edge partners in row order, then its in-edge partners in any order),
-// so rows, work, planning and counting are unchanged.
-constexpr int64_t kKeysPerBlock = 65536;  // keys per block of the bhist and bucket passes
-constexpr int kKeyUnroll = 4;             // keys (or pairs) in flight per thread in the flat passes
+// Every DAG edge of a split row is one pair (owner row slot, partner row
+// slot): the endpoint with the longer out-list owns it, and the other
+// endpoint's list joins the owner's partner range.  The partner pool is the
+// pairs grouped by owner, so the build is a transpose of the pairs:
+//   prep    -- per row slot: its graph (negated when the row is not split),
+//              first build-space id, own list and saturated degree;
+//   mpath   -- merge-path split points of the sequence (row ends + ids) in
+//              blocks of kMP items;
+//   split   -- per block of kMP items, whatever the row lengths: each id's
+//              pair (an out-list degree gather decides the owner), and the
+//              block's pair count per coarse bucket;
+//   level 1 -- each split block's pairs sorted by coarse bucket in LDS and
+//              written as one run per bucket, at positions from the scanned
+//              (bucket-major) counts;
+//   level 2 -- the same within each coarse bucket, by fine bucket, over
+//              tiles of kTile pairs;
+//   final   -- a fine bucket of at most `cap` pairs is sorted by owner in
+//              LDS and written out whole: its owners' partner ranges are one
+//              run of the pool.  A bigger one (hub owners) is counted per
+//              owner by chunks, scanned, and then each chunk is sorted in
+//              LDS and every owner's run reserved on the owner's cursor.
+// Fine bucket = owner >> fs (at most kCoarse * kFine of them), coarse
+// bucket = fine bucket >> 6.  Within an owner the partner order is whatever
+// the LDS ranks give; no count depends on it.
+//
+// Positions: pairs are numbered in owner order over all row slots of the
+// build; the pool position of pair number p of graph g is pci_off(g) -
+// eb(g) + p.  That takes graph g's pairs to start at eb(g): true for a
+// single graph (eb = 0) and for launches of several graphs, where every
+// row is split (batches and the server), so graph g's pairs are the ids of
+// the graphs before it.
 
-// Out-degree of every row slot, saturated at 65535 (8.4 MB for 4M rows, where
-// the rowptr is 33.6 MB): the split's per-edge degree gather reads this.
-constexpr int32_t kDegSat = 65535;
-__global__ __launch_bounds__(kBlock) void partner_deg_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+constexpr int32_t kDegSat = 65535;  // deg16 saturation
+
+// Exclusive scan of n (<= kBlock * per) values of s (in place); returns the
+// sum.  All threads of the block call it.
+template <int kPer>
+__device__ uint32_t block_scan_lds(uint32_t* s, int n, int64_t* s_wave) {
+  const int i0 = threadIdx.x * kPer;
+  uint32_t v[kPer];
+  uint32_t t = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    v[k] = i0 + k < n ? s[i0 + k] : 0u;
+    t += v[k];
+  }
+  int64_t total;
+  uint32_t run = (uint32_t)block_excl_scan((int64_t)t, s_wave, &total);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    if (i0 + k < n) s[i0 + k] = run;
+    run += v[k];
+  }
+  __syncthreads();
+  return (uint32_t)total;
+}
+
+__global__ __launch_bounds__(kBlock) void radix_prep_kernel(LaunchArgs a, int64_t RP, int64_t M, BuildBufs b) {
+  // graphs of this block's first and last slot (uniform), then each lane's
+  const int64_t qa = (int64_t)blockIdx.x * kBlock, qz = min(qa + kBlock, RP) - 1;
+  int lo = find_graph_rows(a.descs, a.ngraphs, qa), hi = find_graph_rows(a.descs, a.ngraphs, qz);
+  const int64_t q = qa + threadIdx.x;
   if (q >= RP) return;
-  const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
-  const int64_t u = q - d.row_base;
-  int64_t du = 0;
-  if (u < d.n_rows) du = a.rp[d.rp_off + u + 1] - a.rp[d.rp_off + u];
-  b.deg16[q] = (uint16_t)min(du, (int64_t)kDegSat);
-}
-
-__global__ __launch_bounds__(kBlock) void partner_split_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
-  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
-  const int gl = threadIdx.x % kBuildG;
-  if (q >= RP) return;  // the whole group
-  const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
-  const int64_t u = q - d.row_base;
-  int32_t c1 = 0;
-  if (u < d.n_rows && u >= d.own_begin && u < d.own_end && (!a.own || a.own[u])) {
-    const int64_t* rp = a.rp + d.rp_off;
-    const int32_t* ci = a.ci + d.ci_off;
-    const int64_t e0 = rp[u], e1 = rp[u + 1], du = e1 - e0;
-    ListRef* out = b.out + d.eb;
-    int32_t* key = b.key + d.eb;
-    const uint16_t* deg = b.deg16 + d.row_base;
-    const ListRef uref = list_ref(d.ci_off + e0, du);
-    const int shift = (int)(threadIdx.x & 63) & ~(kBuildG - 1);
-    const uint32_t below = (1u << gl) - 1u, mask = (1u << kBuildG) - 1u;
-    int32_t cin = 0;
-    for (int64_t eb = e0; eb < e1; eb += kBuildG) {
-      const int64_t e = eb + gl;
-      int v = 0;
-      int64_t vb = 0, dv = 0;
-      bool own = false, in = false;
-      if (e < e1) {
-        v = ci[e];
-        dv = deg[v];
-        if (dv == kDegSat && du >= kDegSat) dv = rp[v + 1] - rp[v];  // both saturated: exact
-        own = dv <= du;
-        in = !own;
-        if (own) vb = rp[v];  // the partner's list: only out-owned edges read the rowptr
-      }
-      const uint32_t go = (uint32_t)(__ballot(own) >> shift) & mask;
-      const uint32_t gi = (uint32_t)(__ballot(in) >> shift) & mask;
-      if (own) {
-        const int64_t at = e0 + c1 + __popc(go & below);
-        out[at] = list_ref(d.ci_off + vb, dv);
-        key[at] = 0;
-      } else if (in) {
-        const int64_t at = e1 - 1 - (cin + __popc(gi & below));
-        out[at] = uref;
-        key[at] = (int32_t)(d.row_base + v) + 1;
-      }
-      c1 += __popc(go);
-      cin += __popc(gi);
-    }
-  }
-  if (gl == 0) b.c1[q] = c1;
-}
-
-// Pairs per owner bucket in keys [blockIdx.x * kKeysPerBlock, ...).
-__global__ __launch_bounds__(kBlock) void partner_bhist_kernel(BuildBufs b, int64_t M, int bshift, int nb) {
-  extern __shared__ uint32_t s_hist[];  // [nb]
-  for (int i = threadIdx.x; i < nb; i += kBlock) s_hist[i] = 0;
-  __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * kKeysPerBlock, i1 = min(i0 + kKeysPerBlock, M);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kKeyUnroll * kBlock) {
-    int32_t k[kKeyUnroll];
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j) k[j] = i + j * kBlock < i1 ? b.key[i + j * kBlock] : 0;
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j)
-      if (k[j] > 0) atomicAdd(&s_hist[(k[j] - 1) >> bshift], 1u);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += kBlock)
-    if (s_hist[i]) atomicAdd(&b.btot[i], (unsigned long long)s_hist[i]);
-}
-
-// After the scan of btot (now bucket starts, btot[nb] = all pairs): bucket
-// fill cursors, and the chunks of each bucket (cstart[nb] = 0, scanned next).
-__global__ __launch_bounds__(kBlock) void bucket_chunks_kernel(BuildBufs b, int nb) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < nb) {
-    const unsigned long long p0 = b.btot[i], p1 = b.btot[i + 1];
-    b.bcur[i] = p0;
-    b.cstart[i] = (int64_t)((p1 - p0 + kChunkPairs - 1) / kChunkPairs);
-  } else if (i == nb) {
-    b.cstart[nb] = 0;
-  }
-}
-
-// Same blocks as bhist: this block's pairs to their buckets.
-__global__ __launch_bounds__(kBlock) void partner_bucket_kernel(BuildBufs b, int64_t M, int bshift, int nb) {
-  extern __shared__ uint32_t s_w[];  // [nb] pairs per bucket, then the next slot of each
-  for (int i = threadIdx.x; i < nb; i += kBlock) s_w[i] = 0;
-  __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * kKeysPerBlock, i1 = min(i0 + kKeysPerBlock, M);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kKeyUnroll * kBlock) {
-    int32_t k[kKeyUnroll];
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j) k[j] = i + j * kBlock < i1 ? b.key[i + j * kBlock] : 0;
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j)
-      if (k[j] > 0) atomicAdd(&s_w[(k[j] - 1) >> bshift], 1u);
-  }
-  __syncthreads();
-  // reserve this block's range of each bucket (pair positions are below 2^32)
-  for (int i = threadIdx.x; i < nb; i += kBlock)
-    if (s_w[i]) s_w[i] = (uint32_t)atomicAdd(&b.bcur[i], (unsigned long long)s_w[i]);
-  __syncthreads();
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kKeyUnroll * kBlock) {
-    int32_t k[kKeyUnroll];
-    ListRef r[kKeyUnroll];
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j) k[j] = i + j * kBlock < i1 ? b.key[i + j * kBlock] : 0;
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j) r[j] = k[j] > 0 ? b.out[i + j * kBlock] : 0;
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j) {
-      if (k[j] <= 0) continue;
-      const uint32_t slot = atomicAdd(&s_w[(k[j] - 1) >> bshift], 1u);
-      b.pkey[slot] = k[j] - 1;
-      b.pref[slot] = r[j];
-    }
-  }
-}
-
-// Chunk blockIdx.x of the bucketed pairs: its bucket, pair range and whether
-// it is its bucket's only chunk.  False for blocks past the last chunk.
-struct PairChunk {
-  int bk;
-  int64_t p0, p1;
-  bool single;
-};
-__device__ __forceinline__ bool pair_chunk(const BuildBufs& b, int nb, PairChunk& c) {
-  const int64_t i = blockIdx.x;
-  if (i >= b.cstart[nb]) return false;
-  int lo = 0, hi = nb - 1;  // last bucket with cstart <= i
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (b.cstart[mid] <= i) lo = mid;
+    if (a.descs[mid].row_base <= q) lo = mid;
     else hi = mid - 1;
   }
-  const int64_t k = i - b.cstart[lo];
-  const int64_t bp0 = (int64_t)b.btot[lo], bp1 = (int64_t)b.btot[lo + 1];
-  c.bk = lo;
-  c.p0 = bp0 + k * kChunkPairs;
-  c.p1 = min(bp1, c.p0 + kChunkPairs);
-  c.single = b.cstart[lo + 1] - b.cstart[lo] == 1;
+  const GraphDesc d = a.descs[lo];
+  const int64_t u = q - d.row_base;
+  const int64_t* rp = a.rp + d.rp_off;
+  const int64_t e0 = rp[u];  // u <= n_rows
+  int64_t du = 0;
+  bool split = false;
+  if (u < d.n_rows) {
+    du = rp[u + 1] - e0;
+    split = u >= d.own_begin && u < d.own_end && (!a.own || a.own[u]);
+  }
+  b.gid[q] = split ? lo : ~lo;
+  b.erow[q] = d.eb + e0;
+  b.lref[q] = u < d.n_rows ? list_ref(d.ci_off + e0, du) : (ListRef)0;
+  b.deg16[q] = (uint16_t)min(du, (int64_t)kDegSat);
+  if (q == RP - 1) b.erow[RP] = M;
+}
+
+// Rows consumed before merge-path item t * kMP: the largest i with
+// erow[i] + i <= t * kMP (erow[i] + i rises strictly with i).
+__global__ __launch_bounds__(kBlock) void radix_mpath_kernel(const int64_t* erow, int64_t RP, int64_t M,
+                                                             int64_t nmp, int64_t* bq) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t > nmp) return;
+  const int64_t k = min(t * kMP, RP + M);
+  int64_t lo = max((int64_t)0, k - M), hi = min(k, RP);
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (erow[mid] + mid <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  bq[t] = lo;
+}
+
+// Split block blk: items [blk * kMP, ...) of the merge path, kMPPer
+// consecutive items per thread.
+__global__ __launch_bounds__(kBlock) void radix_split_kernel(LaunchArgs a, int64_t RP, int64_t M, BuildBufs b,
+                                                             int cs, int nc, int64_t nmp) {
+  __shared__ int64_t s_erow[kMP + 2];
+  __shared__ int32_t s_gid[kMP + 1];
+  __shared__ uint32_t s_hist[kCoarse];
+  const int64_t blk = blockIdx.x;
+  const int64_t k0 = blk * kMP, k1 = min(k0 + kMP, RP + M);
+  const int64_t i0 = b.bq[blk], i1 = b.bq[blk + 1];
+  // rows i0 .. min(i1, RP - 1) (k0 < RP + M, so i0 < RP) and their row ends
+  const int nr = (int)(min(i1, RP - 1) - i0 + 1);
+  for (int x = threadIdx.x; x <= nr; x += kBlock) s_erow[x] = b.erow[i0 + x];
+  for (int x = threadIdx.x; x < nr; x += kBlock) s_gid[x] = b.gid[i0 + x];
+  for (int x = threadIdx.x; x < nc; x += kBlock) s_hist[x] = 0u;
+  __syncthreads();
+  const int64_t kt0 = k0 + (int64_t)threadIdx.x * kMPPer;
+  int64_t E[kMPPer];
+  int32_t R[kMPPer];
+#pragma unroll
+  for (int s = 0; s < kMPPer; ++s) {
+    E[s] = -1;
+    R[s] = 0;
+  }
+  if (kt0 < k1) {
+    int lo = 0, hi = nr - 1;  // this thread's first row: the last with erow + row <= kt0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_erow[mid] + i0 + mid <= kt0) lo = mid;
+      else hi = mid - 1;
+    }
+    int r = lo;
+    int64_t j = kt0 - (i0 + r);
+#pragma unroll
+    for (int s = 0; s < kMPPer; ++s) {
+      if (kt0 + s < k1) {
+        if (j < s_erow[r + 1]) {
+          E[s] = j;
+          R[s] = r;
+          ++j;
+        } else {
+          ++r;  // row r ends here
+        }
+      }
+    }
+  }
+  // the ids of split rows: partner list id, then the partner's degree
+  int32_t qv[kMPPer];
+  int32_t dv[kMPPer];
+  int gc = -1;
+  int64_t rb = 0, cd = 0;
+#pragma unroll
+  for (int s = 0; s < kMPPer; ++s) {
+    qv[s] = -1;
+    if (E[s] >= 0) {
+      const int g = s_gid[R[s]];
+      if (g >= 0) {
+        if (g != gc) {
+          gc = g;
+          rb = a.descs[g].row_base;
+          cd = a.descs[g].ci_off - a.descs[g].eb;
+        }
+        qv[s] = (int32_t)(rb + a.ci[E[s] + cd]);
+      } else {
+        b.key[E[s]] = -1;  // not split: no pair
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < kMPPer; ++s) dv[s] = qv[s] >= 0 ? (int32_t)b.deg16[qv[s]] : 0;
+#pragma unroll
+  for (int s = 0; s < kMPPer; ++s) {
+    if (qv[s] < 0) continue;
+    const int r = R[s];
+    const int64_t du = s_erow[r + 1] - s_erow[r];
+    int64_t dvx = dv[s];
+    if (dvx == kDegSat && du >= kDegSat) dvx = b.erow[qv[s] + 1] - b.erow[qv[s]];  // both saturated: exact
+    const int32_t q = (int32_t)(i0 + r);
+    const bool own = dvx <= du;
+    const int32_t owner = own ? q : qv[s];
+    b.key[E[s]] = owner;
+    b.oth[E[s]] = own ? qv[s] : q;
+    atomicAdd(&s_hist[owner >> cs], 1u);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nc; c += kBlock) b.cnt1[(int64_t)c * nmp + blk] = s_hist[c];
+}
+
+// A tile's pairs sorted by digit in LDS and written as one run per digit:
+// pairs [p0, p1) of (ikey, ioth), digit = (key >> shift) & mask (nd digits),
+// gpos[d] = the tile's first output position of digit d.
+__device__ __forceinline__ void tile_sort_write(const int32_t* __restrict__ ikey, const int32_t* __restrict__ ioth,
+                                                int64_t p0, int64_t p1, int shift, int mask, int nd,
+                                                const uint32_t* s_gpos, uint32_t* s_cnt, int32_t* s_k,
+                                                int32_t* s_o, int64_t* s_wave, int32_t* okey, int32_t* ooth) {
+  int32_t kk[kTilePer], oo[kTilePer];
+  uint32_t rk[kTilePer];
+#pragma unroll
+  for (int s = 0; s < kTilePer; ++s) {
+    const int64_t p = p0 + (int64_t)s * kBlock + threadIdx.x;
+    kk[s] = p < p1 ? ikey[p] : -1;
+    oo[s] = kk[s] >= 0 ? ioth[p] : 0;
+  }
+#pragma unroll
+  for (int s = 0; s < kTilePer; ++s) rk[s] = kk[s] >= 0 ? atomicAdd(&s_cnt[(kk[s] >> shift) & mask], 1u) : 0u;
+  __syncthreads();
+  // s_cnt -> starts; the digits' counts stay readable as start differences
+  __shared__ uint32_t s_st[kCoarse + 1];
+  for (int x = threadIdx.x; x < nd; x += kBlock) s_st[x] = s_cnt[x];
+  __syncthreads();
+  const uint32_t n = block_scan_lds<1>(s_st, nd, s_wave);
+  if (threadIdx.x == 0) s_st[nd] = n;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kTilePer; ++s) {
+    if (kk[s] < 0) continue;
+    const uint32_t at = s_st[(kk[s] >> shift) & mask] + rk[s];
+    s_k[at] = kk[s];
+    s_o[at] = oo[s];
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < n; p += kBlock) {
+    const int d = (s_k[p] >> shift) & mask;
+    const int64_t at = (int64_t)s_gpos[d] + (p - s_st[d]);
+    okey[at] = s_k[p];
+    ooth[at] = s_o[p];
+  }
+}
+
+// Level 1: split block blk's pairs -> coarse runs of pkey/poth.
+__global__ __launch_bounds__(kBlock) void radix_scatter1_kernel(BuildBufs b, int64_t RP, int64_t M, int cs, int nc,
+                                                                int64_t nmp) {
+  __shared__ int32_t s_k[kTile], s_o[kTile];
+  __shared__ uint32_t s_cnt[kCoarse], s_gpos[kCoarse];
+  __shared__ int64_t s_wave[kWaves];
+  const int64_t blk = blockIdx.x;
+  const int64_t k0 = blk * kMP, k1 = min(k0 + kMP, RP + M);
+  const int64_t j0 = k0 - b.bq[blk], j1 = k1 - b.bq[blk + 1];
+  for (int c = threadIdx.x; c < nc; c += kBlock) {
+    s_cnt[c] = 0u;
+    s_gpos[c] = b.cnt1[(int64_t)c * nmp + blk];
+  }
+  __syncthreads();
+  tile_sort_write(b.key, b.oth, j0, j1, cs, 0x7FFFFFFF, nc, s_gpos, s_cnt, s_k, s_o, s_wave, b.pkey, b.poth);
+}
+
+// Coarse bucket c's pairs are [cnt1[c * nmp], cnt1[(c + 1) * nmp]) of the
+// level-1 output (cnt1 scanned, cnt1[nc * nmp] = all pairs).
+__device__ __forceinline__ int64_t coarse_start(const BuildBufs& b, int c, int64_t nmp) {
+  return (int64_t)b.cnt1[(int64_t)c * nmp];
+}
+
+// One workgroup: level-2 tiles per coarse bucket, scanned into tstart[0..nc].
+__global__ __launch_bounds__(kBlock) void radix_tiles2_kernel(BuildBufs b, int nc, int64_t nmp) {
+  __shared__ int64_t s_wave[kWaves];
+  const int c = threadIdx.x;  // nc <= kCoarse <= kBlock
+  int64_t t = 0;
+  if (c < nc) t = (coarse_start(b, c + 1, nmp) - coarse_start(b, c, nmp) + kTile - 1) / kTile;
+  int64_t total;
+  const int64_t at = block_excl_scan(t, s_wave, &total);
+  if (c < nc) b.tstart[c] = at;
+  if (c == 0) b.tstart[nc] = total;
+}
+
+// Level-2 tile t: its coarse bucket, tile index within it and pair range.
+struct Tile2 {
+  int c;
+  int64_t lt, nt, p0, p1;
+};
+__device__ __forceinline__ bool tile2_of(const BuildBufs& b, int64_t t, int nc, int64_t nmp, Tile2& x) {
+  if (t >= b.tstart[nc]) return false;
+  int lo = 0, hi = nc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.tstart[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  x.c = lo;
+  x.nt = b.tstart[lo + 1] - b.tstart[lo];
+  x.lt = t - b.tstart[lo];
+  x.p0 = coarse_start(b, lo, nmp) + x.lt * kTile;
+  x.p1 = min(x.p0 + kTile, coarse_start(b, lo + 1, nmp));
+  return true;
+}
+// cnt2 index of (tile, fine digit d): bucket-major within the coarse bucket.
+__device__ __forceinline__ int64_t cnt2_at(const BuildBufs& b, const Tile2& x, int d) {
+  return b.tstart[x.c] * kFine + (int64_t)d * x.nt + x.lt;
+}
+
+__global__ __launch_bounds__(kBlock) void radix_hist2_kernel(BuildBufs b, int fs, int nc, int64_t nmp) {
+  __shared__ uint32_t s_h[kFine];
+  Tile2 x;
+  if (!tile2_of(b, blockIdx.x, nc, nmp, x)) return;
+  if (threadIdx.x < kFine) s_h[threadIdx.x] = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kTilePer; ++s) {
+    const int64_t p = x.p0 + (int64_t)s * kBlock + threadIdx.x;
+    if (p < x.p1) atomicAdd(&s_h[(b.pkey[p] >> fs) & (kFine - 1)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kFine) b.cnt2[cnt2_at(b, x, threadIdx.x)] = s_h[threadIdx.x];
+}
+
+// Level 2: pkey/poth runs of one coarse bucket -> fine runs of key/oth.
+__global__ __launch_bounds__(kBlock) void radix_scatter2_kernel(BuildBufs b, int fs, int nc, int64_t nmp) {
+  __shared__ int32_t s_k[kTile], s_o[kTile];
+  __shared__ uint32_t s_cnt[kFine], s_gpos[kFine];
+  __shared__ int64_t s_wave[kWaves];
+  Tile2 x;
+  if (!tile2_of(b, blockIdx.x, nc, nmp, x)) return;
+  if (threadIdx.x < kFine) {
+    s_cnt[threadIdx.x] = 0u;
+    s_gpos[threadIdx.x] = b.cnt2[cnt2_at(b, x, threadIdx.x)];
+  }
+  __syncthreads();
+  tile_sort_write(b.pkey, b.poth, x.p0, x.p1, fs, kFine - 1, kFine, s_gpos, s_cnt, s_k, s_o, s_wave, b.key, b.oth);
+}
+
+// Fine bucket f's pairs are [fine_start(f), fine_start(f + 1)) of the
+// level-2 output; f >= 64 * nc is past the last pair.
+__device__ __forceinline__ int64_t fine_start(const BuildBufs& b, int64_t f, int nc, int64_t nmp) {
+  const int c = (int)(f / kFine), d = (int)(f % kFine);
+  if (c >= nc) return coarse_start(b, nc, nmp);
+  const int64_t t0 = b.tstart[c], nt = b.tstart[c + 1] - t0;
+  return nt > 0 ? (int64_t)b.cnt2[t0 * kFine + (int64_t)d * nt] : coarse_start(b, c, nmp);
+}
+
+// One workgroup: chunks of every fine bucket above cap pairs, scanned into
+// chstart[0..nf].
+__global__ __launch_bounds__(kBlock) void radix_chunks_kernel(BuildBufs b, int nf, int nc, int64_t nmp, int cap,
+                                                              int chunk) {
+  constexpr int kPer = kMaxFineBuckets / kBlock;
+  __shared__ int64_t s_wave[kWaves];
+  const int f0 = threadIdx.x * kPer;
+  int64_t v[kPer];
+  int64_t t = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    v[k] = 0;
+    if (f0 + k < nf) {
+      const int64_t n = fine_start(b, f0 + k + 1, nc, nmp) - fine_start(b, f0 + k, nc, nmp);
+      v[k] = n > cap ? (n + chunk - 1) / chunk : 0;
+    }
+    t += v[k];
+  }
+  int64_t total;
+  int64_t run = block_excl_scan(t, s_wave, &total);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    if (f0 + k < nf) b.chstart[f0 + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) b.chstart[nf] = total;
+}
+
+// Chunk x of the big fine buckets: its bucket and pair range.
+struct Chunk {
+  int f;
+  int64_t p0, p1;
+};
+__device__ __forceinline__ bool chunk_of(const BuildBufs& b, int64_t x, int nf, int nc, int64_t nmp, int chunk,
+                                         Chunk& c) {
+  if (x >= b.chstart[nf]) return false;
+  int lo = 0, hi = nf - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.chstart[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  c.f = lo;
+  c.p0 = fine_start(b, lo, nc, nmp) + (x - b.chstart[lo]) * chunk;
+  c.p1 = min(c.p0 + chunk, fine_start(b, lo + 1, nc, nmp));
   return true;
 }
 
-__global__ __launch_bounds__(kBlock) void partner_incount_kernel(BuildBufs b, int bshift, int nb) {
-  extern __shared__ uint32_t s_h[];  // [1 << bshift] in-owned edges per owner of the bucket
-  PairChunk c;
-  if (!pair_chunk(b, nb, c)) return;
-  const int nk = 1 << bshift;
-  const int32_t kb = c.bk << bshift;
-  for (int j = threadIdx.x; j < nk; j += kBlock) s_h[j] = 0;
+// Big buckets, pass 1: pairs per owner (one global add per owner and chunk).
+__global__ __launch_bounds__(kBlock) void radix_count_big_kernel(BuildBufs b, int64_t RP, int fs, int nf, int nc,
+                                                                 int64_t nmp, int chunk) {
+  extern __shared__ uint32_t s_h[];  // [1 << fs]
+  Chunk c;
+  if (!chunk_of(b, blockIdx.x, nf, nc, nmp, chunk, c)) return;
+  const int no = 1 << fs;
+  const int64_t q0 = (int64_t)c.f << fs;
+  for (int o = threadIdx.x; o < no; o += kBlock) s_h[o] = 0u;
   __syncthreads();
-  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_h[b.pkey[p] - kb], 1u);
+  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_h[b.key[p] - q0], 1u);
   __syncthreads();
-  for (int j = threadIdx.x; j < nk; j += kBlock) {
-    const uint32_t h = s_h[j];
-    if (!h) continue;
-    if (c.single) b.c2[kb + j] = (int32_t)h;
-    else atomicAdd(&b.c2[kb + j], (int32_t)h);
-  }
+  for (int o = threadIdx.x; o < no; o += kBlock)
+    if (s_h[o]) atomicAdd(&b.c2[q0 + o], (int32_t)s_h[o]);
 }
 
-// Per row (kCopyG lanes: rows average a few out-owned partners): its
-// out-owned partners to the front of its partner range; cur = absolute slot
-// of its first in-edge partner.
-constexpr int kCopyG = 4;
-__global__ __launch_bounds__(kBlock) void partner_outcopy_kernel(LaunchArgs a, int64_t RP, BuildBufs b) {
-  const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kCopyG;
-  const int gl = threadIdx.x % kCopyG;
-  if (q >= RP) return;
-  const GraphDesc d = a.descs[find_graph_rows(a.descs, a.ngraphs, q)];
-  const int64_t u = q - d.row_base;
-  const int64_t base = d.pci_off + b.off[q] - b.off[d.row_base];
-  const int32_t c = b.c1[q];
-  if (c > 0) {
-    const ListRef* src = b.out + d.eb + a.rp[d.rp_off + u];
-    ListRef* dst = (ListRef*)a.pci + base;
-    for (int j = gl; j < c; j += kCopyG) dst[j] = src[j];
+// Per fine bucket: partner offsets of its owners; a small bucket is placed
+// whole, a big one gets its owners' cursors (absolute pool positions).
+__global__ __launch_bounds__(kBlock) void radix_final_kernel(LaunchArgs a, BuildBufs b, int64_t RP, int fs, int nc,
+                                                             int64_t nmp, int cap) {
+  extern __shared__ int64_t s_mem[];
+  const int no_max = 1 << fs;
+  uint32_t* s_cnt = (uint32_t*)s_mem;           // [no_max]
+  uint32_t* s_cur = s_cnt + no_max;             // [no_max]
+  int32_t* s_g = (int32_t*)(s_cur + no_max);    // [no_max]
+  int32_t* s_oth = s_g + no_max;                // [cap]
+  uint16_t* s_own = (uint16_t*)(s_oth + cap);   // [cap]
+  __shared__ int64_t s_wave[kWaves];
+  const int64_t f = blockIdx.x;
+  const int64_t q0 = f << fs, q1 = min(q0 + no_max, RP);
+  const int no = (int)(q1 - q0);
+  const int64_t p0 = fine_start(b, f, nc, nmp), p1 = fine_start(b, f + 1, nc, nmp), n = p1 - p0;
+  const bool big = n > cap;
+  for (int o = threadIdx.x; o < no_max; o += kBlock) {
+    s_cnt[o] = big && o < no ? (uint32_t)b.c2[q0 + o] : 0u;
+    s_g[o] = o < no ? gid_graph(b.gid[q0 + o]) : 0;
   }
-  if (gl == 0) b.cur[q] = (unsigned long long)(base + c);
-}
-
-__global__ __launch_bounds__(kBlock) void partner_infill_kernel(LaunchArgs a, BuildBufs b, int bshift, int nb) {
-  extern __shared__ uint32_t s_w[];  // [1 << bshift] pairs per owner, then the next rank of each
-  PairChunk c;
-  if (!pair_chunk(b, nb, c)) return;
-  const int nk = 1 << bshift;
-  const int32_t kb = c.bk << bshift;
-  for (int j = threadIdx.x; j < nk; j += kBlock) s_w[j] = 0;
   __syncthreads();
-  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_w[b.pkey[p] - kb], 1u);
+  if (!big) {
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += kBlock) atomicAdd(&s_cnt[b.key[p] - q0], 1u);
+    __syncthreads();
+  }
+  for (int o = threadIdx.x; o < no_max; o += kBlock) s_cur[o] = s_cnt[o];
   __syncthreads();
-  // this chunk's first rank of each owner: 0 when the bucket is one chunk,
-  // else reserved on the owner's counter (kc, zeroed)
-  for (int j = threadIdx.x; j < nk; j += kBlock) {
-    const uint32_t h = s_w[j];
-    if (h) s_w[j] = c.single ? 0u : atomicAdd((uint32_t*)&b.kc[kb + j], h);
+  if (no_max <= kBlock) block_scan_lds<1>(s_cur, no_max, s_wave);
+  else if (no_max <= 4 * kBlock) block_scan_lds<4>(s_cur, no_max, s_wave);
+  else block_scan_lds<kMaxFineOwners / kBlock>(s_cur, no_max, s_wave);
+  for (int o = threadIdx.x; o < no; o += kBlock) {
+    b.off[q0 + o] = p0 + s_cur[o];
+    if (big) {
+      const GraphDesc& d = a.descs[s_g[o]];
+      b.cur[q0 + o] = (unsigned long long)(d.pci_off - d.eb + p0 + s_cur[o]);
+    }
+  }
+  if (big) return;
+  for (int o = threadIdx.x; o < no_max; o += kBlock) s_cnt[o] = s_cur[o];  // cursors
+  __syncthreads();
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+    const int o = b.key[p] - (int32_t)q0;
+    const uint32_t at = atomicAdd(&s_cnt[o], 1u);
+    s_oth[at] = b.oth[p];
+    s_own[at] = (uint16_t)o;
   }
   __syncthreads();
   ListRef* pci = (ListRef*)a.pci;
-  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kKeyUnroll * kBlock) {
-    int32_t k[kKeyUnroll];
-    ListRef ref[kKeyUnroll];
-    unsigned long long at[kKeyUnroll];
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j) {
-      const bool in = p + j * kBlock < c.p1;
-      k[j] = in ? b.pkey[p + j * kBlock] : -1;
-      ref[j] = in ? b.pref[p + j * kBlock] : 0;
-    }
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j) at[j] = k[j] >= 0 ? b.cur[k[j]] : 0;
-#pragma unroll
-    for (int j = 0; j < kKeyUnroll; ++j)
-      if (k[j] >= 0) pci[at[j] + atomicAdd(&s_w[k[j] - kb], 1u)] = ref[j];
+  for (int64_t p = threadIdx.x; p < n; p += kBlock) {
+    const GraphDesc& d = a.descs[s_g[s_own[p]]];
+    pci[d.pci_off - d.eb + p0 + p] = b.lref[s_oth[p]];
+  }
+}
+
+// Big buckets, pass 2: each chunk sorted by owner in LDS; every owner's run
+// reserved on its cursor and written out.
+__global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, BuildBufs b, int64_t RP, int fs, int nf,
+                                                                 int nc, int64_t nmp, int chunk) {
+  extern __shared__ int64_t s_mem[];
+  const int no = 1 << fs;
+  unsigned long long* s_base = (unsigned long long*)s_mem;  // [no]
+  uint32_t* s_cnt = (uint32_t*)(s_base + no);               // [no]
+  uint32_t* s_st = s_cnt + no;                              // [no]
+  int32_t* s_oth = (int32_t*)(s_st + no);                   // [chunk]
+  uint16_t* s_own = (uint16_t*)(s_oth + chunk);             // [chunk]
+  __shared__ int64_t s_wave[kWaves];
+  Chunk c;
+  if (!chunk_of(b, blockIdx.x, nf, nc, nmp, chunk, c)) return;
+  const int64_t q0 = (int64_t)c.f << fs;
+  for (int o = threadIdx.x; o < no; o += kBlock) s_cnt[o] = 0u;
+  __syncthreads();
+  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_cnt[b.key[p] - q0], 1u);
+  __syncthreads();
+  for (int o = threadIdx.x; o < no; o += kBlock) {
+    const uint32_t h = s_cnt[o];
+    s_st[o] = h;
+    if (h) s_base[o] = atomicAdd(&b.cur[q0 + o], (unsigned long long)h);
+  }
+  __syncthreads();
+  if (no <= kBlock) block_scan_lds<1>(s_st, no, s_wave);
+  else if (no <= 4 * kBlock) block_scan_lds<4>(s_st, no, s_wave);
+  else block_scan_lds<kMaxFineOwners / kBlock>(s_st, no, s_wave);
+  for (int o = threadIdx.x; o < no; o += kBlock) s_cnt[o] = s_st[o];  // cursors
+  __syncthreads();
+  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) {
+    const int o = b.key[p] - (int32_t)q0;
+    const uint32_t at = atomicAdd(&s_cnt[o], 1u);
+    s_oth[at] = b.oth[p];
+    s_own[at] = (uint16_t)o;
+  }
+  __syncthreads();
+  ListRef* pci = (ListRef*)a.pci;
+  const int64_t n = c.p1 - c.p0;
+  for (int64_t p = threadIdx.x; p < n; p += kBlock) {
+    const int o = s_own[p];
+    pci[s_base[o] + (p - s_st[o])] = b.lref[s_oth[p]];
   }
 }
 
@@ -1128,51 +1368,54 @@ static hipError_t launch_partner_rows(const LaunchArgs& a, int64_t RP, const Bui
   return hipGetLastError();
 }
 
-static hipError_t launch_partners_bucketed(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b,
-                                           int bshift, hipStream_t s) {
-  const int nb = (int)((RP + ((int64_t)1 << bshift) - 1) >> bshift);
-  const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
-  const unsigned cpblocks = (unsigned)((RP * kCopyG + kBlock - 1) / kBlock);
-  const unsigned kblocks = (unsigned)((M + kKeysPerBlock - 1) / kKeysPerBlock);
-  const unsigned cblocks = (unsigned)(nb + (M + kChunkPairs - 1) / kChunkPairs);  // >= chunks
-  const size_t blds = sizeof(uint32_t) * (size_t)nb, clds = sizeof(uint32_t) << bshift;
-  hipError_t e = launch_zero(b.c2, RP, s);
-  if (e == hipSuccess) e = launch_zero(b.key, M, s);  // keys of rows that are not split stay "none"
-  if (e == hipSuccess) e = launch_zero(b.btot, 2 * ((int64_t)nb + 1), s);
-  if (e == hipSuccess) e = launch_zero((void*)a.vrow, a.P * (int64_t)(sizeof(VRow) / 4), s);  // slots unused
+static hipError_t launch_partners_radix(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b,
+                                        const RadixGeom& x, hipStream_t s) {
+  const int cs = x.fs + 6;
+  const unsigned rblocks = (unsigned)((RP + kBlock - 1) / kBlock);
+  hipError_t e = launch_zero((void*)a.vrow, a.P * (int64_t)(sizeof(VRow) / 4), s);  // slots unused
+  if (e == hipSuccess) e = launch_zero(b.cnt1 + (int64_t)x.nc * x.nmp, 1, s);
+  if (e == hipSuccess) e = launch_zero(b.cnt2, kFine * x.nt2 + 1, s);
+  if (e == hipSuccess) e = launch_zero(b.c2, RP, s);
   if (e != hipSuccess) return e;
-  partner_deg_kernel<<<(unsigned)((RP + kBlock - 1) / kBlock), kBlock, 0, s>>>(a, RP, b);
+  radix_prep_kernel<<<rblocks, kBlock, 0, s>>>(a, RP, M, b);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  partner_split_kernel<<<gblocks, kBlock, 0, s>>>(a, RP, b);
+  radix_mpath_kernel<<<(unsigned)((x.nmp + kBlock) / kBlock), kBlock, 0, s>>>(b.erow, RP, M, x.nmp, b.bq);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if (kblocks > 0) {
-    partner_bhist_kernel<<<kblocks, kBlock, blds, s>>>(b, M, bshift, nb);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  if ((e = launch_scan(LoadU64{b.btot}, StoreU64{b.btot}, (int64_t)nb + 1, b.part, s)) != hipSuccess) return e;
-  bucket_chunks_kernel<<<(unsigned)((nb + kBlock) / kBlock), kBlock, 0, s>>>(b, nb);
+  radix_split_kernel<<<(unsigned)x.nmp, kBlock, 0, s>>>(a, RP, M, b, cs, x.nc, x.nmp);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_scan(LoadI64{b.cstart}, StoreI64{b.cstart}, (int64_t)nb + 1, b.part, s)) != hipSuccess) return e;
-  if (kblocks > 0) {
-    partner_bucket_kernel<<<kblocks, kBlock, blds, s>>>(b, M, bshift, nb);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  partner_incount_kernel<<<cblocks, kBlock, clds, s>>>(b, bshift, nb);
+  const int64_t n1 = (int64_t)x.nc * x.nmp + 1;
+  if ((e = launch_scan(LoadU32{b.cnt1}, StoreU32{b.cnt1}, n1, b.part, s)) != hipSuccess) return e;
+  radix_scatter1_kernel<<<(unsigned)x.nmp, kBlock, 0, s>>>(b, RP, M, cs, x.nc, x.nmp);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_scan(LoadAdd{b.c1, b.c2}, StoreI64{b.off}, RP, b.part, s)) != hipSuccess) return e;
-  partner_outcopy_kernel<<<cpblocks, kBlock, 0, s>>>(a, RP, b);
+  radix_tiles2_kernel<<<1, kBlock, 0, s>>>(b, x.nc, x.nmp);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_zero(b.kc, RP, s)) != hipSuccess) return e;
-  partner_infill_kernel<<<cblocks, kBlock, clds, s>>>(a, b, bshift, nb);
+  radix_hist2_kernel<<<(unsigned)x.nt2, kBlock, 0, s>>>(b, x.fs, x.nc, x.nmp);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int64_t n2 = kFine * x.nt2 + 1;
+  if ((e = launch_scan(LoadU32{b.cnt2}, StoreU32{b.cnt2}, n2, b.part, s)) != hipSuccess) return e;
+  radix_scatter2_kernel<<<(unsigned)x.nt2, kBlock, 0, s>>>(b, x.fs, x.nc, x.nmp);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  radix_chunks_kernel<<<1, kBlock, 0, s>>>(b, x.nf, x.nc, x.nmp, x.cap, x.chunk);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const size_t own_lds = (size_t)1 << x.fs;
+  radix_count_big_kernel<<<(unsigned)x.nch, kBlock, 4 * own_lds, s>>>(b, RP, x.fs, x.nf, x.nc, x.nmp, x.chunk);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  radix_final_kernel<<<(unsigned)x.nf, kBlock, radix_final_lds(x.fs, x.cap), s>>>(a, b, RP, x.fs, x.nc, x.nmp,
+                                                                                    x.cap);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  radix_place_big_kernel<<<(unsigned)x.nch, kBlock, radix_place_lds(x.fs, x.chunk), s>>>(a, b, RP, x.fs, x.nf,
+                                                                                         x.nc, x.nmp, x.chunk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return launch_partner_rows(a, RP, b, s);
 }
 
-hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b, hipStream_t s) {
+hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& bb, hipStream_t s) {
   if (RP <= 0) return hipSuccess;
-  const int bshift = bucket_shift(RP);
-  if (b.key && !b.pedge && bshift >= 0 && M <= b.ids && M < ((int64_t)1 << 32))
-    return launch_partners_bucketed(a, RP, M, b, bshift, s);
+  RadixGeom x;
+  if (bb.key && !bb.pedge && RP <= bb.rows && M <= bb.ids && radix_geom(RP, M, x))
+    return launch_partners_radix(a, RP, M, bb, x, s);
+  BuildBufs b = bb;
+  b.gid = nullptr;  // rows are found by their graph search (no prep pass here)
   const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
   const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
   hipError_t e = launch_zero(b.c2, RP * kXcds, s);

@@ -176,46 +176,84 @@ inline int64_t plan_blocks(int64_t P) { return (P + 255) / 256; }
 constexpr int kXcds = 8;
 
 struct BuildBufs {
-  int32_t* c1;          // [RP] partners owned through out-edges
-  int32_t* c2;          // [kXcds * RP] partners owned through in-edges, per stripe
-  unsigned long long* cur;  // [kXcds * RP] fill cursor (partner slot) of each stripe
-  int64_t* off;         // [RP] partner offsets (pool-global exclusive scan)
+  int32_t* c1;          // [RP] atomic path: partners owned through out-edges
+  int32_t* c2;          // [kXcds * RP] atomic path: partners owned through in-edges, per stripe;
+                        // radix path: pairs per owner of the big fine buckets
+  unsigned long long* cur;  // [kXcds * RP] atomic path: fill cursor of each stripe;
+                            // radix path: pool cursor of each owner of the big fine buckets
+  int64_t* off;         // [RP] partner offsets (exclusive scan over the row slots)
   int32_t* kc;          // [RP] partner rows per DAG row
   int64_t* voff;        // [RP] partner-row offsets (pool-global exclusive scan)
-  int64_t* part;        // [scan_partials(RP)] scan partials
+  int64_t* part;        // [scan_partials(max(RP, ...))] scan partials
   int64_t* pedge;       // null, or [partner pool]: DAG edge position of each partner entry
   int32_t* pvid;        // with pedge: [partner pool] partner vertex of each entry
   int32_t* pown;        // with pedge: [P] owner vertex of each partner row
-  // Bucketed transpose (null when the scratch has no per-id part): per id
-  // of the build (M = sum of the graphs' ids, graph g's at eb):
+  // Radix transpose (null when the scratch has no per-id part).  Build space:
+  // M ids, graph g's at eb; row slots: RP.
+  int32_t* gid;         // [RP] graph of each row slot; ~graph when the row is not split
+  int64_t* erow;        // [RP + 1] first build-space id of each row slot; erow[RP] = M
+  ListRef* lref;        // [RP] each row slot's own list
   uint16_t* deg16;      // [RP] out-degree of each row slot, saturated at 65535
-  int32_t* key;         // [M] owner row slot + 1 of an edge its head owns, else 0
-  ListRef* out;         // [M] out-edge partners of each row, compacted at its front
-  int32_t* pkey;        // [M] owner row slots of the in-owned edges, grouped by bucket
-  ListRef* pref;        // [M] their partners' lists, in the same order
-  unsigned long long* btot;  // [kMaxBuckets + 1] pairs per bucket, then bucket starts
-  unsigned long long* bcur;  // [kMaxBuckets] bucket fill cursors
-  int64_t* cstart;      // [kMaxBuckets + 1] first chunk of each bucket (scan of chunk counts)
-  int64_t ids;          // M capacity
+  int32_t* key;         // [M] owner of each id's pair (-1: none); after level 2, fine-bucket order
+  int32_t* oth;         // [M] partner of each pair, in the order of key
+  int32_t* pkey;        // [M] level-1 (coarse-bucket) order of key
+  int32_t* poth;        // [M] and of oth
+  int64_t* bq;          // [nmp + 1] rows consumed before each split block
+  uint32_t* cnt1;       // [kCoarse * nmp + 1] pairs per (coarse bucket, split block) -> positions
+  uint32_t* cnt2;       // [kFine * nt2 + 1] pairs per (fine digit, level-2 tile) -> positions
+  int64_t* tstart;      // [kCoarse + 1] first level-2 tile of each coarse bucket
+  int64_t* chstart;     // [kMaxFineBuckets + 1] first chunk of each big fine bucket
+  int64_t rows;         // capacity: row slots
+  int64_t ids;          // capacity: ids
 };
 int64_t scan_partials(int64_t n);
 
-// Bucketed transpose: owner row slots are bucketed by their high bits
-// (1 << bshift slots per bucket, at most kMaxBuckets buckets, at most
-// kMaxBucketSlots slots per bucket, so a bucket's counters fit LDS), and a
-// bucket's pairs are processed in chunks of at most kChunkPairs.
-constexpr int kMaxBuckets = 8192;
-constexpr int kMaxBucketShift = 14;
-constexpr int kMinBucketShift = 12;
-constexpr int64_t kChunkPairs = 65536;
-// Bucket shift of a build over RP row slots, or -1 when RP needs more than
-// kMaxBuckets buckets of 2^kMaxBucketShift slots (the build then takes the
-// per-edge atomic path).
-inline int bucket_shift(int64_t RP) {
-  int sh = kMinBucketShift;
-  while (sh <= kMaxBucketShift && (RP + ((int64_t)1 << sh) - 1) >> sh > kMaxBuckets) ++sh;
-  return sh <= kMaxBucketShift ? sh : -1;
+// Radix transpose geometry.  Owner slots fall in fine buckets of 2^fs slots
+// (at most kMaxFineBuckets), 64 fine buckets to a coarse bucket.  Split
+// blocks take kMP merge-path items (row ends and ids), level-2 tiles kTile
+// pairs.  A fine bucket of at most `cap` pairs is sorted whole in LDS; a
+// bigger one goes by chunks of `chunk` pairs.  Dynamic LDS of the final
+// passes stays within kRadixLds.
+constexpr int kMP = 4096;
+constexpr int kMPPer = kMP / kBlock;
+constexpr int kTile = 4096;
+constexpr int kTilePer = kTile / kBlock;
+constexpr int kCoarse = 160;
+constexpr int kFine = 64;
+constexpr int kMaxFineBuckets = kCoarse * kFine;
+constexpr int kMinFineShift = 5;
+constexpr int kMaxFineShift = 11;
+constexpr int kMaxFineOwners = 1 << kMaxFineShift;
+constexpr int64_t kFineCapMax = 8192;
+constexpr int64_t kRadixLds = 65536;
+inline size_t radix_final_lds(int fs, int cap) { return 12 * ((size_t)1 << fs) + 6 * (size_t)cap; }
+inline size_t radix_place_lds(int fs, int chunk) { return 16 * ((size_t)1 << fs) + 6 * (size_t)chunk; }
+struct RadixGeom {
+  int fs, nf, nc;
+  int64_t nmp, nt2, nch;
+  int cap, chunk;
+};
+// False when RP or M is beyond the radix path (the atomic path then builds).
+inline bool radix_geom(int64_t RP, int64_t M, RadixGeom& x) {
+  if (RP <= 0 || RP >= ((int64_t)1 << 31) || M < 0 || M >= ((int64_t)1 << 32)) return false;
+  int fs = kMinFineShift;
+  while (fs <= kMaxFineShift && ((RP + ((int64_t)1 << fs) - 1) >> fs) > kMaxFineBuckets) ++fs;
+  if (fs > kMaxFineShift) return false;
+  const int64_t no = (int64_t)1 << fs;
+  x.fs = fs;
+  x.nf = (int)((RP + no - 1) >> fs);
+  x.nc = (x.nf + kFine - 1) / kFine;
+  x.nmp = (RP + M + kMP - 1) / kMP;
+  x.nt2 = (M + kTile - 1) / kTile + x.nc;
+  const int64_t cap = (kRadixLds - 12 * no) / 6 / 256 * 256, chunk = (kRadixLds - 16 * no) / 6 / 256 * 256;
+  x.cap = (int)(cap < kFineCapMax ? cap : kFineCapMax);
+  x.chunk = (int)(chunk < kFineCapMax ? chunk : kFineCapMax);
+  x.nch = (M + x.chunk - 1) / x.chunk + x.nf;
+  return true;
 }
+// Upper bounds of the per-build arrays for up to `rows` slots and `ids` ids.
+inline int64_t radix_split_blocks(int64_t rows, int64_t ids) { return (rows + ids + kMP - 1) / kMP; }
+inline int64_t radix_tiles2(int64_t ids) { return (ids + kTile - 1) / kTile + kCoarse; }
 
 // Builds the partner lists, rows and counts of every graph of `a` (descs,
 // rp, ci, P as for a count; pci/vrp/vrow/nv are written); RP = sum of

@@ -138,15 +138,29 @@ int64_t Partners::bytes_for(int64_t ci_slots, int64_t row_slots, int32_t ngraphs
 
 // Piece sizes of a build scratch, in allocation order (256-byte pieces).
 static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+// Values the scans of a build may cover: the row slots, and with the radix
+// part the level-1 and level-2 count arrays.
+static int64_t scan_values(int64_t n, int64_t ids) {
+  int64_t v = n;
+  if (ids > 0) {
+    v = std::max<int64_t>(v, (int64_t)tck::kCoarse * tck::radix_split_blocks(n, ids) + 1);
+    v = std::max<int64_t>(v, (int64_t)tck::kFine * tck::radix_tiles2(ids) + 1);
+  }
+  return v;
+}
 static int64_t build_bytes(int64_t n, int64_t ids) {
   // per row: c1, kc (int32), c2 (int32 per XCD stripe), cur (int64 per
-  // stripe), off, voff (int64), scan partials; with the per-id part also
-  // deg16 per row and per id: key, pkey (int32), out, pref (list
-  // references); per bucket: totals, cursors, chunk starts
+  // stripe), off, voff (int64), scan partials; with the radix part also per
+  // row gid, erow, lref, deg16, per id key, oth, pkey, poth (int32), and the
+  // merge-path, count and tile arrays
   int64_t b = up256(4 * n) * 2 + up256(4 * tck::kXcds * n) + up256(8 * tck::kXcds * n) + up256(8 * n) * 2 +
-              up256(8 * tck::scan_partials(n));
-  if (ids > 0)
-    b += up256(2 * n) + up256(4 * ids) * 2 + up256(8 * ids) * 2 + up256(8 * (tck::kMaxBuckets + 1)) * 3;
+              up256(8 * tck::scan_partials(scan_values(n, ids)));
+  if (ids > 0) {
+    const int64_t nmp = tck::radix_split_blocks(n, ids), nt2 = tck::radix_tiles2(ids);
+    b += up256(4 * n) + up256(8 * (n + 1)) + up256(8 * n) + up256(2 * n) + up256(4 * ids) * 4 +
+         up256(8 * (nmp + 1)) + up256(4 * (tck::kCoarse * nmp + 1)) + up256(4 * (tck::kFine * nt2 + 1)) +
+         up256(8 * (tck::kCoarse + 1)) + up256(8 * (tck::kMaxFineBuckets + 1));
+  }
   return b;
 }
 
@@ -173,16 +187,23 @@ tc_status BuildScratch::ensure(int64_t rp_slots, int64_t ids) {
   b.kc = (int32_t*)p;  p += up256(4 * n);
   b.off = (int64_t*)p; p += up256(8 * n);
   b.voff = (int64_t*)p; p += up256(8 * n);
-  b.part = (int64_t*)p; p += up256(8 * tck::scan_partials(n));
+  b.part = (int64_t*)p; p += up256(8 * tck::scan_partials(scan_values(n, ids)));
+  b.rows = n;
   if (ids > 0) {
+    const int64_t nmp = tck::radix_split_blocks(n, ids), nt2 = tck::radix_tiles2(ids);
+    b.gid = (int32_t*)p; p += up256(4 * n);
+    b.erow = (int64_t*)p; p += up256(8 * (n + 1));
+    b.lref = (tck::ListRef*)p; p += up256(8 * n);
     b.deg16 = (uint16_t*)p; p += up256(2 * n);
     b.key = (int32_t*)p; p += up256(4 * ids);
+    b.oth = (int32_t*)p; p += up256(4 * ids);
     b.pkey = (int32_t*)p; p += up256(4 * ids);
-    b.out = (tck::ListRef*)p; p += up256(8 * ids);
-    b.pref = (tck::ListRef*)p; p += up256(8 * ids);
-    b.btot = (unsigned long long*)p; p += up256(8 * (tck::kMaxBuckets + 1));
-    b.bcur = (unsigned long long*)p; p += up256(8 * (tck::kMaxBuckets + 1));
-    b.cstart = (int64_t*)p;
+    b.poth = (int32_t*)p; p += up256(4 * ids);
+    b.bq = (int64_t*)p; p += up256(8 * (nmp + 1));
+    b.cnt1 = (uint32_t*)p; p += up256(4 * (tck::kCoarse * nmp + 1));
+    b.cnt2 = (uint32_t*)p; p += up256(4 * (tck::kFine * nt2 + 1));
+    b.tstart = (int64_t*)p; p += up256(8 * (tck::kCoarse + 1));
+    b.chstart = (int64_t*)p;
     b.ids = ids;
   }
   return TC_OK;

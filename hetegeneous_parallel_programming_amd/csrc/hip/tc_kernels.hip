@@ -836,24 +836,34 @@ __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a,
   const int64_t v0 = b.voff[q] - vbase, p0 = d.pci_off + b.off[q] - pbase, c = b.off[q + 1] - b.off[q];
   const int64_t* rp = a.rp + d.rp_off;
   const ListRef set = list_ref(d.ci_off + rp[u], rp[u + 1] - rp[u]);
+  // an owner with one partner row and its work known from the build: set it
+  // here; the work pass fills rows whose work is still 0
+  const uint32_t w = K == 1 && b.wq ? (uint32_t)min(b.wq[q], 0xFFFFFFFFull) : 0u;
   for (int32_t k = 0; k < K; ++k) {
-    vrow[v0 + k] = VRow{set, 0u, g};
+    vrow[v0 + k] = VRow{set, w, g};
     vrp[v0 + k] = p0 + (c * k) / K;
     if (b.pown) b.pown[d.pr_base + v0 + k] = (int32_t)u;
   }
 }
 
-// Pass 5: work of every partner row (the sum of its partners' list
-// lengths), with a group of kBuildG lanes per slot.
+// Pass 5: work of every partner row still at 0 (the sum of its partners'
+// list lengths), with a group of kBuildG lanes per slot.
 __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
   const int64_t p = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kBuildG;
   const int gl = threadIdx.x % kBuildG;
   if (p >= a.P) return;  // the whole group
   VRow* vr = (VRow*)a.vrow + p;
-  if (vr->set == 0) return;  // unused slot
+  if (vr->set == 0 || vr->work != 0) return;  // unused slot, or work known from the build
   const int64_t pb = a.vrp[p], pe = a.vrp[p + 1];
   int64_t w = 0;
-  for (int64_t e = pb + gl; e < pe; e += kBuildG) w += list_len(a.pci[e]);
+  constexpr int kU = 4;  // list references in flight per lane
+  for (int64_t e0 = pb + gl; e0 < pe; e0 += kU * kBuildG) {
+    ListRef r[kU];
+#pragma unroll
+    for (int j = 0; j < kU; ++j) r[j] = e0 + j * kBuildG < pe ? a.pci[e0 + j * kBuildG] : (ListRef)0;
+#pragma unroll
+    for (int j = 0; j < kU; ++j) w += list_len(r[j]);
+  }
 #pragma unroll
   for (int o = kBuildG / 2; o >= 1; o >>= 1) w += __shfl_xor(w, o, kBuildG);
   if (gl == 0) vr->work = (uint32_t)min(w, (int64_t)0xFFFFFFFF);
@@ -919,6 +929,37 @@ __device__ uint32_t block_scan_lds(uint32_t* s, int n, int64_t* s_wave) {
   return (uint32_t)total;
 }
 
+// LDS cnt[d] += 1 for every lane of the wave with d >= 0 (all 64 lanes
+// call it); returns each such lane's old value (its rank).  Owner-keyed
+// digits are skewed (hub owners), and lanes adding to one LDS word
+// serialise: the wave's kPeel most common digits (met in lane order) take
+// one atomic per digit for all their lanes, the rest one atomic per lane.
+template <int kPeel>
+__device__ __forceinline__ uint32_t wave_count_add(uint32_t* cnt, int d) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  bool done = d < 0;
+  uint32_t r = 0;
+#pragma unroll
+  for (int k = 0; k < kPeel; ++k) {
+    const uint64_t act = __ballot(!done);
+    if (act == 0) break;
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const int dl = __shfl(d, leader);
+    const bool mine = !done && d == dl;
+    const uint64_t peers = __ballot(mine);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[dl], (uint32_t)__popcll(peers));
+    base = __shfl(base, leader);
+    if (mine) {
+      r = base + (uint32_t)__popcll(peers & below);
+      done = true;
+    }
+  }
+  if (!done) r = atomicAdd(&cnt[d], 1u);
+  return r;
+}
+
 __global__ __launch_bounds__(kBlock) void radix_prep_kernel(LaunchArgs a, int64_t RP, int64_t M, BuildBufs b) {
   // graphs of this block's first and last slot (uniform), then each lane's
   const int64_t qa = (int64_t)blockIdx.x * kBlock, qz = min(qa + kBlock, RP) - 1;
@@ -963,139 +1004,167 @@ __global__ __launch_bounds__(kBlock) void radix_mpath_kernel(const int64_t* erow
   bq[t] = lo;
 }
 
-// Split block blk: items [blk * kMP, ...) of the merge path, kMPPer
-// consecutive items per thread.
+// A pair's partner before placement: its list, or (kRefSlot set) the
+// partner's row slot, whose list the placement reads.  Pool offsets stay
+// below 2^38, so bit 63 of a list reference is free.
+constexpr ListRef kRefSlot = 1ull << 63;
+
+// Split block blk: the ids of its merge-path range, kMPPer per thread at a
+// stride of kBlock (coalesced loads and stores); each id's row by a
+// branch-free binary search over the block's rows in LDS.  An edge its head
+// owns carries the row's own list; one its row owns carries the head's slot.
 __global__ __launch_bounds__(kBlock) void radix_split_kernel(LaunchArgs a, int64_t RP, int64_t M, BuildBufs b,
                                                              int cs, int nc, int64_t nmp) {
-  __shared__ int64_t s_erow[kMP + 2];
-  __shared__ int32_t s_gid[kMP + 1];
+  constexpr int kFill = (kMP + 2 + kBlock - 1) / kBlock;
+  __shared__ int32_t s_erow[kMP + 2];  // first id of each row, relative to the block's first row
+  __shared__ int16_t s_gid[kMP + 1];   // graph - g0 (a block spans at most kMP + 1 graphs); ~ when not split
   __shared__ uint32_t s_hist[kCoarse];
   const int64_t blk = blockIdx.x;
   const int64_t k0 = blk * kMP, k1 = min(k0 + kMP, RP + M);
   const int64_t i0 = b.bq[blk], i1 = b.bq[blk + 1];
-  // rows i0 .. min(i1, RP - 1) (k0 < RP + M, so i0 < RP) and their row ends
+  const int64_t j0 = k0 - i0, j1 = k1 - i1;
+  // rows i0 .. min(i1, RP - 1) (k0 < RP + M, so i0 < RP) and the end of the last
   const int nr = (int)(min(i1, RP - 1) - i0 + 1);
-  for (int x = threadIdx.x; x <= nr; x += kBlock) s_erow[x] = b.erow[i0 + x];
-  for (int x = threadIdx.x; x < nr; x += kBlock) s_gid[x] = b.gid[i0 + x];
+  const int64_t ebase = b.erow[i0];
+  const int g0 = gid_graph(b.gid[i0]);
+  const int64_t rb0 = a.descs[g0].row_base, cd0 = a.descs[g0].ci_off - a.descs[g0].eb;
+  {
+    int64_t ev[kFill];
+    int32_t gv[kFill];
+#pragma unroll
+    for (int k = 0; k < kFill; ++k) {
+      const int x = k * kBlock + threadIdx.x;
+      ev[k] = x <= nr ? b.erow[i0 + x] : 0;
+      gv[k] = x < nr ? b.gid[i0 + x] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kFill; ++k) {
+      const int x = k * kBlock + threadIdx.x;
+      if (x <= nr) s_erow[x] = (int32_t)(ev[k] - ebase);
+      if (x < nr) s_gid[x] = (int16_t)(gv[k] >= 0 ? gv[k] - g0 : ~(gid_graph(gv[k]) - g0));
+    }
+  }
   for (int x = threadIdx.x; x < nc; x += kBlock) s_hist[x] = 0u;
   __syncthreads();
-  const int64_t kt0 = k0 + (int64_t)threadIdx.x * kMPPer;
-  int64_t E[kMPPer];
-  int32_t R[kMPPer];
+  int32_t R[kMPPer], qv[kMPPer], dv[kMPPer];
+  int64_t cix[kMPPer];  // colidx position of each id
 #pragma unroll
   for (int s = 0; s < kMPPer; ++s) {
-    E[s] = -1;
-    R[s] = 0;
-  }
-  if (kt0 < k1) {
-    int lo = 0, hi = nr - 1;  // this thread's first row: the last with erow + row <= kt0
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_erow[mid] + i0 + mid <= kt0) lo = mid;
-      else hi = mid - 1;
-    }
-    int r = lo;
-    int64_t j = kt0 - (i0 + r);
+    const int64_t E = j0 + (int64_t)s * kBlock + threadIdx.x;
+    R[s] = -1;
+    if (E < j1) {
+      const int32_t e = (int32_t)(E - ebase);
+      int r = 0;  // the last row whose first id is <= e (s_erow[0] = 0 <= e)
 #pragma unroll
-    for (int s = 0; s < kMPPer; ++s) {
-      if (kt0 + s < k1) {
-        if (j < s_erow[r + 1]) {
-          E[s] = j;
-          R[s] = r;
-          ++j;
-        } else {
-          ++r;  // row r ends here
-        }
-      }
+      for (int bit = 1 << 12; bit > 0; bit >>= 1)
+        if (r + bit < nr && s_erow[r + bit] <= e) r += bit;
+      R[s] = r;
     }
   }
-  // the ids of split rows: partner list id, then the partner's degree
-  int32_t qv[kMPPer];
-  int32_t dv[kMPPer];
-  int gc = -1;
-  int64_t rb = 0, cd = 0;
+  int gc = 0;
+  int64_t rb = rb0, cd = cd0;
 #pragma unroll
   for (int s = 0; s < kMPPer; ++s) {
     qv[s] = -1;
-    if (E[s] >= 0) {
-      const int g = s_gid[R[s]];
-      if (g >= 0) {
-        if (g != gc) {
-          gc = g;
-          rb = a.descs[g].row_base;
-          cd = a.descs[g].ci_off - a.descs[g].eb;
-        }
-        qv[s] = (int32_t)(rb + a.ci[E[s] + cd]);
-      } else {
-        b.key[E[s]] = -1;  // not split: no pair
-      }
+    cix[s] = 0;
+    if (R[s] < 0) continue;
+    const int64_t E = j0 + (int64_t)s * kBlock + threadIdx.x;
+    const int gr = s_gid[R[s]];
+    if (gr < 0) {
+      b.key[E] = -1;  // the row is not split: no pair
+      continue;
     }
+    if (gr != gc) {
+      gc = gr;
+      rb = a.descs[g0 + gr].row_base;
+      cd = a.descs[g0 + gr].ci_off - a.descs[g0 + gr].eb;
+    }
+    cix[s] = E + cd;
+    qv[s] = (int32_t)(rb + a.ci[cix[s]]);
   }
 #pragma unroll
   for (int s = 0; s < kMPPer; ++s) dv[s] = qv[s] >= 0 ? (int32_t)b.deg16[qv[s]] : 0;
 #pragma unroll
   for (int s = 0; s < kMPPer; ++s) {
-    if (qv[s] < 0) continue;
-    const int r = R[s];
-    const int64_t du = s_erow[r + 1] - s_erow[r];
-    int64_t dvx = dv[s];
-    if (dvx == kDegSat && du >= kDegSat) dvx = b.erow[qv[s] + 1] - b.erow[qv[s]];  // both saturated: exact
-    const int32_t q = (int32_t)(i0 + r);
-    const bool own = dvx <= du;
-    const int32_t owner = own ? q : qv[s];
-    b.key[E[s]] = owner;
-    b.oth[E[s]] = own ? qv[s] : q;
-    atomicAdd(&s_hist[owner >> cs], 1u);
+    int digit = -1;
+    if (qv[s] >= 0) {
+      const int64_t E = j0 + (int64_t)s * kBlock + threadIdx.x;
+      const int r = R[s];
+      const int64_t du = s_erow[r + 1] - s_erow[r];
+      int64_t dvx = dv[s];
+      if (dvx == kDegSat && du >= kDegSat) dvx = b.erow[qv[s] + 1] - b.erow[qv[s]];  // both saturated: exact
+      const int32_t q = (int32_t)(i0 + r);
+      int32_t owner;
+      if (dvx <= du) {  // the row owns it: partner = the head, by slot
+        owner = q;
+        b.ref[E] = kRefSlot | (ListRef)(uint32_t)qv[s];
+      } else {  // the head owns it: partner = this row's list
+        owner = qv[s];
+        b.ref[E] = list_ref(cix[s] - (E - ebase - s_erow[r]), du);
+      }
+      b.key[E] = owner;
+      digit = owner >> cs;
+    }
+    wave_count_add<2>(s_hist, digit);
   }
   __syncthreads();
   for (int c = threadIdx.x; c < nc; c += kBlock) b.cnt1[(int64_t)c * nmp + blk] = s_hist[c];
 }
 
 // A tile's pairs sorted by digit in LDS and written as one run per digit:
-// pairs [p0, p1) of (ikey, ioth), digit = (key >> shift) & mask (nd digits),
-// gpos[d] = the tile's first output position of digit d.
-__device__ __forceinline__ void tile_sort_write(const int32_t* __restrict__ ikey, const int32_t* __restrict__ ioth,
+// pairs [p0, p1) of (ikey, iref), digit = (key >> shift) & mask (nd digits),
+// gpos[d] = the tile's first output position of digit d.  LDS holds each
+// sorted key and its place in the tile; the list references are read in
+// sorted order from the tile's own range.
+__device__ __forceinline__ void tile_sort_write(const int32_t* __restrict__ ikey, const ListRef* __restrict__ iref,
                                                 int64_t p0, int64_t p1, int shift, int mask, int nd,
                                                 const uint32_t* s_gpos, uint32_t* s_cnt, int32_t* s_k,
-                                                int32_t* s_o, int64_t* s_wave, int32_t* okey, int32_t* ooth) {
-  int32_t kk[kTilePer], oo[kTilePer];
+                                                uint16_t* s_i, int64_t* s_wave, int32_t* okey, ListRef* oref) {
+  int32_t kk[kTilePer];
   uint32_t rk[kTilePer];
 #pragma unroll
   for (int s = 0; s < kTilePer; ++s) {
     const int64_t p = p0 + (int64_t)s * kBlock + threadIdx.x;
     kk[s] = p < p1 ? ikey[p] : -1;
-    oo[s] = kk[s] >= 0 ? ioth[p] : 0;
   }
 #pragma unroll
-  for (int s = 0; s < kTilePer; ++s) rk[s] = kk[s] >= 0 ? atomicAdd(&s_cnt[(kk[s] >> shift) & mask], 1u) : 0u;
+  for (int s = 0; s < kTilePer; ++s) rk[s] = wave_count_add<2>(s_cnt, kk[s] >= 0 ? (kk[s] >> shift) & mask : -1);
   __syncthreads();
-  // s_cnt -> starts; the digits' counts stay readable as start differences
+  // s_cnt -> starts
   __shared__ uint32_t s_st[kCoarse + 1];
   for (int x = threadIdx.x; x < nd; x += kBlock) s_st[x] = s_cnt[x];
   __syncthreads();
   const uint32_t n = block_scan_lds<1>(s_st, nd, s_wave);
-  if (threadIdx.x == 0) s_st[nd] = n;
-  __syncthreads();
 #pragma unroll
   for (int s = 0; s < kTilePer; ++s) {
     if (kk[s] < 0) continue;
     const uint32_t at = s_st[(kk[s] >> shift) & mask] + rk[s];
     s_k[at] = kk[s];
-    s_o[at] = oo[s];
+    s_i[at] = (uint16_t)(s * kBlock + threadIdx.x);
   }
   __syncthreads();
-  for (uint32_t p = threadIdx.x; p < n; p += kBlock) {
+  ListRef r[kTilePer];
+#pragma unroll
+  for (int s = 0; s < kTilePer; ++s) {
+    const uint32_t p = s * kBlock + threadIdx.x;
+    r[s] = p < n ? iref[p0 + s_i[p]] : (ListRef)0;
+  }
+#pragma unroll
+  for (int s = 0; s < kTilePer; ++s) {
+    const uint32_t p = s * kBlock + threadIdx.x;
+    if (p >= n) continue;
     const int d = (s_k[p] >> shift) & mask;
     const int64_t at = (int64_t)s_gpos[d] + (p - s_st[d]);
     okey[at] = s_k[p];
-    ooth[at] = s_o[p];
+    oref[at] = r[s];
   }
 }
 
-// Level 1: split block blk's pairs -> coarse runs of pkey/poth.
+// Level 1: split block blk's pairs -> coarse runs of pkey/pref.
 __global__ __launch_bounds__(kBlock) void radix_scatter1_kernel(BuildBufs b, int64_t RP, int64_t M, int cs, int nc,
                                                                 int64_t nmp) {
-  __shared__ int32_t s_k[kTile], s_o[kTile];
+  __shared__ int32_t s_k[kTile];
+  __shared__ uint16_t s_i[kTile];
   __shared__ uint32_t s_cnt[kCoarse], s_gpos[kCoarse];
   __shared__ int64_t s_wave[kWaves];
   const int64_t blk = blockIdx.x;
@@ -1106,7 +1175,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter1_kernel(BuildBufs b, int
     s_gpos[c] = b.cnt1[(int64_t)c * nmp + blk];
   }
   __syncthreads();
-  tile_sort_write(b.key, b.oth, j0, j1, cs, 0x7FFFFFFF, nc, s_gpos, s_cnt, s_k, s_o, s_wave, b.pkey, b.poth);
+  tile_sort_write(b.key, b.ref, j0, j1, cs, 0x7FFFFFFF, nc, s_gpos, s_cnt, s_k, s_i, s_wave, b.pkey, b.pref);
 }
 
 // Coarse bucket c's pairs are [cnt1[c * nmp], cnt1[(c + 1) * nmp]) of the
@@ -1158,18 +1227,22 @@ __global__ __launch_bounds__(kBlock) void radix_hist2_kernel(BuildBufs b, int fs
   if (!tile2_of(b, blockIdx.x, nc, nmp, x)) return;
   if (threadIdx.x < kFine) s_h[threadIdx.x] = 0u;
   __syncthreads();
+  int dg[kTilePer];
 #pragma unroll
   for (int s = 0; s < kTilePer; ++s) {
     const int64_t p = x.p0 + (int64_t)s * kBlock + threadIdx.x;
-    if (p < x.p1) atomicAdd(&s_h[(b.pkey[p] >> fs) & (kFine - 1)], 1u);
+    dg[s] = p < x.p1 ? (b.pkey[p] >> fs) & (kFine - 1) : -1;
   }
+#pragma unroll
+  for (int s = 0; s < kTilePer; ++s) wave_count_add<2>(s_h, dg[s]);
   __syncthreads();
   if (threadIdx.x < kFine) b.cnt2[cnt2_at(b, x, threadIdx.x)] = s_h[threadIdx.x];
 }
 
-// Level 2: pkey/poth runs of one coarse bucket -> fine runs of key/oth.
+// Level 2: pkey/pref runs of one coarse bucket -> fine runs of key/ref.
 __global__ __launch_bounds__(kBlock) void radix_scatter2_kernel(BuildBufs b, int fs, int nc, int64_t nmp) {
-  __shared__ int32_t s_k[kTile], s_o[kTile];
+  __shared__ int32_t s_k[kTile];
+  __shared__ uint16_t s_i[kTile];
   __shared__ uint32_t s_cnt[kFine], s_gpos[kFine];
   __shared__ int64_t s_wave[kWaves];
   Tile2 x;
@@ -1179,32 +1252,38 @@ __global__ __launch_bounds__(kBlock) void radix_scatter2_kernel(BuildBufs b, int
     s_gpos[threadIdx.x] = b.cnt2[cnt2_at(b, x, threadIdx.x)];
   }
   __syncthreads();
-  tile_sort_write(b.pkey, b.poth, x.p0, x.p1, fs, kFine - 1, kFine, s_gpos, s_cnt, s_k, s_o, s_wave, b.key, b.oth);
+  tile_sort_write(b.pkey, b.pref, x.p0, x.p1, fs, kFine - 1, kFine, s_gpos, s_cnt, s_k, s_i, s_wave, b.key, b.ref);
 }
 
 // Fine bucket f's pairs are [fine_start(f), fine_start(f + 1)) of the
-// level-2 output; f >= 64 * nc is past the last pair.
-__device__ __forceinline__ int64_t fine_start(const BuildBufs& b, int64_t f, int nc, int64_t nmp) {
+// level-2 output; f >= 64 * nc is past the last pair.  ts = tstart.
+__device__ __forceinline__ int64_t fine_start(const BuildBufs& b, const int64_t* ts, int64_t f, int nc, int64_t nmp) {
   const int c = (int)(f / kFine), d = (int)(f % kFine);
   if (c >= nc) return coarse_start(b, nc, nmp);
-  const int64_t t0 = b.tstart[c], nt = b.tstart[c + 1] - t0;
+  const int64_t t0 = ts[c], nt = ts[c + 1] - t0;
   return nt > 0 ? (int64_t)b.cnt2[t0 * kFine + (int64_t)d * nt] : coarse_start(b, c, nmp);
 }
 
-// One workgroup: chunks of every fine bucket above cap pairs, scanned into
-// chstart[0..nf].
+// One workgroup: fst[f] = first pair of fine bucket f (f <= nf), and the
+// chunks of every fine bucket above cap pairs, scanned into chstart[0..nf].
 __global__ __launch_bounds__(kBlock) void radix_chunks_kernel(BuildBufs b, int nf, int nc, int64_t nmp, int cap,
                                                               int chunk) {
   constexpr int kPer = kMaxFineBuckets / kBlock;
   __shared__ int64_t s_wave[kWaves];
+  __shared__ int64_t s_ts[kCoarse + 1];
+  for (int c = threadIdx.x; c <= nc; c += kBlock) s_ts[c] = b.tstart[c];
+  __syncthreads();
   const int f0 = threadIdx.x * kPer;
+  int64_t st[kPer + 1];
+#pragma unroll
+  for (int k = 0; k <= kPer; ++k) st[k] = f0 + k <= nf ? fine_start(b, s_ts, f0 + k, nc, nmp) : 0;
   int64_t v[kPer];
   int64_t t = 0;
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     v[k] = 0;
     if (f0 + k < nf) {
-      const int64_t n = fine_start(b, f0 + k + 1, nc, nmp) - fine_start(b, f0 + k, nc, nmp);
+      const int64_t n = st[k + 1] - st[k];
       v[k] = n > cap ? (n + chunk - 1) / chunk : 0;
     }
     t += v[k];
@@ -1213,10 +1292,16 @@ __global__ __launch_bounds__(kBlock) void radix_chunks_kernel(BuildBufs b, int n
   int64_t run = block_excl_scan(t, s_wave, &total);
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
-    if (f0 + k < nf) b.chstart[f0 + k] = run;
+    if (f0 + k < nf) {
+      b.chstart[f0 + k] = run;
+      b.fst[f0 + k] = st[k];
+    }
     run += v[k];
   }
-  if (threadIdx.x == 0) b.chstart[nf] = total;
+  if (threadIdx.x == 0) {
+    b.chstart[nf] = total;
+    b.fst[nf] = fine_start(b, s_ts, nf, nc, nmp);
+  }
 }
 
 // Chunk x of the big fine buckets: its bucket and pair range.
@@ -1224,8 +1309,7 @@ struct Chunk {
   int f;
   int64_t p0, p1;
 };
-__device__ __forceinline__ bool chunk_of(const BuildBufs& b, int64_t x, int nf, int nc, int64_t nmp, int chunk,
-                                         Chunk& c) {
+__device__ __forceinline__ bool chunk_of(const BuildBufs& b, int64_t x, int nf, int chunk, Chunk& c) {
   if (x >= b.chstart[nf]) return false;
   int lo = 0, hi = nf - 1;
   while (lo < hi) {
@@ -1234,51 +1318,119 @@ __device__ __forceinline__ bool chunk_of(const BuildBufs& b, int64_t x, int nf, 
     else hi = mid - 1;
   }
   c.f = lo;
-  c.p0 = fine_start(b, lo, nc, nmp) + (x - b.chstart[lo]) * chunk;
-  c.p1 = min(c.p0 + chunk, fine_start(b, lo + 1, nc, nmp));
+  c.p0 = b.fst[lo] + (x - b.chstart[lo]) * chunk;
+  c.p1 = min(c.p0 + chunk, b.fst[lo + 1]);
   return true;
 }
 
 // Big buckets, pass 1: pairs per owner (one global add per owner and chunk).
-__global__ __launch_bounds__(kBlock) void radix_count_big_kernel(BuildBufs b, int64_t RP, int fs, int nf, int nc,
-                                                                 int64_t nmp, int chunk) {
+__global__ __launch_bounds__(kBlock) void radix_count_big_kernel(BuildBufs b, int fs, int nf, int chunk) {
   extern __shared__ uint32_t s_h[];  // [1 << fs]
   Chunk c;
-  if (!chunk_of(b, blockIdx.x, nf, nc, nmp, chunk, c)) return;
+  if (!chunk_of(b, blockIdx.x, nf, chunk, c)) return;
   const int no = 1 << fs;
-  const int64_t q0 = (int64_t)c.f << fs;
+  const int32_t q0 = c.f << fs;
   for (int o = threadIdx.x; o < no; o += kBlock) s_h[o] = 0u;
   __syncthreads();
-  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_h[b.key[p] - q0], 1u);
+  int32_t k[kChunkPer];
+#pragma unroll
+  for (int s = 0; s < kChunkPer; ++s) {
+    const int64_t p = c.p0 + (int64_t)s * kBlock + threadIdx.x;
+    k[s] = p < c.p1 ? b.key[p] - q0 : -1;
+  }
+#pragma unroll
+  for (int s = 0; s < kChunkPer; ++s) wave_count_add<2>(s_h, k[s]);
   __syncthreads();
   for (int o = threadIdx.x; o < no; o += kBlock)
     if (s_h[o]) atomicAdd(&b.c2[q0 + o], (int32_t)s_h[o]);
 }
 
+// Pool position of pair number p owned in graph g.
+__device__ __forceinline__ int64_t pool_pos(const LaunchArgs& a, int g, int64_t p) {
+  return a.descs[g].pci_off - a.descs[g].eb + p;
+}
+
+// Adds v into s_w[o] once per run of equal o across the wave's lanes; o is
+// non-decreasing over the lanes that hold one (o < 0: none, at the end).
+// All 64 lanes call it.
+__device__ __forceinline__ void wave_run_add(unsigned long long* s_w, int o, unsigned long long v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long u = __shfl_up(v, d);
+    const int ou = __shfl_up(o, d);
+    if (lane >= d && ou == o) v += u;
+  }
+  const int on = __shfl_down(o, 1);
+  if (o >= 0 && (lane == 63 || on != o)) atomicAdd(&s_w[o], v);
+}
+
+// Writes the n pairs sorted in LDS (pair s_idx[i] of iref, owner s_own[i])
+// to the pool at dst(o, i), and adds each partner's list length to its
+// owner's work s_w[o]; kUnroll list references in flight per lane (a
+// partner given by its slot is read from lref).
+template <class Dst>
+__device__ __forceinline__ void write_sorted(ListRef* pci, const ListRef* lref, int64_t n, const ListRef* iref,
+                                             const uint16_t* s_idx, const uint16_t* s_own, unsigned long long* s_w,
+                                             Dst dst) {
+  constexpr int kUnroll = 8;
+  for (int64_t i0 = 0; i0 < n; i0 += kUnroll * kBlock) {
+    ListRef r[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      const int64_t i = i0 + (int64_t)j * kBlock + threadIdx.x;
+      r[j] = i < n ? iref[s_idx[i]] : (ListRef)0;
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j)
+      if (r[j] & kRefSlot) r[j] = lref[(uint32_t)r[j]];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      const int64_t i = i0 + (int64_t)j * kBlock + threadIdx.x;
+      int o = -1;
+      if (i < n) {
+        o = s_own[i];
+        pci[dst(o, i)] = r[j];
+      }
+      wave_run_add(s_w, o, (unsigned long long)list_len(r[j]));
+    }
+  }
+}
+
 // Per fine bucket: partner offsets of its owners; a small bucket is placed
 // whole, a big one gets its owners' cursors (absolute pool positions).
-__global__ __launch_bounds__(kBlock) void radix_final_kernel(LaunchArgs a, BuildBufs b, int64_t RP, int fs, int nc,
-                                                             int64_t nmp, int cap) {
+__global__ __launch_bounds__(kBlock) void radix_final_kernel(LaunchArgs a, BuildBufs b, int64_t RP, int fs, int cap) {
   extern __shared__ int64_t s_mem[];
   const int no_max = 1 << fs;
-  uint32_t* s_cnt = (uint32_t*)s_mem;           // [no_max]
+  unsigned long long* s_w = (unsigned long long*)s_mem;  // [no_max] work of each owner
+  uint32_t* s_cnt = (uint32_t*)(s_w + no_max);  // [no_max]
   uint32_t* s_cur = s_cnt + no_max;             // [no_max]
   int32_t* s_g = (int32_t*)(s_cur + no_max);    // [no_max]
-  int32_t* s_oth = s_g + no_max;                // [cap]
-  uint16_t* s_own = (uint16_t*)(s_oth + cap);   // [cap]
+  uint16_t* s_idx = (uint16_t*)(s_g + no_max);  // [cap] place of each sorted pair in the bucket
+  uint16_t* s_own = s_idx + cap;                // [cap]
   __shared__ int64_t s_wave[kWaves];
   const int64_t f = blockIdx.x;
-  const int64_t q0 = f << fs, q1 = min(q0 + no_max, RP);
-  const int no = (int)(q1 - q0);
-  const int64_t p0 = fine_start(b, f, nc, nmp), p1 = fine_start(b, f + 1, nc, nmp), n = p1 - p0;
+  const int32_t q0 = (int32_t)(f << fs);
+  const int no = (int)(min((int64_t)q0 + no_max, RP) - q0);
+  const int64_t p0 = b.fst[f], p1 = b.fst[f + 1], n = p1 - p0;
   const bool big = n > cap;
   for (int o = threadIdx.x; o < no_max; o += kBlock) {
     s_cnt[o] = big && o < no ? (uint32_t)b.c2[q0 + o] : 0u;
     s_g[o] = o < no ? gid_graph(b.gid[q0 + o]) : 0;
+    s_w[o] = 0ull;
   }
   __syncthreads();
+  // a small bucket's owners stay in registers, with their ranks
+  int32_t ko[kFinPer];
+  uint32_t rk[kFinPer];
   if (!big) {
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += kBlock) atomicAdd(&s_cnt[b.key[p] - q0], 1u);
+#pragma unroll
+    for (int s = 0; s < kFinPer; ++s) {
+      const int64_t p = p0 + (int64_t)s * kBlock + threadIdx.x;
+      ko[s] = p < p1 ? b.key[p] - q0 : -1;
+    }
+#pragma unroll
+    for (int s = 0; s < kFinPer; ++s) rk[s] = wave_count_add<2>(s_cnt, ko[s]);
     __syncthreads();
   }
   for (int o = threadIdx.x; o < no_max; o += kBlock) s_cur[o] = s_cnt[o];
@@ -1289,45 +1441,54 @@ __global__ __launch_bounds__(kBlock) void radix_final_kernel(LaunchArgs a, Build
   for (int o = threadIdx.x; o < no; o += kBlock) {
     b.off[q0 + o] = p0 + s_cur[o];
     if (big) {
-      const GraphDesc& d = a.descs[s_g[o]];
-      b.cur[q0 + o] = (unsigned long long)(d.pci_off - d.eb + p0 + s_cur[o]);
+      b.cur[q0 + o] = (unsigned long long)pool_pos(a, s_g[o], p0 + s_cur[o]);
+      b.wq[q0 + o] = 0ull;  // the chunks add to it
     }
   }
   if (big) return;
-  for (int o = threadIdx.x; o < no_max; o += kBlock) s_cnt[o] = s_cur[o];  // cursors
-  __syncthreads();
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
-    const int o = b.key[p] - (int32_t)q0;
-    const uint32_t at = atomicAdd(&s_cnt[o], 1u);
-    s_oth[at] = b.oth[p];
-    s_own[at] = (uint16_t)o;
+#pragma unroll
+  for (int s = 0; s < kFinPer; ++s) {
+    if (ko[s] < 0) continue;
+    const uint32_t at = s_cur[ko[s]] + rk[s];
+    s_idx[at] = (uint16_t)(s * kBlock + threadIdx.x);
+    s_own[at] = (uint16_t)ko[s];
   }
   __syncthreads();
-  ListRef* pci = (ListRef*)a.pci;
-  for (int64_t p = threadIdx.x; p < n; p += kBlock) {
-    const GraphDesc& d = a.descs[s_g[s_own[p]]];
-    pci[d.pci_off - d.eb + p0 + p] = b.lref[s_oth[p]];
-  }
+  write_sorted((ListRef*)a.pci, b.lref, n, b.ref + p0, s_idx, s_own, s_w,
+               [&](int o, int64_t i) { return pool_pos(a, s_g[o], p0 + i); });
+  __syncthreads();
+  for (int o = threadIdx.x; o < no; o += kBlock) b.wq[q0 + o] = s_w[o];
 }
 
 // Big buckets, pass 2: each chunk sorted by owner in LDS; every owner's run
 // reserved on its cursor and written out.
-__global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, BuildBufs b, int64_t RP, int fs, int nf,
-                                                                 int nc, int64_t nmp, int chunk) {
+__global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, BuildBufs b, int fs, int nf, int chunk) {
   extern __shared__ int64_t s_mem[];
   const int no = 1 << fs;
   unsigned long long* s_base = (unsigned long long*)s_mem;  // [no]
-  uint32_t* s_cnt = (uint32_t*)(s_base + no);               // [no]
+  unsigned long long* s_w = s_base + no;                    // [no] work added by this chunk
+  uint32_t* s_cnt = (uint32_t*)(s_w + no);                  // [no]
   uint32_t* s_st = s_cnt + no;                              // [no]
-  int32_t* s_oth = (int32_t*)(s_st + no);                   // [chunk]
-  uint16_t* s_own = (uint16_t*)(s_oth + chunk);             // [chunk]
+  uint16_t* s_idx = (uint16_t*)(s_st + no);                 // [chunk] place of each sorted pair in the chunk
+  uint16_t* s_own = s_idx + chunk;                          // [chunk]
   __shared__ int64_t s_wave[kWaves];
   Chunk c;
-  if (!chunk_of(b, blockIdx.x, nf, nc, nmp, chunk, c)) return;
-  const int64_t q0 = (int64_t)c.f << fs;
-  for (int o = threadIdx.x; o < no; o += kBlock) s_cnt[o] = 0u;
+  if (!chunk_of(b, blockIdx.x, nf, chunk, c)) return;
+  const int32_t q0 = c.f << fs;
+  for (int o = threadIdx.x; o < no; o += kBlock) {
+    s_cnt[o] = 0u;
+    s_w[o] = 0ull;
+  }
   __syncthreads();
-  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) atomicAdd(&s_cnt[b.key[p] - q0], 1u);
+  int32_t ko[kChunkPer];
+  uint32_t rk[kChunkPer];
+#pragma unroll
+  for (int s = 0; s < kChunkPer; ++s) {
+    const int64_t p = c.p0 + (int64_t)s * kBlock + threadIdx.x;
+    ko[s] = p < c.p1 ? b.key[p] - q0 : -1;
+  }
+#pragma unroll
+  for (int s = 0; s < kChunkPer; ++s) rk[s] = wave_count_add<2>(s_cnt, ko[s]);
   __syncthreads();
   for (int o = threadIdx.x; o < no; o += kBlock) {
     const uint32_t h = s_cnt[o];
@@ -1338,21 +1499,19 @@ __global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, B
   if (no <= kBlock) block_scan_lds<1>(s_st, no, s_wave);
   else if (no <= 4 * kBlock) block_scan_lds<4>(s_st, no, s_wave);
   else block_scan_lds<kMaxFineOwners / kBlock>(s_st, no, s_wave);
-  for (int o = threadIdx.x; o < no; o += kBlock) s_cnt[o] = s_st[o];  // cursors
-  __syncthreads();
-  for (int64_t p = c.p0 + threadIdx.x; p < c.p1; p += kBlock) {
-    const int o = b.key[p] - (int32_t)q0;
-    const uint32_t at = atomicAdd(&s_cnt[o], 1u);
-    s_oth[at] = b.oth[p];
-    s_own[at] = (uint16_t)o;
+#pragma unroll
+  for (int s = 0; s < kChunkPer; ++s) {
+    if (ko[s] < 0) continue;
+    const uint32_t at = s_st[ko[s]] + rk[s];
+    s_idx[at] = (uint16_t)(s * kBlock + threadIdx.x);
+    s_own[at] = (uint16_t)ko[s];
   }
   __syncthreads();
-  ListRef* pci = (ListRef*)a.pci;
-  const int64_t n = c.p1 - c.p0;
-  for (int64_t p = threadIdx.x; p < n; p += kBlock) {
-    const int o = s_own[p];
-    pci[s_base[o] + (p - s_st[o])] = b.lref[s_oth[p]];
-  }
+  write_sorted((ListRef*)a.pci, b.lref, c.p1 - c.p0, b.ref + c.p0, s_idx, s_own, s_w,
+               [&](int o, int64_t i) { return (int64_t)s_base[o] + (i - s_st[o]); });
+  __syncthreads();
+  for (int o = threadIdx.x; o < no; o += kBlock)
+    if (s_w[o]) atomicAdd(&b.wq[q0 + o], s_w[o]);
 }
 
 // Rows, row offsets and work after the partner pool is filled (both paths).
@@ -1398,13 +1557,12 @@ static hipError_t launch_partners_radix(const LaunchArgs& a, int64_t RP, int64_t
   radix_chunks_kernel<<<1, kBlock, 0, s>>>(b, x.nf, x.nc, x.nmp, x.cap, x.chunk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const size_t own_lds = (size_t)1 << x.fs;
-  radix_count_big_kernel<<<(unsigned)x.nch, kBlock, 4 * own_lds, s>>>(b, RP, x.fs, x.nf, x.nc, x.nmp, x.chunk);
+  radix_count_big_kernel<<<(unsigned)x.nch, kBlock, 4 * own_lds, s>>>(b, x.fs, x.nf, x.chunk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  radix_final_kernel<<<(unsigned)x.nf, kBlock, radix_final_lds(x.fs, x.cap), s>>>(a, b, RP, x.fs, x.nc, x.nmp,
-                                                                                    x.cap);
+  radix_final_kernel<<<(unsigned)x.nf, kBlock, radix_final_lds(x.fs, x.cap), s>>>(a, b, RP, x.fs, x.cap);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  radix_place_big_kernel<<<(unsigned)x.nch, kBlock, radix_place_lds(x.fs, x.chunk), s>>>(a, b, RP, x.fs, x.nf,
-                                                                                         x.nc, x.nmp, x.chunk);
+  radix_place_big_kernel<<<(unsigned)x.nch, kBlock, radix_place_lds(x.fs, x.chunk), s>>>(a, b, x.fs, x.nf,
+                                                                                         x.chunk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return launch_partner_rows(a, RP, b, s);
 }
@@ -1413,9 +1571,14 @@ hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t M, const Bui
   if (RP <= 0) return hipSuccess;
   RadixGeom x;
   if (bb.key && !bb.pedge && RP <= bb.rows && M <= bb.ids && radix_geom(RP, M, x))
-    return launch_partners_radix(a, RP, M, bb, x, s);
+  {
+    BuildBufs b = bb;
+    b.wq = bb.cur + RP;  // second stripe of the cursors: owners' work
+    return launch_partners_radix(a, RP, M, b, x, s);
+  }
   BuildBufs b = bb;
   b.gid = nullptr;  // rows are found by their graph search (no prep pass here)
+  b.wq = nullptr;   // and every row's work comes from the work pass
   const unsigned blocks = (unsigned)((RP + kBlock - 1) / kBlock);
   const unsigned gblocks = (unsigned)((RP * kBuildG + kBlock - 1) / kBlock);
   hipError_t e = launch_zero(b.c2, RP * kXcds, s);

@@ -195,14 +195,16 @@ struct BuildBufs {
   ListRef* lref;        // [RP] each row slot's own list
   uint16_t* deg16;      // [RP] out-degree of each row slot, saturated at 65535
   int32_t* key;         // [M] owner of each id's pair (-1: none); after level 2, fine-bucket order
-  int32_t* oth;         // [M] partner of each pair, in the order of key
+  ListRef* ref;         // [M] partner list of each pair, in the order of key
   int32_t* pkey;        // [M] level-1 (coarse-bucket) order of key
-  int32_t* poth;        // [M] and of oth
+  ListRef* pref;        // [M] and of ref
   int64_t* bq;          // [nmp + 1] rows consumed before each split block
   uint32_t* cnt1;       // [kCoarse * nmp + 1] pairs per (coarse bucket, split block) -> positions
   uint32_t* cnt2;       // [kFine * nt2 + 1] pairs per (fine digit, level-2 tile) -> positions
   int64_t* tstart;      // [kCoarse + 1] first level-2 tile of each coarse bucket
   int64_t* chstart;     // [kMaxFineBuckets + 1] first chunk of each big fine bucket
+  int64_t* fst;         // [kMaxFineBuckets + 1] first pair of each fine bucket
+  unsigned long long* wq;   // [RP] radix path: each owner's work (its partners' list lengths); else null
   int64_t rows;         // capacity: row slots
   int64_t ids;          // capacity: ids
 };
@@ -225,9 +227,12 @@ constexpr int kMinFineShift = 5;
 constexpr int kMaxFineShift = 11;
 constexpr int kMaxFineOwners = 1 << kMaxFineShift;
 constexpr int64_t kFineCapMax = 8192;
-constexpr int64_t kRadixLds = 65536;
-inline size_t radix_final_lds(int fs, int cap) { return 12 * ((size_t)1 << fs) + 6 * (size_t)cap; }
-inline size_t radix_place_lds(int fs, int chunk) { return 16 * ((size_t)1 << fs) + 6 * (size_t)chunk; }
+constexpr int64_t kChunkMax = 4096;
+constexpr int kFinPer = (int)(kFineCapMax / kBlock);  // pairs per thread of a small bucket
+constexpr int kChunkPer = (int)(kChunkMax / kBlock);  // pairs per thread of a chunk
+constexpr int64_t kRadixLds = 64512;  // 63 KB: room for the kernels' static LDS within 64 KB
+inline size_t radix_final_lds(int fs, int cap) { return 20 * ((size_t)1 << fs) + 4 * (size_t)cap; }
+inline size_t radix_place_lds(int fs, int chunk) { return 24 * ((size_t)1 << fs) + 4 * (size_t)chunk; }
 struct RadixGeom {
   int fs, nf, nc;
   int64_t nmp, nt2, nch;
@@ -245,9 +250,9 @@ inline bool radix_geom(int64_t RP, int64_t M, RadixGeom& x) {
   x.nc = (x.nf + kFine - 1) / kFine;
   x.nmp = (RP + M + kMP - 1) / kMP;
   x.nt2 = (M + kTile - 1) / kTile + x.nc;
-  const int64_t cap = (kRadixLds - 12 * no) / 6 / 256 * 256, chunk = (kRadixLds - 16 * no) / 6 / 256 * 256;
+  const int64_t cap = (kRadixLds - 20 * no) / 4 / 256 * 256, chunk = (kRadixLds - 24 * no) / 4 / 256 * 256;
   x.cap = (int)(cap < kFineCapMax ? cap : kFineCapMax);
-  x.chunk = (int)(chunk < kFineCapMax ? chunk : kFineCapMax);
+  x.chunk = (int)(chunk < kChunkMax ? chunk : kChunkMax);
   x.nch = (M + x.chunk - 1) / x.chunk + x.nf;
   return true;
 }

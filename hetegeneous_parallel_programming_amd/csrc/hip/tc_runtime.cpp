@@ -151,15 +151,16 @@ static int64_t scan_values(int64_t n, int64_t ids) {
 static int64_t build_bytes(int64_t n, int64_t ids) {
   // per row: c1, kc (int32), c2 (int32 per XCD stripe), cur (int64 per
   // stripe), off, voff (int64), scan partials; with the radix part also per
-  // row gid, erow, lref, deg16, per id key, oth, pkey, poth (int32), and the
+  // row gid, erow, lref, deg16, per id key, pkey (int32), ref, pref (list
+  // references), and the
   // merge-path, count and tile arrays
   int64_t b = up256(4 * n) * 2 + up256(4 * tck::kXcds * n) + up256(8 * tck::kXcds * n) + up256(8 * n) * 2 +
               up256(8 * tck::scan_partials(scan_values(n, ids)));
   if (ids > 0) {
     const int64_t nmp = tck::radix_split_blocks(n, ids), nt2 = tck::radix_tiles2(ids);
-    b += up256(4 * n) + up256(8 * (n + 1)) + up256(8 * n) + up256(2 * n) + up256(4 * ids) * 4 +
+    b += up256(4 * n) + up256(8 * (n + 1)) + up256(8 * n) + up256(2 * n) + (up256(4 * ids) + up256(8 * ids)) * 2 +
          up256(8 * (nmp + 1)) + up256(4 * (tck::kCoarse * nmp + 1)) + up256(4 * (tck::kFine * nt2 + 1)) +
-         up256(8 * (tck::kCoarse + 1)) + up256(8 * (tck::kMaxFineBuckets + 1));
+         up256(8 * (tck::kCoarse + 1)) + up256(8 * (tck::kMaxFineBuckets + 1)) * 2;
   }
   return b;
 }
@@ -196,14 +197,15 @@ tc_status BuildScratch::ensure(int64_t rp_slots, int64_t ids) {
     b.lref = (tck::ListRef*)p; p += up256(8 * n);
     b.deg16 = (uint16_t*)p; p += up256(2 * n);
     b.key = (int32_t*)p; p += up256(4 * ids);
-    b.oth = (int32_t*)p; p += up256(4 * ids);
+    b.ref = (tck::ListRef*)p; p += up256(8 * ids);
     b.pkey = (int32_t*)p; p += up256(4 * ids);
-    b.poth = (int32_t*)p; p += up256(4 * ids);
+    b.pref = (tck::ListRef*)p; p += up256(8 * ids);
     b.bq = (int64_t*)p; p += up256(8 * (nmp + 1));
     b.cnt1 = (uint32_t*)p; p += up256(4 * (tck::kCoarse * nmp + 1));
     b.cnt2 = (uint32_t*)p; p += up256(4 * (tck::kFine * nt2 + 1));
     b.tstart = (int64_t*)p; p += up256(8 * (tck::kCoarse + 1));
-    b.chstart = (int64_t*)p;
+    b.chstart = (int64_t*)p; p += up256(8 * (tck::kMaxFineBuckets + 1));
+    b.fst = (int64_t*)p;
     b.ids = ids;
   }
   return TC_OK;

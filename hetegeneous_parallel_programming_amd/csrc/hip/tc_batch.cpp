@@ -5,6 +5,7 @@
 // plain kernel once per graph, as the reference's multi-graph driver does
 // per subgraph (multigpu.cu:190-214: launch, copy back, per graph).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "tc_internal.hpp"
@@ -209,26 +210,88 @@ tc_status tc_batch_create(tc_ctx* ctx, const tc_csr_view* dags, int32_t n_graphs
     b->release();
     return fail(TC_EDEVICE, std::string("batch upload: ") + hipGetErrorString(e));
   }
-  // Partner structures of all graphs in one pass over the pools.
+  // Partner structures of all graphs.  Pairs never cross graphs, so the
+  // build goes by segments of whole graphs, each within the radix build's
+  // limits (row slots, ids; at most kSegIds ids, which also bounds the
+  // scratch), over descriptor copies rebased to the segment's first graph.
   if (n_graphs > 0) {
-    tck::LaunchArgs a{};
-    a.descs = b->d_descs;
-    a.ngraphs = b->n;
-    a.rp = b->rp;
-    a.ci = b->ci;
-    a.P = b->P;
-    b->partners.into(a);
+    struct Seg {
+      int32_t g0, g1;
+      int64_t rows, ids;
+    };
+    // TC_BATCH_SEGMENT_IDS lowers the ids per segment (tests build small
+    // batches in many segments with it)
+    int64_t kSegIds = (int64_t)1 << 27;
+    if (const char* env = std::getenv("TC_BATCH_SEGMENT_IDS")) {
+      const long long v = std::atoll(env);
+      if (v > 0 && v < kSegIds) kSegIds = v;
+    }
+    std::vector<Seg> segs;
+    {
+      Seg cur{0, 0, 0, 0};
+      for (int32_t i = 0; i < n_graphs; ++i) {
+        const tck::GraphDesc& d = b->descs[i];
+        const int64_t r = (int64_t)d.n_rows + 1, m = (i + 1 < n_graphs ? b->descs[i + 1].ci_off : cio) - d.ci_off;
+        tck::RadixGeom x;
+        if (i > cur.g0 && (cur.ids + m > kSegIds || !tck::radix_geom(cur.rows + r, cur.ids + m, x))) {
+          cur.g1 = i;
+          segs.push_back(cur);
+          cur = Seg{i, i, 0, 0};
+        }
+        cur.rows += r;
+        cur.ids += m;
+      }
+      cur.g1 = n_graphs;
+      segs.push_back(cur);
+    }
+    std::vector<tck::GraphDesc> rebased(b->descs);
+    int64_t max_rows = 0, max_ids = 0;
+    for (const Seg& sg : segs) {
+      const tck::GraphDesc& f = b->descs[sg.g0];
+      for (int32_t i = sg.g0; i < sg.g1; ++i) {
+        rebased[i].row_base -= f.row_base;
+        rebased[i].pr_base -= f.pr_base;
+        rebased[i].eb -= f.eb;
+      }
+      max_rows = std::max(max_rows, sg.rows);
+      max_ids = std::max(max_ids, sg.ids);
+    }
+    tck::GraphDesc* d_seg = nullptr;
+    st = dev_alloc((void**)&d_seg, sizeof(tck::GraphDesc) * (size_t)n_graphs, "batch build descs");
+    if (st != TC_OK) {
+      b->release();
+      return st;
+    }
     BuildScratch bs;
-    st = TC_OK;
-    e = hipEventRecord(ctx->ev[0], ctx->stream);
-    if (e == hipSuccess) st = launch_build(a, b->RP, b->cio, bs, ctx->stream);
+    e = hipMemcpy(d_seg, rebased.data(), sizeof(tck::GraphDesc) * (size_t)n_graphs, hipMemcpyHostToDevice);
+    if (e == hipSuccess) st = bs.ensure(max_rows, max_ids);  // once: the segments reuse it in stream order
+    if (e == hipSuccess && st == TC_OK) e = hipEventRecord(ctx->ev[0], ctx->stream);
+    for (const Seg& sg : segs) {
+      if (e != hipSuccess || st != TC_OK) break;
+      const tck::GraphDesc& f = b->descs[sg.g0];
+      const int64_t pr1 = sg.g1 < n_graphs ? b->descs[sg.g1].pr_base : b->P;
+      tck::LaunchArgs a{};
+      a.descs = d_seg + sg.g0;
+      a.ngraphs = sg.g1 - sg.g0;
+      a.rp = b->rp;
+      a.ci = b->ci;
+      a.P = pr1 - f.pr_base;
+      b->partners.into(a);
+      a.vrp += f.pr_base;
+      a.vrow += f.pr_base;
+      a.nv += sg.g0;
+      a.gbase = sg.g0;
+      st = launch_build(a, sg.rows, sg.ids, bs, ctx->stream);
+    }
     if (st == TC_OK && e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
     std::vector<int32_t> nv((size_t)n_graphs);
     if (st == TC_OK && e == hipSuccess)
       e = hipMemcpyAsync(nv.data(), b->partners.nv, sizeof(int32_t) * nv.size(), hipMemcpyDeviceToHost,
                          ctx->stream);
     if (st == TC_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (st != TC_OK || e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);  // nothing may still use the scratch
     bs.release();
+    dev_free(d_seg);
     if (st == TC_OK && e != hipSuccess) st = fail(TC_EDEVICE, std::string("partner build: ") + hipGetErrorString(e));
     if (st != TC_OK) {
       b->release();

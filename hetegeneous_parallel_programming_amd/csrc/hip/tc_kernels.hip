@@ -840,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void partner_rows_fill_kernel(LaunchArgs a,
   // here; the work pass fills rows whose work is still 0
   const uint32_t w = K == 1 && b.wq ? (uint32_t)min(b.wq[q], 0xFFFFFFFFull) : 0u;
   for (int32_t k = 0; k < K; ++k) {
-    vrow[v0 + k] = VRow{set, w, g};
+    vrow[v0 + k] = VRow{set, w, g + a.gbase};
     vrp[v0 + k] = p0 + (c * k) / K;
     if (b.pown) b.pown[d.pr_base + v0 + k] = (int32_t)u;
   }

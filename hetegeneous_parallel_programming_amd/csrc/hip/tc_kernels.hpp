@@ -157,6 +157,7 @@ struct LaunchArgs {
   LocalOut local;
   const uint8_t* own;   // one graph only: null, or per row 1 if it is a pivot row
   int32_t wide_ids;     // 1 when ids reach 2^24: hash sets fold the high id bits into the bucket
+  int32_t gbase;        // partner build of a segment of graphs: index of descs[0] among all graphs
 };
 
 // Ids at or above this need the folded hash (the bucket multiply reads 24 bits).

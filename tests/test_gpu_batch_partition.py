@@ -45,6 +45,26 @@ def test_batch_with_empty_and_tiny_graphs(ctx):
     b.free()
 
 
+@pytest.mark.parametrize("seg_ids", ["1", "40000"])
+def test_batch_built_in_segments(ctx, monkeypatch, seg_ids):
+    """A batch's partner build goes by segments of whole graphs (each with
+    descriptors rebased to its first graph); a small segment size makes many
+    segments, down to one graph each, and the counts and partner rows must
+    equal the one-segment build's and the oracle's."""
+    graphs = Graph.batch(5, 0, 64) + [Graph.from_pairs(7, np.zeros(0, np.int32), np.zeros(0, np.int32)).oriented()]
+    want = np.array([oracle.tc_count(g.out_rowptr(), g.out_colidx()) for g in graphs], np.uint64)
+    whole = ctx.batch(graphs)
+    monkeypatch.setenv("TC_BATCH_SEGMENT_IDS", seg_ids)
+    seg = ctx.batch(graphs)
+    monkeypatch.delenv("TC_BATCH_SEGMENT_IDS")
+    assert seg.info()["partner_rows"] == whole.info()["partner_rows"]
+    for b in (whole, seg):
+        got, _ = b.count()
+        assert np.array_equal(got, want), np.nonzero(got != want)
+    seg.free()
+    whole.free()
+
+
 def test_batch_interleaved_with_single_counts(ctx):
     graphs = Graph.batch(7, 100, 32)
     want = np.array([oracle.tc_count(g.out_rowptr(), g.out_colidx()) for g in graphs], np.uint64)

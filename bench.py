@@ -423,7 +423,7 @@ def serving_mix_capacity(ctx, req, want, k=2000, reps=3):
             "exact": exact}
 
 
-def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clients=8):
+def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clients=8, slots=3):
     """Poisson TC requests from the batched mix plus 1% RMAT-20 through the
     continuous-batching server: saturated throughput; then Poisson arrivals at
     50% and 90% of the batched capacity of the same mix (build included), and
@@ -434,12 +434,12 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
     picks = np.where(rng.random(n_requests) < 0.01, -1, rng.integers(0, len(graphs), n_requests))
     want = np.array([heavy_count if k < 0 else counts[k] for k in picks], np.uint64)
     req = [heavy if k < 0 else graphs[k] for k in picks]
-    out = {"requests": n_requests, "heavy_fraction": float((picks < 0).mean())}
+    out = {"requests": n_requests, "heavy_fraction": float((picks < 0).mean()), "slots": slots}
     cap = serving_mix_capacity(ctx, req, want)
     out["batched_capacity"] = cap
     log(f"serving mix, batched: {cap['graphs_per_s']:.0f} graphs/s with the partner build "
         f"({cap['replay_graphs_per_s']:.0f} replay only; {cap['requests']} requests), exact={cap['exact']}")
-    with Server(ctx, hbm_budget=16 << 30, slots=3, max_batch_graphs=2048, max_batch_rows=16 << 20,
+    with Server(ctx, hbm_budget=16 << 30, slots=slots, max_batch_graphs=2048, max_batch_rows=16 << 20,
                 max_queue=1 << 20, host_staging=4 << 30) as srv:
         for i in range(64):  # warm-up
             srv.submit(req[i], i)
@@ -633,6 +633,7 @@ def main() -> int:
     ap.add_argument("--tune", default="", help="launch parameters, e.g. t_wave=8192,stage_ratio16=24")
     ap.add_argument("--requests", type=int, default=10000, help="requests per serving run")
     ap.add_argument("--clients", type=int, default=8, help="client threads submitting requests")
+    ap.add_argument("--serving-slots", type=int, default=3, help="concurrent launches of the server")
     ap.add_argument("--no-autotune", action="store_true", help="time the batch with the default setting only")
     ap.add_argument("--tune-seconds", type=float, default=12.0, help="autotuner wall-time budget")
     ap.add_argument("--tune-file", default="", help="load the batch setting from / save it to this file")
@@ -765,7 +766,8 @@ def run_side_legs(ctx, args, line) -> None:
             heavy_count = ctx.count(dh)[0]
             dh.free()
             line["extra"]["serving"] = leg_serving(ctx, graphs, counts, heavy, heavy_count,
-                                                   args.requests, args.seed + 100, args.clients)
+                                                   args.requests, args.seed + 100, args.clients,
+                                                   args.serving_slots)
             gold20 = golden_big().get(f"rmat20_seed{args.seed}")
             line["extra"]["serving"]["rmat20_exact_vs_cpu_oracle"] = (
                 gold20["triangles"] == heavy_count if gold20 else None)

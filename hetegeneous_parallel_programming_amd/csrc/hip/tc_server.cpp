@@ -2,14 +2,17 @@
 // device-resident CSR pool.  One server thread owns all device work:
 //   retire  -- finished launches: deliver counts, free their pool extents;
 //   admit   -- waiting requests whose CSR fits the pool (arrival order, with
-//              bounded look-ahead past a blocked head) get their extent;
-//   launch  -- any free slot takes every admitted request (up to the batch
-//              limits) and runs, on its own stream: one staging-copy kernel
-//              that moves all of their CSRs from pinned host memory into
-//              the pool, partner build -> zero -> plan -> per-bin count
-//              kernels, and the copy-back.  (A request that did not fit the
-//              staging area was copied from pageable memory on a copy
-//              stream at admission; its launch waits for that.)
+//              bounded look-ahead past a blocked head) get their extent; a
+//              heavy one (heavy_ids DAG ids or more), or one that did not
+//              fit the pinned staging area, is uploaded by DMA on a copy
+//              stream right away, so it is resident before its launch;
+//   launch  -- a free slot takes every admitted light request (up to the
+//              batch limits); a heavy request goes alone, on slot 0 when
+//              there are several slots, so light launches never wait for
+//              one.  On its own stream the slot runs one staging-copy kernel
+//              that moves the light requests' CSRs from pinned host memory
+//              into the pool, partner build -> zero -> plan -> per-bin count
+//              kernels, and the copy-back.
 // Launch slots run concurrently on separate streams, so requests arriving
 // while others are on the GPU overlap with them on the device.
 #include <algorithm>
@@ -27,6 +30,7 @@ namespace {
 
 constexpr uint64_t kAlign = 256;
 constexpr int64_t kDefaultBatchIds = (int64_t)32 << 20;  // max_batch_ids = 0
+constexpr int64_t kDefaultHeavyIds = (int64_t)4 << 20;   // heavy_ids = 0
 inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 // A request's extent in the HBM pool: rowptr | colidx | partner list
@@ -128,6 +132,7 @@ struct Req {
   char* staged = nullptr;   // or: copy in the pinned staging area
   uint64_t stage_off = 0;
   uint64_t stage_bytes = 0;  // rowptr | pad | colidx, laid out as in the pool
+  bool dma = false;          // uploaded at admission (heavy, or not staged)
   uint64_t off = 0, bytes = 0;
   Upload* upload = nullptr;
   double t_submit = 0, t_admit = 0, t_launch = 0;
@@ -169,6 +174,7 @@ struct tc_server {
   std::deque<tc_request_result> results;
   int64_t outstanding = 0, submitted = 0, completed = 0, rejected = 0, launches = 0, launched_graphs = 0;
   int64_t max_ids = 0;  // DAG ids per launch (cfg.max_batch_ids or its default)
+  int64_t heavy_ids = 0;  // cfg.heavy_ids or its default
   // time accounting (ns): submit (callers, atomic); the server thread's
   // phases, read by get_stats (atomic)
   std::atomic<int64_t> t_submit{0}, t_retire{0}, t_admit{0}, t_launch{0}, t_wait_busy{0}, t_wait_idle{0},
@@ -250,25 +256,30 @@ bool tc_server::admit() {
     }
   }
   if (take.empty()) return false;
-  // Staged requests are copied by their launch's staging kernel.  Pageable
-  // ones go out on the copy stream without waiting: one event marks their
-  // end, and the launches of these requests wait on it on the device.
+  // Light staged requests are copied by their launch's staging kernel.
+  // Heavy or pageable ones go out by DMA on the copy stream now, without
+  // waiting: one event marks their end, and their launches wait on it on
+  // the device.
   hipError_t e = hipSuccess;
-  bool pageable = false;
+  bool dma = false;
   for (Req* r : take) {
-    if (r->staged) continue;
-    pageable = true;
+    if (r->staged && r->nnz < heavy_ids) continue;
+    dma = r->dma = true;
     const Layout L(r->n, r->nnz);
     char* base = pool + r->off;
-    e = hipMemcpyAsync(base, r->rp.data(), sizeof(int64_t) * ((size_t)r->n + 1), hipMemcpyHostToDevice,
-                       copy_stream);
-    if (e == hipSuccess && r->nnz > 0)
-      e = hipMemcpyAsync(base + L.ci, r->ci.data(), sizeof(int32_t) * (size_t)r->nnz, hipMemcpyHostToDevice,
+    if (r->staged) {
+      e = hipMemcpyAsync(base, r->staged, r->stage_bytes, hipMemcpyHostToDevice, copy_stream);
+    } else {
+      e = hipMemcpyAsync(base, r->rp.data(), sizeof(int64_t) * ((size_t)r->n + 1), hipMemcpyHostToDevice,
                          copy_stream);
+      if (e == hipSuccess && r->nnz > 0)
+        e = hipMemcpyAsync(base + L.ci, r->ci.data(), sizeof(int32_t) * (size_t)r->nnz, hipMemcpyHostToDevice,
+                           copy_stream);
+    }
     if (e != hipSuccess) break;
   }
   Upload* up = nullptr;
-  if (e == hipSuccess && pageable) {
+  if (e == hipSuccess && dma) {
     up = new Upload;
     if (!free_events.empty()) {
       up->ev = free_events.back();
@@ -293,7 +304,7 @@ bool tc_server::admit() {
       finish(r, 0, TC_EDEVICE, 0, t);
       continue;
     }
-    if (!r->staged) {
+    if (r->dma) {
       r->upload = up;
       ++up->refs;
     }
@@ -305,15 +316,37 @@ bool tc_server::admit() {
 
 bool tc_server::launch() {
   bool any = false;
-  for (auto& s : slots) {
+  for (size_t si = 0; si < slots.size(); ++si) {
+    Slot& s = slots[si];
     if (s.busy || ready.empty()) continue;
-    int64_t rows = 0, ids = 0;
-    while (!ready.empty() && (int)s.reqs.size() < cfg.max_batch_graphs &&
-           rows + ready.front()->nv_max + 1 <= cfg.max_batch_rows && ids + ready.front()->nnz <= max_ids) {
-      rows += ready.front()->nv_max + 1;
-      ids += ready.front()->nnz;
-      s.reqs.push_back(ready.front());
-      ready.pop_front();
+    // slot 0 of several takes the oldest heavy request first (alone); every
+    // slot then takes light requests in order, up to the batch limits
+    const bool heavy_slot = si == 0 || slots.size() == 1;
+    if (heavy_slot) {
+      for (auto it = ready.begin(); it != ready.end(); ++it) {
+        if ((*it)->nnz < heavy_ids) continue;
+        if (slots.size() > 1 || it == ready.begin()) {  // one slot: in arrival order
+          s.reqs.push_back(*it);
+          ready.erase(it);
+        }
+        break;
+      }
+    }
+    if (s.reqs.empty()) {
+      int64_t rows = 0, ids = 0;
+      for (auto it = ready.begin(); it != ready.end() && (int)s.reqs.size() < cfg.max_batch_graphs;) {
+        Req* r = *it;
+        if (r->nnz >= heavy_ids) {
+          if (slots.size() == 1) break;  // one slot: a heavy request waits its turn
+          ++it;
+          continue;
+        }
+        if (rows + r->nv_max + 1 > cfg.max_batch_rows || ids + r->nnz > max_ids) break;
+        rows += r->nv_max + 1;
+        ids += r->nnz;
+        s.reqs.push_back(r);
+        it = ready.erase(it);
+      }
     }
     if (s.reqs.empty()) continue;
     const int k = (int)s.reqs.size();
@@ -331,7 +364,7 @@ bool tc_server::launch() {
       pb += (int64_t)r->nv_max + 1;
       rb += (int64_t)r->n + 1;
       eb += r->nnz;
-      if (r->staged) {
+      if (r->staged && !r->dma) {
         hc[ncopies++] = tck::StageCopy{staging_dev + r->stage_off, pool + r->off, (int64_t)r->stage_bytes, cblocks};
         cblocks += tck::stage_blocks((int64_t)r->stage_bytes);
       }
@@ -482,10 +515,12 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
   if (cfg->max_batch_rows >= ((int64_t)1 << 31)) return fail(TC_ERANGE, "max_batch_rows must be < 2^31");
   if (cfg->hbm_budget < kAlign) return fail(TC_EINVAL, "hbm_budget too small");
   if (cfg->max_batch_ids < 0) return fail(TC_EINVAL, "max_batch_ids must be >= 0");
+  if (cfg->heavy_ids < 0) return fail(TC_EINVAL, "heavy_ids must be >= 0");
   std::unique_ptr<tc_server> s(new tc_server);
   s->ctx = ctx;
   s->cfg = *cfg;
   s->max_ids = cfg->max_batch_ids > 0 ? cfg->max_batch_ids : kDefaultBatchIds;
+  s->heavy_ids = cfg->heavy_ids > 0 ? cfg->heavy_ids : kDefaultHeavyIds;
   DeviceGuard guard(ctx->device);
   tc_status st = dev_alloc((void**)&s->pool, cfg->hbm_budget, "server pool");
   if (st != TC_OK) return st;

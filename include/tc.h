@@ -233,10 +233,14 @@ tc_status tc_tune_load(tc_ctx* ctx, const char* path, const char* key);
  * which counts one pre-loaded graph per process (successful_push_pull/main.cc:11-36).
  * A server thread admits submitted requests in arrival order when their CSR
  * footprint fits the HBM pool (a request that cannot fit yet waits; up to
- * `lookahead` later ones may pass it), uploads them on a copy stream while
- * earlier launches run, and whenever one of `slots` launch slots is free it
- * launches every admitted request (up to the batch limits) as one batched
- * count on that slot's stream.  Results are delivered by tc_server_poll. */
+ * `lookahead` later ones may pass it); a request of at least `heavy_ids`
+ * DAG ids is uploaded by DMA on a copy stream at admission while earlier
+ * launches run, smaller ones by one copy kernel at the start of their
+ * launch.  Whenever one of `slots` launch slots is free it launches every
+ * admitted light request (up to the batch limits) as one batched count on
+ * that slot's stream; a heavy request goes alone, on slot 0 when there are
+ * several slots, so light requests never wait for one.  Results are
+ * delivered by tc_server_poll. */
 typedef struct tc_server_config {
   uint64_t hbm_budget;       /* bytes of HBM for resident request CSRs            */
   int32_t slots;             /* concurrent launches (streams), 1..8               */
@@ -248,6 +252,8 @@ typedef struct tc_server_config {
                                 into (async DMA to HBM); 0 = pageable copies     */
   int64_t max_batch_ids;     /* DAG ids per launch (sizes per-slot build scratch,
                                 24 bytes per id); 0 = 32 Mi ids                  */
+  int64_t heavy_ids;         /* DAG ids from which a request is heavy: uploaded by
+                                DMA at admission, launched alone; 0 = 4 Mi ids   */
 } tc_server_config;
 
 typedef struct tc_request_result {

@@ -170,6 +170,7 @@ def test_rmat20_request_alone_and_in_the_mix(ctx):
     for rep in range(2):
         r = res[2_000_000 + rep]
         assert r["status"] == 0 and r["count"] == want_heavy, r
+        assert r["batch_graphs"] == 1, r  # a heavy request (>= 4 Mi ids) is launched alone
         for i in range(300):
             r = res[rep * 1000 + i]
             assert r["status"] == 0 and r["count"] == want_light[i], (rep, i, r)
@@ -212,3 +213,25 @@ def test_malformed_requests_are_rejected_and_release_staging(ctx, mix):
     assert res[1]["status"] == 0 and res[1]["count"] == want_big
     assert [res[10 + i]["count"] for i in range(20)] == want[:20]
     assert st["rejected"] == 4 and st["completed"] == 21
+
+
+def test_heavy_requests_go_alone_and_light_ones_pass_them(ctx, mix):
+    """With heavy_ids between the mix graphs (<= 1.81M ids) and its RMAT-2^17
+    request (1.87M ids), that request
+    is uploaded at admission and launched alone (on slot 0), light requests
+    are batched around it on every slot, and all counts stay exact; with one
+    slot the heavy request keeps its place in arrival order."""
+    graphs, want = mix
+    big = len(graphs) - 1
+    for slots in (3, 1):
+        with Server(ctx, hbm_budget=1 << 30, slots=slots, heavy_ids=1_850_000) as srv:
+            order = [big] + list(range(40)) + [big] + list(range(40, 80)) + [big]
+            for t, i in enumerate(order):
+                srv.submit(graphs[i], t)
+            res = collect(srv, len(order))
+        assert len(res) == len(order)
+        for t, i in enumerate(order):
+            r = res[t]
+            assert r["status"] == 0 and r["count"] == want[i], (slots, t, r)
+            if i == big:
+                assert r["batch_graphs"] == 1, (slots, t, r)

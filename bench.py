@@ -423,7 +423,8 @@ def serving_mix_capacity(ctx, req, want, k=2000, reps=3):
             "exact": exact}
 
 
-def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clients=8, slots=3):
+def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clients=8, slots=3,
+                saturated_only=False, fill_graphs=0, fill_wait_us=0):
     """Poisson TC requests from the batched mix plus 1% RMAT-20 through the
     continuous-batching server: saturated throughput; then Poisson arrivals at
     50% and 90% of the batched capacity of the same mix (build included), and
@@ -440,7 +441,7 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
     log(f"serving mix, batched: {cap['graphs_per_s']:.0f} graphs/s with the partner build "
         f"({cap['replay_graphs_per_s']:.0f} replay only; {cap['requests']} requests), exact={cap['exact']}")
     with Server(ctx, hbm_budget=16 << 30, slots=slots, max_batch_graphs=2048, max_batch_rows=16 << 20,
-                max_queue=1 << 20, host_staging=4 << 30) as srv:
+                max_queue=1 << 20, host_staging=4 << 30, fill_graphs=fill_graphs, fill_wait_us=fill_wait_us) as srv:
         for i in range(64):  # warm-up
             srv.submit(req[i], i)
         srv.drain()
@@ -514,9 +515,10 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
             f"exact={sat['exact']}; ms per s: " + ", ".join(f"{k} {v:.0f}" for k, v in sat["time_per_s"].items()))
         # load_50 / load_90: fractions of the batched capacity of this mix;
         # load_50_of_saturated / ...: of the server's own saturated rate
-        for key, frac, base in (("load_50", 0.5, cap["graphs_per_s"]), ("load_90", 0.9, cap["graphs_per_s"]),
-                                ("load_50_of_saturated", 0.5, sat["graphs_per_s"]),
-                                ("load_90_of_saturated", 0.9, sat["graphs_per_s"])):
+        points = () if saturated_only else (
+            ("load_50", 0.5, cap["graphs_per_s"]), ("load_90", 0.9, cap["graphs_per_s"]),
+            ("load_50_of_saturated", 0.5, sat["graphs_per_s"]), ("load_90_of_saturated", 0.9, sat["graphs_per_s"]))
+        for key, frac, base in points:
             lam = frac * base
             arr = np.cumsum(rng.exponential(1.0 / lam, n_requests)) * 1e3
             r = run(arr, clients)
@@ -528,7 +530,8 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
                 f"exact={r['exact']}, mean batch {r['mean_batch']:.1f}")
         out["server_stats"] = srv.stats()
     out["exact_all"] = cap["exact"] and all(
-        out[k]["exact"] for k in ("saturated", "load_50", "load_90", "load_50_of_saturated", "load_90_of_saturated"))
+        out[k]["exact"] for k in ("saturated", "load_50", "load_90", "load_50_of_saturated", "load_90_of_saturated")
+        if k in out)
     return out
 
 
@@ -634,6 +637,9 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=10000, help="requests per serving run")
     ap.add_argument("--clients", type=int, default=8, help="client threads submitting requests")
     ap.add_argument("--serving-slots", type=int, default=3, help="concurrent launches of the server")
+    ap.add_argument("--serving-saturated-only", action="store_true", help="serving leg: the saturated run only")
+    ap.add_argument("--serving-fill", type=int, default=0, help="server fill_graphs (0: its default)")
+    ap.add_argument("--serving-fill-wait-us", type=int, default=0, help="server fill_wait_us (0: its default)")
     ap.add_argument("--no-autotune", action="store_true", help="time the batch with the default setting only")
     ap.add_argument("--tune-seconds", type=float, default=12.0, help="autotuner wall-time budget")
     ap.add_argument("--tune-file", default="", help="load the batch setting from / save it to this file")
@@ -767,7 +773,8 @@ def run_side_legs(ctx, args, line) -> None:
             dh.free()
             line["extra"]["serving"] = leg_serving(ctx, graphs, counts, heavy, heavy_count,
                                                    args.requests, args.seed + 100, args.clients,
-                                                   args.serving_slots)
+                                                   args.serving_slots, args.serving_saturated_only,
+                                                   args.serving_fill, args.serving_fill_wait_us)
             gold20 = golden_big().get(f"rmat20_seed{args.seed}")
             line["extra"]["serving"]["rmat20_exact_vs_cpu_oracle"] = (
                 gold20["triangles"] == heavy_count if gold20 else None)

@@ -7,9 +7,9 @@
 //              fit the pinned staging area, is uploaded by DMA on a copy
 //              stream right away, so it is resident before its launch;
 //   launch  -- a free slot takes every admitted light request (up to the
-//              batch limits); a heavy request goes alone, on slot 0 when
-//              there are several slots, so light launches never wait for
-//              one.  On its own stream the slot runs one staging-copy kernel
+//              batch limits); a heavy request goes alone, on any slot but
+//              the last when there are several, so light launches never
+//              wait for one and always keep a slot.  On its own stream the slot runs one staging-copy kernel
 //              that moves the light requests' CSRs from pinned host memory
 //              into the pool, partner build -> zero -> plan -> per-bin count
 //              kernels, and the copy-back.
@@ -31,6 +31,8 @@ namespace {
 constexpr uint64_t kAlign = 256;
 constexpr int64_t kDefaultBatchIds = (int64_t)32 << 20;  // max_batch_ids = 0
 constexpr int64_t kDefaultHeavyIds = (int64_t)4 << 20;   // heavy_ids = 0
+constexpr int32_t kDefaultFillGraphs = 256;               // fill_graphs = 0
+constexpr int32_t kDefaultFillWaitUs = 2000;              // fill_wait_us = 0
 inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 // A request's extent in the HBM pool: rowptr | colidx | partner list
@@ -175,6 +177,7 @@ struct tc_server {
   int64_t outstanding = 0, submitted = 0, completed = 0, rejected = 0, launches = 0, launched_graphs = 0;
   int64_t max_ids = 0;  // DAG ids per launch (cfg.max_batch_ids or its default)
   int64_t heavy_ids = 0;  // cfg.heavy_ids or its default
+  int32_t fill_graphs = 0, fill_wait_us = 0;  // cfg's, or their defaults
   // time accounting (ns): submit (callers, atomic); the server thread's
   // phases, read by get_stats (atomic)
   std::atomic<int64_t> t_submit{0}, t_retire{0}, t_admit{0}, t_launch{0}, t_wait_busy{0}, t_wait_idle{0},
@@ -319,9 +322,10 @@ bool tc_server::launch() {
   for (size_t si = 0; si < slots.size(); ++si) {
     Slot& s = slots[si];
     if (s.busy || ready.empty()) continue;
-    // slot 0 of several takes the oldest heavy request first (alone); every
-    // slot then takes light requests in order, up to the batch limits
-    const bool heavy_slot = si == 0 || slots.size() == 1;
+    // every slot but the last of several takes the oldest heavy request
+    // first (alone), so light requests always keep one slot; every slot
+    // then takes light requests in order, up to the batch limits
+    const bool heavy_slot = si + 1 < slots.size() || slots.size() == 1;
     if (heavy_slot) {
       for (auto it = ready.begin(); it != ready.end(); ++it) {
         if ((*it)->nnz < heavy_ids) continue;
@@ -330,6 +334,19 @@ bool tc_server::launch() {
           ready.erase(it);
         }
         break;
+      }
+    }
+    if (s.reqs.empty() && fill_graphs > 1) {
+      // sharing the GPU with another launch: wait for a full batch, or for
+      // the oldest light request to have waited fill_wait_us
+      bool others = false;
+      for (size_t sj = 0; sj < slots.size(); ++sj) others |= sj != si && slots[sj].busy;
+      if (others) {
+        int32_t lights = 0;
+        double oldest = 0;
+        for (const Req* r : ready)
+          if (r->nnz < heavy_ids && lights++ == 0) oldest = r->t_submit;
+        if (lights < fill_graphs && (lights == 0 || now_ms() - oldest < fill_wait_us * 1e-3)) continue;
       }
     }
     if (s.reqs.empty()) {
@@ -516,11 +533,14 @@ tc_status tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server**
   if (cfg->hbm_budget < kAlign) return fail(TC_EINVAL, "hbm_budget too small");
   if (cfg->max_batch_ids < 0) return fail(TC_EINVAL, "max_batch_ids must be >= 0");
   if (cfg->heavy_ids < 0) return fail(TC_EINVAL, "heavy_ids must be >= 0");
+  if (cfg->fill_graphs < 0 || cfg->fill_wait_us < 0) return fail(TC_EINVAL, "fill_graphs and fill_wait_us must be >= 0");
   std::unique_ptr<tc_server> s(new tc_server);
   s->ctx = ctx;
   s->cfg = *cfg;
   s->max_ids = cfg->max_batch_ids > 0 ? cfg->max_batch_ids : kDefaultBatchIds;
   s->heavy_ids = cfg->heavy_ids > 0 ? cfg->heavy_ids : kDefaultHeavyIds;
+  s->fill_graphs = cfg->fill_graphs > 0 ? cfg->fill_graphs : kDefaultFillGraphs;
+  s->fill_wait_us = cfg->fill_wait_us > 0 ? cfg->fill_wait_us : kDefaultFillWaitUs;
   DeviceGuard guard(ctx->device);
   tc_status st = dev_alloc((void**)&s->pool, cfg->hbm_budget, "server pool");
   if (st != TC_OK) return st;

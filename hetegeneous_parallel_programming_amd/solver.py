@@ -75,7 +75,8 @@ class AutotuneReport(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", nat.c_u64), ("slots", nat.c_i32), ("max_batch_graphs", nat.c_i32),
                 ("max_batch_rows", nat.c_i64), ("max_queue", nat.c_i32), ("lookahead", nat.c_i32),
-                ("host_staging", nat.c_u64), ("max_batch_ids", nat.c_i64), ("heavy_ids", nat.c_i64)]
+                ("host_staging", nat.c_u64), ("max_batch_ids", nat.c_i64), ("heavy_ids", nat.c_i64),
+                ("fill_graphs", nat.c_i32), ("fill_wait_us", nat.c_i32)]
 
 
 class RequestResult(ctypes.Structure):
@@ -435,9 +436,9 @@ class Server:
     def __init__(self, ctx: Context, hbm_budget: int = 8 << 30, slots: int = 3,
                  max_batch_graphs: int = 1024, max_batch_rows: int = 8 << 20,
                  max_queue: int = 1 << 16, lookahead: int = 64, host_staging: int = 1 << 30,
-                 max_batch_ids: int = 0, heavy_ids: int = 0):
+                 max_batch_ids: int = 0, heavy_ids: int = 0, fill_graphs: int = 0, fill_wait_us: int = 0):
         cfg = ServerConfig(hbm_budget, slots, max_batch_graphs, max_batch_rows, max_queue, lookahead,
-                           host_staging, max_batch_ids, heavy_ids)
+                           host_staging, max_batch_ids, heavy_ids, fill_graphs, fill_wait_us)
         h = ctypes.c_void_p()
         _check(device_lib().tc_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
         self.ctx = ctx

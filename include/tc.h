@@ -238,8 +238,11 @@ tc_status tc_tune_load(tc_ctx* ctx, const char* path, const char* key);
  * launches run, smaller ones by one copy kernel at the start of their
  * launch.  Whenever one of `slots` launch slots is free it launches every
  * admitted light request (up to the batch limits) as one batched count on
- * that slot's stream; a heavy request goes alone, on slot 0 when there are
- * several slots, so light requests never wait for one.  Results are
+ * that slot's stream -- when another launch is running, only once
+ * `fill_graphs` light requests are ready or the oldest has waited
+ * `fill_wait_us`, so launches under load are large; a heavy request goes
+ * alone, on any slot but the last when there are several, so light requests
+ * never wait for one and always keep a slot.  Results are
  * delivered by tc_server_poll. */
 typedef struct tc_server_config {
   uint64_t hbm_budget;       /* bytes of HBM for resident request CSRs            */
@@ -254,6 +257,10 @@ typedef struct tc_server_config {
                                 24 bytes per id); 0 = 32 Mi ids                  */
   int64_t heavy_ids;         /* DAG ids from which a request is heavy: uploaded by
                                 DMA at admission, launched alone; 0 = 4 Mi ids   */
+  int32_t fill_graphs;       /* a light launch that would share the GPU with
+                                another waits until this many light requests are
+                                ready ... (0 = 256; 1 = launch at once)          */
+  int32_t fill_wait_us;      /* ... or the oldest has waited this long (0 = 2000) */
 } tc_server_config;
 
 typedef struct tc_request_result {

@@ -218,7 +218,7 @@ def test_malformed_requests_are_rejected_and_release_staging(ctx, mix):
 def test_heavy_requests_go_alone_and_light_ones_pass_them(ctx, mix):
     """With heavy_ids between the mix graphs (<= 1.81M ids) and its RMAT-2^17
     request (1.87M ids), that request
-    is uploaded at admission and launched alone (on slot 0), light requests
+    is uploaded at admission and launched alone (not on the last slot), light requests
     are batched around it on every slot, and all counts stay exact; with one
     slot the heavy request keeps its place in arrival order."""
     graphs, want = mix

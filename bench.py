@@ -770,11 +770,14 @@ def run_side_legs(ctx, args, line) -> None:
             heavy.orientation()
             dh = ctx.upload(heavy)
             heavy_count = ctx.count(dh)[0]
+            heavy_build_ms = dh.info()["build_ms"]
             dh.free()
+            log(f"rmat20: partner build {heavy_build_ms:.2f} ms on the device")
             line["extra"]["serving"] = leg_serving(ctx, graphs, counts, heavy, heavy_count,
                                                    args.requests, args.seed + 100, args.clients,
                                                    args.serving_slots, args.serving_saturated_only,
                                                    args.serving_fill, args.serving_fill_wait_us)
+            line["extra"]["serving"]["rmat20_build_ms"] = heavy_build_ms
             gold20 = golden_big().get(f"rmat20_seed{args.seed}")
             line["extra"]["serving"]["rmat20_exact_vs_cpu_oracle"] = (
                 gold20["triangles"] == heavy_count if gold20 else None)

@@ -149,6 +149,7 @@ struct tc_ctx {
   tcr::PlanScratch scratch;
   uint64_t* pinned = nullptr;  // 64 B count + 64 B bin counters
   void* noalloc = nullptr;     // device address of the static scratch of tc_count_noalloc
+  uint64_t* plain_out = nullptr;  // device: the single-launch count's word, zero between calls
   int grid() const { return tune.grid > 0 ? tune.grid : count_grid; }
 };
 

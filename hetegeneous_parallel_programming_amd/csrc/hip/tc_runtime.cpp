@@ -330,6 +330,12 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   g->partners.into(a);
   BuildScratch bs;
   hipStream_t s = ctx->stream;
+  // the scratch is allocated before the timed span: build_ms is the device's build
+  st = bs.ensure((int64_t)g->n + 1, g->nnz);
+  if (st != TC_OK) {
+    cleanup();
+    return st;
+  }
   e = hipEventRecord(ctx->ev[0], s);
   if (e == hipSuccess) {
     st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, s);
@@ -408,6 +414,11 @@ tc_status tc_ctx_create(int32_t device, tc_ctx** out) try {
   TC_HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   for (auto& ev : c->ev) TC_HIP_TRY(hipEventCreate(&ev));
   TC_HIP_TRY(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));
+  {
+    tc_status st = dev_alloc((void**)&c->plain_out, 256, "plain count word");
+    if (st != TC_OK) return st;
+    TC_HIP_TRY(hipMemset(c->plain_out, 0, 256));
+  }
   // device address of the allocation-free count's static scratch (this device's copy)
   TC_HIP_TRY(tck::noalloc_scratch_address(&c->noalloc));
   c->count_grid = tck::count_blocks_per_cu() * c->cus;
@@ -425,6 +436,7 @@ void tc_ctx_destroy(tc_ctx* ctx) {
     DeviceGuard guard(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
+    dev_free(ctx->plain_out);
     for (auto& ev : ctx->ev)
       if (ev) (void)hipEventDestroy(ev);
     for (hipStream_t x : ctx->side)
@@ -691,28 +703,31 @@ static tc_status count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stat
 }
 
 // The plain kernel's single launch, its u64 sum at `out` on the device.
-static tc_status plain_count(tc_ctx* ctx, tc_graph* g, uint64_t* out, uint64_t* total, tc_stats* stats) {
+// kept_zero: `out` is the context's plain-count word, zero between calls:
+// no zero launch before the kernel; it is cleared again after the result is
+// read, off the caller's critical path (the next launch on the stream waits
+// for it).
+static tc_status plain_count(tc_ctx* ctx, tc_graph* g, uint64_t* out, uint64_t* total, tc_stats* stats,
+                             bool kept_zero = false) {
   const double t0 = now_ms();
   const int64_t P = (int64_t)g->row_end - g->row_begin;
   hipStream_t s = ctx->stream;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->plain_grid, (P + tck::kWaves - 1) / tck::kWaves));
-  TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  TC_HIP_TRY(tck::launch_zero(out, 2, s));
+  if (!kept_zero) TC_HIP_TRY(tck::launch_zero(out, 2, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
   TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, g->own, out, grid, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, out, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
+  if (kept_zero) TC_HIP_TRY(tck::launch_zero(out, 2, s));
   tc_status st = finish_stats(ctx, stats, t0, false, P);
   if (st == TC_OK && stats) stats->single_launch = 1;
   return st;
 }
 
 static tc_status count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) {
-  tc_status st = ctx->scratch.ensure(0, 1);
-  if (st != TC_OK) return st;
-  return plain_count(ctx, g, ctx->scratch.outs, total, stats);
+  return plain_count(ctx, g, ctx->plain_out, total, stats, true);
 }
 
 // Graphs whose counted rows hold at most this many DAG ids, and whose lists

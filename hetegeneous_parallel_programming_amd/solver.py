@@ -502,13 +502,15 @@ class Server:
             pass
 
 
-def TCSolver(g: Graph, n_gpus: int = 1, chunk_size: int = 1024, ctx: Context | None = None) -> int:
+def TCSolver(g: Graph, n_gpus: int = 1, chunk_size: int = 1024, ctx: Context | None = None,
+             stats: dict | None = None) -> int:
     """Count the triangles of DAG `g` on the GPU; returns the u64 total.
 
     Mirrors ``TCSolver(Graph&, uint64_t& total, int n_gpus, int chunk_size)``
     (successful_push_pull/main.cc:5).  Only one GPU is driven here: n_gpus
     other than 1 raises.  chunk_size is accepted for signature parity; the
-    reference's GPU solvers ignore it too (main.cc:25,28).
+    reference's GPU solvers ignore it too (main.cc:25,28).  `stats`, if
+    given, receives the count's statistics (launch form, plan and count ms).
     """
     if n_gpus != 1:
         raise TcError(nat.TC_EINVAL, "this build drives exactly one GPU (n_gpus must be 1)")
@@ -516,8 +518,10 @@ def TCSolver(g: Graph, n_gpus: int = 1, chunk_size: int = 1024, ctx: Context | N
     c = Context(0) if own else ctx
     try:
         dg = c.upload(g)
-        total, _ = c.count(dg)
+        total, st = c.count(dg)
         dg.free()
+        if stats is not None:
+            stats.update(st)
         return total
     finally:
         if own:

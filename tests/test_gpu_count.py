@@ -254,13 +254,20 @@ def test_ids_beyond_2_24_in_hash_sets(ctx):
 
 
 def test_tc_count_picks_the_single_launch_for_small_graphs(ctx):
-    """tc_count runs the plain kernel's single launch for a graph of at most
-    2^18 DAG ids (ER n=16384 d=16: ~131k) and the binned count above that,
-    whose plan it keeps after the first call; every count is the oracle's."""
-    small = Graph.er(16384, 16, 1).oriented()
+    """tc_count (and TCSolver through it) runs the plain kernel's single
+    launch for a graph of at most 2^18 DAG ids (ER n=16384 d=16, seeds 1-3:
+    ~131k) and the binned count above that, whose plan it keeps after the
+    first call; every count is the oracle's."""
+    from hetegeneous_parallel_programming_amd.solver import TCSolver
+
+    smalls = [Graph.er(16384, 16, seed).oriented() for seed in (1, 2, 3)]
     big = Graph.rmat(1 << 16, 16, 3).oriented()
-    assert small.E() <= 1 << 18 < big.E()
-    for g, single in ((small, True), (big, False)):
+    assert all(g.E() <= 1 << 18 for g in smalls) and big.E() > 1 << 18
+    for g in smalls:  # the reference's boundary takes the same path
+        st = {}
+        assert TCSolver(g, ctx=ctx, stats=st) == oracle.tc_count(g.out_rowptr(), g.out_colidx())
+        assert st["single_launch"]
+    for g, single in [(s, True) for s in smalls] + [(big, False)]:
         want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
         dg = ctx.upload(g)
         (c1, st1), (c2, st2) = ctx.count(dg), ctx.count(dg)

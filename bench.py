@@ -403,28 +403,34 @@ def batch_tuner(args):
     return tune
 
 
-def serving_mix_capacity(ctx, req, want, k=2000, reps=3):
-    """Batched capacity of the serving mix: the first k requests (the same
-    draw: batched-workload graphs plus ~1% RMAT-20) counted by one captured
-    batched launch.  Replay-only graphs/s, and with the batch's partner build
-    included (a served graph arrives without its partner structure)."""
+def serving_mix_capacity(ctx, req, want, heavy, k=2000, reps=3):
+    """Batched capacity of the serving mix: k requests of the run's draw with
+    its heavy share (the first round(k * share) RMAT-20 requests and the first
+    light ones, so a short sample does not under- or over-represent the 1%
+    heavy requests) counted by one captured batched launch.  Replay-only
+    graphs/s, and with the batch's partner build included (a served graph
+    arrives without its partner structure)."""
     k = min(k, len(req))
-    b = ctx.batch(req[:k])
+    hidx = np.nonzero(heavy)[0]
+    lidx = np.nonzero(~heavy)[0]
+    nh = min(len(hidx), int(round(k * len(hidx) / len(req))))
+    pick = np.concatenate([hidx[:nh], lidx[:k - nh]])
+    b = ctx.batch([req[i] for i in pick])
     try:
         build_ms = b.info()["build_ms"]
         b.count()
         ms_total, outs = time_steps(lambda: b.count(), reps, 1)
-        exact = all(np.array_equal(o[0], want[:k]) for o in outs)
+        exact = all(np.array_equal(o[0], want[pick]) for o in outs)
     finally:
         b.free()
     replay_s = ms_total / reps / 1e3
-    return {"requests": k, "replay_ms": replay_s * 1e3, "partner_build_ms": build_ms,
+    return {"requests": k, "heavy_requests": int(nh), "replay_ms": replay_s * 1e3, "partner_build_ms": build_ms,
             "replay_graphs_per_s": k / replay_s, "graphs_per_s": k / (replay_s + build_ms / 1e3),
             "exact": exact}
 
 
 def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clients=8, slots=3,
-                saturated_only=False, fill_graphs=0, fill_wait_us=0):
+                saturated_only=False, fill_graphs=0, fill_wait_us=0, max_ids=0, heavy_frac=0.01):
     """Poisson TC requests from the batched mix plus 1% RMAT-20 through the
     continuous-batching server: saturated throughput; then Poisson arrivals at
     50% and 90% of the batched capacity of the same mix (build included), and
@@ -432,16 +438,18 @@ def leg_serving(ctx, graphs, counts, heavy, heavy_count, n_requests, seed, clien
     from hetegeneous_parallel_programming_amd.solver import Server, clock_ms
 
     rng = np.random.default_rng(seed)
-    picks = np.where(rng.random(n_requests) < 0.01, -1, rng.integers(0, len(graphs), n_requests))
+    picks = np.where(rng.random(n_requests) < heavy_frac, -1, rng.integers(0, len(graphs), n_requests))
     want = np.array([heavy_count if k < 0 else counts[k] for k in picks], np.uint64)
     req = [heavy if k < 0 else graphs[k] for k in picks]
     out = {"requests": n_requests, "heavy_fraction": float((picks < 0).mean()), "slots": slots}
-    cap = serving_mix_capacity(ctx, req, want)
+    cap = serving_mix_capacity(ctx, req, want, picks < 0)
     out["batched_capacity"] = cap
     log(f"serving mix, batched: {cap['graphs_per_s']:.0f} graphs/s with the partner build "
-        f"({cap['replay_graphs_per_s']:.0f} replay only; {cap['requests']} requests), exact={cap['exact']}")
+        f"({cap['replay_graphs_per_s']:.0f} replay only; {cap['requests']} requests, {cap['heavy_requests']} "
+        f"RMAT-20), exact={cap['exact']}")
     with Server(ctx, hbm_budget=16 << 30, slots=slots, max_batch_graphs=2048, max_batch_rows=16 << 20,
-                max_queue=1 << 20, host_staging=4 << 30, fill_graphs=fill_graphs, fill_wait_us=fill_wait_us) as srv:
+                max_queue=1 << 20, host_staging=4 << 30, fill_graphs=fill_graphs, fill_wait_us=fill_wait_us,
+                max_batch_ids=max_ids) as srv:
         for i in range(64):  # warm-up
             srv.submit(req[i], i)
         srv.drain()
@@ -640,6 +648,8 @@ def main() -> int:
     ap.add_argument("--serving-saturated-only", action="store_true", help="serving leg: the saturated run only")
     ap.add_argument("--serving-fill", type=int, default=0, help="server fill_graphs (0: its default)")
     ap.add_argument("--serving-fill-wait-us", type=int, default=0, help="server fill_wait_us (0: its default)")
+    ap.add_argument("--serving-max-ids", type=int, default=0, help="server max_batch_ids (0: its default)")
+    ap.add_argument("--serving-heavy-frac", type=float, default=0.01, help="share of RMAT-20 requests")
     ap.add_argument("--no-autotune", action="store_true", help="time the batch with the default setting only")
     ap.add_argument("--tune-seconds", type=float, default=12.0, help="autotuner wall-time budget")
     ap.add_argument("--tune-file", default="", help="load the batch setting from / save it to this file")
@@ -776,7 +786,8 @@ def run_side_legs(ctx, args, line) -> None:
             line["extra"]["serving"] = leg_serving(ctx, graphs, counts, heavy, heavy_count,
                                                    args.requests, args.seed + 100, args.clients,
                                                    args.serving_slots, args.serving_saturated_only,
-                                                   args.serving_fill, args.serving_fill_wait_us)
+                                                   args.serving_fill, args.serving_fill_wait_us,
+                                                   args.serving_max_ids, args.serving_heavy_frac)
             line["extra"]["serving"]["rmat20_build_ms"] = heavy_build_ms
             gold20 = golden_big().get(f"rmat20_seed{args.seed}")
             line["extra"]["serving"]["rmat20_exact_vs_cpu_oracle"] = (

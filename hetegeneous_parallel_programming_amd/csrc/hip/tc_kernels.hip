@@ -2039,45 +2039,48 @@ __global__ __launch_bounds__(kBlock) void read_kernel(const u32x4* __restrict__ 
   if (acc == 0x9E3779B9u) sink[(int64_t)blockIdx.x * kBlock + threadIdx.x] = acc;
 }
 
-// Staging copy of a server launch: block b finds its copy (the last with
-// first <= b) and moves one kStageChunk of it, 16 bytes per lane and load
-// (the reads cross the host link, so many are kept in flight), then the
-// copy's tail of 4-byte words.
-__global__ __launch_bounds__(kBlock) void stage_copy_kernel(const StageCopy* copies, int32_t n) {
-  const int64_t b = blockIdx.x;
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (copies[mid].first <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  const StageCopy c = copies[lo];
-  const int64_t c0 = (b - c.first) * kStageChunk;
-  if (c0 >= c.bytes) return;
-  const int64_t c1 = min(c0 + kStageChunk, c.bytes);
-  const int64_t v1 = c0 + ((c1 - c0) & ~(int64_t)15);  // 16-byte part of this chunk
-  constexpr int kU = (int)(kStageChunk / (16 * kBlock));  // 16-byte words per lane
-  u32x4 x[kU];
+// Staging copy of a server launch: a small persistent grid strides over the
+// chunks (chunk b of all copies: the copy with the last first <= b, one
+// kStageChunk of it).  Each lane keeps 16-byte loads in flight across the
+// host link; few workgroups hold the CU slots they wait in, so the count
+// kernels of other launches keep their occupancy.
+__global__ __launch_bounds__(kBlock) void stage_copy_kernel(const StageCopy* copies, int32_t n, int64_t chunks) {
+  for (int64_t b = blockIdx.x; b < chunks; b += gridDim.x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (copies[mid].first <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    const StageCopy c = copies[lo];
+    const int64_t c0 = (b - c.first) * kStageChunk;
+    if (c0 >= c.bytes) continue;
+    const int64_t c1 = min(c0 + kStageChunk, c.bytes);
+    const int64_t v1 = c0 + ((c1 - c0) & ~(int64_t)15);  // 16-byte part of this chunk
+    constexpr int kU = (int)(kStageChunk / (16 * kBlock));  // 16-byte words per lane
+    u32x4 x[kU];
 #pragma unroll
-  for (int k = 0; k < kU; ++k) {
-    const int64_t i = c0 + ((int64_t)k * kBlock + threadIdx.x) * 16;
-    if (i < v1) x[k] = *(const u32x4*)(c.src + i);
-  }
+    for (int k = 0; k < kU; ++k) {
+      const int64_t i = c0 + ((int64_t)k * kBlock + threadIdx.x) * 16;
+      if (i < v1) x[k] = *(const u32x4*)(c.src + i);
+    }
 #pragma unroll
-  for (int k = 0; k < kU; ++k) {
-    const int64_t i = c0 + ((int64_t)k * kBlock + threadIdx.x) * 16;
-    if (i < v1) *(u32x4*)(c.dst + i) = x[k];
+    for (int k = 0; k < kU; ++k) {
+      const int64_t i = c0 + ((int64_t)k * kBlock + threadIdx.x) * 16;
+      if (i < v1) *(u32x4*)(c.dst + i) = x[k];
+    }
+    for (int64_t i = v1 + 4 * (int64_t)threadIdx.x; i < c1; i += 4 * kBlock)
+      *(uint32_t*)(c.dst + i) = *(const uint32_t*)(c.src + i);
   }
-  for (int64_t i = v1 + 4 * (int64_t)threadIdx.x; i < c1; i += 4 * kBlock)
-    *(uint32_t*)(c.dst + i) = *(const uint32_t*)(c.src + i);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_stage_copy(const StageCopy* copies, int32_t n, int64_t blocks, hipStream_t s) {
-  if (n <= 0 || blocks <= 0) return hipSuccess;
-  stage_copy_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(copies, n);
+hipError_t launch_stage_copy(const StageCopy* copies, int32_t n, int64_t chunks, hipStream_t s) {
+  if (n <= 0 || chunks <= 0) return hipSuccess;
+  const int64_t grid = chunks < kStageGrid ? chunks : kStageGrid;
+  stage_copy_kernel<<<(unsigned)grid, kBlock, 0, s>>>(copies, n, chunks);
   return hipGetLastError();
 }
 

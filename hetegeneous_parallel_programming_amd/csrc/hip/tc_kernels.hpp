@@ -273,19 +273,20 @@ hipError_t launch_zero(void* p, int64_t words32, hipStream_t s);
 
 // One request's CSR on its way from pinned host memory (src, mapped for the
 // device) to its extent of the device pool (dst): `bytes`, a multiple of 4,
-// both ends 16-byte aligned; blocks [first, first + ceil(bytes / kStageChunk))
-// of the staging copy kernel move it.
+// both ends 16-byte aligned; chunks [first, first + ceil(bytes / kStageChunk))
+// of the staging copy move it.
 struct StageCopy {
   const char* src;
   char* dst;
   int64_t bytes;
   int64_t first;
 };
-constexpr int64_t kStageChunk = (int64_t)64 << 10;  // bytes per block of the staging copy
+constexpr int64_t kStageChunk = (int64_t)64 << 10;  // bytes per chunk of the staging copy
 inline int64_t stage_blocks(int64_t bytes) { return (bytes + kStageChunk - 1) / kStageChunk; }
-// All copies of a launch in one kernel (`blocks` = the sum of their blocks;
-// copies on the device, ordered by first).
-hipError_t launch_stage_copy(const StageCopy* copies, int32_t n, int64_t blocks, hipStream_t s);
+// All copies of a launch in one kernel (`chunks` = the sum of their chunks;
+// copies on the device, ordered by first), on at most kStageGrid workgroups.
+constexpr int64_t kStageGrid = 128;
+hipError_t launch_stage_copy(const StageCopy* copies, int32_t n, int64_t chunks, hipStream_t s);
 // Re-arms a kept plan for another count: tile queue heads and the fallback
 // counter to 0 (bin counts kept), outs[0..nouts) to 0.
 hipError_t launch_rearm(uint32_t* counters, uint64_t* outs, int32_t nouts, hipStream_t s);

@@ -276,3 +276,36 @@ def test_tc_count_picks_the_single_launch_for_small_graphs(ctx):
         assert st1["single_launch"] == single and st2["single_launch"] == single
         if not single:
             assert not st1["plan_kept"] and st2["plan_kept"]
+
+
+def test_both_ends_past_the_saturated_degree(ctx):
+    """An edge whose two ends both have out-degree >= 65535: the build keeps
+    16-bit degrees, which saturate there, and decides the owner on the exact
+    ones.  Vertex 0 points at vertex 1 and 70,000 leaves, vertex 1 at 70,010
+    leaves, 50,000 of them shared: the head (vertex 1) owns the edge, which
+    the saturated degrees would give to the row, and every shared leaf closes
+    one triangle."""
+    a = np.arange(2, 70_002, dtype=np.int32)
+    b = np.concatenate([a[:50_000], np.arange(70_002, 90_012, dtype=np.int32)])
+    n = 90_012
+    rp = np.zeros(n + 1, np.int64)
+    rp[1] = 1 + a.size
+    rp[2:] = rp[1] + b.size
+    ci = np.concatenate([np.array([1], np.int32), a, b])
+    assert b.size > a.size + 1 > 65535
+    assert oracle.tc_count(rp, ci) == 50_000
+    dg = ctx.upload_arrays(rp, ci)
+    assert ctx.count_binned(dg)[0] == 50_000 and ctx.count(dg)[0] == 50_000
+    assert dg.info()["partner_rows"] == Graph.from_csr(rp, ci).work()["partner_rows"]
+    dg.free()
+
+
+def test_radix_build_with_the_widest_buckets(ctx):
+    """12M row slots need fine buckets of 2048 owners (the widest the build
+    takes) and its smallest LDS capacities; every count stays the oracle's."""
+    g = Graph.er(12_000_000, 3, 7).oriented()
+    want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
+    dg = ctx.upload(g)
+    assert ctx.count_binned(dg)[0] == want and ctx.count(dg)[0] == want
+    assert dg.info()["partner_rows"] == g.work()["partner_rows"]
+    dg.free()

@@ -69,19 +69,19 @@ static tc_status capture(tc_ctx* ctx, tc_batch* b) {
   hipError_t e = tck::launch_zero(a.bufs.counters, tck::kNumCounters, s);
   if (e == hipSuccess) e = tck::launch_zero(a.bufs.outs, 2 * (int64_t)b->n, s);
   if (e == hipSuccess) e = tck::launch_plan(a, s);
-  // Heavy bins on the capture stream; the wave bin, the 32-lane bin and the
-  // 16/8-lane bins on three forked branches: each group's workgroups fill
-  // the CUs the others' tails free.
-  constexpr int kFirst[tc_ctx::kSides] = {tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G16};
-  constexpr int kLast[tc_ctx::kSides] = {tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G8};
+  // The bin groups (tck::kBinOrder): group 0 on the capture stream, the
+  // others on forked branches, whose workgroups fill the CUs the other
+  // groups' tails free.
+  static_assert(tck::kBinGroups == tc_ctx::kSides + 1, "one side stream per bin group past the first");
+  const tck::BinGroups& G = tck::kBinOrder;
   if (e == hipSuccess) e = hipEventRecord(ctx->fork, s);
-  for (int i = 0; i < tc_ctx::kSides && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctx->side[i], ctx->fork, 0);
-  if (e == hipSuccess) e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, tck::BIN_GLOBAL, tck::BIN_BLOCK, s);
-  for (int i = 0; i < tc_ctx::kSides && e == hipSuccess; ++i)
-    e = tck::launch_count_bins(a, ctx->cus, ctx->tune.grid, kFirst[i], kLast[i], ctx->side[i]);
-  for (int i = 0; i < tc_ctx::kSides && e == hipSuccess; ++i) {
-    e = hipEventRecord(ctx->join[i], ctx->side[i]);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join[i], 0);
+  for (int i = 1; i < G.n && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctx->side[i - 1], ctx->fork, 0);
+  for (int i = 0; i < G.n && e == hipSuccess; ++i)
+    e = tck::launch_count_order(a, ctx->cus, ctx->tune.grid, G.bin[i], G.len[i], nullptr,
+                                i == 0 ? s : ctx->side[i - 1]);
+  for (int i = 1; i < G.n && e == hipSuccess; ++i) {
+    e = hipEventRecord(ctx->join[i - 1], ctx->side[i - 1]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->join[i - 1], 0);
   }
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->pinned, a.bufs.outs, sizeof(uint64_t) * (size_t)b->n, hipMemcpyDeviceToHost, s);

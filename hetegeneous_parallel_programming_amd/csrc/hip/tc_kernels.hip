@@ -89,6 +89,27 @@ __device__ __forceinline__ int tree_pos(int x, int h) {
 #define TC_IN_FLIGHT 8
 #endif
 constexpr int kInFlight = TC_IN_FLIGHT;  // lookups per lane per iteration
+// Edge ends a lane reads together when it steps past its edge, by group
+// width, and how many such reads a wave-wide group makes before it falls
+// back to a binary search (sub-wave groups: TC_STEP_STAGES_GROUP).
+#ifndef TC_STEP_LA64
+#define TC_STEP_LA64 2
+#endif
+#ifndef TC_STEP_LA32
+#define TC_STEP_LA32 2
+#endif
+#ifndef TC_STEP_LA16
+#define TC_STEP_LA16 2
+#endif
+#ifndef TC_STEP_LA8
+#define TC_STEP_LA8 1
+#endif
+#ifndef TC_STEP_STAGES
+#define TC_STEP_STAGES 2
+#endif
+#ifndef TC_STEP_STAGES_GROUP
+#define TC_STEP_STAGES_GROUP 1
+#endif
 #ifndef TC_IN_FLIGHT_GROUP
 #define TC_IN_FLIGHT_GROUP 6
 #endif
@@ -390,12 +411,35 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
     int cur_k = -1;   // the lane's current edge
     int cur_end = 0;  // t < cur_end  <=>  t lies in the lane's current edge
     gint32* cur_base = (gint32*)ci;
-    // A lane's flat index grows by G per id, so its next edge is almost
-    // always the following one: check that first, search only otherwise.
-    // (t < total = sEnd[G - 1] keeps cur_k + 1 inside the batch.)
+    // A lane's flat index grows by G per id, so its next edge is one of the
+    // next few: a step reads the next kLookAhead ends together (lists of a
+    // few dozen ids make a wave-wide group's lanes pass a couple of edges
+    // per step, a narrow group's one), then, for lanes still past them, up
+    // to kStages - 1 more such reads, and searches only past those.  Each
+    // read moves k over the run of ends at or below t.  t < total =
+    // sEnd[G - 1] keeps cur_k + 1 inside the batch and ends every run inside
+    // the group, so words read past the group's G entries (other groups'
+    // ends, or the Base array after End: LDS of this block) never count.
+    constexpr int kLookAhead = G >= 64 ? TC_STEP_LA64 : G >= 32 ? TC_STEP_LA32 : G >= 16 ? TC_STEP_LA16 : TC_STEP_LA8;
+    constexpr int kStages = G >= 64 ? TC_STEP_STAGES : TC_STEP_STAGES_GROUP;
     auto step_to = [&](int t) {
-      int k = cur_k + 1;
-      int e = sEnd[k];
+      int k = cur_k, e = cur_end;  // e <= t
+#pragma unroll
+      for (int st = 0; st < kStages; ++st) {
+        if (st > 0 && e > t) break;
+        int ev[kLookAhead];
+#pragma unroll
+        for (int i = 0; i < kLookAhead; ++i) ev[i] = sEnd[k + 1 + i];
+        k += 1;
+        e = ev[0];
+        bool run = true;
+#pragma unroll
+        for (int i = 0; i + 1 < kLookAhead; ++i) {
+          run = run && ev[i] <= t;
+          k += run ? 1 : 0;
+          e = run ? ev[i + 1] : e;
+        }
+      }
       if (e <= t) {
         k = find_edge<G>(sEnd, t);
         e = sEnd[k];
@@ -1618,14 +1662,12 @@ struct PRow {
   int64_t pb, pe;
 };
 
+// The row's offsets are loaded beside its record, not after it (vrp has
+// P + 1 entries, so k + 1 is in bounds for every k < P): the two loads are
+// in flight together.  pb and pe mean nothing for an unused slot (set 0).
 __device__ __forceinline__ PRow partner_row(const LaunchArgs& a, int64_t k) {
   const VRow vr = a.vrow[k];
-  PRow pr{vr.g, vr.set, vr.work, 0, 0};
-  if (vr.set != 0) {
-    pr.pb = a.vrp[k];
-    pr.pe = a.vrp[k + 1];
-  }
-  return pr;
+  return PRow{vr.g, vr.set, vr.work, a.vrp[k], a.vrp[k + 1]};
 }
 
 // Work class of a binned partner row: its estimated cost in quarter octaves
@@ -2170,6 +2212,23 @@ hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int6
   launch_bin<BIN_G32>(a, cus, grid, s, tiles[BIN_G32]);
   launch_bin<BIN_G16>(a, cus, grid, s, tiles[BIN_G16]);
   launch_bin<BIN_G8>(a, cus, grid, s, tiles[BIN_G8]);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_order(const LaunchArgs& a, int cus, int grid, const int* order, int n,
+                              const int64_t* tiles, hipStream_t s) {
+  for (int i = 0; i < n; ++i) {
+    const int b = order[i];
+    const int64_t t = tiles ? tiles[b] : -1;
+    switch (b) {
+      case BIN_GLOBAL: launch_bin<BIN_GLOBAL>(a, cus, grid, s, t); break;
+      case BIN_BLOCK: launch_bin<BIN_BLOCK>(a, cus, grid, s, t); break;
+      case BIN_WAVE: launch_bin<BIN_WAVE>(a, cus, grid, s, t); break;
+      case BIN_G32: launch_bin<BIN_G32>(a, cus, grid, s, t); break;
+      case BIN_G16: launch_bin<BIN_G16>(a, cus, grid, s, t); break;
+      default: launch_bin<BIN_G8>(a, cus, grid, s, t); break;
+    }
+  }
   return hipGetLastError();
 }
 

@@ -298,6 +298,51 @@ hipError_t launch_count_bins(const LaunchArgs& a, int cus, int grid, int first, 
 // Same, with each bin's tile count known on the host (tiles[kNumBins]):
 // empty bins are skipped and no grid exceeds its bin's tiles.
 hipError_t launch_count_sized(const LaunchArgs& a, int cus, int grid, const int64_t* tiles, hipStream_t s);
+// Bins order[0..n) in that order, each at full residency or, with tiles
+// (tiles[kNumBins], known on the host), sized as launch_count_sized does.
+hipError_t launch_count_order(const LaunchArgs& a, int cus, int grid, const int* order, int n,
+                              const int64_t* tiles, hipStream_t s);
+
+// Stream groups of the bins: the groups run side by side (group 0 on the
+// library stream, the others on side streams), each group's bins one after
+// another in the order written.  Persistent workgroups hold their CUs until
+// their bin drains, so a bin alone on the device at the end of a count runs
+// at its own issue rate: the groups start the sub-wave bins, whose waves
+// stall the most, beside the others and leave wave-wide work for last.
+#ifndef TC_BIN_ORDER
+#define TC_BIN_ORDER "01|2|3|45"
+#endif
+constexpr int kBinGroups = 4;
+struct BinGroups {
+  int n = 0;                        // groups in use
+  int len[kBinGroups] = {};         // bins per group
+  int bin[kBinGroups][kNumBins] = {};
+  bool ok = false;                  // every bin exactly once, at most kBinGroups groups
+};
+constexpr BinGroups parse_bin_groups(const char* t) {
+  BinGroups g{};
+  int seen = 0, cur = 0;
+  bool ok = true;
+  for (int i = 0; t[i] != 0; ++i) {
+    if (t[i] == '|') {
+      ok = ok && g.len[cur] > 0;
+      ++cur;
+      ok = ok && cur < kBinGroups;
+      if (!ok) break;
+      continue;
+    }
+    const int b = t[i] - '0';
+    ok = ok && b >= 0 && b < kNumBins && !(seen >> b & 1);
+    if (!ok) break;
+    seen |= 1 << b;
+    g.bin[cur][g.len[cur]++] = b;
+  }
+  g.n = cur + 1;
+  g.ok = ok && g.len[cur] > 0 && seen == (1 << kNumBins) - 1;
+  return g;
+}
+constexpr BinGroups kBinOrder = parse_bin_groups(TC_BIN_ORDER);
+static_assert(kBinOrder.ok, "TC_BIN_ORDER: bins 0-5 once each, in at most 4 '|'-separated groups");
 // own: null, or per row 1 if it is a pivot row (rows outside
 // [row_begin, row_end) are never pivots).
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,

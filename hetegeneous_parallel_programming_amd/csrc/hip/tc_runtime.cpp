@@ -561,16 +561,6 @@ tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out) try {
   return tcr::caught();
 }
 
-// Stream group of a bin in tc_count: 0 heavy (global-list, workgroup),
-// 1 wave, 2 32-lane, 3 16- and 8-lane (as many groups as the device's
-// default hardware queues per process, 4).
-static int bin_group(int bin) {
-  if (bin <= tck::BIN_BLOCK) return 0;
-  if (bin == tck::BIN_WAVE) return 1;
-  if (bin == tck::BIN_G32) return 2;
-  return 3;
-}
-
 // Fills the LaunchArgs of a count of g (descs, pools, partners, tune, plan
 // scratch).
 static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
@@ -591,6 +581,13 @@ static tck::LaunchArgs count_args(tc_ctx* ctx, tc_graph* g) {
 }  // extern "C"
 
 namespace tcr {
+
+// Index of the first group with work (it runs on the library stream).
+static int first_group(const bool* any, int n) {
+  for (int gi = 0; gi < n; ++gi)
+    if (any[gi]) return gi;
+  return 0;
+}
 
 // The binned count over plan buffers `bufs`.  kept: the plan in bufs is
 // already built for this graph and tune (its rows per bin in bins[], tiles
@@ -626,36 +623,30 @@ static tc_status binned_count(tc_ctx* ctx, tc_graph* g, const tck::PlanBufs& buf
     }
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  // The bins run as groups on separate streams (default priority): the
-  // global-list and workgroup bins, the wave bin, the 32-lane bin, and the
-  // 16- and 8-lane bins.  Each group's persistent workgroups hold their CUs
-  // until its queues drain; the dispatcher interleaves the groups'
-  // workgroups from the start and the others take over the CUs a group's
-  // tail frees.  One non-empty group runs on the library stream alone.
-  constexpr int kGroups = tc_ctx::kSides + 1;
-  int64_t grp[kGroups][tck::kNumBins] = {};
-  for (int b = 0; b < tck::kNumBins; ++b) grp[bin_group(b)][b] = tiles[b];
-  bool any[kGroups];
-  int used = 0;
-  for (int gi = 0; gi < kGroups; ++gi) {
-    any[gi] = false;
-    for (int b = 0; b < tck::kNumBins; ++b) any[gi] = any[gi] || grp[gi][b] > 0;
-    used += any[gi] ? 1 : 0;
-  }
-  if (used <= 1) {
-    TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, tiles, s));
-  } else {
-    for (int gi = 0; gi < kGroups; ++gi) {
-      if (!any[gi]) continue;
-      hipStream_t gs = gi == 0 ? s : ctx->side[gi - 1];
-      if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs, ctx->ev[1], 0));  // timing starts before any group
-      TC_HIP_TRY(tck::launch_count_sized(a, ctx->cus, ctx->tune.grid, grp[gi], gs));
+  // The bin groups (tck::kBinOrder) run on separate streams (default
+  // priority): each group's persistent workgroups hold their CUs until its
+  // bins drain; the dispatcher interleaves the groups' workgroups from the
+  // start and the others take over the CUs a group's tail frees.  Groups
+  // with no tiles are skipped; one non-empty group runs on the library
+  // stream alone.
+  const tck::BinGroups& G = tck::kBinOrder;
+  bool any[tck::kBinGroups] = {};
+  for (int gi = 0; gi < G.n; ++gi)
+    for (int j = 0; j < G.len[gi]; ++j) any[gi] = any[gi] || tiles[G.bin[gi][j]] > 0;
+  const int g0 = first_group(any, G.n);
+  int side = 0;
+  for (int gi = 0; gi < G.n; ++gi) {
+    if (!any[gi]) continue;
+    hipStream_t gs = s;
+    if (gi != g0) {
+      gs = ctx->side[side++];
+      TC_HIP_TRY(hipStreamWaitEvent(gs, ctx->ev[1], 0));  // timing starts before any group
     }
-    for (int gi = 1; gi < kGroups; ++gi)
-      if (any[gi]) {
-        TC_HIP_TRY(hipEventRecord(ctx->join[gi - 1], ctx->side[gi - 1]));
-        TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[gi - 1], 0));
-      }
+    TC_HIP_TRY(tck::launch_count_order(a, ctx->cus, ctx->tune.grid, G.bin[gi], G.len[gi], tiles, gs));
+  }
+  for (int i = 0; i < side; ++i) {
+    TC_HIP_TRY(hipEventRecord(ctx->join[i], ctx->side[i]));
+    TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[i], 0));
   }
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
@@ -859,15 +850,15 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
   {  // bin groups on the library stream and the side streams, as in tc_count
-    constexpr int kFirst[tc_ctx::kSides + 1] = {tck::BIN_GLOBAL, tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G16};
-    constexpr int kLast[tc_ctx::kSides + 1] = {tck::BIN_BLOCK, tck::BIN_WAVE, tck::BIN_G32, tck::BIN_G8};
-    TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, kFirst[0], kLast[0], s));
-    for (int i = 0; i < tc_ctx::kSides; ++i) {
-      TC_HIP_TRY(hipStreamWaitEvent(ctx->side[i], ctx->ev[1], 0));
-      TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, kFirst[i + 1], kLast[i + 1], ctx->side[i]));
-      TC_HIP_TRY(hipEventRecord(ctx->join[i], ctx->side[i]));
+    const tck::BinGroups& G = tck::kBinOrder;
+    for (int gi = 0; gi < G.n; ++gi) {
+      hipStream_t gs = gi == 0 ? s : ctx->side[gi - 1];
+      if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs, ctx->ev[1], 0));
+      for (int j = 0; j < G.len[gi]; ++j)
+        TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, G.bin[gi][j], G.bin[gi][j], gs));
+      if (gi > 0) TC_HIP_TRY(hipEventRecord(ctx->join[gi - 1], gs));
     }
-    for (int i = 0; i < tc_ctx::kSides; ++i) TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[i], 0));
+    for (int gi = 1; gi < G.n; ++gi) TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[gi - 1], 0));
   }
   if (k > 0)
     TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, g->row_begin, g->row_end, g->own, L.ecount, L.bounds, k, L.pe, s));

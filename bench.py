@@ -42,8 +42,34 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
+# Time and text of the last progress line (read by the side-leg watchdog).
+_progress = [time.monotonic(), "start"]
+_side_legs_done = threading.Event()
+
+
 def log(msg: str) -> None:
+    _progress[0], _progress[1] = time.monotonic(), msg
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def start_watchdog(line: dict, limit_s: float) -> None:
+    """Once the headline measurement is in `line`: if the side legs print
+    nothing for limit_s seconds, print the line (the headline figures, with
+    extra.side_legs_incomplete naming the last step reached) and end the
+    process, so a stalled side leg does not take the headline record with it."""
+    def watch():
+        while True:
+            time.sleep(5.0)
+            silent = time.monotonic() - _progress[0]
+            if silent > limit_s and not _side_legs_done.is_set():
+                line.setdefault("extra", {})["side_legs_incomplete"] = {
+                    "after": _progress[1], "silent_s": round(silent, 1)}
+                print(f"[bench] side legs silent for {silent:.0f} s after: {_progress[1]}; "
+                      f"printing the headline line", file=sys.stderr, flush=True)
+                print(json.dumps(line), flush=True)
+                os._exit(0)
+
+    threading.Thread(target=watch, daemon=True).start()
 
 
 def device_sync() -> None:
@@ -218,9 +244,11 @@ def leg_er(ctx, seed, steps, warmup):
 def local_counts(ctx, g, dg, total, steps=3, warmup=1):
     """tc_count_local on the resident graph: per-edge, per-vertex and per-PE
     attribution, timed beside the binned count, with the reference push/pull
-    per-PE sums of a 4-way vertex split and of a partner-balanced split.
+    per-PE sums of a 4-way vertex split and of a partner-balanced split; then
+    the same without vertex counts (edge and part counts only).
     Size-independent checks: edge counts sum to the total, vertex counts to
-    three times it, part counts to it."""
+    three times it, part counts to it; the edges-only outputs equal the full
+    call's."""
     from hetegeneous_parallel_programming_amd import PartitionedGraph
 
     out = {}
@@ -238,6 +266,17 @@ def local_counts(ctx, g, dg, total, steps=3, warmup=1):
                         "kernel_ms_avg": k_avg, "ms_per_step": ms_total / steps, "exact_sums": exact}
         log(f"local ({balance} split): kernel={k_avg:.3f} ms, per-PE push/pull sums "
             f"{out[balance]['part_counts']}, sums exact={exact}")
+        # the reference's own attribution only (per-edge counts and per-PE
+        # sums, no per-vertex counts): the kernels keep no slot hit counters
+        ms_e, outs_e = time_steps(lambda: ctx.count_local(dg, vertices=False, bounds=bounds), steps, warmup)
+        oe = outs_e[-1][1]
+        exact_e = (all(x[0] == total for x in outs_e)
+                   and np.array_equal(oe["edge_counts"], o["edge_counts"])
+                   and np.array_equal(oe["part_counts"], o["part_counts"]))
+        ke_avg = float(np.mean([x[1]["stats"]["count_ms"] for x in outs_e]))
+        out[balance]["edges_only"] = {"kernel_ms_avg": ke_avg, "ms_per_step": ms_e / steps, "exact": exact_e}
+        log(f"local ({balance} split), edge and part counts only: kernel={ke_avg:.3f} ms, "
+            f"same as with vertex counts={exact_e}")
     out["max_vertex_triangles"] = int(o["vertex_counts"].max())
     return out
 
@@ -327,10 +366,13 @@ def leg_batch(ctx, graphs, seed, reps, tuner=None):
     b = ctx.batch(graphs)
     create_ms = (time.perf_counter() - t0) * 1e3
     build_ms = b.info()["build_ms"]
+    log(f"batch: created in {create_ms:.0f} ms (partner build {build_ms:.1f} ms)")
     each, ms_each, st_each = b.count_each()
     each2, ms_each2, st_each2 = b.count_each()
     base_s = st_each2["total_ms"] / 1e3
+    log(f"batch: plain launch per graph {base_s * 1e3:.0f} ms for all")
     tcc, ms_tcc = per_graph_tc_count(ctx, graphs)
+    log(f"batch: tc_count per graph {ms_tcc.sum():.0f} ms for all")
     if not np.array_equal(tcc, each):
         raise RuntimeError("tc_count per graph differs from the plain launch per graph")
     best_s = float(np.minimum(ms_each2, ms_tcc).sum()) / 1e3
@@ -344,6 +386,7 @@ def leg_batch(ctx, graphs, seed, reps, tuner=None):
         return ms_total / reps / 1e3, [o[1]["count_ms"] for o in outs]
 
     default_s, kms_default = timed()
+    log(f"batch: captured launch, default setting {default_s * 1e3:.2f} ms")
     res = {"graphs": n_graphs, "edges": edges, "sum_count": int(each.sum()),
            "baseline_graphs_per_s": n_graphs / base_s,
            "baseline_p50_ms": float(np.percentile(ms_each2, 50)),
@@ -651,6 +694,9 @@ def main() -> int:
     ap.add_argument("--serving-max-ids", type=int, default=0, help="server max_batch_ids (0: its default)")
     ap.add_argument("--serving-heavy-frac", type=float, default=0.01, help="share of RMAT-20 requests")
     ap.add_argument("--no-autotune", action="store_true", help="time the batch with the default setting only")
+    ap.add_argument("--side-leg-limit", type=float, default=150.0,
+                    help="seconds without progress in the side legs before the headline line is printed "
+                         "alone (0: no limit)")
     ap.add_argument("--tune-seconds", type=float, default=12.0, help="autotuner wall-time budget")
     ap.add_argument("--tune-file", default="", help="load the batch setting from / save it to this file")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -686,7 +732,10 @@ def main() -> int:
             "value": None, "extra": {}}
     if not args.no_rmat:
         line = leg_headline(ctx, args)
+    if args.side_leg_limit > 0:
+        start_watchdog(line, args.side_leg_limit)
     run_side_legs(ctx, args, line)
+    _side_legs_done.set()
     ctx.close()
     print(json.dumps(line), flush=True)
     return 0

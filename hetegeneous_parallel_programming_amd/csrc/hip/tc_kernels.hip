@@ -244,15 +244,21 @@ __device__ __forceinline__ void group_sync() {
   else wave_sync();
 }
 
+// Attribution of a count (the kLocal parameter of the count kernels):
+// none; every triangle to its DAG edge, its three corners (slot hit counters
+// for the third) and, through the edges, the parts of a split (full); or to
+// its DAG edge only, which is all the per-part sums need (edges).
+constexpr int kLocalNone = 0, kLocalFull = 1, kLocalEdges = 2;
+
 // Builds the set of N+(u) (src[0..da), sorted) in `slot` (area words) with
 // the G lanes of a group (gl = lane index in the group): a cuckoo hash when
 // it fits and converges (and the tune allows it), else the tree (da must
 // then fit the tree cap).
-// kLocal: C (area / 2 words: a 16-bit hit counter per slot word, two per
-// word) gets zeroed over the slots the set uses.
+// kLocalFull: C (area / 2 words: a 16-bit hit counter per slot word, two
+// per word) gets zeroed over the slots the set uses.
 // kWide: ids reach 2^24 (kSetHashWide).  fallbacks: counts the groups whose
 // hash build did not converge (one increment per fallen-back set).
-template <int G, bool kLocal, bool kWide>
+template <int G, int kLocal, bool kWide>
 __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, const int32_t* __restrict__ src,
                                                     int da, int gl, const Tune& tune, uint32_t* fallbacks,
                                                     uint32_t* C = nullptr) {
@@ -264,7 +270,7 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
     const int B = 1 << lb;
     for (int x = gl; x < 4 * B; x += G) {
       slot[x] = kEmpty;
-      if constexpr (kLocal)
+      if constexpr (kLocal == kLocalFull)
         if ((x & 1) == 0) C[x >> 1] = 0;
     }
     group_sync<G>();
@@ -280,7 +286,7 @@ __device__ __forceinline__ PivotSet build_pivot_set(int32_t* slot, int area, con
   }
   stage_tree(slot, src, da, gl, G);
   S.p0 = tree_levels(da);
-  if constexpr (kLocal)
+  if constexpr (kLocal == kLocalFull)
     for (int x = gl; x < (1 << S.p0) / 2; x += G) C[x] = 0;
   group_sync<G>();
   return S;
@@ -306,6 +312,34 @@ __device__ __forceinline__ void count_slot_hit(uint32_t* C, int idx) {
   atomicAdd(&C[idx >> 1], 1u << ((idx & 1) << 4));
 }
 
+// Adds one lookup step's hits (f) to the counters EC of their batch edges
+// (kq).  The lanes of a wide group stream one or two edges per step, so
+// per-hit atomics would pile onto one LDS word; instead the wave adds once
+// per distinct hit counter (ballot, one lane adds the popcount; counters
+// told apart by LDS address, so the groups of a wave are kept apart).
+// Call from wave-uniform control flow.  Groups narrower than
+// TC_EDGE_HITS_MIN_G spread over as many edges as groups and add per hit,
+// and so do full local counts (their per-hit branch already bumps a slot
+// counter; a second, per-edge loop beside it costs more than it saves).
+#ifndef TC_EDGE_HITS_MIN_G
+#define TC_EDGE_HITS_MIN_G 64
+#endif
+template <int G, int kLocal>
+__device__ __forceinline__ void edge_hits(uint32_t* EC, int kq, bool f) {
+  if constexpr (kLocal == kLocalEdges && G >= TC_EDGE_HITS_MIN_G) {
+    const uint32_t key = (uint32_t)(uintptr_t)(EC + kq);
+    uint64_t rem = __ballot(f);
+    while (rem) {
+      const int lead = __builtin_ctzll(rem);
+      const uint64_t same = rem & __ballot(key == (uint32_t)__builtin_amdgcn_readlane((int)key, lead));
+      if ((int)(threadIdx.x & 63) == lead) atomicAdd(&EC[kq], (uint32_t)__builtin_popcountll(same));
+      rem &= ~same;
+    }
+  } else {
+    if (f) atomicAdd(&EC[kq], 1u);
+  }
+}
+
 // Adds the set's per-slot hit counters to vcount of the ids in those slots
 // (empty hash slots hold -1, tree padding INT_MAX, tree word 0 -1).
 template <int G>
@@ -329,8 +363,8 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t x) {
 
 // NF lookups of ids w[] in a hash set: two 8-byte LDS reads (one bucket of
 // each table) and four compares per id.  kLocal: a hit also bumps the hit
-// counter of its slot and of its batch edge kq[].
-template <int NF, bool kLocal, bool kWide>
+// counter of its batch edge kq[] and (kLocalFull) of its slot.
+template <int G, int NF, int kLocal, bool kWide>
 __device__ __forceinline__ void hash_probe(const PivotSet& S, const int (&w)[NF], const int (&kq)[NF],
                                            uint32_t& hits, const LocalSink& ls) {
   const char* Tb = (const char*)S.A;
@@ -343,16 +377,16 @@ __device__ __forceinline__ void hash_probe(const PivotSet& S, const int (&w)[NF]
     const int2 y = *(const int2*)(Tb + S.p1 + (b1 << 3));
     const bool f = (x.x == w[q]) | (x.y == w[q]) | (y.x == w[q]) | (y.y == w[q]);
     hits += f ? 1u : 0u;
-    if constexpr (kLocal) {
+    if constexpr (kLocal == kLocalFull) {
       if (f) {
         const int t1 = S.p1 >> 2;  // word offset of table 1
         const int idx = x.x == w[q] ? 2 * (int)b0
                       : x.y == w[q] ? 2 * (int)b0 + 1
                       : y.x == w[q] ? t1 + 2 * (int)b1 : t1 + 2 * (int)b1 + 1;
         count_slot_hit(ls.C, idx);
-        atomicAdd(&ls.EC[kq[q]], 1u);
       }
     }
+    if constexpr (kLocal) edge_hits<G, kLocal>(ls.EC, kq[q], f);
   }
 }
 
@@ -387,7 +421,7 @@ struct RefEdges {
 
 // kWide: hash sets hold keys folded for ids past 2^24 (kSetHashWide); one
 // instantiation per hash kind keeps a single lookup path in the loop.
-template <int G, int kModes, int NF = kInFlight, bool kLocal = false, bool kWide = false, class Src>
+template <int G, int kModes, int NF = kInFlight, int kLocal = kLocalNone, bool kWide = false, class Src>
 __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, const Src src, int da,
                                                 const PivotSet S, int jfirst, int jstride,
                                                 int32_t* sEnd, gint32** sBase, int gl,
@@ -473,7 +507,7 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
         }
       }
       if ((kModes & kModeHash) && S.mode == (kWide ? kSetHashWide : kSetHash)) {
-        hash_probe<NF, kLocal, kWide>(S, w, kq, hits, ls);
+        hash_probe<G, NF, kLocal, kWide>(S, w, kq, hits, ls);
       } else if ((kModes & kModeTree) && ((kModes == kModeTree) || S.mode == kSetTree)) {
         // h descent steps; then the lower bound is k with its trailing one
         // bits and the following zero shifted out (0 = past the end -> -1).
@@ -497,12 +531,9 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
           const unsigned lo = (k >> __builtin_ctz(~k)) >> 1;
           const bool f = A[lo] == w[q];
           hits += f ? 1u : 0u;
-          if constexpr (kLocal) {
-            if (f) {
-              count_slot_hit(ls.C, (int)lo);
-              atomicAdd(&ls.EC[kq[q]], 1u);
-            }
-          }
+          if constexpr (kLocal == kLocalFull)
+            if (f) count_slot_hit(ls.C, (int)lo);
+          if constexpr (kLocal) edge_hits<G, kLocal>(ls.EC, kq[q], f);
         }
       } else if (kModes & kModeHbm) {
         int lo[NF];
@@ -519,22 +550,22 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
           const bool f = A[lo[q]] == w[q];
           hits += f ? 1u : 0u;
           if constexpr (kLocal) {
-            if (f) {
-              atomicAdd(&ls.vcount[w[q]], 1ull);
-              atomicAdd(&ls.EC[kq[q]], 1u);
-            }
+            if constexpr (kLocal == kLocalFull)
+              if (f) atomicAdd(&ls.vcount[w[q]], 1ull);
+            if constexpr (kLocal) edge_hits<G, kLocal>(ls.EC, kq[q], f);
           }
         }
       }
     }
     if constexpr (kLocal) {
       // every edge of the batch is counted in this batch only: store it,
-      // and credit its partner endpoint
+      // and (kLocalFull) credit its partner endpoint
       wave_sync();
       if (j < da) {
         const uint32_t c = ls.EC[gl];
         ls.ecount[ls.pedge[j]] = c;
-        if (c) atomicAdd(&ls.vcount[ls.pvid[j]], (unsigned long long)c);
+        if constexpr (kLocal == kLocalFull)
+          if (c) atomicAdd(&ls.vcount[ls.pvid[j]], (unsigned long long)c);
       }
       ls.EC[gl] = 0;
     }
@@ -1777,20 +1808,26 @@ struct Acc {
 
 // LDS of a local count beside CountLds: per-slot hit counters parallel to
 // the set areas, per-batch-edge hit counters parallel to End.
-template <bool kLocal>
+template <int kLocal>
 struct LocalLds {
   __device__ uint32_t* c() { return nullptr; }
   __device__ uint32_t* ec() { return nullptr; }
 };
 template <>
-struct __align__(16) LocalLds<true> {
+struct __align__(16) LocalLds<kLocalFull> {
   uint32_t C[kAreaBlock / 2];
   uint32_t EC[kBlock];
   __device__ uint32_t* c() { return C; }
   __device__ uint32_t* ec() { return EC; }
 };
+template <>
+struct __align__(16) LocalLds<kLocalEdges> {
+  uint32_t EC[kBlock];
+  __device__ uint32_t* c() { return nullptr; }
+  __device__ uint32_t* ec() { return EC; }
+};
 
-template <int G, int AREA, bool kLocal, bool kWide>
+template <int G, int AREA, int kLocal, bool kWide>
 __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* items, int64_t first,
                                            int64_t last, CountLds& L, LocalLds<kLocal>& LL, bool batch,
                                            Acc& acc) {
@@ -1801,7 +1838,7 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   int32_t* slot = L.A + grp * AREA;
   int32_t* sEnd = L.End + grp * G;
   gint32** sBase = L.Base + grp * G;
-  uint32_t* C = kLocal ? LL.c() + grp * (AREA / 2) : nullptr;
+  uint32_t* C = kLocal == kLocalFull ? LL.c() + grp * (AREA / 2) : nullptr;
   for (int64_t i = first + grp; i < last; i += NGRP) {
     const int64_t item = (int64_t)items[i];
     const PRow pr = partner_row(a, item);
@@ -1811,7 +1848,7 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
                        kLocal ? a.local.pvid + pr.pb : nullptr, a.local.ecount, a.local.vcount};
     const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kLocal, kWide>(
         a.ci, RefEdges{a.pci + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl, ls);
-    if constexpr (kLocal) {
+    if constexpr (kLocal == kLocalFull) {
       group_sync<G>();
       flush_set_hits<G>(slot, C, S, gl, a.local.vcount);
       const uint32_t hr = group_sum<G>(h);  // the row's triangles: each has the owner as a corner
@@ -1823,7 +1860,7 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   }
 }
 
-template <bool kLdsPivot, bool kLocal, bool kWide>
+template <bool kLdsPivot, int kLocal, bool kWide>
 __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, CountLds& L,
                                             LocalLds<kLocal>& LL, bool batch, Acc& acc) {
   const PRow pr = partner_row(a, (int64_t)item);
@@ -1835,7 +1872,7 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t* EC = kLocal ? LL.ec() + wid * 64 : nullptr;
   const int64_t* pedge = kLocal ? a.local.pedge + pr.pb : nullptr;
-  const int32_t* pvid = kLocal ? a.local.pvid + pr.pb : nullptr;
+  const int32_t* pvid = kLocal == kLocalFull ? a.local.pvid + pr.pb : nullptr;
   uint32_t h;
   if (kLdsPivot) {
     const PivotSet S = build_pivot_set<kBlock, kLocal, kWide>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune,
@@ -1844,7 +1881,7 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
     h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kLocal, kWide>(ci, part, np, S, wid * 64, kBlock,
                                                                           L.End + wid * 64, L.Base + wid * 64, lane,
                                                                           ls);
-    if constexpr (kLocal) {
+    if constexpr (kLocal == kLocalFull) {
       __syncthreads();
       flush_set_hits<kBlock>(L.A, LL.c(), S, threadIdx.x, a.local.vcount);
     }
@@ -1854,7 +1891,7 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
     h = pivot_edges<64, kModeHbm, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
                                                      L.Base + wid * 64, lane, ls);
   }
-  if constexpr (kLocal) {  // the owner's share, one atomic per wave
+  if constexpr (kLocal == kLocalFull) {  // the owner's share, one atomic per wave
     const uint32_t hw = group_sum<64>(h);
     if (lane == 0 && hw) atomicAdd(&a.local.vcount[a.local.pown[item]], (unsigned long long)hw);
   }
@@ -1868,16 +1905,24 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
 #define TC_COUNT_WAVES_PER_EU 8
 #endif
 
-// Local variants hold 1.5x the LDS of a plain count: 5 workgroups per CU.
+// Full local variants hold 1.5x the LDS of a plain count (5 workgroups per
+// CU); edges-only ones the plain count's plus 1 KB of edge counters (7).
+#ifndef TC_LOCAL_EDGES_WAVES_PER_EU
+#define TC_LOCAL_EDGES_WAVES_PER_EU 7
+#endif
 constexpr int kLocalWavesPerEU = 5;
+constexpr int waves_per_eu(int kLocal) {
+  return kLocal == kLocalFull ? kLocalWavesPerEU
+       : kLocal == kLocalEdges ? TC_LOCAL_EDGES_WAVES_PER_EU : TC_COUNT_WAVES_PER_EU;
+}
 
 // One persistent kernel per bin, specialised so that each keeps only its own
 // working set in registers.  Blocks pull tiles from the bin's queue head.
-// kLocal: also the per-edge and per-vertex attribution of a.local (one
-// graph only).
+// kLocal: also the per-edge (and, kLocalFull, per-vertex) attribution of
+// a.local (one graph only).
 // kWide: the instantiation for graphs whose ids reach 2^24 (folded hash keys).
-template <int BIN, bool kLocal, bool kWide>
-__global__ __launch_bounds__(kBlock, kLocal ? kLocalWavesPerEU : TC_COUNT_WAVES_PER_EU) void count_kernel(
+template <int BIN, int kLocal, bool kWide>
+__global__ __launch_bounds__(kBlock, waves_per_eu(kLocal)) void count_kernel(
     LaunchArgs a) {
   __shared__ CountLds L;
   __shared__ LocalLds<kLocal> LL;
@@ -1944,59 +1989,87 @@ __device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
   return lo;
 }
 
-// Every counted DAG edge (u, v) with count c goes to the part that computes
-// it in the reference's push/pull kernel
-// (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136): u's part when v
-// is in the same part or d+(u) >= d+(v) (pull), else v's part (push).  A
-// wave walks the edges of 64 consecutive rows as one flattened range (its
-// lanes stay busy whatever the row lengths; a lane finds its row by a binary
-// search over the rows' degree prefix in LDS) and sums runs of edges with
-// the same owner part before one LDS atomic.
-__global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __restrict__ rp,
-                                                             const int32_t* __restrict__ ci, int32_t row_begin,
-                                                             int32_t row_end, const uint8_t* own,
-                                                             const uint32_t* __restrict__ ecount,
-                                                             const int32_t* bounds, int32_t parts,
-                                                             unsigned long long* pe) {
-  __shared__ unsigned long long s_pe[kMaxParts];
-  __shared__ int32_t s_b[kMaxParts + 1];
+// Per-graph tables of the per-part pass: each DAG edge's row (esrc) and each
+// row's out-degree saturated at kDegSat (deg16).  A wave walks the edges of
+// 64 consecutive rows as one flattened range (its lanes stay busy whatever
+// the row lengths; a lane finds its row by a binary search over the rows'
+// degree prefix in LDS).
+__global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __restrict__ rp, int64_t n,
+                                                            int32_t* __restrict__ esrc,
+                                                            uint16_t* __restrict__ deg16) {
   __shared__ int64_t s_e0[kBlock];
-  __shared__ int32_t s_end[kBlock], s_du[kBlock], s_pu[kBlock];
-  for (int i = threadIdx.x; i < parts; i += kBlock) s_pe[i] = 0;
-  for (int i = threadIdx.x; i <= parts; i += kBlock) s_b[i] = bounds[i];
+  __shared__ int32_t s_end[kBlock], s_du[kBlock];
   const int lane = threadIdx.x & 63, w0 = threadIdx.x & ~63;
-  const int64_t u = row_begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   int32_t deg = 0;
-  if (u < row_end && (!own || own[u])) {
+  if (u < n) {
     const int64_t e0 = rp[u];
     deg = (int32_t)(rp[u + 1] - e0);  // a row holds fewer than 2^31 / 64 ids
     s_e0[threadIdx.x] = e0;
+    deg16[u] = (uint16_t)min(deg, kDegSat);
   }
   int total;
   const int before = group_excl_scan<64>(deg, lane, total);
   s_end[threadIdx.x] = before + deg;
   s_du[threadIdx.x] = deg;
-  __syncthreads();  // split points and this wave's rows
-  s_pu[threadIdx.x] = deg ? part_of(s_b, parts, u) : 0;
   wave_sync();
-  int run_owner = 0;
-  unsigned long long run = 0;
   for (int t = lane; t < total; t += 64) {
     const int r = w0 + find_edge<64>(s_end + w0, t);
-    const int32_t du = s_du[r];
-    const int64_t e = s_e0[r] + (t - (s_end[r] - du));
-    const uint32_t c = ecount[e];
-    if (c == 0) continue;
-    const int v = ci[e];
-    const int pu = s_pu[r];
-    const int pv = part_of(s_b, parts, v);
-    const int owner = (pv != pu && du < rp[v + 1] - rp[v]) ? pv : pu;
-    if (owner != run_owner) {
-      if (run) atomicAdd(&s_pe[run_owner], run);
-      run_owner = owner;
-      run = 0;
+    esrc[s_e0[r] + (t - (s_end[r] - s_du[r]))] = (int32_t)(blockIdx.x * kBlock + r);
+  }
+}
+
+// Part of every row under split points b[0..parts] (per call).
+__global__ __launch_bounds__(kBlock) void local_vpart_kernel(const int32_t* __restrict__ bounds, int32_t parts,
+                                                             int64_t n, uint16_t* __restrict__ vpart) {
+  __shared__ int32_t s_b[kMaxParts + 1];
+  for (int i = threadIdx.x; i <= parts; i += kBlock) s_b[i] = bounds[i];
+  __syncthreads();
+  for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < n; u += (int64_t)gridDim.x * kBlock)
+    vpart[u] = (uint16_t)part_of(s_b, parts, u);
+}
+
+// Every counted DAG edge e = (u, v) in [e0, e1) with count c goes to the
+// part that computes it in the reference's push/pull kernel
+// (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136): u's part when v
+// is in the same part or d+(u) >= d+(v) (pull), else v's part (push).
+// Uncounted edges hold c = 0.  Workgroups stream chunks of kPartsChunk
+// consecutive edges (u from esrc, parts from vpart, degrees from deg16;
+// two saturated degrees are compared on rp), and each lane sums runs of
+// edges with the same owner part before one LDS atomic.
+constexpr int kPartsChunk = kBlock * 32;
+__global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __restrict__ rp,
+                                                             const int32_t* __restrict__ ci,
+                                                             const int32_t* __restrict__ esrc,
+                                                             const uint16_t* __restrict__ deg16,
+                                                             const uint16_t* __restrict__ vpart, int64_t e0,
+                                                             int64_t e1, const uint32_t* __restrict__ ecount,
+                                                             int32_t parts, unsigned long long* pe) {
+  __shared__ unsigned long long s_pe[kMaxParts];
+  for (int i = threadIdx.x; i < parts; i += kBlock) s_pe[i] = 0;
+  __syncthreads();
+  int run_owner = 0;
+  unsigned long long run = 0;
+  for (int64_t b0 = e0 + (int64_t)blockIdx.x * kPartsChunk; b0 < e1; b0 += (int64_t)gridDim.x * kPartsChunk) {
+    const int64_t b1 = min(b0 + (int64_t)kPartsChunk, e1);
+    for (int64_t e = b0 + threadIdx.x; e < b1; e += kBlock) {
+      const uint32_t c = ecount[e];
+      if (c == 0) continue;
+      const int u = esrc[e], v = ci[e];
+      const int pu = vpart[u], pv = vpart[v];
+      int owner = pu;
+      if (pv != pu) {
+        const int du = deg16[u], dv = deg16[v];
+        const bool push = (du == kDegSat && dv == kDegSat) ? (rp[u + 1] - rp[u] < rp[v + 1] - rp[v]) : du < dv;
+        owner = push ? pv : pu;
+      }
+      if (owner != run_owner) {
+        if (run) atomicAdd(&s_pe[run_owner], run);
+        run_owner = owner;
+        run = 0;
+      }
+      run += c;
     }
-    run += c;
   }
   if (run) atomicAdd(&s_pe[run_owner], run);
   __syncthreads();
@@ -2160,7 +2233,7 @@ hipError_t launch_plan(const LaunchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int BIN, bool kLocal, bool kWide>
+template <int BIN, int kLocal, bool kWide>
 static int bin_blocks_per_cu() {
   static const int n = [] {
     int k = 0;
@@ -2174,7 +2247,7 @@ static int bin_blocks_per_cu() {
 
 // tiles >= 0: the bin's tile count is known on the host; the grid is then
 // capped at it, and an empty bin is not launched.
-template <int BIN, bool kLocal = false>
+template <int BIN, int kLocal = kLocalNone>
 static void launch_bin(const LaunchArgs& a, int cus, int grid, hipStream_t s, int64_t tiles = -1) {
   if (tiles == 0) return;
   const bool wide = a.wide_ids != 0;
@@ -2232,26 +2305,40 @@ hipError_t launch_count_order(const LaunchArgs& a, int cus, int grid, const int*
   return hipGetLastError();
 }
 
-hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, hipStream_t s) {
-  if (a.ngraphs != 1 || !a.local.pedge || !a.local.pvid || !a.local.pown || !a.local.ecount || !a.local.vcount)
-    return hipErrorInvalidValue;
-  if (first <= BIN_GLOBAL && BIN_GLOBAL <= last) launch_bin<BIN_GLOBAL, true>(a, cus, 0, s);
-  if (first <= BIN_BLOCK && BIN_BLOCK <= last) launch_bin<BIN_BLOCK, true>(a, cus, 0, s);
-  if (first <= BIN_WAVE && BIN_WAVE <= last) launch_bin<BIN_WAVE, true>(a, cus, 0, s);
-  if (first <= BIN_G32 && BIN_G32 <= last) launch_bin<BIN_G32, true>(a, cus, 0, s);
-  if (first <= BIN_G16 && BIN_G16 <= last) launch_bin<BIN_G16, true>(a, cus, 0, s);
-  if (first <= BIN_G8 && BIN_G8 <= last) launch_bin<BIN_G8, true>(a, cus, 0, s);
+template <int kLocal>
+static void launch_local_bins(const LaunchArgs& a, int cus, int first, int last, hipStream_t s) {
+  if (first <= BIN_GLOBAL && BIN_GLOBAL <= last) launch_bin<BIN_GLOBAL, kLocal>(a, cus, 0, s);
+  if (first <= BIN_BLOCK && BIN_BLOCK <= last) launch_bin<BIN_BLOCK, kLocal>(a, cus, 0, s);
+  if (first <= BIN_WAVE && BIN_WAVE <= last) launch_bin<BIN_WAVE, kLocal>(a, cus, 0, s);
+  if (first <= BIN_G32 && BIN_G32 <= last) launch_bin<BIN_G32, kLocal>(a, cus, 0, s);
+  if (first <= BIN_G16 && BIN_G16 <= last) launch_bin<BIN_G16, kLocal>(a, cus, 0, s);
+  if (first <= BIN_G8 && BIN_G8 <= last) launch_bin<BIN_G8, kLocal>(a, cus, 0, s);
+}
+
+hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, bool vertices, hipStream_t s) {
+  if (a.ngraphs != 1 || !a.local.pedge || !a.local.ecount) return hipErrorInvalidValue;
+  if (vertices && (!a.local.pvid || !a.local.pown || !a.local.vcount)) return hipErrorInvalidValue;
+  if (vertices) launch_local_bins<kLocalFull>(a, cus, first, last, s);
+  else launch_local_bins<kLocalEdges>(a, cus, first, last, s);
   return hipGetLastError();
 }
 
-hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                              const uint8_t* own, const uint32_t* ecount, const int32_t* bounds, int32_t parts,
-                              unsigned long long* pe, hipStream_t s) {
+hipError_t launch_local_prep(const int64_t* rp, int64_t n, int32_t* esrc, uint16_t* deg16, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  local_prep_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(rp, n, esrc, deg16);
+  return hipGetLastError();
+}
+
+hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, const int32_t* esrc, const uint16_t* deg16,
+                              uint16_t* vpart, int64_t n, int64_t e0, int64_t e1, const uint32_t* ecount,
+                              const int32_t* bounds, int32_t parts, unsigned long long* pe, int cus, hipStream_t s) {
   if (parts < 1 || parts > kMaxParts || !bounds || !pe) return hipErrorInvalidValue;
-  const int64_t rows = (int64_t)row_end - row_begin;
-  if (rows <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((rows + kBlock - 1) / kBlock);
-  local_parts_kernel<<<blocks, kBlock, 0, s>>>(rp, ci, row_begin, row_end, own, ecount, bounds, parts, pe);
+  if (n <= 0 || e1 <= e0) return hipSuccess;
+  const int64_t vb = min((n + kBlock - 1) / kBlock, (int64_t)cus * 8);
+  local_vpart_kernel<<<(unsigned)vb, kBlock, 0, s>>>(bounds, parts, n, vpart);
+  const int64_t chunks = (e1 - e0 + kPartsChunk - 1) / kPartsChunk;
+  const int64_t pb = min(chunks, (int64_t)cus * 8);
+  local_parts_kernel<<<(unsigned)pb, kBlock, 0, s>>>(rp, ci, esrc, deg16, vpart, e0, e1, ecount, parts, pe);
   return hipGetLastError();
 }
 

@@ -347,16 +347,22 @@ static_assert(kBinOrder.ok, "TC_BIN_ORDER: bins 0-5 once each, in at most 4 '|'-
 // [row_begin, row_end) are never pivots).
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
                         const uint8_t* own, uint64_t* out, int grid, hipStream_t s);
-// Count of one graph that also fills a.local (ecount and vcount zeroed by
-// the caller): same plan, local variants of bins first..last.
-hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, hipStream_t s);
-// After launch_count_local: sums each counted edge's ecount into pe[owner]
-// by the reference push/pull rule over the split bounds[0..parts]
-// (1 <= parts <= kMaxParts; pe zeroed by the caller).
+// Count of one graph that also fills a.local (ecount and, with vertices,
+// vcount zeroed by the caller): same plan, local variants of bins
+// first..last.  Without vertices only the per-edge counts are written (the
+// kernels keep no slot hit counters: less LDS, more waves per SIMD).
+hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, bool vertices, hipStream_t s);
+// Per-graph tables of launch_local_parts over rows [0, n): esrc[e] = the row
+// of DAG edge e, deg16[u] = out-degree of row u saturated at 65535.
+hipError_t launch_local_prep(const int64_t* rp, int64_t n, int32_t* esrc, uint16_t* deg16, hipStream_t s);
+// After launch_count_local: sums the ecount of every edge in [e0, e1) (the
+// counted rows' edges; uncounted ones hold 0) into pe[owner] by the
+// reference push/pull rule over the split bounds[0..parts] (1 <= parts <=
+// kMaxParts; pe zeroed by the caller).  vpart: n entries, written here.
 constexpr int kMaxParts = 1024;
-hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                              const uint8_t* own, const uint32_t* ecount, const int32_t* bounds, int32_t parts,
-                              unsigned long long* pe, hipStream_t s);
+hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, const int32_t* esrc, const uint16_t* deg16,
+                              uint16_t* vpart, int64_t n, int64_t e0, int64_t e1, const uint32_t* ecount,
+                              const int32_t* bounds, int32_t parts, unsigned long long* pe, int cus, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
 // sink: grid * kBlock words (never written for a byte-filled source).
 hipError_t launch_read(const void* src, int64_t bytes, uint32_t* sink, int grid, hipStream_t s);

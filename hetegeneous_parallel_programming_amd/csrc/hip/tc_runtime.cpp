@@ -236,6 +236,9 @@ void LocalBufs::release() {
   dev_free(vcount);
   dev_free(pe);
   dev_free(bounds);
+  dev_free(esrc);
+  dev_free(deg16);
+  dev_free(vpart);
   *this = LocalBufs();
 }
 
@@ -798,11 +801,17 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
     st = dev_alloc((void**)&L.vcount, sizeof(unsigned long long) * ((size_t)g->n + 1), "vertex counts");
   if (st == TC_OK) st = dev_alloc((void**)&L.pe, sizeof(unsigned long long) * tck::kMaxParts, "part counts");
   if (st == TC_OK) st = dev_alloc((void**)&L.bounds, sizeof(int32_t) * (tck::kMaxParts + 1), "split points");
+  const int64_t nr = std::max<int64_t>(g->n, 1);
+  if (st == TC_OK) st = dev_alloc((void**)&L.esrc, sizeof(int32_t) * (size_t)m, "edge rows");
+  if (st == TC_OK) st = dev_alloc((void**)&L.deg16, sizeof(uint16_t) * (size_t)nr, "row degrees");
+  if (st == TC_OK) st = dev_alloc((void**)&L.vpart, sizeof(uint16_t) * (size_t)nr, "row parts");
   if (st == TC_OK) {
     tck::LaunchArgs a = count_args(ctx, g);
     BuildScratch bs;
     st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L);
-    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipError_t e = hipSuccess;
+    if (st == TC_OK) e = tck::launch_local_prep(g->rp, g->n, L.esrc, L.deg16, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     bs.release();
     if (st == TC_OK && e != hipSuccess)
       st = fail(TC_EDEVICE, std::string("partner rebuild: ") + hipGetErrorString(e));
@@ -812,7 +821,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
     return st;
   }
   g->plan_valid = false;  // the rebuild rewrote the partner rows the kept plan lists
-  L.bytes = 16 * m + 4 * P + 8 * ((int64_t)g->n + 1) + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
+  L.bytes = 20 * m + 4 * P + 8 * ((int64_t)g->n + 1) + 4 * nr + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
   return TC_OK;
 }
 
@@ -845,7 +854,9 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
   TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
   TC_HIP_TRY(tck::launch_zero(L.ecount, std::max<int64_t>(g->nnz, 1), s));
-  TC_HIP_TRY(tck::launch_zero(L.vcount, 2 * ((int64_t)g->n + 1), s));
+  // Without vertex_counts the kernels attribute to edges only.
+  const bool vertices = out->vertex_counts != nullptr;
+  if (vertices) TC_HIP_TRY(tck::launch_zero(L.vcount, 2 * ((int64_t)g->n + 1), s));
   TC_HIP_TRY(tck::launch_zero(L.pe, 2 * (int64_t)tck::kMaxParts, s));
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
@@ -855,13 +866,14 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
       hipStream_t gs = gi == 0 ? s : ctx->side[gi - 1];
       if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs, ctx->ev[1], 0));
       for (int j = 0; j < G.len[gi]; ++j)
-        TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, G.bin[gi][j], G.bin[gi][j], gs));
+        TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, G.bin[gi][j], G.bin[gi][j], vertices, gs));
       if (gi > 0) TC_HIP_TRY(hipEventRecord(ctx->join[gi - 1], gs));
     }
     for (int gi = 1; gi < G.n; ++gi) TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[gi - 1], 0));
   }
   if (k > 0)
-    TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, g->row_begin, g->row_end, g->own, L.ecount, L.bounds, k, L.pe, s));
+    TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, L.esrc, L.deg16, L.vpart, g->n, g->edge_begin, g->edge_end,
+                                       L.ecount, L.bounds, k, L.pe, ctx->cus, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,

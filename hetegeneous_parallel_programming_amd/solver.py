@@ -335,7 +335,9 @@ class Context:
         counted rows, CSR order), ``vertex_counts`` (uint64 per row of the
         uploaded graph), ``part_counts`` (uint64 per part of the split
         ``bounds``, the reference push/pull kernel's per-PE sums) and
-        ``stats``; outputs not asked for are None."""
+        ``stats``; outputs not asked for are None.  Without ``vertices`` the
+        kernels attribute each triangle to its edge only (same edge and part
+        counts, less work)."""
         n = int(dg.info()["rows"])
         lo = Local()
         ec = vc = pc = None

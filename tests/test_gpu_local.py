@@ -1,6 +1,7 @@
 """GPU parity of tc_count_local: per-edge counts |N+(u) ∩ N+(v)|, per-vertex
 triangle counts and the reference push/pull per-PE attribution, against
-the oracle (bit-exact), on whole graphs and on halo partitions."""
+the oracle (bit-exact), on whole graphs and on halo partitions; and the
+edges-only kernels (no vertex_counts asked for) against the same."""
 import itertools
 import math
 
@@ -36,6 +37,12 @@ def _check_whole(ctx, g, bounds=None):
     assert np.array_equal(out["vertex_counts"], oracle.tc_per_vertex(rp, ci))
     if bounds is not None:
         assert np.array_equal(out["part_counts"], oracle.tc_pushpull_bounds(rp, ci, bounds))
+    # without vertex_counts the kernels attribute to edges only: same edge and part counts
+    total_e, out_e = ctx.count_local(dg, vertices=False, bounds=bounds)
+    assert total_e == total and out_e["vertex_counts"] is None
+    assert np.array_equal(out_e["edge_counts"], out["edge_counts"])
+    if bounds is not None:
+        assert np.array_equal(out_e["part_counts"], out["part_counts"])
     # the plain count still agrees after the local call rebuilt the partner entries
     assert ctx.count_binned(dg)[0] == total
     dg.free()

@@ -148,3 +148,28 @@ def test_rejects_bad_split(ctx):
     with pytest.raises(TcError):
         ctx.count_local(dg, bounds=np.arange(0, 1026, dtype=np.int32))  # 1025 parts
     dg.free()
+
+
+def test_parts_with_saturated_degrees(ctx):
+    """A pushed edge between two rows whose out-degrees both pass the 16-bit
+    saturation: the per-part pass compares their exact degrees.  Vertex 0
+    points at vertex 1 and 70,000 leaves, vertex 1 at 70,010 leaves, 50,000 of
+    them shared; with 0 and 1 in different parts, d+(0) < d+(1) pushes the
+    edge (0, 1) and its 50,000 triangles to vertex 1's part."""
+    a = np.arange(2, 70_002, dtype=np.int32)
+    b = np.concatenate([a[:50_000], np.arange(70_002, 90_012, dtype=np.int32)])
+    n = 90_012
+    rp = np.zeros(n + 1, np.int64)
+    rp[1] = 1 + a.size
+    rp[2:] = rp[1] + b.size
+    ci = np.concatenate([np.array([1], np.int32), a, b])
+    bounds = [0, 1, n]
+    want = oracle.tc_pushpull_bounds(rp, ci, bounds)
+    assert want.tolist() == [0, 50_000]
+    dg = ctx.upload_arrays(rp, ci)
+    for vertices in (True, False):
+        total, out = ctx.count_local(dg, vertices=vertices, bounds=bounds)
+        assert total == 50_000
+        assert np.array_equal(out["part_counts"], want)
+        assert np.array_equal(out["edge_counts"], oracle.tc_edge_counts(rp, ci))
+    dg.free()

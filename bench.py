@@ -815,6 +815,11 @@ def run_side_legs(ctx, args, line) -> None:
 
     if not args.no_er:
         line["extra"]["er16384"] = leg_er(ctx, args.seed, max(5, args.steps), args.warmup)
+    # the partition legs before the batch and serving legs: the longest side
+    # legs come last
+    if not args.no_partition:
+        line["extra"]["partition_rmat24_p0"] = leg_partition(ctx, args.seed, max(3, args.steps // 4),
+                                                             args.warmup, split=not args.no_split)
     if not args.no_batch or not args.no_serving:
         t0 = time.perf_counter()
         graphs = Graph.batch(args.seed, 0, args.batch_graphs)
@@ -844,9 +849,6 @@ def run_side_legs(ctx, args, line) -> None:
             del heavy
         ctx.tune = default
         del graphs
-    if not args.no_partition:
-        line["extra"]["partition_rmat24_p0"] = leg_partition(ctx, args.seed, max(3, args.steps // 4),
-                                                             args.warmup, split=not args.no_split)
 
 
 if __name__ == "__main__":

@@ -100,8 +100,12 @@ struct LocalBufs {
 struct BuildScratch {
   char* mem = nullptr;
   int64_t cap = 0;
+  char* emem = nullptr;  // the radix path's pair id positions (local rebuilds)
+  int64_t ecap = 0;
   tck::BuildBufs b{};
   tc_status ensure(int64_t rp_slots, int64_t ids);
+  // After ensure: room for the pairs' id positions (b.e1, b.e2) of ids ids.
+  tc_status ensure_edges(int64_t ids);
   void release();
   static int64_t bytes_for(int64_t rp_slots, int64_t ids);
 };

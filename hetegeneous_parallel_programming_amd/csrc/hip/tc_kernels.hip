@@ -1190,11 +1190,15 @@ __global__ __launch_bounds__(kBlock) void radix_split_kernel(LaunchArgs a, int64
 // pairs [p0, p1) of (ikey, iref), digit = (key >> shift) & mask (nd digits),
 // gpos[d] = the tile's first output position of digit d.  LDS holds each
 // sorted key and its place in the tile; the list references are read in
-// sorted order from the tile's own range.
+// sorted order from the tile's own range.  kEdge: each pair's id position
+// rides along (iedge, or the pair's own index when iedge is null) to oedge.
+template <bool kEdge>
 __device__ __forceinline__ void tile_sort_write(const int32_t* __restrict__ ikey, const ListRef* __restrict__ iref,
                                                 int64_t p0, int64_t p1, int shift, int mask, int nd,
                                                 const uint32_t* s_gpos, uint32_t* s_cnt, int32_t* s_k,
-                                                uint16_t* s_i, int64_t* s_wave, int32_t* okey, ListRef* oref) {
+                                                uint16_t* s_i, int64_t* s_wave, int32_t* okey, ListRef* oref,
+                                                const uint32_t* __restrict__ iedge = nullptr,
+                                                uint32_t* oedge = nullptr) {
   int32_t kk[kTilePer];
   uint32_t rk[kTilePer];
 #pragma unroll
@@ -1219,10 +1223,12 @@ __device__ __forceinline__ void tile_sort_write(const int32_t* __restrict__ ikey
   }
   __syncthreads();
   ListRef r[kTilePer];
+  uint32_t ev[kTilePer];
 #pragma unroll
   for (int s = 0; s < kTilePer; ++s) {
     const uint32_t p = s * kBlock + threadIdx.x;
     r[s] = p < n ? iref[p0 + s_i[p]] : (ListRef)0;
+    if constexpr (kEdge) ev[s] = p < n ? (iedge ? iedge[p0 + s_i[p]] : (uint32_t)(p0 + s_i[p])) : 0u;
   }
 #pragma unroll
   for (int s = 0; s < kTilePer; ++s) {
@@ -1232,10 +1238,13 @@ __device__ __forceinline__ void tile_sort_write(const int32_t* __restrict__ ikey
     const int64_t at = (int64_t)s_gpos[d] + (p - s_st[d]);
     okey[at] = s_k[p];
     oref[at] = r[s];
+    if constexpr (kEdge) oedge[at] = ev[s];
   }
 }
 
-// Level 1: split block blk's pairs -> coarse runs of pkey/pref.
+// Level 1: split block blk's pairs -> coarse runs of pkey/pref (kEdge: and
+// their id positions, the pairs' own indexes, to e1).
+template <bool kEdge>
 __global__ __launch_bounds__(kBlock) void radix_scatter1_kernel(BuildBufs b, int64_t RP, int64_t M, int cs, int nc,
                                                                 int64_t nmp) {
   __shared__ int32_t s_k[kTile];
@@ -1250,7 +1259,8 @@ __global__ __launch_bounds__(kBlock) void radix_scatter1_kernel(BuildBufs b, int
     s_gpos[c] = b.cnt1[(int64_t)c * nmp + blk];
   }
   __syncthreads();
-  tile_sort_write(b.key, b.ref, j0, j1, cs, 0x7FFFFFFF, nc, s_gpos, s_cnt, s_k, s_i, s_wave, b.pkey, b.pref);
+  tile_sort_write<kEdge>(b.key, b.ref, j0, j1, cs, 0x7FFFFFFF, nc, s_gpos, s_cnt, s_k, s_i, s_wave, b.pkey, b.pref,
+                        nullptr, b.e1);
 }
 
 // Coarse bucket c's pairs are [cnt1[c * nmp], cnt1[(c + 1) * nmp]) of the
@@ -1315,6 +1325,7 @@ __global__ __launch_bounds__(kBlock) void radix_hist2_kernel(BuildBufs b, int fs
 }
 
 // Level 2: pkey/pref runs of one coarse bucket -> fine runs of key/ref.
+template <bool kEdge>
 __global__ __launch_bounds__(kBlock) void radix_scatter2_kernel(BuildBufs b, int fs, int nc, int64_t nmp) {
   __shared__ int32_t s_k[kTile];
   __shared__ uint16_t s_i[kTile];
@@ -1327,7 +1338,8 @@ __global__ __launch_bounds__(kBlock) void radix_scatter2_kernel(BuildBufs b, int
     s_gpos[threadIdx.x] = b.cnt2[cnt2_at(b, x, threadIdx.x)];
   }
   __syncthreads();
-  tile_sort_write(b.pkey, b.pref, x.p0, x.p1, fs, kFine - 1, kFine, s_gpos, s_cnt, s_k, s_i, s_wave, b.key, b.ref);
+  tile_sort_write<kEdge>(b.pkey, b.pref, x.p0, x.p1, fs, kFine - 1, kFine, s_gpos, s_cnt, s_k, s_i, s_wave, b.key,
+                        b.ref, b.e1, b.e2);
 }
 
 // Fine bucket f's pairs are [fine_start(f), fine_start(f + 1)) of the
@@ -1443,11 +1455,12 @@ __device__ __forceinline__ void wave_run_add(unsigned long long* s_w, int o, uns
 // Writes the n pairs sorted in LDS (pair s_idx[i] of iref, owner s_own[i])
 // to the pool at dst(o, i), and adds each partner's list length to its
 // owner's work s_w[o]; kUnroll list references in flight per lane (a
-// partner given by its slot is read from lref).
-template <class Dst>
+// partner given by its slot is read from lref).  kEdge: each pair's id
+// position (iedge) goes to pedge at the same place.
+template <bool kEdge, class Dst>
 __device__ __forceinline__ void write_sorted(ListRef* pci, const ListRef* lref, int64_t n, const ListRef* iref,
                                              const uint16_t* s_idx, const uint16_t* s_own, unsigned long long* s_w,
-                                             Dst dst) {
+                                             Dst dst, const uint32_t* iedge = nullptr, int64_t* pedge = nullptr) {
   constexpr int kUnroll = 8;
   for (int64_t i0 = 0; i0 < n; i0 += kUnroll * kBlock) {
     ListRef r[kUnroll];
@@ -1465,7 +1478,9 @@ __device__ __forceinline__ void write_sorted(ListRef* pci, const ListRef* lref, 
       int o = -1;
       if (i < n) {
         o = s_own[i];
-        pci[dst(o, i)] = r[j];
+        const int64_t at = dst(o, i);
+        pci[at] = r[j];
+        if constexpr (kEdge) pedge[at] = (int64_t)iedge[s_idx[i]];
       }
       wave_run_add(s_w, o, (unsigned long long)list_len(r[j]));
     }
@@ -1474,6 +1489,7 @@ __device__ __forceinline__ void write_sorted(ListRef* pci, const ListRef* lref, 
 
 // Per fine bucket: partner offsets of its owners; a small bucket is placed
 // whole, a big one gets its owners' cursors (absolute pool positions).
+template <bool kEdge>
 __global__ __launch_bounds__(kBlock) void radix_final_kernel(LaunchArgs a, BuildBufs b, int64_t RP, int fs, int cap) {
   extern __shared__ int64_t s_mem[];
   const int no_max = 1 << fs;
@@ -1529,14 +1545,16 @@ __global__ __launch_bounds__(kBlock) void radix_final_kernel(LaunchArgs a, Build
     s_own[at] = (uint16_t)ko[s];
   }
   __syncthreads();
-  write_sorted((ListRef*)a.pci, b.lref, n, b.ref + p0, s_idx, s_own, s_w,
-               [&](int o, int64_t i) { return pool_pos(a, s_g[o], p0 + i); });
+  write_sorted<kEdge>((ListRef*)a.pci, b.lref, n, b.ref + p0, s_idx, s_own, s_w,
+                      [&](int o, int64_t i) { return pool_pos(a, s_g[o], p0 + i); }, kEdge ? b.e2 + p0 : nullptr,
+                      b.pedge);
   __syncthreads();
   for (int o = threadIdx.x; o < no; o += kBlock) b.wq[q0 + o] = s_w[o];
 }
 
 // Big buckets, pass 2: each chunk sorted by owner in LDS; every owner's run
 // reserved on its cursor and written out.
+template <bool kEdge>
 __global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, BuildBufs b, int fs, int nf, int chunk) {
   extern __shared__ int64_t s_mem[];
   const int no = 1 << fs;
@@ -1582,11 +1600,31 @@ __global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, B
     s_own[at] = (uint16_t)ko[s];
   }
   __syncthreads();
-  write_sorted((ListRef*)a.pci, b.lref, c.p1 - c.p0, b.ref + c.p0, s_idx, s_own, s_w,
-               [&](int o, int64_t i) { return (int64_t)s_base[o] + (i - s_st[o]); });
+  write_sorted<kEdge>((ListRef*)a.pci, b.lref, c.p1 - c.p0, b.ref + c.p0, s_idx, s_own, s_w,
+                      [&](int o, int64_t i) { return (int64_t)s_base[o] + (i - s_st[o]); },
+                      kEdge ? b.e2 + c.p0 : nullptr, b.pedge);
   __syncthreads();
   for (int o = threadIdx.x; o < no; o += kBlock)
     if (s_w[o]) atomicAdd(&b.wq[q0 + o], s_w[o]);
+}
+
+// Radix path of a local rebuild (one graph): the partner vertex of every
+// entry of partner row k, from the entry's DAG id position e (pedge): the
+// DAG edge is (esrc[e], ci[e]); the partner is the end that is not the row's
+// owner pown[k].  One wave per row, grid-stride.
+__global__ __launch_bounds__(kBlock) void partner_vid_kernel(LaunchArgs a, int64_t P, BuildBufs b) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6, nw = ((int64_t)gridDim.x * kBlock) >> 6;
+  for (int64_t k = w0; k < P; k += nw) {
+    if (a.vrow[k].set == 0) continue;
+    const int32_t x = b.pown[k];
+    const int64_t i0 = a.vrp[k], i1 = a.vrp[k + 1];
+    for (int64_t i = i0 + lane; i < i1; i += 64) {
+      const int64_t e = b.pedge[i];
+      const int32_t r = b.esrc[e];
+      b.pvid[i] = r == x ? a.ci[e] : r;
+    }
+  }
 }
 
 // Rows, row offsets and work after the partner pool is filled (both paths).
@@ -1602,6 +1640,7 @@ static hipError_t launch_partner_rows(const LaunchArgs& a, int64_t RP, const Bui
   return hipGetLastError();
 }
 
+template <bool kEdge>
 static hipError_t launch_partners_radix(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& b,
                                         const RadixGeom& x, hipStream_t s) {
   const int cs = x.fs + 6;
@@ -1619,7 +1658,7 @@ static hipError_t launch_partners_radix(const LaunchArgs& a, int64_t RP, int64_t
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int64_t n1 = (int64_t)x.nc * x.nmp + 1;
   if ((e = launch_scan(LoadU32{b.cnt1}, StoreU32{b.cnt1}, n1, b.part, s)) != hipSuccess) return e;
-  radix_scatter1_kernel<<<(unsigned)x.nmp, kBlock, 0, s>>>(b, RP, M, cs, x.nc, x.nmp);
+  radix_scatter1_kernel<kEdge><<<(unsigned)x.nmp, kBlock, 0, s>>>(b, RP, M, cs, x.nc, x.nmp);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   radix_tiles2_kernel<<<1, kBlock, 0, s>>>(b, x.nc, x.nmp);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1627,29 +1666,40 @@ static hipError_t launch_partners_radix(const LaunchArgs& a, int64_t RP, int64_t
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int64_t n2 = kFine * x.nt2 + 1;
   if ((e = launch_scan(LoadU32{b.cnt2}, StoreU32{b.cnt2}, n2, b.part, s)) != hipSuccess) return e;
-  radix_scatter2_kernel<<<(unsigned)x.nt2, kBlock, 0, s>>>(b, x.fs, x.nc, x.nmp);
+  radix_scatter2_kernel<kEdge><<<(unsigned)x.nt2, kBlock, 0, s>>>(b, x.fs, x.nc, x.nmp);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   radix_chunks_kernel<<<1, kBlock, 0, s>>>(b, x.nf, x.nc, x.nmp, x.cap, x.chunk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const size_t own_lds = (size_t)1 << x.fs;
   radix_count_big_kernel<<<(unsigned)x.nch, kBlock, 4 * own_lds, s>>>(b, x.fs, x.nf, x.chunk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  radix_final_kernel<<<(unsigned)x.nf, kBlock, radix_final_lds(x.fs, x.cap), s>>>(a, b, RP, x.fs, x.cap);
+  radix_final_kernel<kEdge><<<(unsigned)x.nf, kBlock, radix_final_lds(x.fs, x.cap), s>>>(a, b, RP, x.fs, x.cap);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  radix_place_big_kernel<<<(unsigned)x.nch, kBlock, radix_place_lds(x.fs, x.chunk), s>>>(a, b, x.fs, x.nf,
-                                                                                         x.chunk);
+  radix_place_big_kernel<kEdge><<<(unsigned)x.nch, kBlock, radix_place_lds(x.fs, x.chunk), s>>>(a, b, x.fs, x.nf,
+                                                                                                x.chunk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  return launch_partner_rows(a, RP, b, s);
+  if ((e = launch_partner_rows(a, RP, b, s)) != hipSuccess) return e;
+  if constexpr (kEdge) {
+    if (a.P > 0) {
+      const int64_t blocks = min((a.P * 64 + kBlock - 1) / kBlock, (int64_t)65536);
+      partner_vid_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(a, a.P, b);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_partners(const LaunchArgs& a, int64_t RP, int64_t M, const BuildBufs& bb, hipStream_t s) {
   if (RP <= 0) return hipSuccess;
   RadixGeom x;
-  if (bb.key && !bb.pedge && RP <= bb.rows && M <= bb.ids && radix_geom(RP, M, x))
-  {
+  // A local rebuild (pedge) takes the radix path for one graph with the id
+  // positions' scratch and every edge's row.
+  const bool local = bb.pedge != nullptr;
+  const bool radix_local = local && bb.e1 && bb.e2 && bb.esrc && bb.pvid && bb.pown && a.ngraphs == 1;
+  if (bb.key && (!local || radix_local) && RP <= bb.rows && M <= bb.ids && radix_geom(RP, M, x)) {
     BuildBufs b = bb;
     b.wq = bb.cur + RP;  // second stripe of the cursors: owners' work
-    return launch_partners_radix(a, RP, M, b, x, s);
+    return local ? launch_partners_radix<true>(a, RP, M, b, x, s) : launch_partners_radix<false>(a, RP, M, b, x, s);
   }
   BuildBufs b = bb;
   b.gid = nullptr;  // rows are found by their graph search (no prep pass here)

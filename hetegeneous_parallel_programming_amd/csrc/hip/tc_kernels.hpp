@@ -206,6 +206,13 @@ struct BuildBufs {
   int64_t* chstart;     // [kMaxFineBuckets + 1] first chunk of each big fine bucket
   int64_t* fst;         // [kMaxFineBuckets + 1] first pair of each fine bucket
   unsigned long long* wq;   // [RP] radix path: each owner's work (its partners' list lengths); else null
+  // Radix path with pedge (a local rebuild of one graph): each pair's DAG id
+  // position rides along the two levels (e1, e2) and lands in pedge beside
+  // the partner's list; esrc[e] = the row of DAG id e gives the partner
+  // vertices (pvid).  Null otherwise.
+  uint32_t* e1;         // [M] level-1 order of the pairs' id positions
+  uint32_t* e2;         // [M] level-2 (fine-bucket) order
+  const int32_t* esrc;  // [M]
   int64_t rows;         // capacity: row slots
   int64_t ids;          // capacity: ids
 };

@@ -211,19 +211,40 @@ tc_status BuildScratch::ensure(int64_t rp_slots, int64_t ids) {
   return TC_OK;
 }
 
+tc_status BuildScratch::ensure_edges(int64_t ids) {
+  const int64_t need = 2 * up256(4 * std::max<int64_t>(ids, 1));
+  if (need > ecap) {
+    dev_free(emem);
+    emem = nullptr;
+    ecap = 0;
+    tc_status st = dev_alloc((void**)&emem, (size_t)need, "partner build id positions");
+    if (st != TC_OK) return st;
+    ecap = need;
+  }
+  b.e1 = (uint32_t*)emem;
+  b.e2 = (uint32_t*)(emem + need / 2);
+  return TC_OK;
+}
+
 void BuildScratch::release() {
   dev_free(mem);
+  dev_free(emem);
   *this = BuildScratch();
 }
 
 tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t M, BuildScratch& bs, hipStream_t s,
                        const LocalBufs* loc) {
-  // the local rebuild writes edge positions beside the partners: atomic path
-  tc_status st = bs.ensure(RP, loc ? 0 : M);
+  // The local rebuild also writes each entry's edge position and partner
+  // vertex: the radix path carries the positions along (one graph, its
+  // edges' rows in loc->esrc, filled before); the per-edge atomic path
+  // writes them directly where the radix geometry does not fit.
+  tc_status st = bs.ensure(RP, M);
+  if (st == TC_OK && loc) st = bs.ensure_edges(M);
   if (st != TC_OK) return st;
   bs.b.pedge = loc ? loc->pedge : nullptr;
   bs.b.pvid = loc ? loc->pvid : nullptr;
   bs.b.pown = loc ? loc->pown : nullptr;
+  bs.b.esrc = loc ? loc->esrc : nullptr;
   TC_HIP_TRY(tck::launch_partners(a, RP, M, bs.b, s));
   return TC_OK;
 }
@@ -808,9 +829,8 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
   if (st == TC_OK) {
     tck::LaunchArgs a = count_args(ctx, g);
     BuildScratch bs;
-    st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L);
-    hipError_t e = hipSuccess;
-    if (st == TC_OK) e = tck::launch_local_prep(g->rp, g->n, L.esrc, L.deg16, ctx->stream);
+    hipError_t e = tck::launch_local_prep(g->rp, g->n, L.esrc, L.deg16, ctx->stream);  // the rebuild reads esrc
+    if (e == hipSuccess) st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     bs.release();
     if (st == TC_OK && e != hipSuccess)

@@ -200,8 +200,9 @@ def leg_er(ctx, seed, steps, warmup):
     launch at this size), the binned count (its plan kept after the first
     call) and the plain kernel on the resident DAG, beside the CPU oracle's
     count of the same DAG on all host cores (median of 3); and TCSolver's
-    one-shot call (upload + partner build + count + free).  A ~131k-edge
-    graph is launch-bound on the GPU."""
+    one-shot call (tc_count_csr: copy in, one launch, count back).  A
+    ~131k-edge graph is launch-bound on the GPU: each is timed over at least
+    50 calls."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, and here the timed CPU figure beside the GPU one
     from hetegeneous_parallel_programming_amd import Graph
@@ -836,7 +837,7 @@ def run_side_legs(ctx, args, line) -> None:
     from hetegeneous_parallel_programming_amd import Graph
 
     if not args.no_er:
-        line["extra"]["er16384"] = leg_er(ctx, args.seed, max(5, args.steps), args.warmup)
+        line["extra"]["er16384"] = leg_er(ctx, args.seed, max(50, args.steps), max(5, args.warmup))
     # the partition legs before the batch and serving legs: the longest side
     # legs come last
     if not args.no_partition:

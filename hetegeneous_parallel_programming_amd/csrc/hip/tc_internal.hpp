@@ -157,6 +157,10 @@ struct tc_ctx {
   uint64_t* pinned = nullptr;  // 64 B count + 64 B bin counters
   void* noalloc = nullptr;     // device address of the static scratch of tc_count_noalloc
   uint64_t* plain_out = nullptr;  // device: the single-launch count's word, zero between calls
+  // tc_count_csr's copy of a small graph: rowptr, then colidx at the next
+  // 256-byte boundary (grow-only, kept until destroy)
+  uint8_t* once_dev = nullptr;
+  size_t once_cap = 0;  // bytes
   int grid() const { return tune.grid > 0 ? tune.grid : count_grid; }
 };
 

@@ -123,6 +123,7 @@ def device_lib():
     S(L, "tc_graph_free", nat.c_i32, [P, P])
     S(L, "tc_graph_bytes", nat.c_i64, [P])
     S(L, "tc_count", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
+    S(L, "tc_count_csr", nat.c_i32, [P, ctypes.POINTER(CsrView), nat.c_u32, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_plain", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_binned", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
     S(L, "tc_count_noalloc", nat.c_i32, [P, P, nat.u64p, ctypes.POINTER(Stats)])
@@ -303,6 +304,15 @@ class Context:
         total = nat.c_u64(0)
         st = Stats()
         _check(device_lib().tc_count(self._h, dg._h, ctypes.byref(total), ctypes.byref(st)))
+        return int(total.value), st.as_dict()
+
+    def count_csr(self, g: Graph, check: int = nat.TC_CHECK_SHAPE):
+        """tc_count_csr: one count of the host DAG g, nothing kept on the
+        device (a small graph is counted without a partner structure)."""
+        total = nat.c_u64(0)
+        st = Stats()
+        _check(device_lib().tc_count_csr(self._h, ctypes.byref(g.view()), check, ctypes.byref(total),
+                                         ctypes.byref(st)))
         return int(total.value), st.as_dict()
 
     def count_binned(self, dg: DeviceGraph):
@@ -519,9 +529,7 @@ def TCSolver(g: Graph, n_gpus: int = 1, chunk_size: int = 1024, ctx: Context | N
     own = ctx is None
     c = Context(0) if own else ctx
     try:
-        dg = c.upload(g)
-        total, st = c.count(dg)
-        dg.free()
+        total, st = c.count_csr(g)
         if stats is not None:
             stats.update(st)
         return total

@@ -125,6 +125,16 @@ tc_status tc_graph_get_info(const tc_graph* g, tc_graph_info* out);
  * (successful_push_pull/multigpu_nvshmem_pushpull.cu:69-83) around
  * warp_vertex (bs_warp_vertex.cuh:2-35). */
 tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
+
+/* tc_count_csr: one count of a host DAG, nothing kept -- the whole of
+ * TCSolver(Graph&, uint64_t& total, ...) (successful_push_pull/main.cc:5;
+ * multigpu_nvshmem_pushpull.cu:29-85: device copy, launch, copy-back).
+ * check as for tc_graph_upload.  A graph that tc_count would count by its
+ * single plain launch (at most 2^18 DAG ids, lists below 1024) is copied
+ * into a device buffer the context keeps for this call (grown when needed)
+ * and counted without a partner structure; any other graph is uploaded,
+ * counted and freed.  stats->single_launch tells which. */
+tc_status tc_count_csr(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, uint64_t* total, tc_stats* stats);
 /* tc_count_binned: always the degree-binned count (tc_count takes it for all
  * but small graphs; see below).  The binned count's plan is built on a
  * graph's first binned count and kept in the graph (device bytes: 10 per

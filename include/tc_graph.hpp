@@ -317,12 +317,8 @@ class Device {
 
   // Count the DAG `g` on the device (binned kernels).
   AccType count(const Graph& g, tc_stats* st = nullptr) {
-    tc_graph* dg = nullptr;
-    dev_check(tc_graph_upload(ctx_, &g.view(), TC_CHECK_SHAPE, &dg));
     uint64_t total = 0;
-    tc_status s = tc_count(ctx_, dg, &total, st);
-    tc_graph_free(ctx_, dg);
-    dev_check(s);
+    dev_check(tc_count_csr(ctx_, &g.view(), TC_CHECK_SHAPE, &total, st));
     return total;
   }
 

@@ -309,3 +309,32 @@ def test_radix_build_with_the_widest_buckets(ctx):
     assert ctx.count_binned(dg)[0] == want and ctx.count(dg)[0] == want
     assert dg.info()["partner_rows"] == g.work()["partner_rows"]
     dg.free()
+
+
+def test_count_csr_one_shot(ctx):
+    """tc_count_csr counts a host DAG in one call with nothing kept: small
+    graphs (ER n=16384 d=16, seeds 1-3) by the single launch without a
+    partner structure, copied to the context's grow-only buffer (a larger
+    graph, a smaller one again, an edgeless one and one of 2.2M vertices in
+    between), a bigger graph through upload + tc_count + free; every count the oracle's; malformed
+    arrays are rejected as tc_graph_upload rejects them."""
+    from hetegeneous_parallel_programming_amd.solver import TCSolver
+
+    cases = [Graph.er(16384, 16, s).oriented() for s in (1, 2, 3)]
+    cases += [Graph.er(100, 4, 9).oriented(), Graph.from_pairs(5, np.zeros(0, np.int32), np.zeros(0, np.int32)),
+              Graph.rmat(1 << 16, 16, 3).oriented(), Graph.er(3000, 8, 4).oriented()]
+    # 2.2M vertices, two triangles: the copy's rowptr is far longer than its
+    # colidx, and the buffer grows to it
+    far = np.array([0, 1, 0, 2_100_000, 2_100_001, 2_100_000], np.int32)
+    near = np.array([1, 2, 2, 2_100_001, 2_199_999, 2_199_999], np.int32)
+    cases += [Graph.from_pairs(2_200_000, far, near).oriented(), Graph.er(16384, 16, 1).oriented()]
+    for g in cases:
+        want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
+        got, st = ctx.count_csr(g)
+        assert got == want, (got, want)
+        max_out = int(np.diff(g.out_rowptr()).max()) if g.V() else 0
+        assert st["single_launch"] == (g.E() <= 1 << 18 and max_out < 1024)
+        assert TCSolver(g, ctx=ctx) == want
+    rp = np.array([0, 2, 3], np.int64)
+    with pytest.raises(TcError):
+        ctx.count_csr(Graph.from_csr(rp, np.array([1, 1, 0], np.int32), check=0), check=7)

@@ -87,9 +87,7 @@ struct LocalBufs {
   unsigned long long* vcount = nullptr;
   unsigned long long* pe = nullptr;  // [tck::kMaxParts]
   int32_t* bounds = nullptr;         // [tck::kMaxParts + 1]
-  int32_t* esrc = nullptr;           // [nnz] row of each DAG edge
-  uint16_t* deg16 = nullptr;         // [n] out-degrees, saturated at 65535
-  uint16_t* vpart = nullptr;         // [n] part of each row under the call's split
+  int32_t* esrc = nullptr;           // [nnz] row u of each DAG edge (u, v) | (d+(u) < d+(v)) << 31
   int64_t bytes = 0;
   void release();
 };

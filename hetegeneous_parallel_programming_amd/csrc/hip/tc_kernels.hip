@@ -1610,7 +1610,8 @@ __global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, B
 
 // Radix path of a local rebuild (one graph): the partner vertex of every
 // entry of partner row k, from the entry's DAG id position e (pedge): the
-// DAG edge is (esrc[e], ci[e]); the partner is the end that is not the row's
+// DAG edge is (esrc[e] without bit 31, ci[e]); the partner is the end that
+// is not the row's
 // owner pown[k].  One wave per row, grid-stride.
 __global__ __launch_bounds__(kBlock) void partner_vid_kernel(LaunchArgs a, int64_t P, BuildBufs b) {
   const int lane = threadIdx.x & 63;
@@ -1621,7 +1622,7 @@ __global__ __launch_bounds__(kBlock) void partner_vid_kernel(LaunchArgs a, int64
     const int64_t i0 = a.vrp[k], i1 = a.vrp[k + 1];
     for (int64_t i = i0 + lane; i < i1; i += 64) {
       const int64_t e = b.pedge[i];
-      const int32_t r = b.esrc[e];
+      const int32_t r = b.esrc[e] & 0x7fffffff;  // bit 31: the per-part pass's push bit
       b.pvid[i] = r == x ? a.ci[e] : r;
     }
   }
@@ -2039,14 +2040,24 @@ __device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
   return lo;
 }
 
-// Per-graph tables of the per-part pass: each DAG edge's row (esrc) and each
-// row's out-degree saturated at kDegSat (deg16).  A wave walks the edges of
-// 64 consecutive rows as one flattened range (its lanes stay busy whatever
-// the row lengths; a lane finds its row by a binary search over the rows'
-// degree prefix in LDS).
-__global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __restrict__ rp, int64_t n,
-                                                            int32_t* __restrict__ esrc,
-                                                            uint16_t* __restrict__ deg16) {
+// Out-degree of every row, as 32 bits (the prep's head-degree table).
+__global__ __launch_bounds__(kBlock) void local_deg_kernel(const int64_t* __restrict__ rp, int64_t n,
+                                                           int32_t* __restrict__ deg) {
+  for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < n; u += (int64_t)gridDim.x * kBlock)
+    deg[u] = (int32_t)(rp[u + 1] - rp[u]);
+}
+
+// Per-graph table of the per-part pass: for each DAG edge e = (u, v),
+// esrc[e] = u | (d+(u) < d+(v)) << 31 (the row, and whether the reference's
+// push/pull kernel pushes the edge to v's part when v is remote).  A wave
+// walks the edges of 64 consecutive rows as one flattened range (its lanes
+// stay busy whatever the row lengths; a lane finds its row by a binary
+// search over the rows' degree prefix in LDS); d+(v) is one 4-byte read of
+// the degree table dtab.
+__global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci,
+                                                            const int32_t* __restrict__ dtab, int64_t n,
+                                                            int32_t* __restrict__ esrc) {
   __shared__ int64_t s_e0[kBlock];
   __shared__ int32_t s_end[kBlock], s_du[kBlock];
   const int lane = threadIdx.x & 63, w0 = threadIdx.x & ~63;
@@ -2056,7 +2067,6 @@ __global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __res
     const int64_t e0 = rp[u];
     deg = (int32_t)(rp[u + 1] - e0);  // a row holds fewer than 2^31 / 64 ids
     s_e0[threadIdx.x] = e0;
-    deg16[u] = (uint16_t)min(deg, kDegSat);
   }
   int total;
   const int before = group_excl_scan<64>(deg, lane, total);
@@ -2065,18 +2075,11 @@ __global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __res
   wave_sync();
   for (int t = lane; t < total; t += 64) {
     const int r = w0 + find_edge<64>(s_end + w0, t);
-    esrc[s_e0[r] + (t - (s_end[r] - s_du[r]))] = (int32_t)(blockIdx.x * kBlock + r);
+    const int64_t e = s_e0[r] + (t - (s_end[r] - s_du[r]));
+    const int32_t v = ci[e];
+    const bool push = s_du[r] < dtab[v];
+    esrc[e] = (int32_t)((uint32_t)(blockIdx.x * kBlock + r) | (push ? 0x80000000u : 0u));
   }
-}
-
-// Part of every row under split points b[0..parts] (per call).
-__global__ __launch_bounds__(kBlock) void local_vpart_kernel(const int32_t* __restrict__ bounds, int32_t parts,
-                                                             int64_t n, uint16_t* __restrict__ vpart) {
-  __shared__ int32_t s_b[kMaxParts + 1];
-  for (int i = threadIdx.x; i <= parts; i += kBlock) s_b[i] = bounds[i];
-  __syncthreads();
-  for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < n; u += (int64_t)gridDim.x * kBlock)
-    vpart[u] = (uint16_t)part_of(s_b, parts, u);
 }
 
 // Every counted DAG edge e = (u, v) in [e0, e1) with count c goes to the
@@ -2084,41 +2087,60 @@ __global__ __launch_bounds__(kBlock) void local_vpart_kernel(const int32_t* __re
 // (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136): u's part when v
 // is in the same part or d+(u) >= d+(v) (pull), else v's part (push).
 // Uncounted edges hold c = 0.  Workgroups stream chunks of kPartsChunk
-// consecutive edges (u from esrc, parts from vpart, degrees from deg16;
-// two saturated degrees are compared on rp), and each lane sums runs of
-// edges with the same owner part before one LDS atomic.
+// consecutive edges, reading only streams: c, then u and the push bit from
+// esrc, v from ci (kPartsUnroll edges per lane at a time, all loads issued
+// before any is used).  Parts come from the split points in LDS: the lane
+// keeps its last row's part and range in registers (a chunk's rows are
+// consecutive), so only a pushed edge's head outside that range is
+// searched.  Each lane sums runs of edges with the same owner part before
+// one LDS atomic.
 constexpr int kPartsChunk = kBlock * 32;
-__global__ __launch_bounds__(kBlock) void local_parts_kernel(const int64_t* __restrict__ rp,
-                                                             const int32_t* __restrict__ ci,
+constexpr int kPartsUnroll = 4;
+__global__ __launch_bounds__(kBlock) void local_parts_kernel(const int32_t* __restrict__ ci,
                                                              const int32_t* __restrict__ esrc,
-                                                             const uint16_t* __restrict__ deg16,
-                                                             const uint16_t* __restrict__ vpart, int64_t e0,
-                                                             int64_t e1, const uint32_t* __restrict__ ecount,
-                                                             int32_t parts, unsigned long long* pe) {
+                                                             const int32_t* __restrict__ bounds, int32_t parts,
+                                                             int64_t e0, int64_t e1,
+                                                             const uint32_t* __restrict__ ecount,
+                                                             unsigned long long* pe) {
   __shared__ unsigned long long s_pe[kMaxParts];
+  __shared__ int32_t s_b[kMaxParts + 1];
   for (int i = threadIdx.x; i < parts; i += kBlock) s_pe[i] = 0;
+  for (int i = threadIdx.x; i <= parts; i += kBlock) s_b[i] = bounds[i];
   __syncthreads();
   int run_owner = 0;
   unsigned long long run = 0;
+  int pu = 0, lo = 0, hi = 0;  // the last row's part and its vertex range [lo, hi)
   for (int64_t b0 = e0 + (int64_t)blockIdx.x * kPartsChunk; b0 < e1; b0 += (int64_t)gridDim.x * kPartsChunk) {
     const int64_t b1 = min(b0 + (int64_t)kPartsChunk, e1);
-    for (int64_t e = b0 + threadIdx.x; e < b1; e += kBlock) {
-      const uint32_t c = ecount[e];
-      if (c == 0) continue;
-      const int u = esrc[e], v = ci[e];
-      const int pu = vpart[u], pv = vpart[v];
-      int owner = pu;
-      if (pv != pu) {
-        const int du = deg16[u], dv = deg16[v];
-        const bool push = (du == kDegSat && dv == kDegSat) ? (rp[u + 1] - rp[u] < rp[v + 1] - rp[v]) : du < dv;
-        owner = push ? pv : pu;
+    for (int64_t eb = b0 + threadIdx.x; eb < b1; eb += (int64_t)kBlock * kPartsUnroll) {
+      uint32_t c[kPartsUnroll], x[kPartsUnroll];
+      int v[kPartsUnroll];
+#pragma unroll
+      for (int j = 0; j < kPartsUnroll; ++j) {
+        const int64_t e = eb + (int64_t)j * kBlock;
+        const bool in = e < b1;
+        c[j] = in ? ecount[e] : 0u;
+        x[j] = in ? (uint32_t)esrc[e] : 0u;
+        v[j] = in ? ci[e] : 0;
       }
-      if (owner != run_owner) {
-        if (run) atomicAdd(&s_pe[run_owner], run);
-        run_owner = owner;
-        run = 0;
+#pragma unroll
+      for (int j = 0; j < kPartsUnroll; ++j) {
+        if (c[j] == 0) continue;
+        const int u = (int)(x[j] & 0x7fffffffu);
+        if (u < lo || u >= hi) {
+          pu = part_of(s_b, parts, u);
+          lo = s_b[pu];
+          hi = s_b[pu + 1];
+        }
+        int owner = pu;
+        if ((x[j] >> 31) && (v[j] < lo || v[j] >= hi)) owner = part_of(s_b, parts, v[j]);
+        if (owner != run_owner) {
+          if (run) atomicAdd(&s_pe[run_owner], run);
+          run_owner = owner;
+          run = 0;
+        }
+        run += c[j];
       }
-      run += c;
     }
   }
   if (run) atomicAdd(&s_pe[run_owner], run);
@@ -2373,22 +2395,23 @@ hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last,
   return hipGetLastError();
 }
 
-hipError_t launch_local_prep(const int64_t* rp, int64_t n, int32_t* esrc, uint16_t* deg16, hipStream_t s) {
+hipError_t launch_local_prep(const int64_t* rp, const int32_t* ci, int64_t n, int32_t* deg, int32_t* esrc,
+                             int cus, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  local_prep_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(rp, n, esrc, deg16);
+  const int64_t blocks = (n + kBlock - 1) / kBlock;
+  local_deg_kernel<<<(unsigned)min(blocks, (int64_t)cus * 8), kBlock, 0, s>>>(rp, n, deg);
+  local_prep_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(rp, ci, deg, n, esrc);
   return hipGetLastError();
 }
 
-hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, const int32_t* esrc, const uint16_t* deg16,
-                              uint16_t* vpart, int64_t n, int64_t e0, int64_t e1, const uint32_t* ecount,
-                              const int32_t* bounds, int32_t parts, unsigned long long* pe, int cus, hipStream_t s) {
+hipError_t launch_local_parts(const int32_t* ci, const int32_t* esrc, int64_t n, int64_t e0, int64_t e1,
+                              const uint32_t* ecount, const int32_t* bounds, int32_t parts, unsigned long long* pe,
+                              int cus, hipStream_t s) {
   if (parts < 1 || parts > kMaxParts || !bounds || !pe) return hipErrorInvalidValue;
   if (n <= 0 || e1 <= e0) return hipSuccess;
-  const int64_t vb = min((n + kBlock - 1) / kBlock, (int64_t)cus * 8);
-  local_vpart_kernel<<<(unsigned)vb, kBlock, 0, s>>>(bounds, parts, n, vpart);
   const int64_t chunks = (e1 - e0 + kPartsChunk - 1) / kPartsChunk;
   const int64_t pb = min(chunks, (int64_t)cus * 8);
-  local_parts_kernel<<<(unsigned)pb, kBlock, 0, s>>>(rp, ci, esrc, deg16, vpart, e0, e1, ecount, parts, pe);
+  local_parts_kernel<<<(unsigned)pb, kBlock, 0, s>>>(ci, esrc, bounds, parts, e0, e1, ecount, pe);
   return hipGetLastError();
 }
 

@@ -350,6 +350,18 @@ constexpr BinGroups parse_bin_groups(const char* t) {
 }
 constexpr BinGroups kBinOrder = parse_bin_groups(TC_BIN_ORDER);
 static_assert(kBinOrder.ok, "TC_BIN_ORDER: bins 0-5 once each, in at most 4 '|'-separated groups");
+// The groups of the edges-only local count (without vertex counts).  The
+// workgroup bin goes first on its own: the wave bin then waits for CUs
+// behind it and the 32-/16-lane bins, and runs at the end with the whole
+// device, beside the short 8-lane bin (RMAT-22 edge and part counts on
+// MI355X: 11.8 ms, against 12.2-12.6 ms with TC_BIN_ORDER, 11.9 ms with the
+// 8-lane bin after the wave bin).  The full local count keeps TC_BIN_ORDER,
+// which measured ~0.25 ms faster for it.
+#ifndef TC_EDGES_BIN_ORDER
+#define TC_EDGES_BIN_ORDER "1|02|3|45"
+#endif
+constexpr BinGroups kEdgesBinOrder = parse_bin_groups(TC_EDGES_BIN_ORDER);
+static_assert(kEdgesBinOrder.ok, "TC_EDGES_BIN_ORDER: bins 0-5 once each, in at most 4 '|'-separated groups");
 // own: null, or per row 1 if it is a pivot row (rows outside
 // [row_begin, row_end) are never pivots).
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
@@ -359,17 +371,20 @@ hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin,
 // first..last.  Without vertices only the per-edge counts are written (the
 // kernels keep no slot hit counters: less LDS, more waves per SIMD).
 hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, bool vertices, hipStream_t s);
-// Per-graph tables of launch_local_parts over rows [0, n): esrc[e] = the row
-// of DAG edge e, deg16[u] = out-degree of row u saturated at 65535.
-hipError_t launch_local_prep(const int64_t* rp, int64_t n, int32_t* esrc, uint16_t* deg16, hipStream_t s);
+// Per-graph table of launch_local_parts over rows [0, n): for DAG edge
+// e = (u, v), esrc[e] = u | (d+(u) < d+(v)) << 31.  deg: n words of
+// scratch (each row's out-degree, left there).
+hipError_t launch_local_prep(const int64_t* rp, const int32_t* ci, int64_t n, int32_t* deg, int32_t* esrc,
+                             int cus, hipStream_t s);
 // After launch_count_local: sums the ecount of every edge in [e0, e1) (the
 // counted rows' edges; uncounted ones hold 0) into pe[owner] by the
 // reference push/pull rule over the split bounds[0..parts] (1 <= parts <=
-// kMaxParts; pe zeroed by the caller).  vpart: n entries, written here.
+// kMaxParts, bounds[0] = 0, bounds[parts] = n, non-decreasing; pe zeroed by
+// the caller).
 constexpr int kMaxParts = 1024;
-hipError_t launch_local_parts(const int64_t* rp, const int32_t* ci, const int32_t* esrc, const uint16_t* deg16,
-                              uint16_t* vpart, int64_t n, int64_t e0, int64_t e1, const uint32_t* ecount,
-                              const int32_t* bounds, int32_t parts, unsigned long long* pe, int cus, hipStream_t s);
+hipError_t launch_local_parts(const int32_t* ci, const int32_t* esrc, int64_t n, int64_t e0, int64_t e1,
+                              const uint32_t* ecount, const int32_t* bounds, int32_t parts, unsigned long long* pe,
+                              int cus, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, int grid, hipStream_t s);
 // sink: grid * kBlock words (never written for a byte-filled source).
 hipError_t launch_read(const void* src, int64_t bytes, uint32_t* sink, int grid, hipStream_t s);

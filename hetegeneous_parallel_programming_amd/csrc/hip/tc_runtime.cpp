@@ -258,8 +258,6 @@ void LocalBufs::release() {
   dev_free(pe);
   dev_free(bounds);
   dev_free(esrc);
-  dev_free(deg16);
-  dev_free(vpart);
   *this = LocalBufs();
 }
 
@@ -873,14 +871,14 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
     st = dev_alloc((void**)&L.vcount, sizeof(unsigned long long) * ((size_t)g->n + 1), "vertex counts");
   if (st == TC_OK) st = dev_alloc((void**)&L.pe, sizeof(unsigned long long) * tck::kMaxParts, "part counts");
   if (st == TC_OK) st = dev_alloc((void**)&L.bounds, sizeof(int32_t) * (tck::kMaxParts + 1), "split points");
-  const int64_t nr = std::max<int64_t>(g->n, 1);
   if (st == TC_OK) st = dev_alloc((void**)&L.esrc, sizeof(int32_t) * (size_t)m, "edge rows");
-  if (st == TC_OK) st = dev_alloc((void**)&L.deg16, sizeof(uint16_t) * (size_t)nr, "row degrees");
-  if (st == TC_OK) st = dev_alloc((void**)&L.vpart, sizeof(uint16_t) * (size_t)nr, "row parts");
   if (st == TC_OK) {
     tck::LaunchArgs a = count_args(ctx, g);
     BuildScratch bs;
-    hipError_t e = tck::launch_local_prep(g->rp, g->n, L.esrc, L.deg16, ctx->stream);  // the rebuild reads esrc
+    // the rebuild reads esrc; vcount (zeroed by every call that counts
+    // vertices) holds the prep's row degrees meanwhile
+    hipError_t e = tck::launch_local_prep(g->rp, g->ci, g->n, reinterpret_cast<int32_t*>(L.vcount), L.esrc,
+                                          ctx->cus, ctx->stream);
     if (e == hipSuccess) st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     bs.release();
@@ -892,7 +890,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
     return st;
   }
   g->plan_valid = false;  // the rebuild rewrote the partner rows the kept plan lists
-  L.bytes = 20 * m + 4 * P + 8 * ((int64_t)g->n + 1) + 4 * nr + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
+  L.bytes = 20 * m + 4 * P + 8 * ((int64_t)g->n + 1) + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
   return TC_OK;
 }
 
@@ -932,7 +930,7 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
   {  // bin groups on the library stream and the side streams, as in tc_count
-    const tck::BinGroups& G = tck::kBinOrder;
+    const tck::BinGroups& G = vertices ? tck::kBinOrder : tck::kEdgesBinOrder;
     for (int gi = 0; gi < G.n; ++gi) {
       hipStream_t gs = gi == 0 ? s : ctx->side[gi - 1];
       if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs, ctx->ev[1], 0));
@@ -943,8 +941,8 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
     for (int gi = 1; gi < G.n; ++gi) TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[gi - 1], 0));
   }
   if (k > 0)
-    TC_HIP_TRY(tck::launch_local_parts(g->rp, g->ci, L.esrc, L.deg16, L.vpart, g->n, g->edge_begin, g->edge_end,
-                                       L.ecount, L.bounds, k, L.pe, ctx->cus, s));
+    TC_HIP_TRY(tck::launch_local_parts(g->ci, L.esrc, g->n, g->edge_begin, g->edge_end, L.ecount, L.bounds, k, L.pe,
+                                       ctx->cus, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, a.bufs.outs, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipMemcpyAsync(ctx->pinned + 8, a.bufs.counters, sizeof(uint32_t) * tck::kNumCounters,

@@ -852,6 +852,7 @@ __global__ __launch_bounds__(kBlock) void partner_fill_kernel(LaunchArgs a, int6
       if (b.pedge) {  // both ways it is edge (u, v)
         b.pedge[d.pci_off + slot] = d.ci_off + e;
         b.pvid[d.pci_off + slot] = own ? v : (int32_t)u;
+        if (b.esrc) b.esrc[d.ci_off + e] = (int32_t)((uint32_t)u | (own ? 0u : 0x80000000u));
       }
     }
     w += __popc(gm);
@@ -957,7 +958,8 @@ __global__ __launch_bounds__(kBlock) void partner_work_kernel(LaunchArgs a) {
 //              blocks of kMP items;
 //   split   -- per block of kMP items, whatever the row lengths: each id's
 //              pair (an out-list degree gather decides the owner), and the
-//              block's pair count per coarse bucket;
+//              block's pair count per coarse bucket (a local rebuild also
+//              marks each edge's row word in esrc where the head owns it);
 //   level 1 -- each split block's pairs sorted by coarse bucket in LDS and
 //              written as one run per bucket, at positions from the scanned
 //              (bucket-major) counts;
@@ -1178,6 +1180,8 @@ __global__ __launch_bounds__(kBlock) void radix_split_kernel(LaunchArgs a, int64
         b.ref[E] = list_ref(cix[s] - (E - ebase - s_erow[r]), du);
       }
       b.key[E] = owner;
+      // a local rebuild (one graph): the edge's row word, bit 31 when the head owns it
+      if (b.esrc) b.esrc[cix[s]] = (int32_t)((uint32_t)(q - rb0) | (owner == q ? 0u : 0x80000000u));
       digit = owner >> cs;
     }
     wave_count_add<2>(s_hist, digit);
@@ -1611,8 +1615,7 @@ __global__ __launch_bounds__(kBlock) void radix_place_big_kernel(LaunchArgs a, B
 // Radix path of a local rebuild (one graph): the partner vertex of every
 // entry of partner row k, from the entry's DAG id position e (pedge): the
 // DAG edge is (esrc[e] without bit 31, ci[e]); the partner is the end that
-// is not the row's
-// owner pown[k].  One wave per row, grid-stride.
+// is not the row's owner pown[k].  One wave per row, grid-stride.
 __global__ __launch_bounds__(kBlock) void partner_vid_kernel(LaunchArgs a, int64_t P, BuildBufs b) {
   const int lane = threadIdx.x & 63;
   const int64_t w0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6, nw = ((int64_t)gridDim.x * kBlock) >> 6;
@@ -1622,7 +1625,7 @@ __global__ __launch_bounds__(kBlock) void partner_vid_kernel(LaunchArgs a, int64
     const int64_t i0 = a.vrp[k], i1 = a.vrp[k + 1];
     for (int64_t i = i0 + lane; i < i1; i += 64) {
       const int64_t e = b.pedge[i];
-      const int32_t r = b.esrc[e] & 0x7fffffff;  // bit 31: the per-part pass's push bit
+      const int32_t r = b.esrc[e] & 0x7fffffff;
       b.pvid[i] = r == x ? a.ci[e] : r;
     }
   }
@@ -2040,23 +2043,14 @@ __device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
   return lo;
 }
 
-// Out-degree of every row, as 32 bits (the prep's head-degree table).
-__global__ __launch_bounds__(kBlock) void local_deg_kernel(const int64_t* __restrict__ rp, int64_t n,
-                                                           int32_t* __restrict__ deg) {
-  for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < n; u += (int64_t)gridDim.x * kBlock)
-    deg[u] = (int32_t)(rp[u + 1] - rp[u]);
-}
-
-// Per-graph table of the per-part pass: for each DAG edge e = (u, v),
-// esrc[e] = u | (d+(u) < d+(v)) << 31 (the row, and whether the reference's
-// push/pull kernel pushes the edge to v's part when v is remote).  A wave
-// walks the edges of 64 consecutive rows as one flattened range (its lanes
-// stay busy whatever the row lengths; a lane finds its row by a binary
-// search over the rows' degree prefix in LDS); d+(v) is one 4-byte read of
-// the degree table dtab.
-__global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __restrict__ rp,
-                                                            const int32_t* __restrict__ ci,
-                                                            const int32_t* __restrict__ dtab, int64_t n,
+// Per-graph table of the per-part pass, first half: esrc[e] = the row u of
+// each DAG edge e = (u, v).  The local rebuild then sets bit 31 where v owns
+// the edge, i.e. d+(u) < d+(v) (the owner rule), which is when the
+// reference's push/pull kernel pushes the edge to v's part.  A wave walks
+// the edges of 64 consecutive rows as one flattened range (its lanes stay
+// busy whatever the row lengths; a lane finds its row by a binary search
+// over the rows' degree prefix in LDS).
+__global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __restrict__ rp, int64_t n,
                                                             int32_t* __restrict__ esrc) {
   __shared__ int64_t s_e0[kBlock];
   __shared__ int32_t s_end[kBlock], s_du[kBlock];
@@ -2075,10 +2069,7 @@ __global__ __launch_bounds__(kBlock) void local_prep_kernel(const int64_t* __res
   wave_sync();
   for (int t = lane; t < total; t += 64) {
     const int r = w0 + find_edge<64>(s_end + w0, t);
-    const int64_t e = s_e0[r] + (t - (s_end[r] - s_du[r]));
-    const int32_t v = ci[e];
-    const bool push = s_du[r] < dtab[v];
-    esrc[e] = (int32_t)((uint32_t)(blockIdx.x * kBlock + r) | (push ? 0x80000000u : 0u));
+    esrc[s_e0[r] + (t - (s_end[r] - s_du[r]))] = (int32_t)(blockIdx.x * kBlock + r);
   }
 }
 
@@ -2395,12 +2386,9 @@ hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last,
   return hipGetLastError();
 }
 
-hipError_t launch_local_prep(const int64_t* rp, const int32_t* ci, int64_t n, int32_t* deg, int32_t* esrc,
-                             int cus, hipStream_t s) {
+hipError_t launch_local_prep(const int64_t* rp, int64_t n, int32_t* esrc, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int64_t blocks = (n + kBlock - 1) / kBlock;
-  local_deg_kernel<<<(unsigned)min(blocks, (int64_t)cus * 8), kBlock, 0, s>>>(rp, n, deg);
-  local_prep_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(rp, ci, deg, n, esrc);
+  local_prep_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(rp, n, esrc);
   return hipGetLastError();
 }
 

@@ -209,10 +209,12 @@ struct BuildBufs {
   // Radix path with pedge (a local rebuild of one graph): each pair's DAG id
   // position rides along the two levels (e1, e2) and lands in pedge beside
   // the partner's list; esrc[e] = the row of DAG id e gives the partner
-  // vertices (pvid).  Null otherwise.
+  // vertices (pvid).  Either path of a local rebuild (the radix split, or
+  // the atomic fill) sets bit 31 of esrc[e] where the edge's head owns it.
+  // Null otherwise.
   uint32_t* e1;         // [M] level-1 order of the pairs' id positions
   uint32_t* e2;         // [M] level-2 (fine-bucket) order
-  const int32_t* esrc;  // [M]
+  int32_t* esrc;        // [M]
   int64_t rows;         // capacity: row slots
   int64_t ids;          // capacity: ids
 };
@@ -371,11 +373,10 @@ hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin,
 // first..last.  Without vertices only the per-edge counts are written (the
 // kernels keep no slot hit counters: less LDS, more waves per SIMD).
 hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, bool vertices, hipStream_t s);
-// Per-graph table of launch_local_parts over rows [0, n): for DAG edge
-// e = (u, v), esrc[e] = u | (d+(u) < d+(v)) << 31.  deg: n words of
-// scratch (each row's out-degree, left there).
-hipError_t launch_local_prep(const int64_t* rp, const int32_t* ci, int64_t n, int32_t* deg, int32_t* esrc,
-                             int cus, hipStream_t s);
+// Per-graph table of launch_local_parts over rows [0, n): esrc[e] = the row
+// u of DAG edge e = (u, v); the local rebuild (launch_build with pedge and
+// esrc) then sets bit 31 where v owns the edge, i.e. d+(u) < d+(v).
+hipError_t launch_local_prep(const int64_t* rp, int64_t n, int32_t* esrc, hipStream_t s);
 // After launch_count_local: sums the ecount of every edge in [e0, e1) (the
 // counted rows' edges; uncounted ones hold 0) into pe[owner] by the
 // reference push/pull rule over the split bounds[0..parts] (1 <= parts <=

@@ -875,10 +875,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
   if (st == TC_OK) {
     tck::LaunchArgs a = count_args(ctx, g);
     BuildScratch bs;
-    // the rebuild reads esrc; vcount (zeroed by every call that counts
-    // vertices) holds the prep's row degrees meanwhile
-    hipError_t e = tck::launch_local_prep(g->rp, g->ci, g->n, reinterpret_cast<int32_t*>(L.vcount), L.esrc,
-                                          ctx->cus, ctx->stream);
+    hipError_t e = tck::launch_local_prep(g->rp, g->n, L.esrc, ctx->stream);  // the rebuild reads and marks esrc
     if (e == hipSuccess) st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     bs.release();

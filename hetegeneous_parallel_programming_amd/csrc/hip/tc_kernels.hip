@@ -437,6 +437,12 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
       src(j, b, c);
       base = (gint32*)ci + b;
     }
+    // kLocal: the batch edge's DAG position, loaded before the batch is
+    // streamed so that the store after it does not wait for the load
+    // (RMAT-22 edges-only local count ~0.05 ms faster on MI355X)
+    int64_t ej = 0;
+    if constexpr (kLocal)
+      if (j < da) ej = ls.pedge[j];
     int total;
     const int excl = group_excl_scan<G>(c, gl, total);
     sEnd[gl] = excl + c;
@@ -563,7 +569,7 @@ __device__ __forceinline__ uint32_t pivot_edges(const int32_t* __restrict__ ci, 
       wave_sync();
       if (j < da) {
         const uint32_t c = ls.EC[gl];
-        ls.ecount[ls.pedge[j]] = c;
+        ls.ecount[ej] = c;
         if constexpr (kLocal == kLocalFull)
           if (c) atomicAdd(&ls.vcount[ls.pvid[j]], (unsigned long long)c);
       }

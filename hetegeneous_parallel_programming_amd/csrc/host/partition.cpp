@@ -61,6 +61,22 @@ tc_status tch_partition_range(const tc_csr_view* dag, int32_t k, int32_t part, t
   return tch::caught();
 }
 
+// PARTNER weights, in streamed-id equivalents scaled by kWeightOne (fixed
+// point, so that every platform finds the same split): per DAG edge the ids
+// the partner scheme streams, min(d+(u), d+(v)), plus kPartnerEdgeIds for
+// its list reference, and its share of its owner's partner rows
+// (ceil(c / TCH_PARTNERS_PER_ROW) rows over the owner's c edges), each
+// row costing kPartnerRowIds (the row record, the owner's set built in LDS,
+// the plan entry).  The two constants are a least-squares fit of the count
+// kernel times of the eight parts of RMAT scale 24's 4-way vertex and
+// partner splits on MI355X (0.52 ms per G streamed ids, 0.78 ms per M
+// partner rows; largest error 0.34 ms); they took the slowest part of the
+// partner split from 10.95 to 10.52 ms against edge weights of 2 ids and no
+// row term.
+constexpr int64_t kWeightOne = 1 << 16;
+constexpr int64_t kPartnerEdgeIds = 6;
+constexpr int64_t kPartnerRowIds = 1503;
+
 tc_status tch_partition_bounds(const tc_csr_view* dag, int32_t k, int32_t balance, int32_t* bounds) try {
   if (!dag || !bounds) return fail(TC_EINVAL, "null argument");
   if (k < 1) return fail(TC_EINVAL, "need k >= 1");
@@ -72,21 +88,43 @@ tc_status tch_partition_bounds(const tc_csr_view* dag, int32_t k, int32_t balanc
     return TC_OK;
   }
   const int64_t* rp = dag->rowptr;
+  const int32_t* ci = dag->colidx;
+  // PARTNER: the edges each vertex owns (the endpoint with the longer
+  // out-list, the row on a tie), for each edge's share of its owner's
+  // partner rows.
+  std::unique_ptr<int32_t[]> own;
+  if (balance == TCH_BALANCE_PARTNER) {
+    own.reset(new int32_t[(size_t)n + 1]());
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t u = 0; u < n; ++u) {
+      const int64_t du = rp[u + 1] - rp[u];
+      for (int64_t x = rp[u]; x < rp[u + 1]; ++x) {
+        const int32_t v = ci[x];
+        const int32_t o = rp[v + 1] - rp[v] > du ? v : (int32_t)u;
+        __atomic_fetch_add(&own[o], 1, __ATOMIC_RELAXED);
+      }
+    }
+  }
   std::unique_ptr<int64_t[]> w(new int64_t[(size_t)n + 1]);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t u = 0; u < n; ++u) {
     const int64_t du = rp[u + 1] - rp[u];
     int64_t c = du;
-    if (balance != TCH_BALANCE_EDGES) {
-      c = (balance == TCH_BALANCE_WEDGES) ? du : 2 * du;
+    if (balance == TCH_BALANCE_WEDGES) {
+      for (int64_t x = rp[u]; x < rp[u + 1]; ++x) c += rp[ci[x] + 1] - rp[ci[x]];
+    } else if (balance == TCH_BALANCE_PARTNER) {
+      c = 0;
       for (int64_t x = rp[u]; x < rp[u + 1]; ++x) {
-        const int32_t v = dag->colidx[x];
+        const int32_t v = ci[x];
         const int64_t dv = rp[v + 1] - rp[v];
-        c += (balance == TCH_BALANCE_WEDGES) ? dv : std::min(du, dv);
+        const int64_t oc = own[dv > du ? v : (int32_t)u];
+        c += (std::min(du, dv) + kPartnerEdgeIds) * kWeightOne +
+             kPartnerRowIds * kWeightOne * ((oc + TCH_PARTNERS_PER_ROW - 1) / TCH_PARTNERS_PER_ROW) / oc;
       }
     }
     w[u] = c;
   }
+  own.reset();
   std::unique_ptr<int64_t[]> pre(new int64_t[(size_t)n + 1]);
   exclusive_scan(w.get(), n, pre.get());
   w.reset();

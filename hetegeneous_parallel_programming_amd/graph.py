@@ -283,7 +283,8 @@ class PartitionedGraph:
     ``balance`` picks the split points of :meth:`halo_partition`: "vertices"
     is the reference's ceil(n/k) split (graph_partition.cc:88-107); "edges",
     "wedges" and "partner" balance d+(u), the edge-iterator elements or the
-    partner-scheme ints of the owned rows (``tch_partition_bounds``).  The
+    partner-scheme cost of the owned rows (streamed ids plus per-edge and
+    per-partner-row costs measured on MI355X; ``tch_partition_bounds``).  The
     vertex-induced subgraphs of :meth:`edgecut_induced_partition1D` always
     use the reference split."""
 

@@ -123,8 +123,12 @@ tc_status tch_partition_range(const tc_csr_view* dag, int32_t k, int32_t part,
  *   EDGES     d+(u)
  *   WEDGES    d+(u) + sum over v in N+(u) of d+(v)   (the elements an
  *             edge-iterator count streams, bs_warp_vertex.cuh:13-22)
- *   PARTNER   sum over v in N+(u) of 2 + min(d+(u), d+(v))  (the ints the
- *             partner-scheme count reads for the row's edges)
+ *   PARTNER   sum over v in N+(u) of 65536 * (6 + min(d+(u), d+(v)))
+ *             + floor(65536 * 1503 * ceil(c_o / TCH_PARTNERS_PER_ROW) / c_o),
+ *             c_o = the edges the edge's owner o owns (o = v when
+ *             d+(v) > d+(u), else u): the ids the partner-scheme count
+ *             streams, plus per edge and per partner row the costs measured
+ *             on MI355X, in streamed-id equivalents (fixed point)
  * and each split point is the row where the prefix of weights is nearest to
  * p/k of the total.  No counterpart in the reference, whose vertex split puts
  * about half of an RMAT graph's edges into part 0. */

@@ -192,8 +192,28 @@ def _row_weights(rp, ci, balance):
         return deg
     if balance == "wedges":
         return deg + np.bincount(u, weights=dv, minlength=deg.size).astype(np.int64)
-    return 2 * deg + np.bincount(u, weights=np.minimum(deg[u], dv),
-                                 minlength=deg.size).astype(np.int64)
+    # partner: streamed ids + 6 per edge + 1503 per partner row, shared by
+    # the owner's edges; fixed point x 2^16, summed per row exactly
+    owner = np.where(dv > deg[u], ci, u)
+    oc = np.bincount(owner, minlength=deg.size).astype(np.int64)[owner]
+    per_edge = ((np.minimum(deg[u], dv) + 6) << 16) + (1503 << 16) * ((oc + 1023) // 1024) // oc
+    pre = np.concatenate([[0], np.cumsum(per_edge, dtype=np.int64)])
+    return pre[rp[1:]] - pre[rp[:-1]]
+
+
+def test_partner_weights_share_each_owners_rows():
+    """Each owner's partner rows, shared over the edges it owns, add up to
+    the partner rows of the library's work model (Graph.work, the same count
+    the device build makes), up to the fixed point's floor per edge."""
+    d = Graph.rmat(1 << 14, 16, 5).oriented()
+    rp, ci = d.out_rowptr(), d.out_colidx()
+    deg = np.diff(rp).astype(np.int64)
+    u = np.repeat(np.arange(rp.size - 1), deg)
+    dv = deg[ci]
+    base = ((np.minimum(deg[u], dv) + 6) << 16).sum()
+    rows_q = int(_row_weights(rp, ci, "partner").sum() - base)
+    want = d.work()["partner_rows"] * (1503 << 16)
+    assert want - ci.size <= rows_q <= want
 
 
 @pytest.mark.parametrize("balance", ["vertices", "edges", "wedges", "partner"])

@@ -256,7 +256,8 @@ def local_counts(ctx, g, dg, total, steps=3, warmup=1):
     """tc_count_local on the resident graph: per-edge, per-vertex and per-PE
     attribution, timed beside the binned count, with the reference push/pull
     per-PE sums of a 4-way vertex split and of a partner-balanced split; then
-    the same without vertex counts (edge and part counts only).
+    the same without vertex counts (edge and part counts only), and the part
+    counts alone.
     Size-independent checks: edge counts sum to the total, vertex counts to
     three times it, part counts to it; the edges-only outputs equal the full
     call's."""
@@ -299,6 +300,17 @@ def local_counts(ctx, g, dg, total, steps=3, warmup=1):
         out[balance]["edges_only"] = {"kernel_ms_avg": ke_avg, "ms_per_step": ms_e / steps, "exact": exact_e}
         log(f"local ({balance} split), edge and part counts only: kernel={ke_avg:.3f} ms, "
             f"same as with vertex counts={exact_e}")
+        # the per-PE sums alone (what the reference's push/pull kernel
+        # reports): each partner row's hits go to its owner's part
+        ms_p, outs_p = time_steps(lambda: ctx.count_local(dg, edges=False, vertices=False, bounds=bounds),
+                                  steps, warmup)
+        op = outs_p[-1][1]
+        exact_p = (all(x[0] == total for x in outs_p)
+                   and np.array_equal(op["part_counts"], o["part_counts"]))
+        kp_avg = float(np.mean([x[1]["stats"]["count_ms"] for x in outs_p]))
+        out[balance]["parts_only"] = {"kernel_ms_avg": kp_avg, "ms_per_step": ms_p / steps, "exact": exact_p}
+        log(f"local ({balance} split), part counts only: kernel={kp_avg:.3f} ms, "
+            f"same as with vertex counts={exact_p}")
     out["max_vertex_triangles"] = int(o["vertex_counts"].max())
     return out
 

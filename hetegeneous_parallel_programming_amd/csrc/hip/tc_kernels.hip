@@ -244,11 +244,10 @@ __device__ __forceinline__ void group_sync() {
   else wave_sync();
 }
 
-// Attribution of a count (the kLocal parameter of the count kernels):
-// none; every triangle to its DAG edge, its three corners (slot hit counters
-// for the third) and, through the edges, the parts of a split (full); or to
-// its DAG edge only, which is all the per-part sums need (edges).
-constexpr int kLocalNone = 0, kLocalFull = 1, kLocalEdges = 2;
+// Attribution of a count: the kLocal parameter of the count kernels
+// (kLocalNone .. kLocalParts, declared with LocalOut).  Inside a row's
+// intersections the parts mode works as the plain count does.
+constexpr int edge_local(int kLocal) { return kLocal == kLocalParts ? kLocalNone : kLocal; }
 
 // Builds the set of N+(u) (src[0..da), sorted) in `slot` (area words) with
 // the G lanes of a group (gl = lane index in the group): a cuckoo hash when
@@ -1854,6 +1853,29 @@ static_assert(kCapG8 <= kAreaG8 && kCapG16 <= kAreaG16 && kCapG32 <= kAreaG32 &&
 
 // Per-lane running count; in batch mode it is flushed to outs[g] whenever
 // the lane moves to a pivot of another graph.
+// Part of vertex u under split points b[0..k] (b[0] = 0 <= u < b[k]): the
+// last p with b[p] <= u, which skips empty parts.
+__device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
+  int lo = 0, hi = k;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (b[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Part of a partner row's owner under the call's split (kLocalParts).  The
+// PE that computes a DAG edge in the reference's push/pull kernel
+// (successful_push_pull/bs_warp_vertex_nvshmem.cuh:127-136: u's part when v
+// is local or d+(u) >= d+(v), else v's) is always its owner's part: the
+// owner is u exactly when d+(v) <= d+(u), and when v owns it u and v are in
+// one part or the edge is pushed to v's.  So a row's hits all go to one
+// part.
+__device__ __forceinline__ int owner_part(const LaunchArgs& a, int64_t item) {
+  return part_of(a.local.bounds, a.local.parts, a.local.pown[item]);
+}
+
 struct Acc {
   uint64_t sum = 0;
   int g = -1;
@@ -1898,15 +1920,16 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
   int32_t* slot = L.A + grp * AREA;
   int32_t* sEnd = L.End + grp * G;
   gint32** sBase = L.Base + grp * G;
+  constexpr int kEL = edge_local(kLocal);
   uint32_t* C = kLocal == kLocalFull ? LL.c() + grp * (AREA / 2) : nullptr;
   for (int64_t i = first + grp; i < last; i += NGRP) {
     const int64_t item = (int64_t)items[i];
     const PRow pr = partner_row(a, item);
-    const PivotSet S = build_pivot_set<G, kLocal, kWide>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set),
-                                                         gl, a.tune, a.bufs.counters + kCntFallback, C);
-    const LocalSink ls{C, kLocal ? LL.ec() + grp * G : nullptr, kLocal ? a.local.pedge + pr.pb : nullptr,
-                       kLocal ? a.local.pvid + pr.pb : nullptr, a.local.ecount, a.local.vcount};
-    const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kLocal, kWide>(
+    const PivotSet S = build_pivot_set<G, kEL, kWide>(slot, AREA, a.ci + list_start(pr.set), list_len(pr.set),
+                                                      gl, a.tune, a.bufs.counters + kCntFallback, C);
+    const LocalSink ls{C, kEL ? LL.ec() + grp * G : nullptr, kEL ? a.local.pedge + pr.pb : nullptr,
+                       kEL ? a.local.pvid + pr.pb : nullptr, a.local.ecount, a.local.vcount};
+    const uint32_t h = pivot_edges<G, kModeHash | kModeTree, G == 64 ? kInFlight : kInFlightGroup, kEL, kWide>(
         a.ci, RefEdges{a.pci + pr.pb}, (int)(pr.pe - pr.pb), S, 0, G, sEnd, sBase, gl, ls);
     if constexpr (kLocal == kLocalFull) {
       group_sync<G>();
@@ -1914,7 +1937,8 @@ __device__ __forceinline__ void group_tile(const LaunchArgs& a, const uint64_t* 
       const uint32_t hr = group_sum<G>(h);  // the row's triangles: each has the owner as a corner
       if (gl == 0 && hr) atomicAdd(&a.local.vcount[a.local.pown[item]], (unsigned long long)hr);
     }
-    acc.to_graph(pr.g, batch, a.bufs.outs);
+    if constexpr (kLocal == kLocalParts) acc.to_graph(owner_part(a, item), true, a.local.pe);
+    else acc.to_graph(pr.g, batch, a.bufs.outs);
     acc.sum += h;
     wave_sync();
   }
@@ -1930,17 +1954,18 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   const RefEdges part{a.pci + pr.pb};
   const int np = (int)(pr.pe - pr.pb);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint32_t* EC = kLocal ? LL.ec() + wid * 64 : nullptr;
-  const int64_t* pedge = kLocal ? a.local.pedge + pr.pb : nullptr;
+  constexpr int kEL = edge_local(kLocal);
+  uint32_t* EC = kEL ? LL.ec() + wid * 64 : nullptr;
+  const int64_t* pedge = kEL ? a.local.pedge + pr.pb : nullptr;
   const int32_t* pvid = kLocal == kLocalFull ? a.local.pvid + pr.pb : nullptr;
   uint32_t h;
   if (kLdsPivot) {
-    const PivotSet S = build_pivot_set<kBlock, kLocal, kWide>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune,
-                                                              a.bufs.counters + kCntFallback, LL.c());
+    const PivotSet S = build_pivot_set<kBlock, kEL, kWide>(L.A, kAreaBlock, set, ds, threadIdx.x, a.tune,
+                                                           a.bufs.counters + kCntFallback, LL.c());
     const LocalSink ls{LL.c(), EC, pedge, pvid, a.local.ecount, a.local.vcount};
-    h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kLocal, kWide>(ci, part, np, S, wid * 64, kBlock,
-                                                                          L.End + wid * 64, L.Base + wid * 64, lane,
-                                                                          ls);
+    h = pivot_edges<64, kModeHash | kModeTree, kInFlight, kEL, kWide>(ci, part, np, S, wid * 64, kBlock,
+                                                                       L.End + wid * 64, L.Base + wid * 64, lane,
+                                                                       ls);
     if constexpr (kLocal == kLocalFull) {
       __syncthreads();
       flush_set_hits<kBlock>(L.A, LL.c(), S, threadIdx.x, a.local.vcount);
@@ -1948,14 +1973,15 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
   } else {
     const PivotSet S{set, kSetHbm, ds, 0};
     const LocalSink ls{nullptr, EC, pedge, pvid, a.local.ecount, a.local.vcount};
-    h = pivot_edges<64, kModeHbm, kInFlight, kLocal>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
-                                                     L.Base + wid * 64, lane, ls);
+    h = pivot_edges<64, kModeHbm, kInFlight, kEL>(ci, part, np, S, wid * 64, kBlock, L.End + wid * 64,
+                                                  L.Base + wid * 64, lane, ls);
   }
   if constexpr (kLocal == kLocalFull) {  // the owner's share, one atomic per wave
     const uint32_t hw = group_sum<64>(h);
     if (lane == 0 && hw) atomicAdd(&a.local.vcount[a.local.pown[item]], (unsigned long long)hw);
   }
-  acc.to_graph(pr.g, batch, a.bufs.outs);
+  if constexpr (kLocal == kLocalParts) acc.to_graph(owner_part(a, (int64_t)item), true, a.local.pe);
+  else acc.to_graph(pr.g, batch, a.bufs.outs);
   acc.sum += h;
 }
 
@@ -1973,7 +1999,7 @@ __device__ __forceinline__ void block_pivot(const LaunchArgs& a, uint64_t item, 
 constexpr int kLocalWavesPerEU = 5;
 constexpr int waves_per_eu(int kLocal) {
   return kLocal == kLocalFull ? kLocalWavesPerEU
-       : kLocal == kLocalEdges ? TC_LOCAL_EDGES_WAVES_PER_EU : TC_COUNT_WAVES_PER_EU;
+       : kLocal == kLocalEdges ? TC_LOCAL_EDGES_WAVES_PER_EU : TC_COUNT_WAVES_PER_EU;  // parts: the plain count's
 }
 
 // One persistent kernel per bin, specialised so that each keeps only its own
@@ -1986,8 +2012,9 @@ __global__ __launch_bounds__(kBlock, waves_per_eu(kLocal)) void count_kernel(
     LaunchArgs a) {
   __shared__ CountLds L;
   __shared__ LocalLds<kLocal> LL;
-  const bool batch = !kLocal && a.ngraphs > 1;
-  if constexpr (kLocal) LL.EC[threadIdx.x] = 0;  // ordered by the barrier below
+  // batch: per-graph sums; kLocalParts: per-part sums, kept the same way
+  const bool batch = (!kLocal && a.ngraphs > 1) || kLocal == kLocalParts;
+  if constexpr (edge_local(kLocal) != kLocalNone) LL.EC[threadIdx.x] = 0;  // ordered by the barrier below
   if (threadIdx.x == 0) {
     int64_t seg = 0;
     for (int b = 0; b < BIN; ++b) seg += a.bufs.counters[kCntBinCount + b];
@@ -2015,14 +2042,16 @@ __global__ __launch_bounds__(kBlock, waves_per_eu(kLocal)) void count_kernel(
     else if (BIN == BIN_G16) group_tile<16, kAreaG16, kLocal, kWide>(a, items, first, last, L, LL, batch, acc);
     else group_tile<8, kAreaG8, kLocal, kWide>(a, items, first, last, L, LL, batch, acc);
     if (batch) {
-      // Items are ordered by graph, so a tile almost always belongs to one
-      // graph: then one block-reduced atomic; otherwise one per lane.
-      const int tg = a.vrow[items[first]].g;
+      // Items are ordered by graph (parts: by owner, so by part), so a tile
+      // almost always belongs to one: then one block-reduced atomic;
+      // otherwise one per lane.
+      uint64_t* outs = kLocal == kLocalParts ? a.local.pe : a.bufs.outs;
+      const int tg = kLocal == kLocalParts ? owner_part(a, (int64_t)items[first]) : a.vrow[items[first]].g;
       if (!__syncthreads_or(acc.sum != 0 && acc.g != tg)) {
         const uint64_t tot = block_sum(acc.sum, L.red);
-        if (threadIdx.x == 0 && tot) atomicAdd(&a.bufs.outs[tg], tot);
+        if (threadIdx.x == 0 && tot) atomicAdd(&outs[tg], tot);
       } else if (acc.sum) {
-        atomicAdd(&a.bufs.outs[acc.g], acc.sum);
+        atomicAdd(&outs[acc.g], acc.sum);
       }
       acc.sum = 0;
     }
@@ -2037,17 +2066,6 @@ __global__ __launch_bounds__(kBlock, waves_per_eu(kLocal)) void count_kernel(
 // ---------------------------------------------------------------------------
 // local count, second pass: per-edge counts to vertices and parts
 // ---------------------------------------------------------------------------
-// Part of vertex u under split points b[0..k] (b[0] = 0 <= u < b[k]): the
-// last p with b[p] <= u, which skips empty parts.
-__device__ __forceinline__ int part_of(const int32_t* b, int k, int64_t u) {
-  int lo = 0, hi = k;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (b[mid] <= u) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // Per-graph table of the per-part pass, first half: esrc[e] = the row u of
 // each DAG edge e = (u, v).  The local rebuild then sets bit 31 where v owns
@@ -2384,11 +2402,22 @@ static void launch_local_bins(const LaunchArgs& a, int cus, int first, int last,
   if (first <= BIN_G8 && BIN_G8 <= last) launch_bin<BIN_G8, kLocal>(a, cus, 0, s);
 }
 
-hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, bool vertices, hipStream_t s) {
-  if (a.ngraphs != 1 || !a.local.pedge || !a.local.ecount) return hipErrorInvalidValue;
-  if (vertices && (!a.local.pvid || !a.local.pown || !a.local.vcount)) return hipErrorInvalidValue;
-  if (vertices) launch_local_bins<kLocalFull>(a, cus, first, last, s);
-  else launch_local_bins<kLocalEdges>(a, cus, first, last, s);
+hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, int mode, hipStream_t s) {
+  if (a.ngraphs != 1 || !a.local.pown) return hipErrorInvalidValue;
+  if (mode == kLocalParts) {
+    if (!a.local.bounds || !a.local.pe || a.local.parts < 1 || a.local.parts > kMaxParts) return hipErrorInvalidValue;
+    launch_local_bins<kLocalParts>(a, cus, first, last, s);
+    return hipGetLastError();
+  }
+  if (!a.local.pedge || !a.local.ecount) return hipErrorInvalidValue;
+  if (mode == kLocalFull) {
+    if (!a.local.pvid || !a.local.vcount) return hipErrorInvalidValue;
+    launch_local_bins<kLocalFull>(a, cus, first, last, s);
+  } else if (mode == kLocalEdges) {
+    launch_local_bins<kLocalEdges>(a, cus, first, last, s);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

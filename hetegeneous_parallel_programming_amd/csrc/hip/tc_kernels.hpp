@@ -134,12 +134,22 @@ struct VRow {
 // partner pool); pown: owner vertex of every partner row (per plan index);
 // ecount: per DAG edge position, |N+(u) ∩ N+(v)| of edge (u, v); vcount: per
 // vertex, the triangles with a corner there.  All null in a plain count.
+// Attribution modes of a local count: none (the plain count); every
+// triangle to its DAG edge, its three corners (slot hit counters for the
+// third) and, through the edges, the parts of a split (full); to its DAG
+// edge only (edges); or only to the part of a split that computes it, one
+// sum per partner row (parts: no per-edge or per-vertex output).
+constexpr int kLocalNone = 0, kLocalFull = 1, kLocalEdges = 2, kLocalParts = 3;
+
 struct LocalOut {
   const int64_t* pedge;
   const int32_t* pvid;
   const int32_t* pown;
   uint32_t* ecount;
   unsigned long long* vcount;
+  const int32_t* bounds = nullptr;  // kLocalParts: split points [parts + 1]
+  int32_t parts = 0;
+  uint64_t* pe = nullptr;           // kLocalParts: per-part sums [parts], zeroed by the caller
 };
 
 struct LaunchArgs {
@@ -368,11 +378,12 @@ static_assert(kEdgesBinOrder.ok, "TC_EDGES_BIN_ORDER: bins 0-5 once each, in at 
 // [row_begin, row_end) are never pivots).
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
                         const uint8_t* own, uint64_t* out, int grid, hipStream_t s);
-// Count of one graph that also fills a.local (ecount and, with vertices,
-// vcount zeroed by the caller): same plan, local variants of bins
-// first..last.  Without vertices only the per-edge counts are written (the
-// kernels keep no slot hit counters: less LDS, more waves per SIMD).
-hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, bool vertices, hipStream_t s);
+// Count of one graph that also fills a.local: same plan, local variants of
+// bins first..last.  mode kLocalFull: ecount and vcount (zeroed by the
+// caller); kLocalEdges: ecount only (the kernels keep no slot hit counters:
+// less LDS, more waves per SIMD); kLocalParts: pe only (bounds, parts, pe
+// set; the plain count's kernels, each row's hits added to its owner's part).
+hipError_t launch_count_local(const LaunchArgs& a, int cus, int first, int last, int mode, hipStream_t s);
 // Per-graph table of launch_local_parts over rows [0, n): esrc[e] = the row
 // u of DAG edge e = (u, v); the local rebuild (launch_build with pedge and
 // esrc) then sets bit 31 where v owns the edge, i.e. d+(u) < d+(v).

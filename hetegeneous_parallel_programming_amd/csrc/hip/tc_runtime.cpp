@@ -912,32 +912,40 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   if (st != TC_OK) return st;
   LocalBufs& L = g->local;
   tck::LaunchArgs a = count_args(ctx, g);
+  // Without vertex_counts the kernels attribute to edges only; with part
+  // sums alone, each partner row's hits go straight to its owner's part
+  // (the part that computes its edges under the reference's push/pull rule),
+  // with no per-edge counts or per-part pass.
+  const bool vertices = out->vertex_counts != nullptr;
+  const bool parts_only = k > 0 && !vertices && !out->edge_counts;
+  const int mode = vertices ? tck::kLocalFull : parts_only ? tck::kLocalParts : tck::kLocalEdges;
   a.local = tck::LocalOut{L.pedge, L.pvid, L.pown, L.ecount, L.vcount};
+  a.local.bounds = L.bounds;
+  a.local.parts = k;
+  a.local.pe = reinterpret_cast<uint64_t*>(L.pe);
   hipStream_t s = ctx->stream;
   if (k > 0)
     TC_HIP_TRY(hipMemcpyAsync(L.bounds, out->bounds, sizeof(int32_t) * ((size_t)k + 1), hipMemcpyHostToDevice, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[0], s));
   TC_HIP_TRY(tck::launch_zero(a.bufs.counters, tck::kNumCounters, s));
   TC_HIP_TRY(tck::launch_zero(a.bufs.outs, 2, s));
-  TC_HIP_TRY(tck::launch_zero(L.ecount, std::max<int64_t>(g->nnz, 1), s));
-  // Without vertex_counts the kernels attribute to edges only.
-  const bool vertices = out->vertex_counts != nullptr;
+  if (!parts_only) TC_HIP_TRY(tck::launch_zero(L.ecount, std::max<int64_t>(g->nnz, 1), s));
   if (vertices) TC_HIP_TRY(tck::launch_zero(L.vcount, 2 * ((int64_t)g->n + 1), s));
   TC_HIP_TRY(tck::launch_zero(L.pe, 2 * (int64_t)tck::kMaxParts, s));
   TC_HIP_TRY(tck::launch_plan(a, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
   {  // bin groups on the library stream and the side streams, as in tc_count
-    const tck::BinGroups& G = vertices ? tck::kBinOrder : tck::kEdgesBinOrder;
+    const tck::BinGroups& G = mode == tck::kLocalEdges ? tck::kEdgesBinOrder : tck::kBinOrder;
     for (int gi = 0; gi < G.n; ++gi) {
       hipStream_t gs = gi == 0 ? s : ctx->side[gi - 1];
       if (gi > 0) TC_HIP_TRY(hipStreamWaitEvent(gs, ctx->ev[1], 0));
       for (int j = 0; j < G.len[gi]; ++j)
-        TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, G.bin[gi][j], G.bin[gi][j], vertices, gs));
+        TC_HIP_TRY(tck::launch_count_local(a, ctx->cus, G.bin[gi][j], G.bin[gi][j], mode, gs));
       if (gi > 0) TC_HIP_TRY(hipEventRecord(ctx->join[gi - 1], gs));
     }
     for (int gi = 1; gi < G.n; ++gi) TC_HIP_TRY(hipStreamWaitEvent(s, ctx->join[gi - 1], 0));
   }
-  if (k > 0)
+  if (k > 0 && !parts_only)
     TC_HIP_TRY(tck::launch_local_parts(g->ci, L.esrc, g->n, g->edge_begin, g->edge_end, L.ecount, L.bounds, k, L.pe,
                                        ctx->cus, s));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
@@ -955,6 +963,11 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
     TC_HIP_TRY(hipMemcpy(out->vertex_counts, L.vcount, sizeof(uint64_t) * (size_t)g->n, hipMemcpyDeviceToHost));
   if (k > 0)
     TC_HIP_TRY(hipMemcpy(out->part_counts, L.pe, sizeof(uint64_t) * (size_t)k, hipMemcpyDeviceToHost));
+  if (parts_only) {  // every row's hits went to a part
+    uint64_t sum = 0;
+    for (int32_t p = 0; p < k; ++p) sum += out->part_counts[p];
+    *total = sum;
+  }
   return finish_stats(ctx, stats, t0, true, g->nv, 1, fallbacks);
 } catch (...) {
   return tcr::caught();

@@ -347,7 +347,9 @@ class Context:
         ``bounds``, the reference push/pull kernel's per-PE sums) and
         ``stats``; outputs not asked for are None.  Without ``vertices`` the
         kernels attribute each triangle to its edge only (same edge and part
-        counts, less work)."""
+        counts, less work); with ``bounds`` alone (no edges, no vertices)
+        each partner row's triangles go to its owner's part (same part
+        counts, the plain count's kernels)."""
         n = int(dg.info()["rows"])
         lo = Local()
         ec = vc = pc = None

@@ -183,14 +183,17 @@ tc_status tc_count_noalloc(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* 
  *                    bs_warp_vertex_nvshmem.cuh:127-136): an edge's count
  *                    goes to u's part when v is in the same part or
  *                    d+(u) >= d+(v), else to v's part.
- * The reference emits none of these; they make a split's attribution
- * checkable.  A call without vertex_counts attributes each triangle to its
- * edge only (edge_counts and part_counts are the same; RMAT-22: 12.4 ms
- * against 19.8 ms with vertex_counts).  One graph only (not batches); the
- * first call adds the partner entries' edge positions, per-edge and
- * per-vertex buffers and the per-part pass's tables (each edge's row, 16-bit
- * row degrees, each row's part) to the graph (tc_graph_get_info reports the
- * bytes). */
+ * The reference emits none of these but the per-PE sums; they make a
+ * split's attribution checkable.  A call without vertex_counts attributes
+ * each triangle to its edge only (edge_counts and part_counts are the same;
+ * RMAT-22: 11.8 ms against 19.5 ms with vertex_counts).  A call with
+ * part_counts alone (edge_counts and vertex_counts null, parts > 0) adds
+ * each partner row's triangles to its owner's part, which is the part that
+ * computes the row's edges under the push/pull rule: no per-edge counts,
+ * the plain count's kernels.  One graph only (not batches); the first call
+ * adds the partner entries' edge positions and owners, per-edge and
+ * per-vertex buffers and the per-part pass's table (each edge's row and push
+ * bit) to the graph (tc_graph_get_info reports the bytes). */
 typedef struct tc_local {
   uint32_t* edge_counts;
   uint64_t* vertex_counts;

@@ -43,6 +43,11 @@ def _check_whole(ctx, g, bounds=None):
     assert np.array_equal(out_e["edge_counts"], out["edge_counts"])
     if bounds is not None:
         assert np.array_equal(out_e["part_counts"], out["part_counts"])
+        # part sums alone: each partner row's hits go to its owner's part, with
+        # no per-edge counts; the same sums and their total
+        total_p, out_p = ctx.count_local(dg, edges=False, vertices=False, bounds=bounds)
+        assert total_p == total and out_p["edge_counts"] is None and out_p["vertex_counts"] is None
+        assert np.array_equal(out_p["part_counts"], out["part_counts"])
     # the plain count still agrees after the local call rebuilt the partner entries
     assert ctx.count_binned(dg)[0] == total
     dg.free()
@@ -152,7 +157,8 @@ def test_rejects_bad_split(ctx):
 
 def test_parts_with_saturated_degrees(ctx):
     """A pushed edge between two rows whose out-degrees both pass the 16-bit
-    saturation: the per-part pass compares their exact degrees.  Vertex 0
+    saturation: the per-part pass (and, for part sums alone, the owner rule
+    of the partner build) compares their exact degrees.  Vertex 0
     points at vertex 1 and 70,000 leaves, vertex 1 at 70,010 leaves, 50,000 of
     them shared; with 0 and 1 in different parts, d+(0) < d+(1) pushes the
     edge (0, 1) and its 50,000 triangles to vertex 1's part."""
@@ -172,4 +178,7 @@ def test_parts_with_saturated_degrees(ctx):
         assert total == 50_000
         assert np.array_equal(out["part_counts"], want)
         assert np.array_equal(out["edge_counts"], oracle.tc_edge_counts(rp, ci))
+    # part sums alone: the edge's owner (vertex 1, by exact degrees) takes it
+    total, out = ctx.count_local(dg, edges=False, vertices=False, bounds=bounds)
+    assert total == 50_000 and np.array_equal(out["part_counts"], want)
     dg.free()

@@ -77,12 +77,12 @@ struct Partners {
 };
 
 // Buffers of a local count of one graph: the partner entries' DAG edge
-// positions and partner vertices, the partner rows' owners, per-edge and
-// per-vertex counts, and per-part sums with their split points.
+// positions and partner vertices, per-edge and per-vertex counts, and
+// per-part sums with their split points (a call with part sums alone
+// allocates only those two; the partner rows' owners belong to the graph).
 struct LocalBufs {
   int64_t* pedge = nullptr;
   int32_t* pvid = nullptr;
-  int32_t* pown = nullptr;
   uint32_t* ecount = nullptr;
   unsigned long long* vcount = nullptr;
   unsigned long long* pe = nullptr;  // [tck::kMaxParts]
@@ -113,7 +113,7 @@ struct BuildScratch {
 // loc: null, or where to write each partner entry's DAG edge position and
 // partner vertex and each partner row's owner (pedge, pvid, pown).
 tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t M, BuildScratch& bs, hipStream_t s,
-                       const LocalBufs* loc = nullptr);
+                       const LocalBufs* loc = nullptr, int32_t* pown = nullptr);
 
 // Makes ctx's device current for the scope of one API call.
 struct DeviceGuard {
@@ -171,6 +171,7 @@ struct tc_graph {
   int32_t row_begin = 0, row_end = 0;
   int64_t edge_begin = 0, edge_end = 0;  // DAG edge positions of rows [row_begin, row_end)
   uint8_t* own = nullptr;                 // null, or per row 1 if it is a pivot row
+  int32_t* pown = nullptr;  // [nv_max + 1] owner vertex of each partner row, written by every build
   int32_t max_out = 0;
   tcr::Partners partners;
   tcr::LocalBufs local;

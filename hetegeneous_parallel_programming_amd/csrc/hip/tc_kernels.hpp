@@ -198,7 +198,7 @@ struct BuildBufs {
   int64_t* part;        // [scan_partials(max(RP, ...))] scan partials
   int64_t* pedge;       // null, or [partner pool]: DAG edge position of each partner entry
   int32_t* pvid;        // with pedge: [partner pool] partner vertex of each entry
-  int32_t* pown;        // with pedge: [P] owner vertex of each partner row
+  int32_t* pown;        // null, or (one graph) [P] owner vertex of each partner row
   // Radix transpose (null when the scratch has no per-id part).  Build space:
   // M ids, graph g's at eb; row slots: RP.
   int32_t* gid;         // [RP] graph of each row slot; ~graph when the row is not split

@@ -233,7 +233,7 @@ void BuildScratch::release() {
 }
 
 tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t M, BuildScratch& bs, hipStream_t s,
-                       const LocalBufs* loc) {
+                       const LocalBufs* loc, int32_t* pown) {
   // The local rebuild also writes each entry's edge position and partner
   // vertex: the radix path carries the positions along (one graph, its
   // edges' rows in loc->esrc, filled before); the per-edge atomic path
@@ -243,7 +243,7 @@ tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t M, BuildScr
   if (st != TC_OK) return st;
   bs.b.pedge = loc ? loc->pedge : nullptr;
   bs.b.pvid = loc ? loc->pvid : nullptr;
-  bs.b.pown = loc ? loc->pown : nullptr;
+  bs.b.pown = pown;  // one graph: each partner row's owner
   bs.b.esrc = loc ? loc->esrc : nullptr;
   TC_HIP_TRY(tck::launch_partners(a, RP, M, bs.b, s));
   return TC_OK;
@@ -252,7 +252,6 @@ tc_status launch_build(const tck::LaunchArgs& a, int64_t RP, int64_t M, BuildScr
 void LocalBufs::release() {
   dev_free(pedge);
   dev_free(pvid);
-  dev_free(pown);
   dev_free(ecount);
   dev_free(vcount);
   dev_free(pe);
@@ -319,6 +318,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
     dev_free(g->ci);
     dev_free(g->d_desc);
     dev_free(g->own);
+    dev_free(g->pown);
     g->partners.release();
   };
   tc_status st = dev_alloc((void**)&g->rp, sizeof(int64_t) * ((size_t)g->n + 1), "rowptr");
@@ -327,6 +327,8 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   if (st == TC_OK) st = dev_alloc((void**)&g->ci, sizeof(int32_t) * (size_t)std::max<int64_t>(g->nnz, 1), "colidx");
   if (st == TC_OK) st = dev_alloc((void**)&g->d_desc, sizeof(tck::GraphDesc), "graph desc");
   if (st == TC_OK) st = g->partners.alloc(g->nnz, (int64_t)g->nv_max + 1, 1);
+  if (st == TC_OK)
+    st = dev_alloc((void**)&g->pown, sizeof(int32_t) * ((size_t)g->nv_max + 1), "partner row owners");
   if (st != TC_OK) {
     cleanup();
     return st;
@@ -360,7 +362,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   }
   e = hipEventRecord(ctx->ev[0], s);
   if (e == hipSuccess) {
-    st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, s);
+    st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, s, nullptr, g->pown);
     if (st == TC_OK) e = hipEventRecord(ctx->ev[1], s);
     if (st == TC_OK && e == hipSuccess)
       e = hipMemcpyAsync(ctx->pinned, g->partners.nv, sizeof(int32_t), hipMemcpyDeviceToHost, s);
@@ -377,7 +379,7 @@ static tc_status upload_csr(tc_ctx* ctx, const tc_csr_view* dag, int32_t row_beg
   g->build_ms = ms;
   g->nv = *(const int32_t*)ctx->pinned;
   g->bytes = sizeof(int64_t) * ((int64_t)g->n + 1) + sizeof(int32_t) * g->nnz + sizeof(tck::GraphDesc) +
-             g->partners.bytes + (owned ? (int64_t)g->n : 0);
+             g->partners.bytes + (owned ? (int64_t)g->n : 0) + sizeof(int32_t) * ((int64_t)g->nv_max + 1);
   *out = g.release();
   return TC_OK;
 }
@@ -560,6 +562,7 @@ tc_status tc_graph_free(tc_ctx* ctx, tc_graph* g) try {
   dev_free(g->ci);
   dev_free(g->d_desc);
   dev_free(g->own);
+  dev_free(g->pown);
   g->partners.release();
   g->local.release();
   g->plan.release();
@@ -858,25 +861,38 @@ tc_status tc_count_noalloc(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* 
 
 // First local count of g: its buffers, and a partner rebuild that also
 // records every partner entry's DAG edge position.
+// The split points and per-part sums of a local count (what a call with
+// part sums alone needs beside the graph's partner-row owners).
+constexpr int64_t kSplitBytes = 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
+static tc_status ensure_split(LocalBufs& L) {
+  if (L.pe) return TC_OK;
+  tc_status st = dev_alloc((void**)&L.bounds, sizeof(int32_t) * (tck::kMaxParts + 1), "split points");
+  if (st == TC_OK) st = dev_alloc((void**)&L.pe, sizeof(unsigned long long) * tck::kMaxParts, "part counts");
+  if (st != TC_OK) {
+    dev_free(L.bounds);
+    L.bounds = nullptr;
+    return st;
+  }
+  L.bytes = std::max(L.bytes, kSplitBytes);
+  return TC_OK;
+}
+
 static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
   LocalBufs& L = g->local;
   if (L.pedge) return TC_OK;
   const int64_t m = std::max<int64_t>(g->nnz, 1);
-  const int64_t P = (int64_t)g->nv_max + 1;
   tc_status st = dev_alloc((void**)&L.pedge, sizeof(int64_t) * (size_t)m, "partner edge positions");
   if (st == TC_OK) st = dev_alloc((void**)&L.pvid, sizeof(int32_t) * (size_t)m, "partner vertices");
-  if (st == TC_OK) st = dev_alloc((void**)&L.pown, sizeof(int32_t) * (size_t)P, "partner row owners");
   if (st == TC_OK) st = dev_alloc((void**)&L.ecount, sizeof(uint32_t) * (size_t)m, "edge counts");
   if (st == TC_OK)
     st = dev_alloc((void**)&L.vcount, sizeof(unsigned long long) * ((size_t)g->n + 1), "vertex counts");
-  if (st == TC_OK) st = dev_alloc((void**)&L.pe, sizeof(unsigned long long) * tck::kMaxParts, "part counts");
-  if (st == TC_OK) st = dev_alloc((void**)&L.bounds, sizeof(int32_t) * (tck::kMaxParts + 1), "split points");
+  if (st == TC_OK) st = ensure_split(L);
   if (st == TC_OK) st = dev_alloc((void**)&L.esrc, sizeof(int32_t) * (size_t)m, "edge rows");
   if (st == TC_OK) {
     tck::LaunchArgs a = count_args(ctx, g);
     BuildScratch bs;
     hipError_t e = tck::launch_local_prep(g->rp, g->n, L.esrc, ctx->stream);  // the rebuild reads and marks esrc
-    if (e == hipSuccess) st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L);
+    if (e == hipSuccess) st = launch_build(a, (int64_t)g->n + 1, g->nnz, bs, ctx->stream, &L, g->pown);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     bs.release();
     if (st == TC_OK && e != hipSuccess)
@@ -887,7 +903,7 @@ static tc_status ensure_local(tc_ctx* ctx, tc_graph* g) {
     return st;
   }
   g->plan_valid = false;  // the rebuild rewrote the partner rows the kept plan lists
-  L.bytes = 20 * m + 4 * P + 8 * ((int64_t)g->n + 1) + 8 * tck::kMaxParts + 4 * (tck::kMaxParts + 1);
+  L.bytes = 20 * m + 8 * ((int64_t)g->n + 1) + kSplitBytes;
   return TC_OK;
 }
 
@@ -907,19 +923,19 @@ tc_status tc_count_local(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_local* ou
   DeviceGuard guard(ctx->device);
   const double t0 = now_ms();
   const int64_t P = (int64_t)g->nv_max + 1;
-  tc_status st = ctx->scratch.ensure(P, 1);
-  if (st == TC_OK) st = ensure_local(ctx, g);
-  if (st != TC_OK) return st;
-  LocalBufs& L = g->local;
-  tck::LaunchArgs a = count_args(ctx, g);
   // Without vertex_counts the kernels attribute to edges only; with part
   // sums alone, each partner row's hits go straight to its owner's part
-  // (the part that computes its edges under the reference's push/pull rule),
-  // with no per-edge counts or per-part pass.
+  // (the part that computes its edges under the reference's push/pull rule):
+  // no per-edge counts, per-part pass or local rebuild.
   const bool vertices = out->vertex_counts != nullptr;
   const bool parts_only = k > 0 && !vertices && !out->edge_counts;
   const int mode = vertices ? tck::kLocalFull : parts_only ? tck::kLocalParts : tck::kLocalEdges;
-  a.local = tck::LocalOut{L.pedge, L.pvid, L.pown, L.ecount, L.vcount};
+  tc_status st = ctx->scratch.ensure(P, 1);
+  if (st == TC_OK) st = parts_only ? ensure_split(g->local) : ensure_local(ctx, g);
+  if (st != TC_OK) return st;
+  LocalBufs& L = g->local;
+  tck::LaunchArgs a = count_args(ctx, g);
+  a.local = tck::LocalOut{L.pedge, L.pvid, g->pown, L.ecount, L.vcount};
   a.local.bounds = L.bounds;
   a.local.parts = k;
   a.local.pe = reinterpret_cast<uint64_t*>(L.pe);

@@ -190,10 +190,12 @@ tc_status tc_count_noalloc(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* 
  * part_counts alone (edge_counts and vertex_counts null, parts > 0) adds
  * each partner row's triangles to its owner's part, which is the part that
  * computes the row's edges under the push/pull rule: no per-edge counts,
- * the plain count's kernels.  One graph only (not batches); the first call
- * adds the partner entries' edge positions and owners, per-edge and
- * per-vertex buffers and the per-part pass's table (each edge's row and push
- * bit) to the graph (tc_graph_get_info reports the bytes). */
+ * the plain count's kernels, and no local buffers beyond the split points
+ * and sums (the partner rows' owners come with every upload).  One graph
+ * only (not batches); the first call with edge or vertex outputs adds the
+ * partner entries' edge positions, per-edge and per-vertex buffers and the
+ * per-part pass's table (each edge's row and push bit) to the graph
+ * (tc_graph_get_info reports the bytes). */
 typedef struct tc_local {
   uint32_t* edge_counts;
   uint64_t* vertex_counts;

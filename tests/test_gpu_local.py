@@ -182,3 +182,31 @@ def test_parts_with_saturated_degrees(ctx):
     total, out = ctx.count_local(dg, edges=False, vertices=False, bounds=bounds)
     assert total == 50_000 and np.array_equal(out["part_counts"], want)
     dg.free()
+
+
+def test_part_sums_alone_on_a_fresh_upload(ctx):
+    """Part sums alone need only the partner rows' owners, which every upload
+    keeps: the first such call adds just the split points and sums (no
+    per-edge buffers, no rebuild), matches the oracle's push/pull sums, and
+    agrees with a full call made afterwards; an owned-row mask too."""
+    g = Graph.rmat(1 << 15, 16, 8).oriented()
+    rp, ci = g.out_rowptr(), g.out_colidx()
+    dg = ctx.upload(g)
+    before = dg.info()["bytes"]
+    for balance in ("vertices", "partner"):
+        bounds = PartitionedGraph(g, 4, balance).bounds()
+        total, out = ctx.count_local(dg, edges=False, vertices=False, bounds=bounds)
+        assert total == oracle.tc_count(rp, ci)
+        assert np.array_equal(out["part_counts"], oracle.tc_pushpull_bounds(rp, ci, bounds))
+    assert dg.info()["bytes"] - before < 16 * 1024
+    total_f, out_f = ctx.count_local(dg, bounds=bounds)  # rebuilds the partner entries
+    total_p, out_p = ctx.count_local(dg, edges=False, vertices=False, bounds=bounds)
+    assert total_p == total_f and np.array_equal(out_p["part_counts"], out_f["part_counts"])
+    dg.free()
+    odd = (np.arange(g.V()) % 2).astype(np.uint8)
+    dg = ctx.upload(g, owned=odd)
+    total_p, out_p = ctx.count_local(dg, edges=False, vertices=False, bounds=bounds)
+    total_f, out_f = ctx.count_local(dg, bounds=bounds)
+    assert total_p == total_f == oracle.tc_count_rows(rp, ci, np.flatnonzero(odd))
+    assert np.array_equal(out_p["part_counts"], out_f["part_counts"])
+    dg.free()

@@ -265,7 +265,7 @@ def local_counts(ctx, g, dg, total, steps=3, warmup=1):
 
     out = {}
     # the first local call rebuilds the partner structure with each entry's
-    # edge position and fills the per-part tables; no host outputs here
+    # edge position and writes the per-part pass's row words; no host outputs here
     t0 = time.perf_counter()
     ctx.count_local(dg, edges=False, vertices=False)
     first_ms = (time.perf_counter() - t0) * 1e3
@@ -273,7 +273,7 @@ def local_counts(ctx, g, dg, total, steps=3, warmup=1):
     ctx.count_local(dg, edges=False, vertices=False)
     next_ms = (time.perf_counter() - t0) * 1e3
     out["first_call_ms"], out["next_call_ms"] = first_ms, next_ms
-    log(f"local: first call {first_ms:.1f} ms (partner rebuild with edge positions, per-part tables), "
+    log(f"local: first call {first_ms:.1f} ms (partner rebuild with edge positions, per-part row words), "
         f"next {next_ms:.1f} ms (no host outputs)")
     for balance in ("vertices", "partner"):
         bounds = PartitionedGraph(g, 4, balance).bounds()

@@ -154,7 +154,12 @@ struct tc_ctx {
   tcr::PlanScratch scratch;
   uint64_t* pinned = nullptr;  // 64 B count + 64 B bin counters
   void* noalloc = nullptr;     // device address of the static scratch of tc_count_noalloc
-  uint64_t* plain_out = nullptr;  // device: the single-launch count's word, zero between calls
+  // device: the single-launch count's two words, used in turn; each launch
+  // clears the other one for the next call (plain_dirty: a call that did
+  // not finish left them unknown)
+  uint64_t* plain_out = nullptr;
+  int plain_turn = 0;
+  bool plain_dirty = false;
   // tc_count_csr's copy of a small graph: rowptr, then colidx at the next
   // 256-byte boundary (grow-only, kept until destroy)
   uint8_t* once_dev = nullptr;

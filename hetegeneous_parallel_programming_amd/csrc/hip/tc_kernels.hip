@@ -2177,9 +2177,12 @@ struct __align__(16) PlainLds {
 __global__ __launch_bounds__(kBlock, TC_COUNT_WAVES_PER_EU) void plain_kernel(const int64_t* __restrict__ rp,
                                                        const int32_t* __restrict__ ci,
                                                        int32_t row_begin, int32_t row_end,
-                                                       const uint8_t* own, uint64_t* out) {
+                                                       const uint8_t* own, uint64_t* out,
+                                                       uint64_t* zero_next) {
   __shared__ PlainLds L;
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the next call's word, cleared here instead of by a launch of its own
+  if (zero_next && blockIdx.x == 0 && threadIdx.x == 0) *zero_next = 0;
   const int64_t nw = (int64_t)gridDim.x * kWaves;
   uint64_t acc = 0;
   for (int64_t u = row_begin + (int64_t)blockIdx.x * kWaves + wid; u < row_end; u += nw) {
@@ -2439,8 +2442,8 @@ hipError_t launch_local_parts(const int32_t* ci, const int32_t* esrc, int64_t n,
 }
 
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                        const uint8_t* own, uint64_t* out, int grid, hipStream_t s) {
-  plain_kernel<<<grid, kBlock, 0, s>>>(rp, ci, row_begin, row_end, own, out);
+                        const uint8_t* own, uint64_t* out, int grid, hipStream_t s, uint64_t* zero_next) {
+  plain_kernel<<<grid, kBlock, 0, s>>>(rp, ci, row_begin, row_end, own, out, zero_next);
   return hipGetLastError();
 }
 

@@ -375,9 +375,10 @@ static_assert(kBinOrder.ok, "TC_BIN_ORDER: bins 0-5 once each, in at most 4 '|'-
 constexpr BinGroups kEdgesBinOrder = parse_bin_groups(TC_EDGES_BIN_ORDER);
 static_assert(kEdgesBinOrder.ok, "TC_EDGES_BIN_ORDER: bins 0-5 once each, in at most 4 '|'-separated groups");
 // own: null, or per row 1 if it is a pivot row (rows outside
-// [row_begin, row_end) are never pivots).
+// [row_begin, row_end) are never pivots).  out must hold 0; zero_next (null,
+// or another word) is set to 0 by the kernel.
 hipError_t launch_plain(const int64_t* rp, const int32_t* ci, int32_t row_begin, int32_t row_end,
-                        const uint8_t* own, uint64_t* out, int grid, hipStream_t s);
+                        const uint8_t* own, uint64_t* out, int grid, hipStream_t s, uint64_t* zero_next = nullptr);
 // Count of one graph that also fills a.local: same plan, local variants of
 // bins first..last.  mode kLocalFull: ecount and vcount (zeroed by the
 // caller); kLocalEdges: ecount only (the kernels keep no slot hit counters:

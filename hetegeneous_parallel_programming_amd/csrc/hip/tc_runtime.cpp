@@ -730,14 +730,27 @@ static tc_status plain_count(tc_ctx* ctx, tc_graph* g, uint64_t* out, uint64_t* 
   const int64_t P = (int64_t)g->row_end - g->row_begin;
   hipStream_t s = ctx->stream;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->plain_grid, (P + tck::kWaves - 1) / tck::kWaves));
-  if (!kept_zero) TC_HIP_TRY(tck::launch_zero(out, 2, s));
+  // kept_zero: out is the context's pair of words, one per call in turn
+  uint64_t* word = out;
+  uint64_t* next = nullptr;
+  if (kept_zero) {
+    if (ctx->plain_dirty) TC_HIP_TRY(tck::launch_zero(out, 4, s));
+    ctx->plain_dirty = true;
+    word = out + ctx->plain_turn;
+    next = out + (1 - ctx->plain_turn);
+  } else {
+    TC_HIP_TRY(tck::launch_zero(out, 2, s));
+  }
   TC_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, g->own, out, grid, s));
+  TC_HIP_TRY(tck::launch_plain(g->rp, g->ci, g->row_begin, g->row_end, g->own, word, grid, s, next));
   TC_HIP_TRY(hipEventRecord(ctx->ev[2], s));
-  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, out, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TC_HIP_TRY(hipMemcpyAsync(ctx->pinned, word, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   TC_HIP_TRY(hipStreamSynchronize(s));
   *total = ctx->pinned[0];
-  if (kept_zero) TC_HIP_TRY(tck::launch_zero(out, 2, s));
+  if (kept_zero) {
+    ctx->plain_turn = 1 - ctx->plain_turn;
+    ctx->plain_dirty = false;
+  }
   tc_status st = finish_stats(ctx, stats, t0, false, P);
   if (st == TC_OK && stats) stats->single_launch = 1;
   return st;

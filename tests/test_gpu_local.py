@@ -117,16 +117,25 @@ def test_partitions_add_up_to_the_whole(ctx):
     for balance in ("vertices", "partner"):
         acc[:] = 0
         pg = PartitionedGraph(g, 3, balance)
+        bounds = pg.bounds()
+        parts_acc = np.zeros(3, np.uint64)
         for p in range(3):
             part = pg.halo_partition(p)
             dg = ctx.upload_partition(part)
-            total, out = ctx.count_local(dg)
+            # part sums alone first (no local rebuild yet): an edge's owner may
+            # be a halo vertex, whose degree comes from the halo cache
+            total_p, out_p = ctx.count_local(dg, edges=False, vertices=False, bounds=bounds)
+            total, out = ctx.count_local(dg, bounds=bounds)
             b, e = part.row_begin, part.row_end
             assert np.array_equal(out["edge_counts"], want_e[rp[b]:rp[e]])
             assert total == int(out["edge_counts"].sum())
+            assert total_p == total and np.array_equal(out_p["part_counts"], out["part_counts"])
+            parts_acc += out_p["part_counts"]
             acc += out["vertex_counts"]
             dg.free()
         assert np.array_equal(acc, want_v)
+        # over the partitions, each PE's sum is the whole graph's push/pull sum
+        assert np.array_equal(parts_acc, oracle.tc_pushpull_bounds(rp, ci, bounds))
 
 
 def test_repeated_calls_and_partial_outputs(ctx):

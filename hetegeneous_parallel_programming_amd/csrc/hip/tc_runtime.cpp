@@ -786,23 +786,33 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats) t
 
 tc_status tc_count_csr(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, uint64_t* total, tc_stats* stats) try {
   if (!ctx || !total) return fail(TC_EINVAL, "null argument");
+  if (!dag) return fail(TC_EINVAL, "null csr view");
   int32_t max_out = 0;
-  tc_status st = check_dag(dag, check, &max_out);
-  if (st != TC_OK) return st;
+  tc_status st;
+  auto whole = [&]() {  // upload + tc_count + free
+    tc_graph* g = nullptr;
+    tc_status s2 = upload_csr(ctx, dag, 0, dag->n_rows, nullptr, max_out, &g);
+    if (s2 != TC_OK) return s2;
+    s2 = tc_count(ctx, g, total, stats);
+    tc_graph_free(ctx, g);
+    return s2;
+  };
+  // A graph small enough by its id count for the single plain launch (no
+  // partner structure) is copied to the context's grow-only buffer first,
+  // and checked on the host while the copies run; the launch follows the
+  // check.  (Staging the arrays in pinned memory for one copy measured no
+  // faster on MI355X.)
+  const bool small = dag->n_rows >= 0 && dag->nnz >= 0 && dag->nnz <= tcr::kPlainMaxIds && dag->rowptr &&
+                     (dag->nnz == 0 || dag->colidx);
+  if (!small) {
+    st = check_dag(dag, check, &max_out);
+    if (st != TC_OK) return st;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    return whole();
+  }
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
-  if (dag->nnz > tcr::kPlainMaxIds || max_out >= tck::kCapWave) {  // the whole graph path
-    tc_graph* g = nullptr;
-    st = upload_csr(ctx, dag, 0, dag->n_rows, nullptr, max_out, &g);
-    if (st != TC_OK) return st;
-    st = tc_count(ctx, g, total, stats);
-    tc_graph_free(ctx, g);
-    return st;
-  }
-  // A graph tc_count would count by the single plain launch: no partner
-  // structure; its arrays go to the context's grow-only buffer. (Staging
-  // them in pinned memory for one copy measured no faster on MI355X: an ER
-  // n=16384 d=16 call took ~0.10 ms either way.)
   const size_t rp_bytes = sizeof(int64_t) * ((size_t)dag->n_rows + 1);
   const size_t ci_off = (rp_bytes + 255) & ~(size_t)255;
   const size_t bytes = ci_off + sizeof(int32_t) * (size_t)std::max<int64_t>(dag->nnz, 1);
@@ -815,10 +825,19 @@ tc_status tc_count_csr(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, uint
     ctx->once_cap = bytes;
   }
   hipStream_t s = ctx->stream;
-  TC_HIP_TRY(hipMemcpyAsync(ctx->once_dev, dag->rowptr, rp_bytes, hipMemcpyHostToDevice, s));
-  if (dag->nnz > 0)
-    TC_HIP_TRY(hipMemcpyAsync(ctx->once_dev + ci_off, dag->colidx, sizeof(int32_t) * (size_t)dag->nnz,
-                              hipMemcpyHostToDevice, s));
+  hipError_t ce = hipMemcpyAsync(ctx->once_dev, dag->rowptr, rp_bytes, hipMemcpyHostToDevice, s);
+  if (ce == hipSuccess && dag->nnz > 0)
+    ce = hipMemcpyAsync(ctx->once_dev + ci_off, dag->colidx, sizeof(int32_t) * (size_t)dag->nnz,
+                        hipMemcpyHostToDevice, s);
+  if (ce != hipSuccess) {
+    (void)hipStreamSynchronize(s);  // a copy already queued may still read the caller's arrays
+    return fail(TC_EDEVICE, std::string("one-shot copy: ") + hipGetErrorString(ce));
+  }
+  st = check_dag(dag, check, &max_out);
+  if (st != TC_OK || max_out >= tck::kCapWave) {
+    TC_HIP_TRY(hipStreamSynchronize(s));  // the copies read the caller's arrays
+    return st != TC_OK ? st : whole();
+  }
   tc_graph g;  // a view of the copy: no partners, nothing to free
   g.ctx = ctx;
   g.rp = reinterpret_cast<int64_t*>(ctx->once_dev);

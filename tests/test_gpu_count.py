@@ -335,6 +335,15 @@ def test_count_csr_one_shot(ctx):
         max_out = int(np.diff(g.out_rowptr()).max()) if g.V() else 0
         assert st["single_launch"] == (g.E() <= 1 << 18 and max_out < 1024)
         assert TCSolver(g, ctx=ctx) == want
+    # few ids but one out-list of 1100 (past a wave's LDS set): copied, then
+    # counted through the whole-graph path; row 0 -> 1..1100, i -> i + 1
+    k = 1100
+    rp = np.concatenate([[0], k + np.arange(0, k + 1)]).astype(np.int64)
+    rp[-1] = rp[-2]  # vertex 1100 has no out-edge
+    ci = np.concatenate([np.arange(1, k + 1), np.arange(2, k + 1)]).astype(np.int32)
+    fan = Graph.from_csr(rp, ci)
+    got, st = ctx.count_csr(fan)
+    assert got == oracle.tc_count(rp, ci) == k - 1 and not st["single_launch"]
     rp = np.array([0, 2, 3], np.int64)
     with pytest.raises(TcError):
         ctx.count_csr(Graph.from_csr(rp, np.array([1, 1, 0], np.int32), check=0), check=7)

@@ -527,7 +527,7 @@ def TCSolver(g: Graph, n_gpus: int = 1, chunk_size: int = 1024, ctx: Context | N
     given, receives the count's statistics (launch form, plan and count ms).
     """
     if n_gpus != 1:
-        raise TcError(nat.TC_EINVAL, "this build drives exactly one GPU (n_gpus must be 1)")
+        raise TcError(nat.TC_EINVAL, "this library drives exactly one GPU (n_gpus must be 1)")
     own = ctx is None
     c = Context(0) if own else ctx
     try:

@@ -406,7 +406,7 @@ inline void tc_solve(Graph& g, uint64_t& total, int n_gpus, int chunk_size) {
   (void)chunk_size;
   if (!g.is_metis()) {
     if (n_gpus != 1)
-      throw Error(TC_EINVAL, "this build drives exactly one GPU per process: a whole graph needs n_gpus = 1 (METIS parts take the PE count)");
+      throw Error(TC_EINVAL, "this library drives exactly one GPU per process: a whole graph needs n_gpus = 1 (METIS parts take the PE count)");
     Device dev(pe_device());
     total = dev.count(g);
     return;

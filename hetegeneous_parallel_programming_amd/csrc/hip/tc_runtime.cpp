@@ -760,14 +760,29 @@ static tc_status count_plain(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats
   return plain_count(ctx, g, ctx->plain_out, total, stats, true);
 }
 
-// Graphs whose counted rows hold at most this many DAG ids, and whose lists
-// all fit a wave's LDS tree, are counted by the single plain launch in
-// tc_count: below this size its one launch beats planning and the binned
-// kernels' launches (measured on MI355X).
+// Graphs that tc_count counts by the single plain launch (every list must
+// fit a wave's LDS tree): those whose counted rows hold at most kPlainMaxIds
+// DAG ids, where one launch beats planning and the binned kernels' launches;
+// and flat graphs up to kPlainFlatMaxIds ids -- at most kPlainFlatMaxRows
+// rows and out-lists of at most kPlainFlatMaxOut, which bounds the plain
+// kernel's per-edge work -- where it still does.  Measured on MI355X (plain
+// against binned, resident): ER n=65536 d=32 0.086 / 0.200 ms, ER n=131072
+// d=32 0.136 / 0.211 ms, ER n=131072 d=16 0.102 / 0.126 ms; against that,
+// ER n=262144 d=8 (more rows) 0.150 / 0.114 ms and RMAT n=32768 d=16 (lists
+// to 189) 0.392 / 0.263 ms.
 constexpr int64_t kPlainMaxIds = (int64_t)1 << 18;
+constexpr int64_t kPlainFlatMaxIds = (int64_t)1 << 21;
+constexpr int64_t kPlainFlatMaxRows = (int64_t)1 << 17;
+constexpr int32_t kPlainFlatMaxOut = 64;
+
+static bool plain_fits(int64_t rows, int64_t ids, int32_t max_out) {
+  if (max_out >= tck::kCapWave) return false;
+  return ids <= kPlainMaxIds ||
+         (ids <= kPlainFlatMaxIds && rows <= kPlainFlatMaxRows && max_out <= kPlainFlatMaxOut);
+}
 
 static bool small_graph(const tc_graph* g) {
-  return g->edge_end - g->edge_begin <= kPlainMaxIds && g->max_out < tck::kCapWave;
+  return plain_fits((int64_t)g->row_end - g->row_begin, g->edge_end - g->edge_begin, g->max_out);
 }
 
 }  // namespace tcr
@@ -797,22 +812,22 @@ tc_status tc_count_csr(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, uint
     tc_graph_free(ctx, g);
     return s2;
   };
-  // A graph small enough by its id count for the single plain launch (no
-  // partner structure) is copied to the context's grow-only buffer first,
-  // and checked on the host while the copies run; the launch follows the
-  // check.  (Staging the arrays in pinned memory for one copy measured no
-  // faster on MI355X.)
+  // A graph that tc_count would count by the single plain launch needs no
+  // partner structure: it is copied to the context's grow-only buffer and
+  // counted there.  One of at most kPlainMaxIds ids is copied first and
+  // checked on the host while the copies run (the launch follows the
+  // check); a larger one is checked first, so that one that turns out to
+  // need the whole-graph path is not copied for nothing.  (Staging the
+  // arrays in pinned memory for one copy measured no faster on MI355X.)
   const bool small = dag->n_rows >= 0 && dag->nnz >= 0 && dag->nnz <= tcr::kPlainMaxIds && dag->rowptr &&
                      (dag->nnz == 0 || dag->colidx);
   if (!small) {
     st = check_dag(dag, check, &max_out);
     if (st != TC_OK) return st;
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    DeviceGuard guard(ctx->device);
-    return whole();
   }
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   DeviceGuard guard(ctx->device);
+  if (!small && !tcr::plain_fits(dag->n_rows, dag->nnz, max_out)) return whole();
   const size_t rp_bytes = sizeof(int64_t) * ((size_t)dag->n_rows + 1);
   const size_t ci_off = (rp_bytes + 255) & ~(size_t)255;
   const size_t bytes = ci_off + sizeof(int32_t) * (size_t)std::max<int64_t>(dag->nnz, 1);
@@ -833,10 +848,12 @@ tc_status tc_count_csr(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, uint
     (void)hipStreamSynchronize(s);  // a copy already queued may still read the caller's arrays
     return fail(TC_EDEVICE, std::string("one-shot copy: ") + hipGetErrorString(ce));
   }
-  st = check_dag(dag, check, &max_out);
-  if (st != TC_OK || max_out >= tck::kCapWave) {
-    TC_HIP_TRY(hipStreamSynchronize(s));  // the copies read the caller's arrays
-    return st != TC_OK ? st : whole();
+  if (small) {
+    st = check_dag(dag, check, &max_out);
+    if (st != TC_OK || !tcr::plain_fits(dag->n_rows, dag->nnz, max_out)) {
+      TC_HIP_TRY(hipStreamSynchronize(s));  // the copies read the caller's arrays
+      return st != TC_OK ? st : whole();
+    }
   }
   tc_graph g;  // a view of the copy: no partners, nothing to free
   g.ctx = ctx;

@@ -130,7 +130,7 @@ tc_status tc_count(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
  * TCSolver(Graph&, uint64_t& total, ...) (successful_push_pull/main.cc:5;
  * multigpu_nvshmem_pushpull.cu:29-85: device copy, launch, copy-back).
  * check as for tc_graph_upload.  A graph that tc_count would count by its
- * single plain launch (at most 2^18 DAG ids, lists below 1024) is copied
+ * single plain launch (see below) is copied
  * into a device buffer the context keeps for this call (grown when needed)
  * and counted without a partner structure; any other graph is uploaded,
  * counted and freed.  stats->single_launch tells which. */
@@ -140,10 +140,11 @@ tc_status tc_count_csr(tc_ctx* ctx, const tc_csr_view* dag, uint32_t check, uint
  * graph's first binned count and kept in the graph (device bytes: 10 per
  * partner-row slot) for later counts under the same tc_tune. */
 tc_status tc_count_binned(tc_ctx* ctx, tc_graph* g, uint64_t* total, tc_stats* stats);
-/* tc_count on a graph whose counted rows hold at most 2^18 DAG ids, with
- * every list under 1024 ids, runs tc_count_plain's single launch instead
- * (stats->single_launch = 1): at that size one launch is faster than
- * planning plus the binned kernels' launches.  Both are exact.
+/* tc_count on a graph whose every list is under 1024 ids and whose counted
+ * rows hold at most 2^18 DAG ids -- or, for a flat graph, at most 2^21 ids
+ * in at most 2^17 rows with lists of at most 64 -- runs tc_count_plain's
+ * single launch instead (stats->single_launch = 1): there one launch is
+ * faster than planning plus the binned kernels' launches.  Both are exact.
  * tc_count_plain: one launch of the plain kernel, one wave64 per pivot,
  * grid-stride over pivots at full residency -- the direct restatement of
  * warp_vertex (bs_warp_vertex.cuh:2-35, grid a17 of

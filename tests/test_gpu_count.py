@@ -253,21 +253,33 @@ def test_ids_beyond_2_24_in_hash_sets(ctx):
     assert st["hash_fallbacks"] == 0, st
 
 
+def _plain_fits(g):
+    """The single-launch rule: lists below 1024 and at most 2^18 DAG ids, or
+    a flat graph of at most 2^21 ids, 2^17 rows and out-lists of 64."""
+    mo = int(np.diff(g.out_rowptr()).max()) if g.V() else 0
+    return mo < 1024 and (g.E() <= 1 << 18 or (g.E() <= 1 << 21 and g.V() <= 1 << 17 and mo <= 64))
+
+
 def test_tc_count_picks_the_single_launch_for_small_graphs(ctx):
     """tc_count (and TCSolver through it) runs the plain kernel's single
     launch for a graph of at most 2^18 DAG ids (ER n=16384 d=16, seeds 1-3:
-    ~131k) and the binned count above that, whose plan it keeps after the
-    first call; every count is the oracle's."""
+    ~131k) and for a flat one up to 2^21 ids (ER n=65536 d=32: 1M ids, lists
+    below 64), and the binned count for a skewed one (RMAT 2^16, lists to
+    253) or one with more rows (ER n=262144 d=8), whose plan it keeps after
+    the first call; every count is the oracle's."""
     from hetegeneous_parallel_programming_amd.solver import TCSolver
 
     smalls = [Graph.er(16384, 16, seed).oriented() for seed in (1, 2, 3)]
+    flat = Graph.er(65536, 32, 4).oriented()
     big = Graph.rmat(1 << 16, 16, 3).oriented()
-    assert all(g.E() <= 1 << 18 for g in smalls) and big.E() > 1 << 18
-    for g in smalls:  # the reference's boundary takes the same path
+    many_rows = Graph.er(262144, 8, 5).oriented()
+    assert all(g.E() <= 1 << 18 for g in smalls) and big.E() > 1 << 18 and flat.E() > 1 << 18
+    assert _plain_fits(flat) and not _plain_fits(big) and not _plain_fits(many_rows)
+    for g in smalls + [flat]:  # the reference's boundary takes the same path
         st = {}
         assert TCSolver(g, ctx=ctx, stats=st) == oracle.tc_count(g.out_rowptr(), g.out_colidx())
         assert st["single_launch"]
-    for g, single in [(s, True) for s in smalls] + [(big, False)]:
+    for g, single in [(s, True) for s in smalls] + [(flat, True), (big, False), (many_rows, False)]:
         want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
         dg = ctx.upload(g)
         (c1, st1), (c2, st2) = ctx.count(dg), ctx.count(dg)
@@ -332,8 +344,7 @@ def test_count_csr_one_shot(ctx):
         want = oracle.tc_count(g.out_rowptr(), g.out_colidx())
         got, st = ctx.count_csr(g)
         assert got == want, (got, want)
-        max_out = int(np.diff(g.out_rowptr()).max()) if g.V() else 0
-        assert st["single_launch"] == (g.E() <= 1 << 18 and max_out < 1024)
+        assert st["single_launch"] == _plain_fits(g)
         assert TCSolver(g, ctx=ctx) == want
     # few ids but one out-list of 1100 (past a wave's LDS set): copied, then
     # counted through the whole-graph path; row 0 -> 1..1100, i -> i + 1
